@@ -1,0 +1,183 @@
+/*
+ * pic_hip.h - C ABI of the MI355X (gfx950) charge-deposition library.
+ *
+ * Plain pointers and sizes only.  Every call returns a pic_status; on failure
+ * pic_hip_last_error() holds a message for the calling thread.  No call throws.
+ *
+ * Reference interfaces replaced (paths relative to the ehocine/pic-deposition
+ * repository root):
+ *   - depositChargeCudaSoA / depositChargeCudaAoS
+ *       src/deposition/cuda/deposition_cuda.hpp:11-12,
+ *       src/deposition/cuda/deposition_cuda.cu:288-337
+ *     -> pic_hip_deposit_host (synchronous drop-in: FP64 host in, FP64 rho out).
+ *   - the device side of depositChargeSoA / depositChargeTimedSoA
+ *       src/deposition/deposition.cpp:120-132,181-200,257-310
+ *     -> pic_hip_create / pic_hip_upload / pic_hip_deposit / pic_hip_get_rho
+ *        (device-resident particles, asynchronous on the context's stream).
+ *   - sortParticlesByCell / particleCellIndex
+ *       src/sorting/cell_sort.cpp:8-46, cell_sort.hpp:8-11
+ *     -> pic_hip_sort, pic_hip_sort_perm, pic_hip_get_keys.
+ *   - gatherFieldsSoA / pushParticlesSoA   src/deposition/deposition.cpp:335-374
+ *     -> pic_hip_gather_push.
+ *   - PoissonSolverFFT::solve               src/pic/poisson_fft.cpp:45-94
+ *     -> pic_hip_poisson.
+ *   - FieldGrid::integratedRho / fieldEnergy, ParticlesSoA::totalCharge /
+ *     kineticEnergy, totalAbsChargeSoA      src/pic/field_grid.cpp:24-38,
+ *     src/pic/particles_aos.cpp:182-196, src/pic/diagnostics.cpp:18-51
+ *     -> pic_hip_diagnostics (FP64 sums on the device).
+ *   - applyNeutralizingBackground           src/pic/validation.cpp:56-61,
+ *     src/pic/simulation.cpp:52-59
+ *     -> pic_hip_add_background.
+ */
+#ifndef PIC_HIP_H
+#define PIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PIC_OK = 0,
+    PIC_ERR_INVALID_ARGUMENT = 1,
+    PIC_ERR_HIP = 2,             /* HIP runtime error (message has the HIP error string) */
+    PIC_ERR_OUT_OF_MEMORY = 3,
+    PIC_ERR_UNSUPPORTED = 4,     /* e.g. Esirkepov move longer than 15 cells */
+    PIC_ERR_NOT_SORTED = 5,      /* GPU_Priv deposit without a cell order (call pic_hip_sort) */
+    PIC_ERR_FFT = 6
+} pic_status;
+
+/* DepositionScheme, reference src/deposition/deposition.hpp:11, same order as
+ * schemeToInt (deposition_cuda.cu:275-286).  3 is the reference's
+ * trajectory-averaged Esirkepov density (ngp.cpp:13-84), computed for real
+ * (the reference CUDA path substitutes CIC). */
+typedef enum { PIC_SCHEME_NGP = 0, PIC_SCHEME_CIC = 1, PIC_SCHEME_TSC = 2, PIC_SCHEME_ESIRKEPOV = 3 } pic_scheme;
+
+/* DepositionBackend, reference deposition.hpp:13 (names CPU / GPU_Atomics /
+ * GPU_Priv, deposition.cpp:390-415).  CPU is not served by this library. */
+typedef enum { PIC_BACKEND_CPU = 0, PIC_BACKEND_GPU_ATOMICS = 1, PIC_BACKEND_GPU_PRIV = 2 } pic_backend;
+
+/* pic::Domain (src/pic/domain.hpp:7-44): dx = Lx / Nx, dy = Ly / Ny. */
+typedef struct {
+    double Lx, Ly;
+    int32_t Nx, Ny;
+} pic_domain;
+
+/* FP64 device sums, see pic_hip_diagnostics. */
+typedef struct {
+    double rho_sum;          /* sum of rho over cells */
+    double integrated_rho;   /* rho_sum * cellVolume          (field_grid.cpp:24-30) */
+    double field_energy;     /* 0.5 * sum(Ex^2+Ey^2) * cellVolume (field_grid.cpp:32-38) */
+    double kinetic_energy;   /* sum 0.5 m v^2                 (particles_aos.cpp:190-196) */
+    double total_charge;     /* sum q                          (particles_aos.cpp:182-188) */
+    double total_abs_charge; /* sum |q|                        (diagnostics.cpp:18-24) */
+    double charge_error;     /* |integrated_rho - total_charge| / max(sum|q|, 1e-12) (diagnostics.cpp:26-29) */
+} pic_diagnostics;
+
+/* On-device particle initialiser (build-defined; Philox4x32-10 draws keyed by
+ * seed, counter = particle index, so the result does not depend on launch
+ * shape).  kind 0: x, y uniform; kind 1: x drawn from n(x) ~ 1 + amp*cos(k x)
+ * (k = 2 pi mode / Lx), y uniform; kind 2: x, y uniform, first half of the
+ * particles at vx = +amp, second half at -amp (the split of
+ * initializeTwoStream, particles_aos.cpp:142-160).  Every kind then adds
+ * pert*sin(k x) to vx, N(0, vth^2) to vx and vy, and U(-vmax, vmax) to vx and
+ * vy.  Positions are rounded to FP32 and wrapped into [0, L). */
+typedef struct {
+    int32_t kind;
+    uint64_t seed;
+    double amp, pert, vth, vmax;
+    int32_t mode;
+    double q, m;
+} pic_init_params;
+
+typedef struct pic_hip_ctx pic_hip_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pic_hip_last_error(void);
+const char* pic_hip_version(void);
+
+/* ---- drop-in seam --------------------------------------------------------
+ * Replaces depositChargeCudaSoA (deposition_cuda.cu:335-337 -> launchDeposition
+ * :288-326).  Synchronous; caller-owned FP64 SoA in (never reordered), FP64 rho
+ * out (all Nx*Ny cells overwritten, as the reference's device memset + D2H).
+ * Positions and charges are rounded to FP32 on the way in.  backend
+ * GPU_ATOMICS deposits in the given order; GPU_PRIV sorts a device copy by
+ * cell first.  `sorted` mirrors DepositionConfig::sorted: the device copy is
+ * cell-sorted before depositing (for GPU_PRIV it always is).  vx, vy and dt
+ * are read only for the Esirkepov scheme (DepositionConfig::esirkepov_dt). */
+pic_status pic_hip_deposit_host(const double* x, const double* y, const double* vx, const double* vy,
+                                const double* q, size_t n, const pic_domain* domain, int scheme, int backend,
+                                int sorted, double dt, double* rho_out);
+
+/* ---- device-resident context --------------------------------------------
+ * One HIP stream per context; calls enqueue work asynchronously on it and are
+ * ordered; pic_hip_sync waits and reports kernel-side failures.  A context is
+ * not thread-safe; use one per host thread. */
+pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx** out);
+pic_status pic_hip_destroy(pic_hip_ctx* ctx);
+/* The context's hipStream_t, for event timing by the caller. */
+void* pic_hip_stream(pic_hip_ctx* ctx);
+pic_status pic_hip_sync(pic_hip_ctx* ctx);
+size_t pic_hip_count(pic_hip_ctx* ctx);
+
+/* LDS tiling of the GPU_PRIV deposit: tiles of tile_x by tile_y cells,
+ * particles_per_lane in {4, 8, 16}.  Defaults are chosen at create time. */
+pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int tile_x, int tile_y, int particles_per_lane);
+pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int* tile_x, int* tile_y, int* particles_per_lane);
+
+/* Particles: FP32 SoA x, y, vx, vy, q, m.  NULL vx / vy upload zeros, NULL m
+ * uploads ones; x, y, q are required.  Upload clears the cell order. */
+pic_status pic_hip_upload(pic_hip_ctx* ctx, const float* x, const float* y, const float* vx, const float* vy,
+                          const float* q, const float* m, size_t n);
+/* Copies the current device order back; NULL pointers are skipped.  Synchronous. */
+pic_status pic_hip_download(pic_hip_ctx* ctx, float* x, float* y, float* vx, float* vy, float* q, float* m);
+pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* params, size_t n);
+
+/* Cell sort.  pic_hip_sort computes every particle's cell key
+ * (particleCellIndex), a stable permutation by key, applies it to all six
+ * streams, and builds the cell offset table.  pic_hip_sort_perm computes the
+ * same without moving the particles and copies out (synchronously) the sorted
+ * keys, perm[i] = source index of the i-th sorted particle, and the Nx*Ny+1
+ * cell offsets; NULL outputs are skipped. */
+pic_status pic_hip_sort(pic_hip_ctx* ctx);
+pic_status pic_hip_sort_perm(pic_hip_ctx* ctx, int32_t* keys_sorted, int32_t* perm, int32_t* cell_offsets);
+/* Keys of the current order (synchronous copy out). */
+pic_status pic_hip_get_keys(pic_hip_ctx* ctx, int32_t* keys);
+
+/* Deposit into the context's FP32 rho (all cells overwritten).  GPU_PRIV
+ * needs a cell order from pic_hip_sort; particles that moved out of their
+ * cell since then are still deposited exactly (global-atomic overflow path). */
+pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt);
+pic_status pic_hip_get_rho(pic_hip_ctx* ctx, float* rho);
+pic_status pic_hip_get_rho_f64(pic_hip_ctx* ctx, double* rho);
+pic_status pic_hip_set_rho(pic_hip_ctx* ctx, const float* rho);
+
+/* rho += value on every cell (neutralising background 1/V). */
+pic_status pic_hip_add_background(pic_hip_ctx* ctx, double value);
+/* phi, Ex, Ey from rho: spectral solve with k = 0 removed (FP64 mean), FP32
+ * FFTs, E = -forward difference of phi. */
+pic_status pic_hip_poisson(pic_hip_ctx* ctx);
+pic_status pic_hip_get_fields(pic_hip_ctx* ctx, float* phi, float* Ex, float* Ey);
+pic_status pic_hip_set_fields(pic_hip_ctx* ctx, const float* Ex, const float* Ey);
+/* do_gather: v += -(q/m) E(x) dt (CIC interpolation); then do_push:
+ * x += v dt wrapped periodically. */
+pic_status pic_hip_gather_push(pic_hip_ctx* ctx, double dt, int do_gather, int do_push);
+/* FP64 sums over rho, Ex/Ey and the particles (synchronous). */
+pic_status pic_hip_diagnostics(pic_hip_ctx* ctx, pic_diagnostics* out);
+
+/* ---- kernel timing (build-defined; no reference counterpart) -------------
+ * Between begin and end, every deposit records a pair of HIP events on the
+ * context's stream around its main scatter kernel (the tiled kernel for
+ * GPU_PRIV, the atomic kernel for GPU_ATOMICS).  end waits for the stream and
+ * returns the summed kernel time and the number of launches recorded
+ * (at most max_launches). */
+pic_status pic_hip_profile_begin(pic_hip_ctx* ctx, int max_launches);
+pic_status pic_hip_profile_end(pic_hip_ctx* ctx, double* total_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIC_HIP_H */

@@ -1,0 +1,741 @@
+// FP64 CPU oracle for the charge-deposition hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: the parity tests, the
+// smoke check and the CPU-baseline leg of the benchmark call it; the HIP product
+// library never links or calls it.
+//
+// It restates, in double precision and in the reference's arithmetic order, the
+// algorithms of ehocine/pic-deposition listed below (paths are relative to the
+// reference repository root).  Every exported function names the lines it
+// follows.  Deliberate deviations are marked DEVIATION.
+//
+// Pinning (which independent check fixes each function) is listed in the header
+// comment of oracle.py next to the Python binding.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include <omp.h>
+
+extern "C" {
+
+// Mirrors pic::Domain (src/pic/domain.hpp:7-44).  dx, dy are derived exactly as
+// Domain::updateDerived (domain.hpp:17-20): dx = Lx / Nx in double.
+typedef struct {
+    double Lx, Ly;
+    int Nx, Ny;
+} or_domain;
+
+}  // extern "C"
+
+namespace {
+
+struct Dom {
+    double Lx, Ly, dx, dy;
+    int Nx, Ny;
+    explicit Dom(const or_domain* d)
+        : Lx(d->Lx), Ly(d->Ly), dx(d->Lx / d->Nx), dy(d->Ly / d->Ny), Nx(d->Nx), Ny(d->Ny) {}
+    double cellVolume() const { return dx * dy; }          // domain.hpp:22
+    double domainVolume() const { return Lx * Ly; }        // domain.hpp:23
+    int cellIndex(int ix, int iy) const { return iy * Nx + ix; }  // domain.hpp:25
+    // domain.hpp:27-32
+    void wrapPosition(double& x, double& y) const {
+        x = std::fmod(x, Lx);
+        if (x < 0.0) x += Lx;
+        y = std::fmod(y, Ly);
+        if (y < 0.0) y += Ly;
+    }
+    // domain.hpp:34-43
+    void cellCoords(double x, double y, int& ix, int& iy, double& wx, double& wy) const {
+        wx = x / dx;
+        wy = y / dy;
+        ix = static_cast<int>(std::floor(wx));
+        iy = static_cast<int>(std::floor(wy));
+        wx -= ix;
+        wy -= iy;
+        ix = (ix % Nx + Nx) % Nx;
+        iy = (iy % Ny + Ny) % Ny;
+    }
+};
+
+// src/deposition/ngp.cpp:9
+inline int wrapIndex(int i, int n) { return (i % n + n) % n; }
+
+// src/deposition/ngp.cpp:28-34
+void tscWeights(double xi, double w[3], int& i0) {
+    i0 = static_cast<int>(std::floor(xi - 0.5));
+    const double x = xi - static_cast<double>(i0);
+    w[0] = 0.5 * (1.5 - x) * (1.5 - x);
+    w[1] = 0.75 - (x - 1.0) * (x - 1.0);
+    w[2] = 0.5 * (x - 0.5) * (x - 0.5);
+}
+
+struct EW {
+    int index;
+    double weight;
+};
+
+// src/deposition/ngp.cpp:36-84 (trajectory path fractions on floor cells,
+// renormalised to a partition of unity; exact CIC pair when |x1-x0| < 1e-15).
+void esirkepovWeights1DSparse(double x0, double x1, double dx, int n, std::vector<EW>& weights) {
+    weights.clear();
+    if (std::abs(x1 - x0) < 1e-15) {
+        int ix = static_cast<int>(std::floor(x0 / dx));
+        const double xi = x0 / dx - static_cast<double>(ix);
+        weights.push_back({wrapIndex(ix, n), 1.0 - xi});
+        weights.push_back({wrapIndex(ix + 1, n), xi});
+        return;
+    }
+    const double inv_dx = 1.0 / dx;
+    double x_curr = x0;
+    const double x_end = x1;
+    int cell = static_cast<int>(std::floor(x_curr * inv_dx));
+    if (x_end > x0) {
+        while (x_curr < x_end - 1e-15) {
+            const double x_next = std::min(x_end, (cell + 1) * dx);
+            const double w = (x_next - x_curr) * inv_dx;
+            weights.push_back({wrapIndex(cell, n), w});
+            x_curr = x_next;
+            cell++;
+        }
+    } else {
+        while (x_curr > x_end + 1e-15) {
+            const double x_next = std::max(x_end, cell * dx);
+            const double w = (x_curr - x_next) * inv_dx;
+            weights.push_back({wrapIndex(cell, n), w});
+            x_curr = x_next;
+            cell--;
+        }
+    }
+    double total = 0.0;
+    for (const auto& e : weights) total += e.weight;
+    if (total > 1e-300) {
+        const double inv_total = 1.0 / total;
+        for (auto& e : weights) e.weight *= inv_total;
+    }
+}
+
+// Per-particle scatter of each scheme into `rho`, in the serial-path order of
+// src/deposition/ngp.cpp.  `inv_vol_mul` selects between the serial NGP form
+// (q / cellVolume, ngp.cpp:118) and the OpenMP form (q * inv_vol,
+// deposition_parallel.cpp:63).
+inline void depositNGPOne(const Dom& d, double x, double y, double q, double* rho, bool inv_vol_mul) {
+    int ix, iy;
+    double wx, wy;
+    d.cellCoords(x, y, ix, iy, wx, wy);
+    const int jx = wx >= 0.5 ? wrapIndex(ix + 1, d.Nx) : ix;
+    const int jy = wy >= 0.5 ? wrapIndex(iy + 1, d.Ny) : iy;
+    const double v = inv_vol_mul ? q * (1.0 / d.cellVolume()) : q / d.cellVolume();
+    rho[static_cast<size_t>(d.cellIndex(jx, jy))] += v;
+}
+
+// src/deposition/ngp.cpp:140-157
+inline void depositCICOne(const Dom& d, double inv_vol, double x, double y, double qp, double* rho) {
+    int ix, iy;
+    double wx, wy;
+    d.cellCoords(x, y, ix, iy, wx, wy);
+    const double q = qp * inv_vol;
+    const int ixp = wrapIndex(ix + 1, d.Nx);
+    const int iyp = wrapIndex(iy + 1, d.Ny);
+    rho[static_cast<size_t>(d.cellIndex(ix, iy))] += q * (1.0 - wx) * (1.0 - wy);
+    rho[static_cast<size_t>(d.cellIndex(ixp, iy))] += q * wx * (1.0 - wy);
+    rho[static_cast<size_t>(d.cellIndex(ix, iyp))] += q * (1.0 - wx) * wy;
+    rho[static_cast<size_t>(d.cellIndex(ixp, iyp))] += q * wx * wy;
+}
+
+// src/deposition/ngp.cpp:181-202 (serial-path periodic wrap)
+inline void depositTSCOne(const Dom& d, double inv_vol, double x, double y, double qp, double* rho) {
+    const double fx = x / d.dx;
+    const double fy = y / d.dy;
+    int ix0, iy0;
+    double wx[3], wy[3];
+    tscWeights(fx, wx, ix0);
+    tscWeights(fy, wy, iy0);
+    const double q = qp * inv_vol;
+    for (int dy = 0; dy < 3; ++dy) {
+        const int jy = wrapIndex(iy0 + dy, d.Ny);
+        for (int dx = 0; dx < 3; ++dx) {
+            const int jx = wrapIndex(ix0 + dx, d.Nx);
+            rho[static_cast<size_t>(d.cellIndex(jx, jy))] += q * wx[dx] * wy[dy];
+        }
+    }
+}
+
+// src/deposition/ngp.cpp:13-24
+inline void depositEsirkepovOne(const Dom& d, double inv_vol, double x, double y, double vx, double vy, double q,
+                                double dt, double* rho, std::vector<EW>& xs, std::vector<EW>& ys) {
+    esirkepovWeights1DSparse(x, x + vx * dt, d.dx, d.Nx, xs);
+    esirkepovWeights1DSparse(y, y + vy * dt, d.dy, d.Ny, ys);
+    for (const auto& sx : xs)
+        for (const auto& sy : ys)
+            rho[static_cast<size_t>(d.cellIndex(sx.index, sy.index))] += q * sx.weight * sy.weight * inv_vol;
+}
+
+void depositRange(const Dom& d, int scheme, const double* x, const double* y, const double* vx, const double* vy,
+                  const double* q, int64_t begin, int64_t end, double dt, double* rho, bool parallel_forms) {
+    const double inv_vol = 1.0 / d.cellVolume();
+    std::vector<EW> xs, ys;
+    xs.reserve(8);
+    ys.reserve(8);
+    for (int64_t i = begin; i < end; ++i) {
+        switch (scheme) {
+            case 0: depositNGPOne(d, x[i], y[i], q[i], rho, parallel_forms); break;
+            case 1: depositCICOne(d, inv_vol, x[i], y[i], q[i], rho); break;
+            case 2: depositTSCOne(d, inv_vol, x[i], y[i], q[i], rho); break;
+            default:
+                // deposition_parallel.cpp:552-557 multiplies q * wx * wy * inv_vol with q
+                // unscaled, the same expression as ngp.cpp:20-21.
+                depositEsirkepovOne(d, inv_vol, x[i], y[i], vx ? vx[i] : 0.0, vy ? vy[i] : 0.0, q[i], dt, rho, xs,
+                                    ys);
+                break;
+        }
+    }
+}
+
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Deposition
+// ---------------------------------------------------------------------------
+
+// Serial deposition, src/deposition/ngp.cpp:108-228 (SoA variants), with
+// FieldGrid::clearRho first (field_grid.cpp:22, deposition.cpp:121).
+// scheme: 0 NGP, 1 CIC, 2 TSC, 3 Esirkepov (density variant, needs vx, vy, dt).
+int or_deposit(int scheme, const double* x, const double* y, const double* vx, const double* vy, const double* q,
+               int64_t n, const or_domain* dom, double dt, double* rho) {
+    if (scheme < 0 || scheme > 3 || !dom || dom->Nx <= 0 || dom->Ny <= 0) return 1;
+    Dom d(dom);
+    std::fill(rho, rho + static_cast<size_t>(d.Nx) * d.Ny, 0.0);
+    depositRange(d, scheme, x, y, vx, vy, q, 0, n, dt, rho, false);
+    return 0;
+}
+
+// OpenMP privatised deposition, src/deposition/deposition_parallel.cpp:18-47:
+// static contiguous chunk per thread, private full-grid rho_local, merge under
+// a critical section.  DEVIATION: TSC and NGP use the serial-path periodic
+// wrap (ngp.cpp:9) because deposition_parallel.cpp:108-111 indexes rho_local[-1]
+// for x < 0.5*dx.  The merge is done in thread order (deterministic) instead of
+// critical-section arrival order.
+int or_deposit_omp(int scheme, const double* x, const double* y, const double* vx, const double* vy,
+                   const double* q, int64_t n, const or_domain* dom, double dt, int num_threads, double* rho) {
+    if (scheme < 0 || scheme > 3 || !dom || dom->Nx <= 0 || dom->Ny <= 0) return 1;
+    Dom d(dom);
+    const size_t ncells = static_cast<size_t>(d.Nx) * d.Ny;
+    std::fill(rho, rho + ncells, 0.0);
+    if (num_threads <= 1) {
+        depositRange(d, scheme, x, y, vx, vy, q, 0, n, dt, rho, true);
+        return 0;
+    }
+    std::vector<std::vector<double>> locals(static_cast<size_t>(num_threads));
+#pragma omp parallel num_threads(num_threads)
+    {
+        const int tid = omp_get_thread_num();
+        const int nth = omp_get_num_threads();
+        const int64_t chunk = (n + nth - 1) / nth;
+        const int64_t begin = static_cast<int64_t>(tid) * chunk;
+        const int64_t end = std::min<int64_t>(begin + chunk, n);
+        std::vector<double>& local = locals[static_cast<size_t>(tid)];
+        local.assign(ncells, 0.0);
+        if (begin < end) depositRange(d, scheme, x, y, vx, vy, q, begin, end, dt, local.data(), true);
+    }
+    for (int t = 0; t < num_threads; ++t) {
+        const std::vector<double>& local = locals[static_cast<size_t>(t)];
+        if (local.empty()) continue;
+        for (size_t i = 0; i < ncells; ++i) rho[i] += local[i];
+    }
+    return 0;
+}
+
+// Timed OpenMP deposition in the manner of depositChargeTimedSoA
+// (deposition.cpp:257-310): 3 warm-up deposits, then `repeats` timed ones.
+// Returns mean milliseconds; *std_ms receives the sample standard deviation.
+double or_deposit_omp_timed(int scheme, const double* x, const double* y, const double* vx, const double* vy,
+                            const double* q, int64_t n, const or_domain* dom, double dt, int num_threads,
+                            int repeats, double* rho, double* std_ms) {
+    for (int i = 0; i < 3; ++i) or_deposit_omp(scheme, x, y, vx, vy, q, n, dom, dt, num_threads, rho);
+    std::vector<double> samples;
+    for (int i = 0; i < repeats; ++i) {
+        const double t0 = omp_get_wtime();
+        or_deposit_omp(scheme, x, y, vx, vy, q, n, dom, dt, num_threads, rho);
+        samples.push_back((omp_get_wtime() - t0) * 1000.0);
+    }
+    double sum = 0.0;
+    for (double v : samples) sum += v;
+    const double mean = samples.empty() ? 0.0 : sum / samples.size();
+    double var = 0.0;
+    for (double v : samples) var += (v - mean) * (v - mean);
+    if (std_ms) *std_ms = samples.size() > 1 ? std::sqrt(var / (samples.size() - 1)) : 0.0;
+    return mean;
+}
+
+// ---------------------------------------------------------------------------
+// Cell sort (src/sorting/cell_sort.cpp)
+// ---------------------------------------------------------------------------
+
+// particleCellIndex, cell_sort.cpp:8-13
+void or_cell_keys(const double* x, const double* y, int64_t n, const or_domain* dom, int32_t* keys) {
+    Dom d(dom);
+    for (int64_t i = 0; i < n; ++i) {
+        int ix, iy;
+        double wx, wy;
+        d.cellCoords(x[i], y[i], ix, iy, wx, wy);
+        keys[i] = d.cellIndex(ix, iy);
+    }
+}
+
+// Permutation that sorts particles by cell key, cell_sort.cpp:22-46.
+// DEVIATION: stable (std::stable_sort) because the reference's std::sort tie
+// order is an introsort artefact.  perm[i] = source index of the i-th sorted particle.
+void or_sort_perm(const int32_t* keys, int64_t n, int32_t* perm) {
+    std::vector<int32_t> order(static_cast<size_t>(n));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+    std::copy(order.begin(), order.end(), perm);
+}
+
+// Exclusive-prefix cell offsets of a key array: off[c] = number of keys < c,
+// off[G] = n.  (Build-defined table; the reference has none.)
+void or_cell_offsets(const int32_t* keys, int64_t n, int ncells, int32_t* off) {
+    std::vector<int64_t> cnt(static_cast<size_t>(ncells) + 1, 0);
+    for (int64_t i = 0; i < n; ++i) cnt[static_cast<size_t>(keys[i]) + 1]++;
+    for (int c = 0; c < ncells; ++c) cnt[static_cast<size_t>(c) + 1] += cnt[static_cast<size_t>(c)];
+    for (int c = 0; c <= ncells; ++c) off[c] = static_cast<int32_t>(cnt[static_cast<size_t>(c)]);
+}
+
+// ---------------------------------------------------------------------------
+// Gather and push (src/deposition/deposition.cpp:335-374)
+// ---------------------------------------------------------------------------
+
+// gatherFieldsSoA, deposition.cpp:335-356: CIC interpolation of Ex, Ey at
+// cellCoords, v += -(q/m) E dt.
+void or_gather(const double* x, const double* y, double* vx, double* vy, const double* q, const double* m, int64_t n,
+               const double* Ex, const double* Ey, const or_domain* dom, double dt) {
+    Dom d(dom);
+    for (int64_t i = 0; i < n; ++i) {
+        int ix, iy;
+        double wx, wy;
+        d.cellCoords(x[i], y[i], ix, iy, wx, wy);
+        const int ixp = (ix + 1) % d.Nx;
+        const int iyp = (iy + 1) % d.Ny;
+        const size_t i00 = static_cast<size_t>(d.cellIndex(ix, iy));
+        const size_t i10 = static_cast<size_t>(d.cellIndex(ixp, iy));
+        const size_t i01 = static_cast<size_t>(d.cellIndex(ix, iyp));
+        const size_t i11 = static_cast<size_t>(d.cellIndex(ixp, iyp));
+        const double ex = Ex[i00] * (1.0 - wx) * (1.0 - wy) + Ex[i10] * wx * (1.0 - wy) +
+                          Ex[i01] * (1.0 - wx) * wy + Ex[i11] * wx * wy;
+        const double ey = Ey[i00] * (1.0 - wx) * (1.0 - wy) + Ey[i10] * wx * (1.0 - wy) +
+                          Ey[i01] * (1.0 - wx) * wy + Ey[i11] * wx * wy;
+        vx[i] += -q[i] / m[i] * ex * dt;
+        vy[i] += -q[i] / m[i] * ey * dt;
+    }
+}
+
+// pushParticlesSoA, deposition.cpp:367-374: drift then fmod wrap.
+void or_push(double* x, double* y, const double* vx, const double* vy, int64_t n, const or_domain* dom, double dt) {
+    Dom d(dom);
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] += vx[i] * dt;
+        y[i] += vy[i] * dt;
+        d.wrapPosition(x[i], y[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions and diagnostics (field_grid.cpp:24-38, diagnostics.cpp:10-51,
+// particles_aos.cpp:182-196)
+// ---------------------------------------------------------------------------
+
+double or_integrated_rho(const double* rho, const or_domain* dom) {
+    Dom d(dom);
+    double sum = 0.0;
+    const size_t nc = static_cast<size_t>(d.Nx) * d.Ny;
+    for (size_t i = 0; i < nc; ++i) sum += rho[i];
+    return sum * d.cellVolume();
+}
+
+double or_field_energy(const double* Ex, const double* Ey, const or_domain* dom) {
+    Dom d(dom);
+    double sum = 0.0;
+    const size_t nc = static_cast<size_t>(d.Nx) * d.Ny;
+    for (size_t i = 0; i < nc; ++i) sum += Ex[i] * Ex[i] + Ey[i] * Ey[i];
+    return 0.5 * sum * d.cellVolume();
+}
+
+double or_kinetic_energy(const double* vx, const double* vy, const double* m, int64_t n) {
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; ++i) sum += 0.5 * m[i] * (vx[i] * vx[i] + vy[i] * vy[i]);
+    return sum;
+}
+
+double or_total_charge(const double* q, int64_t n) {
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; ++i) sum += q[i];
+    return sum;
+}
+
+double or_total_abs_charge(const double* q, int64_t n) {
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; ++i) sum += std::abs(q[i]);
+    return sum;
+}
+
+// diagnostics.cpp:26-29
+double or_charge_error(double integrated_rho, double total_charge, double abs_charge_sum) {
+    const double denom = std::max(abs_charge_sum, 1e-12);
+    return std::abs(integrated_rho - total_charge) / denom;
+}
+
+// ---------------------------------------------------------------------------
+// Initialisers (src/pic/particles_aos.cpp), libstdc++ <random> draws
+// ---------------------------------------------------------------------------
+
+// particles_aos.cpp:67-81
+void or_init_uniform_maxwellian(int64_t n, const or_domain* dom, double T, unsigned seed, double* x, double* y,
+                                double* vx, double* vy, double* q, double* m) {
+    Dom d(dom);
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> ux(0.0, d.Lx);
+    std::uniform_real_distribution<double> uy(0.0, d.Ly);
+    const double stdv = std::sqrt(T);
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = ux(rng);
+        y[i] = uy(rng);
+        {
+            std::normal_distribution<double> dist(0.0, stdv);  // sampleNormal, particles_aos.cpp:11-14
+            vx[i] = dist(rng);
+        }
+        {
+            std::normal_distribution<double> dist(0.0, stdv);
+            vy[i] = dist(rng);
+        }
+        m[i] = 1.0;
+        q[i] = (i % 2 == 0) ? 1.0 : -1.0;
+    }
+}
+
+// particles_aos.cpp:83-96 (deterministic lattice; the seed is unused)
+void or_init_langmuir(int64_t n, const or_domain* dom, double amplitude, int mode, double* x, double* y, double* vx,
+                      double* vy, double* q, double* m) {
+    Dom d(dom);
+    const double k = 2.0 * M_PI * static_cast<double>(mode) / d.Lx;
+    const double qp = -1.0 / static_cast<double>(n);
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = d.Lx * static_cast<double>(i) / static_cast<double>(n);
+        y[i] = 0.5 * d.Ly;
+        vx[i] = amplitude * std::sin(k * x[i]);
+        vy[i] = 0.0;
+        m[i] = 1.0;
+        q[i] = qp;
+    }
+}
+
+// particles_aos.cpp:142-160 (quasi1d != 0: particles_aos.cpp:162-180)
+void or_init_two_stream(int64_t n, const or_domain* dom, double beam_velocity, double perturbation, unsigned seed,
+                        int quasi1d, double* x, double* y, double* vx, double* vy, double* q, double* m) {
+    Dom d(dom);
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> ux(0.0, d.Lx);
+    std::uniform_real_distribution<double> uy(0.0, d.Ly);
+    const double qp = -1.0 / static_cast<double>(n);
+    const double k = 2.0 * M_PI / d.Lx;
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = ux(rng);
+        y[i] = quasi1d ? 0.5 * d.Ly : uy(rng);
+        const double sign = (i < n / 2) ? 1.0 : -1.0;
+        const double pert = perturbation * std::sin(k * x[i]);
+        vx[i] = sign * beam_velocity + pert;
+        vy[i] = 0.0;
+        m[i] = 1.0;
+        q[i] = qp;
+    }
+}
+
+// Raw mt19937 words, for pinning the Philox/mt19937 helpers in tests.
+void or_mt19937_words(unsigned seed, int64_t n, uint32_t* out) {
+    std::mt19937 rng(seed);
+    for (int64_t i = 0; i < n; ++i) out[i] = static_cast<uint32_t>(rng());
+}
+
+// ---------------------------------------------------------------------------
+// FFT Poisson (src/pic/poisson_fft.cpp:10-94)
+// ---------------------------------------------------------------------------
+// FFTW is absent here.  The reference's separable R2HC/HC2R solve divides
+// every half-complex coefficient by k^2 with |k| taken from its frequency,
+// which in exact arithmetic is the complex spectral solve
+//   phi_hat(k) = cv * rho_hat(k) / |k|^2,  phi_hat(0) = 0,
+// followed by E = -forward difference of phi.  This oracle computes that with
+// its own FP64 radix-2 complex FFT (power-of-two grids) or a direct DFT.
+
+static void fft1d(std::vector<std::complex<double>>& a, bool inverse) {
+    const size_t n = a.size();
+    if (n <= 1) return;
+    if ((n & (n - 1)) == 0) {
+        for (size_t i = 1, j = 0; i < n; ++i) {
+            size_t bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) std::swap(a[i], a[j]);
+        }
+        for (size_t len = 2; len <= n; len <<= 1) {
+            const double ang = 2.0 * M_PI / static_cast<double>(len) * (inverse ? 1.0 : -1.0);
+            for (size_t i = 0; i < n; i += len) {
+                for (size_t k = 0; k < len / 2; ++k) {
+                    const std::complex<double> w(std::cos(ang * k), std::sin(ang * k));
+                    const std::complex<double> u = a[i + k];
+                    const std::complex<double> v = a[i + k + len / 2] * w;
+                    a[i + k] = u + v;
+                    a[i + k + len / 2] = u - v;
+                }
+            }
+        }
+        return;
+    }
+    std::vector<std::complex<double>> out(n);
+    for (size_t k = 0; k < n; ++k) {
+        std::complex<double> s(0.0, 0.0);
+        for (size_t t = 0; t < n; ++t) {
+            const double ang = 2.0 * M_PI * static_cast<double>((k * t) % n) / static_cast<double>(n) *
+                               (inverse ? 1.0 : -1.0);
+            s += a[t] * std::complex<double>(std::cos(ang), std::sin(ang));
+        }
+        out[k] = s;
+    }
+    a.swap(out);
+}
+
+static void fft2d(std::vector<std::complex<double>>& a, int nx, int ny, bool inverse) {
+    std::vector<std::complex<double>> row(static_cast<size_t>(nx)), col(static_cast<size_t>(ny));
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) row[static_cast<size_t>(i)] = a[static_cast<size_t>(j) * nx + i];
+        fft1d(row, inverse);
+        for (int i = 0; i < nx; ++i) a[static_cast<size_t>(j) * nx + i] = row[static_cast<size_t>(i)];
+    }
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) col[static_cast<size_t>(j)] = a[static_cast<size_t>(j) * nx + i];
+        fft1d(col, inverse);
+        for (int j = 0; j < ny; ++j) a[static_cast<size_t>(j) * nx + i] = col[static_cast<size_t>(j)];
+    }
+}
+
+// PoissonSolverFFT::solve, poisson_fft.cpp:45-94 (k^2 table :23-35).
+void or_poisson(const double* rho, const or_domain* dom, double* phi, double* Ex, double* Ey) {
+    Dom d(dom);
+    const int nx = d.Nx, ny = d.Ny;
+    const size_t n = static_cast<size_t>(nx) * ny;
+    const double cell_vol = d.cellVolume();
+    std::vector<std::complex<double>> a(n);
+    for (size_t i = 0; i < n; ++i) a[i] = std::complex<double>(rho[i] * cell_vol, 0.0);
+    fft2d(a, nx, ny, false);
+    a[0] = 0.0;
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+            if (i == 0 && j == 0) continue;
+            const double kx = 2.0 * M_PI * ((i <= nx / 2) ? i : i - nx) / d.Lx;
+            const double ky = 2.0 * M_PI * ((j <= ny / 2) ? j : j - ny) / d.Ly;
+            a[static_cast<size_t>(j) * nx + i] /= (kx * kx + ky * ky);
+        }
+    }
+    fft2d(a, nx, ny, true);
+    const double norm = 1.0 / static_cast<double>(n);
+    for (size_t i = 0; i < n; ++i) phi[i] = a[i].real() * norm;
+    const double idx_dx = 1.0 / d.dx;
+    const double idx_dy = 1.0 / d.dy;
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+            const int ip = (i + 1) % nx;
+            const int jp = (j + 1) % ny;
+            const size_t id = static_cast<size_t>(j) * nx + i;
+            Ex[id] = -((phi[static_cast<size_t>(j) * nx + ip] - phi[id]) * idx_dx);
+            Ey[id] = -((phi[static_cast<size_t>(jp) * nx + i] - phi[id]) * idx_dy);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Physics estimators (src/pic/validation.cpp)
+// ---------------------------------------------------------------------------
+
+// validation.cpp:19-54
+double or_dominant_frequency(const double* signal, int64_t n, double dt) {
+    if (n < 8) return 0.0;
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; ++i) mean += signal[i];
+    mean /= static_cast<double>(n);
+    std::vector<double> c(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) c[static_cast<size_t>(i)] = signal[i] - mean;
+    const int nn = static_cast<int>(n);
+    double best_power = 0.0;
+    int best_k = 1;
+    for (int k = 1; k < nn / 2; ++k) {
+        double re = 0.0, im = 0.0;
+        for (int t = 0; t < nn; ++t) {
+            const double phase = 2.0 * M_PI * static_cast<double>(k * t) / static_cast<double>(nn);
+            re += c[static_cast<size_t>(t)] * std::cos(phase);
+            im -= c[static_cast<size_t>(t)] * std::sin(phase);
+        }
+        const double power = re * re + im * im;
+        if (power > best_power) {
+            best_power = power;
+            best_k = k;
+        }
+    }
+    return static_cast<double>(best_k) / (static_cast<double>(nn) * dt);
+}
+
+// validation.cpp:66-71
+double or_plasma_frequency(const or_domain* dom, int64_t num_particles) {
+    Dom d(dom);
+    const double n0 = static_cast<double>(num_particles) / d.domainVolume();
+    const double q_macro = 1.0 / static_cast<double>(num_particles);
+    return std::sqrt(n0 * q_macro * q_macro);
+}
+
+// validation.cpp:78-85
+double or_resonant_beam_velocity(const or_domain* dom, int64_t num_particles, int mode) {
+    Dom d(dom);
+    const double omega_p = or_plasma_frequency(dom, num_particles);
+    const double k = 2.0 * M_PI * static_cast<double>(mode) / d.Lx;
+    if (k < 1e-30) return 0.0;
+    return omega_p / k;
+}
+
+// validation.cpp:99-119
+double or_fourier_mode_amplitude(const double* Ex, const or_domain* dom, int mode_x) {
+    Dom d(dom);
+    const double k = 2.0 * M_PI * static_cast<double>(mode_x) / d.Lx;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < d.Ny; ++j) {
+        for (int i = 0; i < d.Nx; ++i) {
+            const double xx = (static_cast<double>(i) + 0.5) * d.dx;
+            const double ex = Ex[static_cast<size_t>(j) * d.Nx + i];
+            re += ex * std::cos(k * xx);
+            im += ex * std::sin(k * xx);
+        }
+    }
+    const double norm = static_cast<double>(d.Nx * d.Ny);
+    re /= norm;
+    im /= norm;
+    return std::sqrt(re * re + im * im);
+}
+
+// validation.cpp:121-152
+double or_growth_rate(const double* signal, int64_t n, double dt, int record_start_step) {
+    if (n < 8) return 0.0;
+    const int64_t start = n / 4;
+    const int64_t end = (3 * n) / 4;
+    if (end <= start + 2) return 0.0;
+    double st = 0.0, sl = 0.0, stt = 0.0, stl = 0.0;
+    int count = 0;
+    for (int64_t i = start; i < end; ++i) {
+        const double amp = std::max(signal[i], 1e-30);
+        const double t = static_cast<double>(record_start_step + static_cast<int>(i)) * dt;
+        const double l = std::log(amp);
+        st += t;
+        sl += l;
+        stt += t * t;
+        stl += t * l;
+        ++count;
+    }
+    const double denom = static_cast<double>(count) * stt - st * st;
+    if (std::abs(denom) < 1e-30) return 0.0;
+    return (static_cast<double>(count) * stl - st * sl) / denom;
+}
+
+// ---------------------------------------------------------------------------
+// Esirkepov charge-conserving current (build-defined; no reference
+// counterpart).  Esirkepov, Comput. Phys. Commun. 135 (2001) 144, eqs. (21)-(23)
+// specialised to 2D with CIC (first-order) shape factors S at the old and new
+// positions.  Jx is stored on the x-face at (i+1/2, j) in array slot (i, j),
+// Jy on the y-face at (i, j+1/2) in slot (i, j).  rho_old / rho_new receive
+// the CIC densities at x0 and x1 = x0 + v dt (no wrap of x1 needed: indices
+// are wrapped).  Continuity: (rho_new - rho_old)/dt + Dx Jx + Dy Jy = 0 with
+// backward differences Dx Jx(i,j) = (Jx(i,j) - Jx(i-1,j))/dx.
+// ---------------------------------------------------------------------------
+
+static void cicShape1D(double xi, int& i0, double w[2]) {
+    i0 = static_cast<int>(std::floor(xi));
+    const double f = xi - i0;
+    w[0] = 1.0 - f;
+    w[1] = f;
+}
+
+int or_esirkepov_current(const double* x, const double* y, const double* vx, const double* vy, const double* q,
+                         int64_t n, const or_domain* dom, double dt, double* rho_old, double* rho_new, double* Jx,
+                         double* Jy) {
+    Dom d(dom);
+    const size_t nc = static_cast<size_t>(d.Nx) * d.Ny;
+    std::fill(rho_old, rho_old + nc, 0.0);
+    std::fill(rho_new, rho_new + nc, 0.0);
+    std::fill(Jx, Jx + nc, 0.0);
+    std::fill(Jy, Jy + nc, 0.0);
+    const double inv_vol = 1.0 / d.cellVolume();
+    for (int64_t p = 0; p < n; ++p) {
+        const double fx0 = x[p] / d.dx, fy0 = y[p] / d.dy;
+        const double fx1 = (x[p] + vx[p] * dt) / d.dx, fy1 = (y[p] + vy[p] * dt) / d.dy;
+        if (std::abs(fx1 - fx0) > 1.0 || std::abs(fy1 - fy0) > 1.0) return 2;
+        // Common 3x3 frame anchored at the old floor cell minus one.
+        const int bx = static_cast<int>(std::floor(fx0)) - 1;
+        const int by = static_cast<int>(std::floor(fy0)) - 1;
+        double S0x[4] = {0, 0, 0, 0}, S1x[4] = {0, 0, 0, 0}, S0y[4] = {0, 0, 0, 0}, S1y[4] = {0, 0, 0, 0};
+        int i0;
+        double w[2];
+        cicShape1D(fx0, i0, w);
+        S0x[i0 - bx] += w[0];
+        S0x[i0 - bx + 1] += w[1];
+        cicShape1D(fx1, i0, w);
+        S1x[i0 - bx] += w[0];
+        S1x[i0 - bx + 1] += w[1];
+        cicShape1D(fy0, i0, w);
+        S0y[i0 - by] += w[0];
+        S0y[i0 - by + 1] += w[1];
+        cicShape1D(fy1, i0, w);
+        S1y[i0 - by] += w[0];
+        S1y[i0 - by + 1] += w[1];
+        const double qv = q[p] * inv_vol;
+        // Esirkepov decomposition W (eq. 21 in 2D).
+        double Wx[4][4], Wy[4][4];
+        for (int a = 0; a < 4; ++a) {
+            for (int b = 0; b < 4; ++b) {
+                const double dSx = S1x[a] - S0x[a];
+                const double dSy = S1y[b] - S0y[b];
+                Wx[a][b] = dSx * (S0y[b] + 0.5 * dSy);
+                Wy[a][b] = dSy * (S0x[a] + 0.5 * dSx);
+            }
+        }
+        for (int b = 0; b < 4; ++b) {
+            double acc = 0.0;
+            for (int a = 0; a < 4; ++a) {
+                acc -= qv * d.dx / dt * Wx[a][b];
+                const int gx = wrapIndex(bx + a, d.Nx), gy = wrapIndex(by + b, d.Ny);
+                Jx[static_cast<size_t>(gy) * d.Nx + gx] += acc;
+            }
+        }
+        for (int a = 0; a < 4; ++a) {
+            double acc = 0.0;
+            for (int b = 0; b < 4; ++b) {
+                acc -= qv * d.dy / dt * Wy[a][b];
+                const int gx = wrapIndex(bx + a, d.Nx), gy = wrapIndex(by + b, d.Ny);
+                Jy[static_cast<size_t>(gy) * d.Nx + gx] += acc;
+            }
+        }
+        for (int a = 0; a < 4; ++a) {
+            for (int b = 0; b < 4; ++b) {
+                const int gx = wrapIndex(bx + a, d.Nx), gy = wrapIndex(by + b, d.Ny);
+                rho_old[static_cast<size_t>(gy) * d.Nx + gx] += qv * S0x[a] * S0y[b];
+                rho_new[static_cast<size_t>(gy) * d.Nx + gx] += qv * S1x[a] * S1y[b];
+            }
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

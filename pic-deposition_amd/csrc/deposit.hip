@@ -1,0 +1,524 @@
+// Charge-deposition kernels for gfx950.
+//
+//  * deposit_atomic_kernel  - "GPU_Atomics": one particle per lane, FP32 hardware
+//    atomics into rho.  Works on any particle order.  Counterpart of the
+//    reference depositKernelSoA (src/deposition/cuda/deposition_cuda.cu:205-210).
+//  * deposit_tile_kernel    - "GPU_Priv": needs cell-sorted particles and the
+//    cell offset table produced by the sort.  One wavefront owns one tile of
+//    T x R cells and a private LDS window of (T+3) x (R+3) nodes (halo -1..+2,
+//    enough for NGP/CIC/TSC/Esirkepov-density stencils).  Each lane streams K
+//    consecutive particles with float4 loads, accumulates same-cell runs in
+//    registers, the wave then combines runs that straddle lanes with a DPP
+//    segmented scan, and only one lane per cell adds into LDS.  LDS adds use
+//    the returned old value to carry the exact rounding error (TwoSum) into a
+//    compensation window, so the per-tile partial sums are Kahan-compensated.
+//    Particles whose current cell is outside the tile (drift since the last
+//    sort) take an overflow path with global atomics.
+//  * deposit_merge_kernel   - writes rho node by node by adding the windows of
+//    the (at most 4 x 4) tiles that cover it in a fixed order, again with
+//    compensation.  Together with the per-wave windows this makes the sorted
+//    path's result independent of scheduling.
+// The reference's own privatised kernel (deposition_cuda.cu:212-241) scans all
+// particles per tile; this design reads each particle once.
+
+#include "pic_common.hpp"
+#include "pic_kernels.hpp"
+
+namespace pichip {
+
+namespace {
+
+// LDS visibility between lanes of one wavefront (in-order LDS queue + no
+// compiler reordering across this point).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// DPP controls (GFX9 encoding).
+constexpr int kRowShr = 0x110;
+constexpr int kRowBcast15 = 0x142;
+constexpr int kRowBcast31 = 0x143;
+constexpr int kWaveShr1 = 0x138;
+constexpr int kWaveShl1 = 0x130;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_i(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+// One Hillis-Steele step of the linear recurrence B_i = b_i + a_i * B_{i-1}.
+template <int CTRL, int ROW_MASK, int V>
+__device__ __forceinline__ void lin_scan_step(float (&B)[V], float& A) {
+    const float As = dpp_f<CTRL, ROW_MASK>(1.0f, A);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const float Bs = dpp_f<CTRL, ROW_MASK>(0.0f, B[k]);
+        B[k] = fmaf(A, Bs, B[k]);
+    }
+    A *= As;
+}
+
+// Inclusive wave-wide scan of B_i = b_i + a_i * B_{i-1} with a_i in {0, 1}:
+// a segmented sum where a_i = 0 starts a new segment at lane i.
+template <int V>
+__device__ __forceinline__ void wave_segmented_scan(float (&B)[V], float A) {
+    lin_scan_step<kRowShr + 1, 0xf>(B, A);
+    lin_scan_step<kRowShr + 2, 0xf>(B, A);
+    lin_scan_step<kRowShr + 4, 0xf>(B, A);
+    lin_scan_step<kRowShr + 8, 0xf>(B, A);
+    lin_scan_step<kRowBcast15, 0xa>(B, A);
+    lin_scan_step<kRowBcast31, 0xc>(B, A);
+}
+
+// Compensated add of a cell's V stencil values into the wave's LDS window.
+template <int SX, int V>
+__device__ __forceinline__ void emit_cell(float* __restrict__ ws, float* __restrict__ wc, int wbase, int WX,
+                                          const float (&z)[V]) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const float v = z[k];
+        if (v != 0.0f) {
+            const int a = wbase + (k / SX) * WX + (k % SX);
+            const float old = atomicAdd(ws + a, v);
+            atomicAdd(wc + a, two_sum_err(old, v));
+        }
+    }
+}
+
+// General reference-Esirkepov walk (esirkepovWeights1DSparse,
+// src/deposition/ngp.cpp:36-84) for displacements of any length, in cell units.
+// Returns the number of support cells (<= kEskMaxCells) or -1 if longer.
+constexpr int kEskMaxCells = 16;
+__device__ int esk_walk_1d(float x0, float v, float dt, float inv_dx, int* cell, float* w) {
+    const float disp = v * dt;
+    const float f0 = x0 * inv_dx;
+    const int c0 = (int)floorf(f0);
+    if (fabsf(disp) < 1e-15f) {
+        const float xi = f0 - (float)c0;
+        cell[0] = c0;
+        w[0] = 1.0f - xi;
+        cell[1] = c0 + 1;
+        w[1] = xi;
+        return 2;
+    }
+    const float f1 = (x0 + disp) * inv_dx;
+    int n = 0;
+    float total = 0.0f;
+    float cur = f0;
+    int c = c0;
+    if (disp > 0.0f) {
+        while (cur < f1) {
+            if (n == kEskMaxCells) return -1;
+            const float nxt = fminf(f1, (float)(c + 1));
+            cell[n] = c;
+            w[n] = nxt - cur;
+            total += w[n];
+            ++n;
+            cur = nxt;
+            ++c;
+        }
+    } else {
+        while (cur > f1) {
+            if (n == kEskMaxCells) return -1;
+            const float nxt = fmaxf(f1, (float)c);
+            cell[n] = c;
+            w[n] = cur - nxt;
+            total += w[n];
+            ++n;
+            cur = nxt;
+            --c;
+        }
+    }
+    if (n == 0) {  // displacement below FP32 resolution: all weight on the home cell
+        cell[0] = c0;
+        w[0] = 1.0f;
+        return 1;
+    }
+    if (total > 1e-30f) {
+        const float inv = 1.0f / total;
+        for (int i = 0; i < n; ++i) w[i] *= inv;
+    }
+    return n;
+}
+
+// Scatter one particle with global FP32 atomics (used by GPU_Atomics and by the
+// tile kernel's overflow path).
+template <int S>
+__device__ __forceinline__ void scatter_global(float x, float y, float vx, float vy, float qv, float dt,
+                                               const DevDomain& d, float* __restrict__ grid, int* __restrict__ status) {
+    using St = Stencil<S>;
+    float w[St::V];
+    int cx, cy;
+    const bool ok = St::weights(x, y, vx, vy, qv, dt, d, cx, cy, w);
+    if (ok) {
+#pragma unroll
+        for (int k = 0; k < St::V; ++k) {
+            if (w[k] != 0.0f) {
+                const int gx = wrap_index(cx + St::OX + k % St::SX, d.Nx);
+                const int gy = wrap_index(cy + St::OY + k / St::SX, d.Ny);
+                unsafeAtomicAdd(grid + (size_t)gy * d.Nx + gx, w[k]);
+            }
+        }
+        return;
+    }
+    // Only the Esirkepov density can fail the fixed stencil (long moves).
+    int cxs[kEskMaxCells], cys[kEskMaxCells];
+    float wxs[kEskMaxCells], wys[kEskMaxCells];
+    const int nx = esk_walk_1d(x, vx, dt, d.inv_dx, cxs, wxs);
+    const int ny = esk_walk_1d(y, vy, dt, d.inv_dy, cys, wys);
+    if (nx < 0 || ny < 0) {
+        atomicOr(status, 1);
+        return;
+    }
+    for (int i = 0; i < nx; ++i) {
+        const int gx = wrap_index(cxs[i], d.Nx);
+        for (int j = 0; j < ny; ++j) {
+            const int gy = wrap_index(cys[j], d.Ny);
+            unsafeAtomicAdd(grid + (size_t)gy * d.Nx + gx, qv * wxs[i] * wys[j]);
+        }
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void deposit_atomic_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                             const float* __restrict__ pvx,
+                                                             const float* __restrict__ pvy,
+                                                             const float* __restrict__ pq, int64_t n, DevDomain d,
+                                                             float dt, float* __restrict__ rho,
+                                                             int* __restrict__ status) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float vx = Stencil<S>::kNeedsVelocity ? pvx[i] : 0.0f;
+    const float vy = Stencil<S>::kNeedsVelocity ? pvy[i] : 0.0f;
+    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * d.inv_vol, dt, d, rho, status);
+}
+
+template <int S, int K>
+__global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                           const float* __restrict__ pvx,
+                                                           const float* __restrict__ pvy,
+                                                           const float* __restrict__ pq,
+                                                           const int* __restrict__ cell_off, DevDomain d, TileGeom g,
+                                                           float dt, float* __restrict__ partial,
+                                                           float* __restrict__ overflow, int* __restrict__ ovf_count,
+                                                           int* __restrict__ status) {
+    using St = Stencil<S>;
+    constexpr int V = St::V;
+    constexpr int Q = K / 4;
+    static_assert(K % 4 == 0, "K must be a multiple of 4 (float4 loads)");
+    extern __shared__ float smem[];
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + wv;
+    if (tile >= g.ntiles) return;  // wave-uniform
+
+    float* win_s = smem + (size_t)wv * g.lds_per_wave;
+    float* win_c = win_s + g.wsize;
+    int* tab = reinterpret_cast<int*>(win_c + g.wsize);
+    int* t_qstart = tab;              // [R]   first aligned quad of row r
+    int* t_rs = tab + g.R;            // [R]   first particle of row r
+    int* t_re = tab + 2 * g.R;        // [R]   one past the last particle of row r
+    int* t_qcum = tab + 3 * g.R;      // [R+1] cumulative quad counts
+
+    const int tx = tile % g.ntx, ty = tile / g.ntx;
+    const int x0 = tx * g.T, y0 = ty * g.R;
+    const int tw = min(g.T, d.Nx - x0), th = min(g.R, d.Ny - y0);
+
+    for (int i = lane; i < g.wsize; i += 64) {
+        win_s[i] = 0.0f;
+        win_c[i] = 0.0f;
+    }
+    // Row table: lane r < th describes row y0 + r of the tile.
+    int nq = 0;
+    if (lane < th) {
+        const int c = (y0 + lane) * d.Nx + x0;
+        const int s = cell_off[c], e = cell_off[c + tw];
+        const int qs = s >> 2, qe = (e + 3) >> 2;
+        nq = e > s ? qe - qs : 0;
+        t_qstart[lane] = qs;
+        t_rs[lane] = s;
+        t_re[lane] = e;
+    }
+    // Inclusive prefix of nq over the (at most 64) row lanes.
+    int incl = nq;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane < th) t_qcum[lane + 1] = incl;
+    if (lane == 0) t_qcum[0] = 0;
+    wave_sync();
+    const int total_q = t_qcum[th];
+
+    const int WX = g.WX;
+    for (int base = 0; base < total_q; base += 64 * Q) {
+        const int vq0 = base + lane * Q;
+        float4 X[Q], Y[Q], QQ[Q], VX[Q], VY[Q];
+        int p0[Q], lo[Q], hi[Q];
+        int r = 0;
+        while (r + 1 < th && vq0 >= t_qcum[r + 1]) ++r;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const int vq = vq0 + j;
+            p0[j] = -1;
+            lo[j] = 0;
+            hi[j] = 0;
+            if (vq < total_q) {
+                while (vq >= t_qcum[r + 1]) ++r;
+                const int pq4 = t_qstart[r] + (vq - t_qcum[r]);
+                p0[j] = pq4 * 4;
+                lo[j] = t_rs[r];
+                hi[j] = t_re[r];
+                X[j] = *reinterpret_cast<const float4*>(px + p0[j]);
+                Y[j] = *reinterpret_cast<const float4*>(py + p0[j]);
+                QQ[j] = *reinterpret_cast<const float4*>(pq + p0[j]);
+                if (St::kNeedsVelocity) {
+                    VX[j] = *reinterpret_cast<const float4*>(pvx + p0[j]);
+                    VY[j] = *reinterpret_cast<const float4*>(pvy + p0[j]);
+                }
+            }
+        }
+
+        float acc[V], H[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            acc[k] = 0.0f;
+            H[k] = 0.0f;
+        }
+        int hk = -1, ck = -1;
+        bool multi = false;
+
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int p = p0[j] + e;
+                if (p0[j] >= 0 && p >= lo[j] && p < hi[j]) {
+                    const float xx = e == 0 ? X[j].x : e == 1 ? X[j].y : e == 2 ? X[j].z : X[j].w;
+                    const float yy = e == 0 ? Y[j].x : e == 1 ? Y[j].y : e == 2 ? Y[j].z : Y[j].w;
+                    const float qq = e == 0 ? QQ[j].x : e == 1 ? QQ[j].y : e == 2 ? QQ[j].z : QQ[j].w;
+                    float vxx = 0.0f, vyy = 0.0f;
+                    if (St::kNeedsVelocity) {
+                        vxx = e == 0 ? VX[j].x : e == 1 ? VX[j].y : e == 2 ? VX[j].z : VX[j].w;
+                        vyy = e == 0 ? VY[j].x : e == 1 ? VY[j].y : e == 2 ? VY[j].z : VY[j].w;
+                    }
+                    const float qv = qq * d.inv_vol;
+                    float w[V];
+                    int cx, cy;
+                    const bool ok = St::weights(xx, yy, vxx, vyy, qv, dt, d, cx, cy, w);
+                    const int lx = wrap_index(cx, d.Nx) - x0;
+                    const int ly = wrap_index(cy, d.Ny) - y0;
+                    if (!ok || (unsigned)lx >= (unsigned)tw || (unsigned)ly >= (unsigned)th) {
+                        scatter_global<S>(xx, yy, vxx, vyy, qv, dt, d, overflow, status);
+                        atomicAdd(ovf_count, 1);
+                    } else {
+                        const int lk = (ly << 12) | lx;
+                        const bool nseg = lk != ck;
+                        if (nseg && ck >= 0) {
+                            if (!multi) {
+#pragma unroll
+                                for (int k = 0; k < V; ++k) H[k] = acc[k];
+                                multi = true;
+                            } else {
+                                const int wb = ((ck >> 12) + 1 + St::OY) * WX + (ck & 4095) + 1 + St::OX;
+                                emit_cell<St::SX, V>(win_s, win_c, wb, WX, acc);
+                            }
+                        }
+                        const float keep = nseg ? 0.0f : 1.0f;
+#pragma unroll
+                        for (int k = 0; k < V; ++k) acc[k] = fmaf(acc[k], keep, w[k]);
+                        if (nseg) {
+                            ck = lk;
+                            if (hk < 0) hk = lk;
+                        }
+                    }
+                }
+            }
+        }
+
+        // Combine the runs that straddle lanes.  Lane i's outgoing run is acc
+        // with key ck; it continues lane i-1's outgoing run when hk == ck(i-1).
+        const int ck_prev = dpp_i<kWaveShr1, 0xf>(-2, ck);
+        const bool cont = hk >= 0 && hk == ck_prev;
+        const float a = (cont && !multi) ? 1.0f : 0.0f;
+        float Z[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) Z[k] = acc[k];
+        wave_segmented_scan<V>(Z, a);
+        float Zp[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) Zp[k] = dpp_f<kWaveShr1, 0xf>(0.0f, Z[k]);
+        if (multi) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) H[k] = cont ? H[k] + Zp[k] : H[k];
+            const int wb = ((hk >> 12) + 1 + St::OY) * WX + (hk & 4095) + 1 + St::OX;
+            emit_cell<St::SX, V>(win_s, win_c, wb, WX, H);
+        }
+        const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
+        if (ck >= 0 && !cont_next) {
+            const int wb = ((ck >> 12) + 1 + St::OY) * WX + (ck & 4095) + 1 + St::OX;
+            emit_cell<St::SX, V>(win_s, win_c, wb, WX, Z);
+        }
+    }
+
+    wave_sync();
+    float* out = partial + (size_t)tile * g.wsize;
+    for (int i = lane; i < g.wsize; i += 64) out[i] = win_s[i] + win_c[i];
+}
+
+// Tiles (along one axis) whose window can hold node g: those owning cells
+// g-2 .. g+1 (window halo is 1 before and 2 after the tile).  Writes up to 4
+// distinct tile indices in a fixed order.
+__device__ __forceinline__ int window_candidates(int gnode, int T, int nt, int n, int (&t)[4]) {
+    int cnt = 0;
+#pragma unroll
+    for (int o = -2; o <= 1; ++o) {
+        const int c = wrap_index(gnode + o, n);
+        const int ti = c / T;
+        bool dup = false;
+        for (int i = 0; i < cnt; ++i) dup |= (t[i] == ti);
+        if (!dup) t[cnt++] = ti;
+    }
+    (void)nt;
+    return cnt;
+}
+
+__global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
+                                                            float* __restrict__ overflow,
+                                                            const int* __restrict__ ovf_count, DevDomain d,
+                                                            TileGeom g, float* __restrict__ rho) {
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    if (gx >= d.Nx) return;
+    int txs[4], tys[4];
+    const int ncx = window_candidates(gx, g.T, g.ntx, d.Nx, txs);
+    const int ncy = window_candidates(gy, g.R, g.nty, d.Ny, tys);
+    float s = 0.0f, c = 0.0f;
+    for (int iy = 0; iy < ncy; ++iy) {
+        const int ty = tys[iy];
+        const int y0 = ty * g.R;
+        const int th = min(g.R, d.Ny - y0);
+        // images gy' = gy + m*Ny inside [y0 - 1, y0 + th + 2)
+        int gyi = gy;
+        while (gyi - d.Ny >= y0 - 1) gyi -= d.Ny;
+        while (gyi < y0 - 1) gyi += d.Ny;
+        for (; gyi < y0 + th + 2; gyi += d.Ny) {
+            const int wy = gyi - (y0 - 1);
+            for (int ix = 0; ix < ncx; ++ix) {
+                const int tx = txs[ix];
+                const int x0 = tx * g.T;
+                const int tw = min(g.T, d.Nx - x0);
+                int gxi = gx;
+                while (gxi - d.Nx >= x0 - 1) gxi -= d.Nx;
+                while (gxi < x0 - 1) gxi += d.Nx;
+                for (; gxi < x0 + tw + 2; gxi += d.Nx) {
+                    const int wx = gxi - (x0 - 1);
+                    const float v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
+                    c += two_sum_err(s, v);
+                    s += v;
+                }
+            }
+        }
+    }
+    float out = s + c;
+    const size_t node = (size_t)gy * d.Nx + gx;
+    if (*ovf_count > 0) {
+        out += overflow[node];
+        overflow[node] = 0.0f;
+    }
+    rho[node] = out;
+}
+
+__global__ void reset_counter_kernel(int* __restrict__ c) { *c = 0; }
+
+__global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, size_t n, float v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int S>
+void launch_atomic(const DepositArgs& a) {
+    const int64_t blocks = (a.n + 255) / 256;
+    if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
+    if (a.n > 0)
+        hipLaunchKernelGGL(deposit_atomic_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, a.stream, a.x, a.y, a.vx,
+                           a.vy, a.q, a.n, a.dom, a.dt, a.rho, a.status);
+    if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
+}
+
+template <int S, int K>
+void launch_tile(const DepositArgs& a) {
+    const TileGeom& g = a.geom;
+    const int waves_per_block = 4;
+    const unsigned blocks = (unsigned)((g.ntiles + waves_per_block - 1) / waves_per_block);
+    const size_t lds = (size_t)waves_per_block * g.lds_per_wave * sizeof(float);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, a.stream, a.ovf_count);
+    if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
+    hipLaunchKernelGGL((deposit_tile_kernel<S, K>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream, a.x,
+                       a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.overflow, a.ovf_count, a.status);
+    if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
+    hipLaunchKernelGGL(deposit_merge_kernel, dim3((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny), dim3(256), 0,
+                       a.stream, a.partial, a.overflow, a.ovf_count, a.dom, g, a.rho);
+}
+
+template <int S>
+void launch_tile_k(const DepositArgs& a) {
+    switch (a.ppl) {
+        case 4: launch_tile<S, 4>(a); break;
+        case 16: launch_tile<S, 16>(a); break;
+        default: launch_tile<S, 8>(a); break;
+    }
+}
+
+}  // namespace
+
+TileGeom make_tile_geom(int Nx, int Ny, int T, int R) {
+    TileGeom g;
+    g.T = T < 1 ? 1 : (T > Nx ? Nx : T);
+    g.R = R < 1 ? 1 : (R > Ny ? Ny : R);
+    if (g.T > 4096) g.T = 4096;
+    if (g.R > 64) g.R = 64;
+    g.ntx = (Nx + g.T - 1) / g.T;
+    g.nty = (Ny + g.R - 1) / g.R;
+    g.ntiles = g.ntx * g.nty;
+    g.WX = g.T + 3;
+    g.WY = g.R + 3;
+    g.wsize = g.WX * g.WY;
+    g.lds_per_wave = 2 * g.wsize + 4 * g.R + 1;
+    return g;
+}
+
+void fill_async(float* p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+hipError_t launch_deposit(const DepositArgs& a) {
+    if (a.backend == 1) {
+        fill_async(a.rho, (size_t)a.dom.Nx * a.dom.Ny, 0.0f, a.stream);
+        switch (a.scheme) {
+            case NGP: launch_atomic<NGP>(a); break;
+            case CIC: launch_atomic<CIC>(a); break;
+            case TSC: launch_atomic<TSC>(a); break;
+            default: launch_atomic<ESK>(a); break;
+        }
+    } else {
+        switch (a.scheme) {
+            case NGP: launch_tile_k<NGP>(a); break;
+            case CIC: launch_tile_k<CIC>(a); break;
+            case TSC: launch_tile_k<TSC>(a); break;
+            default: launch_tile_k<ESK>(a); break;
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pichip

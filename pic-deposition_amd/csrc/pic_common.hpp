@@ -1,0 +1,212 @@
+// Device-side building blocks shared by the deposition, sort and step kernels.
+//
+// Geometry follows pic::Domain (reference src/pic/domain.hpp:7-44): a periodic
+// Lx x Ly box of Nx x Ny cells, cell (ix, iy) has row-major key iy*Nx + ix, a
+// particle's cell is floor(x/dx) wrapped into [0, Nx).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pichip {
+
+enum Scheme { NGP = 0, CIC = 1, TSC = 2, ESK = 3 };
+
+struct DevDomain {
+    int Nx, Ny;
+    double dx, dy;        // FP64 cell sizes: cell keys are computed exactly as the reference does
+    double Lx, Ly;
+    float inv_dx, inv_dy;  // FP32 reciprocals for shape-function weights
+    float inv_vol;         // 1 / (dx*dy) rounded to FP32
+};
+
+// Reference wrapIndex (src/deposition/ngp.cpp:9): ((i % n) + n) % n, with the
+// common in-range case taken without integer division.
+__device__ __forceinline__ int wrap_index(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;
+    if (i >= n && i < 2 * n) return i - n;
+    if (i < 0 && i >= -n) return i + n;
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// Cell key of a position exactly as particleCellIndex (src/sorting/cell_sort.cpp:8-13)
+// via Domain::cellCoords (domain.hpp:34-43): FP64 division, floor, wrap.
+__device__ __forceinline__ int cell_key_exact(float x, float y, const DevDomain& d) {
+    const double fx = (double)x / d.dx;
+    const double fy = (double)y / d.dy;
+    const int ix = wrap_index((int)floor(fx), d.Nx);
+    const int iy = wrap_index((int)floor(fy), d.Ny);
+    return iy * d.Nx + ix;
+}
+
+// Knuth TwoSum error term: (a + b) == fl(a + b) + two_sum_err(a, b) exactly.
+__device__ __forceinline__ float two_sum_err(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    return (a - (s - bb)) + (b - bb);
+}
+
+// ---------------------------------------------------------------------------
+// Shape functions.  Each scheme maps one particle to a home cell (cx, cy) =
+// unwrapped floor of the position in cell units, and V weights on a fixed
+// SX x SY node stencil whose lower-left corner is (cx + OX, cy + OY).  Slot k
+// is node (cx + OX + k % SX, cy + OY + k / SX).  Weights include q / cellVolume.
+// ---------------------------------------------------------------------------
+template <int S>
+struct Stencil;
+
+// NGP: src/deposition/ngp.cpp:108-120.  Nearest node is ix or ix+1 per axis
+// (wx >= 0.5 picks ix+1), so the stencil is 2x2 with one non-zero slot.
+template <>
+struct Stencil<NGP> {
+    static constexpr int SX = 2, SY = 2, V = 4, OX = 0, OY = 0;
+    static constexpr bool kNeedsVelocity = false;
+    __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
+                                                   int& cx, int& cy, float (&w)[V]) {
+        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
+        const float flx = floorf(fx), fly = floorf(fy);
+        cx = (int)flx;
+        cy = (int)fly;
+        const bool rx = (fx - flx) >= 0.5f, ry = (fy - fly) >= 0.5f;
+        w[0] = (!rx && !ry) ? qv : 0.0f;
+        w[1] = (rx && !ry) ? qv : 0.0f;
+        w[2] = (!rx && ry) ? qv : 0.0f;
+        w[3] = (rx && ry) ? qv : 0.0f;
+        return true;
+    }
+};
+
+// CIC: src/deposition/ngp.cpp:140-157, same product order q*(1-wx)*(1-wy).
+template <>
+struct Stencil<CIC> {
+    static constexpr int SX = 2, SY = 2, V = 4, OX = 0, OY = 0;
+    static constexpr bool kNeedsVelocity = false;
+    __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
+                                                   int& cx, int& cy, float (&w)[V]) {
+        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
+        const float flx = floorf(fx), fly = floorf(fy);
+        cx = (int)flx;
+        cy = (int)fly;
+        const float wx = fx - flx, wy = fy - fly;
+        const float ux = 1.0f - wx, uy = 1.0f - wy;
+        w[0] = qv * ux * uy;
+        w[1] = qv * wx * uy;
+        w[2] = qv * ux * wy;
+        w[3] = qv * wx * wy;
+        return true;
+    }
+};
+
+// TSC: tscWeights (src/deposition/ngp.cpp:28-34) with i0 = floor(fx - 0.5).
+// Written in terms of the in-cell fraction f = fx - floor(fx): for f >= 0.5,
+// i0 = cx and x' = f; for f < 0.5, i0 = cx - 1 and x' = f + 1.  The quadratic
+// B-spline values are the same polynomials re-expanded around f, which keeps
+// every subtraction exact in FP32.  Slots cover node offsets -1..2 per axis.
+__device__ __forceinline__ void tsc_1d(float f, float (&w)[4]) {
+    if (f >= 0.5f) {
+        const float a = 1.5f - f, b = f - 1.0f, c = f - 0.5f;
+        w[0] = 0.0f;
+        w[1] = 0.5f * a * a;
+        w[2] = 0.75f - b * b;
+        w[3] = 0.5f * c * c;
+    } else {
+        const float a = 0.5f - f, c = f + 0.5f;
+        w[0] = 0.5f * a * a;
+        w[1] = 0.75f - f * f;
+        w[2] = 0.5f * c * c;
+        w[3] = 0.0f;
+    }
+}
+
+template <>
+struct Stencil<TSC> {
+    static constexpr int SX = 4, SY = 4, V = 16, OX = -1, OY = -1;
+    static constexpr bool kNeedsVelocity = false;
+    __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
+                                                   int& cx, int& cy, float (&w)[V]) {
+        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
+        const float flx = floorf(fx), fly = floorf(fy);
+        cx = (int)flx;
+        cy = (int)fly;
+        float wx[4], wy[4];
+        tsc_1d(fx - flx, wx);
+        tsc_1d(fy - fly, wy);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) wx[a] *= qv;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) w[b * 4 + a] = wx[a] * wy[b];
+        return true;
+    }
+};
+
+// Reference "Esirkepov" density (esirkepovWeights1DSparse, src/deposition/ngp.cpp:36-84):
+// per axis, path-length fractions of the straight move x0 -> x0 + v*dt over
+// the floor cells it crosses, renormalised to sum 1; a static particle
+// (|dx| < 1e-15) gets the CIC pair.  For |v*dt| < 1 cell the support is
+// within offsets -1..1 of the home cell.  Returns false when a move crosses
+// more than one cell boundary (handled by the general per-particle path).
+__device__ __forceinline__ bool esk_1d(float x0, float v, float dt, float inv_dx, int& c0, float (&w)[3]) {
+    const float disp = v * dt;
+    const float x1 = x0 + disp;
+    const float f0 = x0 * inv_dx;
+    const float fl = floorf(f0);
+    c0 = (int)fl;
+    w[0] = w[1] = w[2] = 0.0f;
+    // The reference tests |x1 - x0| < 1e-15 in FP64; in FP32 x1 == x0 already
+    // for displacements far above 1e-15, so the displacement itself is tested.
+    if (fabsf(disp) < 1e-15f) {
+        const float xi = f0 - fl;
+        w[1] = 1.0f - xi;
+        w[2] = xi;
+        return true;
+    }
+    const float f1 = x1 * inv_dx;
+    if (disp > 0.0f) {
+        const float b = fl + 1.0f;  // upper face of the home cell, in cell units
+        if (f1 <= b) {
+            w[1] = 1.0f;
+            return true;
+        }
+        if (f1 > b + 1.0f) return false;
+        const float wa = b - f0, wb = f1 - b;
+        const float inv = 1.0f / (wa + wb);
+        w[1] = wa * inv;
+        w[2] = wb * inv;
+        return true;
+    }
+    // Moving left: the home cell's lower face is at fl.
+    if (f1 >= fl) {
+        w[1] = 1.0f;
+        return true;
+    }
+    if (f1 < fl - 1.0f) return false;
+    const float wa = f0 - fl, wb = fl - f1;
+    const float inv = 1.0f / (wa + wb);
+    w[1] = wa * inv;
+    w[0] = wb * inv;
+    return true;
+}
+
+template <>
+struct Stencil<ESK> {
+    static constexpr int SX = 3, SY = 3, V = 9, OX = -1, OY = -1;
+    static constexpr bool kNeedsVelocity = true;
+    __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
+                                                   const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
+        float wx[3], wy[3];
+        const bool okx = esk_1d(x, vx, dt, d.inv_dx, cx, wx);
+        const bool oky = esk_1d(y, vy, dt, d.inv_dy, cy, wy);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) wx[a] *= qv;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) w[b * 3 + a] = wx[a] * wy[b];
+        return okx && oky;
+    }
+};
+
+}  // namespace pichip

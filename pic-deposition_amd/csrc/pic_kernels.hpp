@@ -1,0 +1,109 @@
+// Host-side launch interfaces of the HIP kernels (one translation unit each:
+// deposit.hip, sort.hip, step.hip).  Used only by api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pic_common.hpp"
+
+namespace pichip {
+
+// Tiling of the grid for the LDS-privatised deposit: tiles of T cells along x
+// by R cells along y; each tile's LDS window spans nodes (x0-1 .. x0+T+1) x
+// (y0-1 .. y0+R+1), i.e. WX = T+3 by WY = R+3.
+struct TileGeom {
+    int T, R;
+    int ntx, nty, ntiles;
+    int WX, WY, wsize;
+    int lds_per_wave;  // floats: 2 windows + row tables
+};
+
+TileGeom make_tile_geom(int Nx, int Ny, int T, int R);
+
+struct DepositArgs {
+    const float *x, *y, *vx, *vy, *q;
+    int64_t n;
+    const int* cell_off;  // G+1 offsets of the cell-sorted order (tile backend only)
+    DevDomain dom;
+    int scheme;   // Scheme
+    int backend;  // 1 = global atomics, 2 = LDS tiles
+    int ppl;      // particles per lane per chunk (4, 8, 16)
+    TileGeom geom;
+    float dt;
+    float* rho;       // [Ny][Nx] FP32 out
+    float* partial;   // ntiles * wsize
+    float* overflow;  // [Ny][Nx], kept zero between calls
+    int* ovf_count;
+    int* status;      // bit 0: Esirkepov move longer than the general walk supports
+    hipStream_t stream;
+    hipEvent_t ev_begin = nullptr;  // if set, recorded around the main scatter kernel
+    hipEvent_t ev_end = nullptr;
+};
+
+hipError_t launch_deposit(const DepositArgs& a);
+void fill_async(float* p, size_t n, float v, hipStream_t s);
+
+// ---- sort.hip -------------------------------------------------------------
+struct SortBuffers {
+    int* keys;       // [n] cell key of each particle, current order
+    int* keys_alt;   // [n]
+    int* perm;       // [n]
+    int* perm_alt;   // [n]
+    uint32_t* hist;  // radix histogram scratch (digit-major, blocks minor)
+    size_t hist_cap;
+    uint32_t* scan_tmp;  // scan scratch
+    size_t scan_tmp_cap;
+    int* cell_off;   // [G+1]
+};
+
+// Bytes of histogram / scan scratch the sort needs for n particles.
+size_t sort_hist_elems(int64_t n);
+size_t sort_scan_tmp_elems(size_t m);
+
+// Computes keys for the current order and a stable permutation sorting them;
+// on return perm[i] is the source index of the i-th particle in sorted order,
+// keys_sorted (== sb.keys) the sorted keys, and cell_off the G+1 offsets.
+hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, const DevDomain& d, SortBuffers& sb,
+                                 hipStream_t s);
+// dst[i] = src[perm[i]] for one float stream.
+void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n, hipStream_t s);
+void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s);
+
+// ---- step.hip -------------------------------------------------------------
+void launch_gather_push(float* x, float* y, float* vx, float* vy, const float* q, const float* m, int64_t n,
+                        const float* Ex, const float* Ey, const DevDomain& d, float dt, int do_gather, int do_push,
+                        hipStream_t s);
+
+// FP64 sums: out[0] = sum(rho), out[1] = sum(Ex^2+Ey^2), out[2] = sum(0.5 m v^2),
+// out[3] = sum(q), out[4] = sum(|q|).  `partials` holds 5 * nblocks doubles.
+void launch_diagnostics(const float* rho, const float* Ex, const float* Ey, int64_t G, const float* vx,
+                        const float* vy, const float* q, const float* m, int64_t n, double* partials, int nblocks,
+                        double* out, hipStream_t s);
+
+// Particle initialisers (Philox4x32-10 counter-based draws; see step.hip).
+struct InitParams {
+    int kind;       // 0 uniform positions, 1 x drawn from n(x) ~ 1 + amp cos(k x), 2 two beams +-amp
+    uint64_t seed;
+    double amp;     // density perturbation amplitude (kind 1) or beam speed (kind 2)
+    double pert;    // velocity perturbation: vx += pert * sin(k x)
+    double vth;     // Maxwellian spread per velocity component
+    double vmax;    // uniform velocity U(-vmax, vmax) per component
+    int mode;       // k = 2 pi mode / Lx
+    double qp, mp;  // per-particle charge and mass
+};
+void launch_init(float* x, float* y, float* vx, float* vy, float* q, float* m, int64_t n, const DevDomain& d,
+                 const InitParams& p, hipStream_t s);
+
+// Spectral Poisson solve helpers (hipFFT plans live in api.hip).
+// in[i] = (rho[i] + bg) * cv - mean, mean from the FP64 charge sum.
+void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, double cv, const double* rho_sum,
+                         hipStream_t s);
+// Divide the half spectrum [Ny][Nx/2+1] by |k|^2 and by Nx*Ny; zero k = 0.
+void launch_poisson_scale(float2* spec, const DevDomain& d, hipStream_t s);
+// Ex = -(phi(i+1,j) - phi(i,j))/dx, Ey = -(phi(i,j+1) - phi(i,j))/dy.
+void launch_efield(const float* phi, float* Ex, float* Ey, const DevDomain& d, hipStream_t s);
+void launch_add_scalar(float* p, int64_t n, float v, hipStream_t s);
+
+}  // namespace pichip

@@ -1,0 +1,299 @@
+// Kernels that close the timestep: CIC gather + leapfrog push, the spectral
+// Poisson solve around hipFFT, FP64 diagnostics, and on-device particle
+// initialisers.
+
+#include "pic_common.hpp"
+#include "pic_kernels.hpp"
+
+namespace pichip {
+
+namespace {
+
+// gatherFieldsSoA (reference src/deposition/deposition.cpp:335-356) followed by
+// pushParticlesSoA (deposition.cpp:367-374) in FP32.  The push reduces the
+// displacement modulo L before adding it, so that |v dt| >> L does not cost
+// FP32 position bits; in exact arithmetic this equals fmod(x + v dt, L).
+__global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px, float* __restrict__ py,
+                                                          float* __restrict__ pvx, float* __restrict__ pvy,
+                                                          const float* __restrict__ pq, const float* __restrict__ pm,
+                                                          int64_t n, const float* __restrict__ Ex,
+                                                          const float* __restrict__ Ey, DevDomain d, float dt,
+                                                          int do_gather, int do_push) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = px[i], y = py[i];
+    float vx = pvx[i], vy = pvy[i];
+    if (do_gather) {
+        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
+        const float flx = floorf(fx), fly = floorf(fy);
+        const float wx = fx - flx, wy = fy - fly;
+        const int ix = wrap_index((int)flx, d.Nx), iy = wrap_index((int)fly, d.Ny);
+        const int ixp = ix + 1 == d.Nx ? 0 : ix + 1;
+        const int iyp = iy + 1 == d.Ny ? 0 : iy + 1;
+        const size_t r0 = (size_t)iy * d.Nx, r1 = (size_t)iyp * d.Nx;
+        const float ux = 1.0f - wx, uy = 1.0f - wy;
+        const float ex = Ex[r0 + ix] * ux * uy + Ex[r0 + ixp] * wx * uy + Ex[r1 + ix] * ux * wy + Ex[r1 + ixp] * wx * wy;
+        const float ey = Ey[r0 + ix] * ux * uy + Ey[r0 + ixp] * wx * uy + Ey[r1 + ix] * ux * wy + Ey[r1 + ixp] * wx * wy;
+        const float qm = -pq[i] / pm[i];
+        vx += qm * ex * dt;
+        vy += qm * ey * dt;
+        pvx[i] = vx;
+        pvy[i] = vy;
+    }
+    if (do_push) {
+        const float Lx = (float)d.Lx, Ly = (float)d.Ly;
+        x += fmodf(vx * dt, Lx);
+        y += fmodf(vy * dt, Ly);
+        x = fmodf(x, Lx);
+        if (x < 0.0f) x += Lx;
+        y = fmodf(y, Ly);
+        if (y < 0.0f) y += Ly;
+        px[i] = x;
+        py[i] = y;
+    }
+}
+
+// Five FP64 sums in one pass; fixed-shape tree per block, then a single block
+// adds the block partials in index order (deterministic).
+__global__ __launch_bounds__(256) void diag_partial_kernel(const float* __restrict__ rho,
+                                                           const float* __restrict__ Ex,
+                                                           const float* __restrict__ Ey, int64_t G,
+                                                           const float* __restrict__ vx,
+                                                           const float* __restrict__ vy,
+                                                           const float* __restrict__ q,
+                                                           const float* __restrict__ m, int64_t n,
+                                                           double* __restrict__ partials) {
+    double s[5] = {0, 0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += stride) {
+        if (rho) s[0] += (double)rho[i];
+        if (Ex) {
+            const double ex = Ex[i], ey = Ey[i];
+            s[1] += ex * ex + ey * ey;
+        }
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double a = vx[i], b = vy[i], mm = m[i], qq = q[i];
+        s[2] += 0.5 * mm * (a * a + b * b);
+        s[3] += qq;
+        s[4] += fabs(qq);
+    }
+    __shared__ double red[5][256];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void diag_final_kernel(const double* __restrict__ partials, int nblocks,
+                                                         double* __restrict__ out) {
+    __shared__ double red[256];
+    for (int k = 0; k < 5; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(size_t)k * nblocks + b];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = red[0];
+        __syncthreads();
+    }
+}
+
+// ---- Philox4x32-10 (Salmon et al., SC'11) ----------------------------------
+struct U4 {
+    uint32_t v[4];
+};
+
+__device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)M0 * c0;
+        const uint64_t p1 = (uint64_t)M1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = (uint32_t)p0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+        k0 += W0;
+        k1 += W1;
+    }
+    U4 o;
+    o.v[0] = c0;
+    o.v[1] = c1;
+    o.v[2] = c2;
+    o.v[3] = c3;
+    return o;
+}
+
+// 53-bit uniform in [0, 1) from two words.
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
+    const uint64_t r = (((uint64_t)a << 32) | b) >> 11;
+    return (double)r * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ float wrap_pos(double x, double L) {
+    float f = (float)x;
+    const float Lf = (float)L;
+    if (f >= Lf) f -= Lf;
+    if (f < 0.0f) f += Lf;
+    if (f >= Lf) f = 0.0f;
+    return f;
+}
+
+__global__ __launch_bounds__(256) void init_kernel(float* __restrict__ px, float* __restrict__ py,
+                                                   float* __restrict__ pvx, float* __restrict__ pvy,
+                                                   float* __restrict__ pq, float* __restrict__ pm, int64_t n,
+                                                   DevDomain d, InitParams p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k0 = (uint32_t)p.seed, k1 = (uint32_t)(p.seed >> 32);
+    const U4 a = philox((uint32_t)i, (uint32_t)((uint64_t)i >> 32), 0u, 0u, k0, k1);
+    const U4 b = philox((uint32_t)i, (uint32_t)((uint64_t)i >> 32), 1u, 0u, k0, k1);
+    const double ux = u01(a.v[0], a.v[1]);
+    const double uy = u01(a.v[2], a.v[3]);
+    const double kx = 2.0 * M_PI * (double)p.mode / d.Lx;
+    double x = ux * d.Lx;
+    if (p.kind == 1 && p.amp != 0.0) {
+        // Inverse CDF of n(x) ~ 1 + amp cos(kx) on [0, Lx) by Newton iteration.
+        const double target = ux * d.Lx;
+        for (int it = 0; it < 8; ++it) {
+            const double F = x + p.amp / kx * sin(kx * x) - target;
+            const double dF = 1.0 + p.amp * cos(kx * x);
+            x -= F / dF;
+        }
+    }
+    const double y = uy * d.Ly;
+    double vx = 0.0, vy = 0.0;
+    if (p.vmax > 0.0) {
+        vx = (2.0 * u01(b.v[0], b.v[1]) - 1.0) * p.vmax;
+        vy = (2.0 * u01(b.v[2], b.v[3]) - 1.0) * p.vmax;
+    }
+    if (p.vth > 0.0) {
+        // Box-Muller on (1 - u) in (0, 1].
+        const double r = sqrt(-2.0 * log(1.0 - u01(b.v[0], b.v[1])));
+        const double th = 2.0 * M_PI * u01(b.v[2], b.v[3]);
+        vx += p.vth * r * cos(th);
+        vy += p.vth * r * sin(th);
+    }
+    if (p.kind == 2) vx += (i < n / 2 ? 1.0 : -1.0) * p.amp;
+    if (p.pert != 0.0) vx += p.pert * sin(kx * x);
+    px[i] = wrap_pos(x, d.Lx);
+    py[i] = wrap_pos(y, d.Ly);
+    pvx[i] = (float)vx;
+    pvy[i] = (float)vy;
+    pq[i] = (float)p.qp;
+    pm[i] = (float)p.mp;
+}
+
+
+// ---- spectral Poisson (reference src/pic/poisson_fft.cpp:45-94) ------------
+// The reference zeroes the k = 0 coefficient; here the mean of rho*cv is
+// removed in FP64 before the FP32 transform as well, so the DC term never
+// enters the single-precision FFT.
+__global__ __launch_bounds__(256) void poisson_prep_kernel(const float* __restrict__ rho, float* __restrict__ in,
+                                                           int64_t G, double bg, double cv,
+                                                           const double* __restrict__ rho_sum) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    const double mean = (*rho_sum + bg * (double)G) * cv / (double)G;
+    in[i] = (float)(((double)rho[i] + bg) * cv - mean);
+}
+
+// k^2 table of poisson_fft.cpp:23-35 evaluated on the fly; the 1/(Nx*Ny) of
+// poisson_fft.cpp:66 is folded in.
+__global__ __launch_bounds__(256) void poisson_scale_kernel(float2* __restrict__ spec, DevDomain d) {
+    const int hx = d.Nx / 2 + 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= hx) return;
+    const size_t id = (size_t)j * hx + i;
+    if (i == 0 && j == 0) {
+        spec[id] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const double kx = 2.0 * M_PI * (double)((i <= d.Nx / 2) ? i : i - d.Nx) / d.Lx;
+    const double ky = 2.0 * M_PI * (double)((j <= d.Ny / 2) ? j : j - d.Ny) / d.Ly;
+    const float f = (float)(1.0 / ((kx * kx + ky * ky) * (double)d.Nx * (double)d.Ny));
+    const float2 v = spec[id];
+    spec[id] = make_float2(v.x * f, v.y * f);
+}
+
+__global__ __launch_bounds__(256) void efield_kernel(const float* __restrict__ phi, float* __restrict__ Ex,
+                                                     float* __restrict__ Ey, DevDomain d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= d.Nx) return;
+    const int ip = i + 1 == d.Nx ? 0 : i + 1;
+    const int jp = j + 1 == d.Ny ? 0 : j + 1;
+    const size_t id = (size_t)j * d.Nx + i;
+    const float p = phi[id];
+    Ex[id] = -((phi[(size_t)j * d.Nx + ip] - p) * (float)(1.0 / d.dx));
+    Ey[id] = -((phi[(size_t)jp * d.Nx + i] - p) * (float)(1.0 / d.dy));
+}
+
+__global__ __launch_bounds__(256) void add_scalar_kernel(float* __restrict__ p, int64_t n, float v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] += v;
+}
+
+}  // namespace
+
+void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, double cv, const double* rho_sum,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(poisson_prep_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, rho, in, G, bg, cv,
+                       rho_sum);
+}
+
+void launch_poisson_scale(float2* spec, const DevDomain& d, hipStream_t s) {
+    const int hx = d.Nx / 2 + 1;
+    hipLaunchKernelGGL(poisson_scale_kernel, dim3((unsigned)((hx + 255) / 256), (unsigned)d.Ny), dim3(256), 0, s, spec,
+                       d);
+}
+
+void launch_efield(const float* phi, float* Ex, float* Ey, const DevDomain& d, hipStream_t s) {
+    hipLaunchKernelGGL(efield_kernel, dim3((unsigned)((d.Nx + 255) / 256), (unsigned)d.Ny), dim3(256), 0, s, phi, Ex,
+                       Ey, d);
+}
+
+void launch_add_scalar(float* p, int64_t n, float v, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(add_scalar_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_gather_push(float* x, float* y, float* vx, float* vy, const float* q, const float* m, int64_t n,
+                        const float* Ex, const float* Ey, const DevDomain& d, float dt, int do_gather, int do_push,
+                        hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(gather_push_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, vx, vy, q, m,
+                           n, Ex, Ey, d, dt, do_gather, do_push);
+}
+
+void launch_diagnostics(const float* rho, const float* Ex, const float* Ey, int64_t G, const float* vx,
+                        const float* vy, const float* q, const float* m, int64_t n, double* partials, int nblocks,
+                        double* out, hipStream_t s) {
+    hipLaunchKernelGGL(diag_partial_kernel, dim3(nblocks), dim3(256), 0, s, rho, Ex, Ey, G, vx, vy, q, m, n,
+                       partials);
+    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, s, partials, nblocks, out);
+}
+
+void launch_init(float* x, float* y, float* vx, float* vy, float* q, float* m, int64_t n, const DevDomain& d,
+                 const InitParams& p, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, vx, vy, q, m, n, d,
+                           p);
+}
+
+}  // namespace pichip

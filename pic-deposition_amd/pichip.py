@@ -1,0 +1,341 @@
+"""ctypes binding of libpic_hip.so (the C ABI in include/pic_hip.h).
+
+This is the Python face of the HIP path: every call goes to the gfx950
+kernels.  There is no CPU fallback: if the shared library is missing or cannot
+be loaded, importing this module raises.
+
+Names mirror the reference's API (ehocine/pic-deposition):
+``DepositionScheme`` / ``DepositionBackend`` (src/deposition/deposition.hpp:11-13),
+``scheme_from_string`` / ``backend_from_string`` (deposition.cpp:402-415),
+``deposit_charge`` (depositChargeSoA, deposition.cpp:181-200, GPU branch).
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpic_hip.so")
+
+
+class PicError(RuntimeError):
+    """A pic_hip call returned a non-zero pic_status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"pic_hip error {status}: {message}")
+        self.status = status
+
+
+class DepositionScheme(enum.IntEnum):
+    NGP = 0
+    CIC = 1
+    TSC = 2
+    Esirkepov = 3
+
+
+class DepositionBackend(enum.IntEnum):
+    CPU = 0
+    GPUAtomics = 1
+    GPUPrivatized = 2
+
+
+_SCHEME_NAMES = {"NGP": 0, "CIC": 1, "TSC": 2, "Esirkepov": 3}
+_BACKEND_NAMES = {"CPU": 0, "GPU_Atomics": 1, "GPU_Priv": 2}
+
+
+def scheme_from_string(name: str) -> DepositionScheme:
+    """schemeFromString (deposition.cpp:402-408): unknown names raise."""
+    if name not in _SCHEME_NAMES:
+        raise RuntimeError("Unknown deposition scheme: " + name)
+    return DepositionScheme(_SCHEME_NAMES[name])
+
+
+def backend_from_string(name: str) -> DepositionBackend:
+    """backendFromString (deposition.cpp:410-415): unknown names raise."""
+    if name not in _BACKEND_NAMES:
+        raise RuntimeError("Unknown backend: " + name)
+    return DepositionBackend(_BACKEND_NAMES[name])
+
+
+def scheme_name(s: int) -> str:
+    return {0: "NGP", 1: "CIC", 2: "TSC", 3: "Esirkepov"}.get(int(s), "Unknown")
+
+
+def backend_name(b: int) -> str:
+    return {0: "CPU", 1: "GPU_Atomics", 2: "GPU_Priv"}.get(int(b), "Unknown")
+
+
+class Domain(ctypes.Structure):
+    """pic_domain / pic::Domain: dx = Lx / Nx."""
+
+    _fields_ = [("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32)]
+
+    @property
+    def dx(self) -> float:
+        return self.Lx / self.Nx
+
+    @property
+    def dy(self) -> float:
+        return self.Ly / self.Ny
+
+    def cell_volume(self) -> float:
+        return self.dx * self.dy
+
+
+class Diagnostics(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("rho_sum", "integrated_rho", "field_energy", "kinetic_energy",
+                                                "total_charge", "total_abs_charge", "charge_error")]
+
+
+class InitParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("seed", ctypes.c_uint64), ("amp", ctypes.c_double),
+                ("pert", ctypes.c_double), ("vth", ctypes.c_double), ("vmax", ctypes.c_double),
+                ("mode", ctypes.c_int32), ("q", ctypes.c_double), ("m", ctypes.c_double)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `make -C pic-deposition_amd` (or __graft_entry__.build())")
+    return ctypes.CDLL(LIB_PATH)
+
+
+_lib = _load()
+
+_P = ctypes.c_void_p
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+_SIGS = {
+    "pic_hip_last_error": (ctypes.c_char_p, []),
+    "pic_hip_version": (ctypes.c_char_p, []),
+    "pic_hip_deposit_host": (ctypes.c_int, [_f64p, _f64p, _f64p, _f64p, _f64p, ctypes.c_size_t,
+                                            ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_double, _f64p]),
+    "pic_hip_create": (ctypes.c_int, [ctypes.POINTER(Domain), ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "pic_hip_destroy": (ctypes.c_int, [_P]),
+    "pic_hip_stream": (_P, [_P]),
+    "pic_hip_sync": (ctypes.c_int, [_P]),
+    "pic_hip_count": (ctypes.c_size_t, [_P]),
+    "pic_hip_set_tiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pic_hip_get_tiling": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "pic_hip_upload": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_size_t]),
+    "pic_hip_download": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    "pic_hip_init_particles": (ctypes.c_int, [_P, ctypes.POINTER(InitParams), ctypes.c_size_t]),
+    "pic_hip_sort": (ctypes.c_int, [_P]),
+    "pic_hip_sort_perm": (ctypes.c_int, [_P, _i32p, _i32p, _i32p]),
+    "pic_hip_get_keys": (ctypes.c_int, [_P, _i32p]),
+    "pic_hip_deposit": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "pic_hip_get_rho": (ctypes.c_int, [_P, _f32p]),
+    "pic_hip_get_rho_f64": (ctypes.c_int, [_P, _f64p]),
+    "pic_hip_set_rho": (ctypes.c_int, [_P, _f32p]),
+    "pic_hip_add_background": (ctypes.c_int, [_P, ctypes.c_double]),
+    "pic_hip_poisson": (ctypes.c_int, [_P]),
+    "pic_hip_get_fields": (ctypes.c_int, [_P, _f32p, _f32p, _f32p]),
+    "pic_hip_set_fields": (ctypes.c_int, [_P, _f32p, _f32p]),
+    "pic_hip_gather_push": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "pic_hip_diagnostics": (ctypes.c_int, [_P, ctypes.POINTER(Diagnostics)]),
+    "pic_hip_profile_begin": (ctypes.c_int, [_P, ctypes.c_int]),
+    "pic_hip_profile_end": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise PicError(status, _lib.pic_hip_last_error().decode())
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def version() -> str:
+    return _lib.pic_hip_version().decode()
+
+
+def make_domain(Lx: float, Ly: float, Nx: int, Ny: int) -> Domain:
+    return Domain(float(Lx), float(Ly), int(Nx), int(Ny))
+
+
+def deposit_host(x, y, q, domain: Domain, scheme=DepositionScheme.CIC, backend=DepositionBackend.GPUAtomics,
+                 sorted_=False, vx=None, vy=None, dt=0.0) -> np.ndarray:
+    """pic_hip_deposit_host: FP64 host arrays in, FP64 rho[Ny, Nx] out (synchronous)."""
+    n = len(x)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    vx = None if vx is None else np.ascontiguousarray(vx, dtype=np.float64)
+    vy = None if vy is None else np.ascontiguousarray(vy, dtype=np.float64)
+    rho = np.zeros(domain.Nx * domain.Ny, dtype=np.float64)
+    _check(_lib.pic_hip_deposit_host(_ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double), _ptr(vx, ctypes.c_double),
+                                     _ptr(vy, ctypes.c_double), _ptr(q, ctypes.c_double), n, ctypes.byref(domain),
+                                     int(scheme), int(backend), int(bool(sorted_)), float(dt),
+                                     _ptr(rho, ctypes.c_double)))
+    return rho.reshape(domain.Ny, domain.Nx)
+
+
+@dataclass
+class Tiling:
+    tile_x: int
+    tile_y: int
+    particles_per_lane: int
+
+
+class Context:
+    """A device-resident PIC instance (pic_hip_ctx)."""
+
+    def __init__(self, domain: Domain, capacity: int):
+        self.domain = domain
+        h = _P()
+        _check(_lib.pic_hip_create(ctypes.byref(domain), int(capacity), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.pic_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return _lib.pic_hip_stream(self._h) or 0
+
+    @property
+    def count(self) -> int:
+        return _lib.pic_hip_count(self._h)
+
+    def sync(self) -> None:
+        _check(_lib.pic_hip_sync(self._h))
+
+    def set_tiling(self, tile_x: int, tile_y: int, particles_per_lane: int = 8) -> None:
+        _check(_lib.pic_hip_set_tiling(self._h, tile_x, tile_y, particles_per_lane))
+
+    def tiling(self) -> Tiling:
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.pic_hip_get_tiling(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return Tiling(a.value, b.value, c.value)
+
+    def upload(self, x, y, q, vx=None, vy=None, m=None) -> None:
+        arrs = [_f32(a) for a in (x, y, vx, vy, q, m)]
+        n = len(arrs[0])
+        for a in arrs:
+            if a is not None and len(a) != n:
+                raise ValueError("all particle arrays must have the same length")
+        _check(_lib.pic_hip_upload(self._h, *[_ptr(a, ctypes.c_float) for a in arrs], n))
+
+    def download(self):
+        n = self.count
+        out = [np.empty(n, dtype=np.float32) for _ in range(6)]
+        _check(_lib.pic_hip_download(self._h, *[_ptr(a, ctypes.c_float) for a in out]))
+        return dict(zip(("x", "y", "vx", "vy", "q", "m"), out))
+
+    def init_particles(self, n: int, kind: int = 0, seed: int = 42, amp: float = 0.0, pert: float = 0.0,
+                       vth: float = 0.0, vmax: float = 0.0, mode: int = 1, q: float = 1.0, m: float = 1.0) -> None:
+        p = InitParams(kind, seed, amp, pert, vth, vmax, mode, q, m)
+        _check(_lib.pic_hip_init_particles(self._h, ctypes.byref(p), int(n)))
+
+    def sort(self) -> None:
+        _check(_lib.pic_hip_sort(self._h))
+
+    def sort_perm(self):
+        n = self.count
+        keys = np.empty(n, dtype=np.int32)
+        perm = np.empty(n, dtype=np.int32)
+        off = np.empty(self.domain.Nx * self.domain.Ny + 1, dtype=np.int32)
+        _check(_lib.pic_hip_sort_perm(self._h, _ptr(keys, ctypes.c_int32), _ptr(perm, ctypes.c_int32),
+                                      _ptr(off, ctypes.c_int32)))
+        return keys, perm, off
+
+    def keys(self) -> np.ndarray:
+        k = np.empty(self.count, dtype=np.int32)
+        _check(_lib.pic_hip_get_keys(self._h, _ptr(k, ctypes.c_int32)))
+        return k
+
+    def deposit(self, scheme=DepositionScheme.CIC, backend=DepositionBackend.GPUPrivatized, dt: float = 0.0) -> None:
+        _check(_lib.pic_hip_deposit(self._h, int(scheme), int(backend), float(dt)))
+
+    def rho(self) -> np.ndarray:
+        r = np.empty(self.domain.Nx * self.domain.Ny, dtype=np.float32)
+        _check(_lib.pic_hip_get_rho(self._h, _ptr(r, ctypes.c_float)))
+        return r.reshape(self.domain.Ny, self.domain.Nx)
+
+    def set_rho(self, rho) -> None:
+        r = np.ascontiguousarray(rho, dtype=np.float32).ravel()
+        _check(_lib.pic_hip_set_rho(self._h, _ptr(r, ctypes.c_float)))
+
+    def add_background(self, value: float) -> None:
+        _check(_lib.pic_hip_add_background(self._h, float(value)))
+
+    def poisson(self) -> None:
+        _check(_lib.pic_hip_poisson(self._h))
+
+    def fields(self):
+        G = self.domain.Nx * self.domain.Ny
+        phi, ex, ey = (np.empty(G, dtype=np.float32) for _ in range(3))
+        _check(_lib.pic_hip_get_fields(self._h, _ptr(phi, ctypes.c_float), _ptr(ex, ctypes.c_float),
+                                       _ptr(ey, ctypes.c_float)))
+        shp = (self.domain.Ny, self.domain.Nx)
+        return phi.reshape(shp), ex.reshape(shp), ey.reshape(shp)
+
+    def set_fields(self, ex, ey) -> None:
+        a, b = _f32(np.ravel(ex)), _f32(np.ravel(ey))
+        _check(_lib.pic_hip_set_fields(self._h, _ptr(a, ctypes.c_float), _ptr(b, ctypes.c_float)))
+
+    def gather_push(self, dt: float, gather: bool = True, push: bool = True) -> None:
+        _check(_lib.pic_hip_gather_push(self._h, float(dt), int(gather), int(push)))
+
+    def profile_begin(self, max_launches: int) -> None:
+        _check(_lib.pic_hip_profile_begin(self._h, int(max_launches)))
+
+    def profile_end(self):
+        """(summed main-kernel milliseconds, launches) since profile_begin."""
+        ms, k = ctypes.c_double(), ctypes.c_int()
+        _check(_lib.pic_hip_profile_end(self._h, ctypes.byref(ms), ctypes.byref(k)))
+        return ms.value, k.value
+
+    def diagnostics(self) -> Diagnostics:
+        d = Diagnostics()
+        _check(_lib.pic_hip_diagnostics(self._h, ctypes.byref(d)))
+        return d
+
+
+def deposit_charge(particles: dict, domain: Domain, scheme="CIC", backend="GPU_Priv", sorted_=False,
+                   esirkepov_dt=0.0) -> np.ndarray:
+    """GPU branch of depositChargeSoA (deposition.cpp:181-200): host FP64 SoA in,
+    rho (FP64) out; the caller's particle order is not changed."""
+    s = scheme_from_string(scheme) if isinstance(scheme, str) else DepositionScheme(scheme)
+    b = backend_from_string(backend) if isinstance(backend, str) else DepositionBackend(backend)
+    if b == DepositionBackend.CPU:
+        raise RuntimeError("CPU backend requested from the HIP library; the CPU schemes live in the host code")
+    return deposit_host(particles["x"], particles["y"], particles["q"], domain, s, b, sorted_,
+                        particles.get("vx"), particles.get("vy"), esirkepov_dt)

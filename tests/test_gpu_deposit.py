@@ -1,0 +1,217 @@
+"""GPU deposition parity against the FP64 oracle (all four schemes, both GPU
+backends).  Tolerance: relative L2 <= 1e-6 per scheme (FP32 positions,
+weights and accumulation on the device; the oracle deposits the same FP32
+positions promoted to FP64)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL_L2 = 1e-6
+SCHEMES = [0, 1, 2, 3]
+BACKENDS = [1, 2]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32)
+
+
+def uniform_particles(n, L=(1.0, 1.0), seed=0, vmax=0.0):
+    rng = np.random.default_rng(seed)
+    x = f32(rng.uniform(0, L[0], n))
+    y = f32(rng.uniform(0, L[1], n))
+    x[x >= np.float32(L[0])] = 0
+    y[y >= np.float32(L[1])] = 0
+    q = f32(rng.uniform(0.5, 1.5, n) / 64)
+    vx = f32(rng.uniform(-vmax, vmax, n))
+    vy = f32(rng.uniform(-vmax, vmax, n))
+    return x, y, q, vx, vy
+
+
+def gpu_rho(hip, dom, x, y, q, vx, vy, scheme, backend, dt, tiling=None):
+    with hip.Context(dom, len(x)) as c:
+        if tiling:
+            c.set_tiling(*tiling)
+        c.upload(x, y, q, vx, vy)
+        if backend == 2:
+            c.sort()
+        c.deposit(scheme, backend, dt)
+        return c.rho().astype(np.float64)
+
+
+def oracle_rho(orc, dom, x, y, q, vx, vy, scheme, dt):
+    od = orc.domain(dom.Lx, dom.Ly, dom.Nx, dom.Ny)
+    return orc.deposit(scheme, x.astype(np.float64), y.astype(np.float64), q.astype(np.float64), od,
+                       vx.astype(np.float64), vy.astype(np.float64), dt)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_parity_64x64_64ppc(hip, orc, scheme, backend):
+    say("parity 64x64 64ppc", scheme, backend)
+    dom = hip.make_domain(1.0, 1.0, 64, 64)
+    dt = 0.01
+    x, y, q, vx, vy = uniform_particles(64 * 64 * 64, seed=scheme, vmax=0.5 / 64 / dt)
+    g = gpu_rho(hip, dom, x, y, q, vx, vy, scheme, backend, dt)
+    r = oracle_rho(orc, dom, x, y, q, vx, vy, scheme, dt)
+    assert rel_l2(g, r) <= REL_L2
+    assert np.abs(g - r).max() <= 1e-5 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("tiling", [(32, 8, 8), (64, 4, 16), (16, 16, 4), (3, 3, 4), (7, 5, 8), (128, 1, 8)])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_tile_shapes(hip, orc, scheme, tiling):
+    say("tiling", scheme, tiling)
+    dom = hip.make_domain(1.0, 1.0, 70, 37)  # ragged against every tile shape
+    dt = 0.01
+    x, y, q, vx, vy = uniform_particles(70 * 37 * 20, seed=7, vmax=0.5 / 70 / dt)
+    g = gpu_rho(hip, dom, x, y, q, vx, vy, scheme, 2, dt, tiling)
+    r = oracle_rho(orc, dom, x, y, q, vx, vy, scheme, dt)
+    assert rel_l2(g, r) <= REL_L2
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1.0, 1.0), (2, 3, 1.0, 1.0), (4, 4, 1.0, 1.0), (100, 50, 1.0, 1.0),
+                                   (64, 32, 2.0, 1.0), (128, 128, 128.0, 128.0)])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_grid_shapes_and_domains(hip, orc, scheme, shape):
+    nx, ny, lx, ly = shape
+    say("grid", scheme, shape)
+    dom = hip.make_domain(lx, ly, nx, ny)
+    dt = 0.01
+    n = max(nx * ny * 16, 1000)
+    x, y, q, vx, vy = uniform_particles(n, (lx, ly), seed=3, vmax=0.4 * (lx / nx) / dt)
+    for backend in BACKENDS:
+        g = gpu_rho(hip, dom, x, y, q, vx, vy, scheme, backend, dt)
+        r = oracle_rho(orc, dom, x, y, q, vx, vy, scheme, dt)
+        assert rel_l2(g, r) <= REL_L2, backend
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_edge_positions(hip, orc, scheme):
+    say("edge positions", scheme)
+    dom = hip.make_domain(1.0, 1.0, 16, 16)
+    faces = f32(np.arange(17) / 16.0)
+    x = np.concatenate([faces, f32([0.0, np.nextafter(np.float32(1), np.float32(0)), 1.0, 0.5, 0.03125, 0.999])])
+    y = np.concatenate([faces[::-1], f32([0.0, 0.0, 0.5, 1.0, 0.999, np.nextafter(np.float32(1), np.float32(0))])])
+    n = len(x)
+    q = f32(np.linspace(0.5, 1.5, n))
+    vx = f32(np.where(np.arange(n) % 2, 1.0, -1.0) * 0.4 / 16 / 0.01)
+    vy = f32(np.zeros(n))
+    for backend in BACKENDS:
+        g = gpu_rho(hip, dom, x, y, q, vx, vy, scheme, backend, 0.01)
+        r = oracle_rho(orc, dom, x, y, q, vx, vy, scheme, 0.01)
+        assert rel_l2(g, r) <= REL_L2, backend
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_empty_and_single(hip, orc, backend):
+    say("empty/single", backend)
+    dom = hip.make_domain(1.0, 1.0, 8, 8)
+    e = f32([])
+    g = gpu_rho(hip, dom, e, e, e, e, e, 1, backend, 0.0)
+    assert np.all(g == 0)
+    x, y, q = f32([0.3]), f32([0.55]), f32([2.0])
+    z = f32([0.0])
+    for s in SCHEMES:
+        g = gpu_rho(hip, dom, x, y, q, z, z, s, backend, 0.0)
+        r = oracle_rho(orc, dom, x, y, q, z, z, s, 0.0)
+        assert rel_l2(g, r) <= REL_L2
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_drifted_particles_take_overflow_path(hip, orc, scheme):
+    """Sort, then move particles (as between re-sorts) and deposit with the
+    stale cell order: exact, via the global-atomic overflow path."""
+    say("drift", scheme)
+    dom = hip.make_domain(1.0, 1.0, 32, 32)
+    dt = 0.01
+    x, y, q, vx, vy = uniform_particles(32 * 32 * 40, seed=11, vmax=1.5 / 32 / dt)
+    with hip.Context(dom, len(x)) as c:
+        c.upload(x, y, q, vx, vy)
+        c.sort()
+        c.gather_push(dt, gather=False, push=True)  # up to 1.5 cells of drift
+        c.deposit(scheme, 2, dt)
+        g = c.rho().astype(np.float64)
+        p = c.download()
+    r = oracle_rho(orc, dom, p["x"], p["y"], p["q"], p["vx"], p["vy"], scheme, dt)
+    assert rel_l2(g, r) <= REL_L2
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_sorted_path_is_deterministic(hip, scheme):
+    say("determinism", scheme)
+    dom = hip.make_domain(1.0, 1.0, 64, 64)
+    x, y, q, vx, vy = uniform_particles(64 * 64 * 32, seed=5, vmax=0.3 / 64 / 0.01)
+    with hip.Context(dom, len(x)) as c:
+        c.upload(x, y, q, vx, vy)
+        c.sort()
+        c.deposit(scheme, 2, 0.01)
+        a = c.rho().copy()
+        c.deposit(scheme, 2, 0.01)
+        b = c.rho().copy()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_charge_conservation_like_reference_gpu_tests(hip, orc):
+    """main_test.cpp:158-180 (GPU CIC, 128x128, 5000 Maxwellian particles,
+    seed 123).  The reference asks 1e-9 in FP64; the FP32 path is held to
+    1e-6 here."""
+    say("charge conservation")
+    od = orc.domain(1.0, 1.0, 128, 128)
+    p = orc.init("maxwellian", 5000, od, seed=123)
+    dom = hip.make_domain(1.0, 1.0, 128, 128)
+    for backend in BACKENDS:
+        for scheme in SCHEMES:
+            rho = hip.deposit_host(p["x"], p["y"], p["q"], dom, scheme, backend, False, p["vx"], p["vy"], 0.0)
+            err = orc.charge_error(orc.integrated_rho(rho, od), orc.total_charge(p["q"]),
+                                   orc.total_abs_charge(p["q"]))
+            assert err <= 1e-6, (backend, scheme, err)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_drop_in_host_call_matches_oracle(hip, orc, scheme):
+    say("drop-in", scheme)
+    od = orc.domain(1.0, 1.0, 32, 32)
+    p = orc.init("maxwellian", 20000, od, seed=123)
+    dom = hip.make_domain(1.0, 1.0, 32, 32)
+    vx, vy = p["vx"] * 0.01, p["vy"] * 0.01  # keep Esirkepov moves under a cell
+    x32, y32 = p["x"].astype(np.float32).astype(np.float64), p["y"].astype(np.float32).astype(np.float64)
+    r = orc.deposit(scheme, x32, y32, p["q"], od, vx.astype(np.float32), vy.astype(np.float32), 0.01)
+    for backend in BACKENDS:
+        for sorted_ in (False, True):
+            rho = hip.deposit_host(p["x"], p["y"], p["q"], dom, scheme, backend, sorted_, vx, vy, 0.01)
+            assert rel_l2(rho, r) <= REL_L2, (backend, sorted_)
+    # the caller's arrays are untouched
+    assert np.array_equal(p["x"], orc.init("maxwellian", 20000, od, seed=123)["x"])
+
+
+def test_golden_noise_vs_grid_on_gpu(hip, orc):
+    """GPU deposit of the reference study's particles reproduces the
+    reference-held spectral-noise scalars (noise_vs_grid.csv) to FP32 accuracy."""
+    import json
+    import os
+
+    say("golden noise")
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_scalars.json")))
+    gold = gold["noise_vs_grid"]
+    names = {"NGP": 0, "CIC": 1, "TSC": 2, "Esirkepov": 3}
+    for g, name, val in gold["rows"]:
+        od = orc.domain(1.0, 1.0, g, g)
+        p = orc.init("maxwellian", gold["num_particles"], od, seed=gold["seed"], T=1.0)
+        dom = hip.make_domain(1.0, 1.0, g, g)
+        rho = hip.deposit_host(p["x"], p["y"], p["q"], dom, names[name], 2, True, p["vx"], p["vy"], 0.0)
+        assert orc.spectral_noise_level(rho) == pytest.approx(val, rel=1e-5), (g, name)

@@ -1,0 +1,301 @@
+"""The FP64 oracle pinned against the reference build, closed forms and the
+reference-held scalars (CPU only)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _need_ref(orc):
+    if not orc.ref_available():
+        pytest.skip("reference build oracle/_ref not present (needs the reference checkout at build time)")
+
+
+def _maxwellian(orc, n, nx, seed=123, L=1.0):
+    dom = orc.domain(L, L, nx, nx)
+    return dom, orc.init("maxwellian", n, dom, seed=seed)
+
+
+# ---- against the reference's own code ---------------------------------------
+
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+def test_serial_deposit_bit_exact_vs_reference(orc, scheme):
+    _need_ref(orc)
+    dom, p = _maxwellian(orc, 3000, 32)
+    dt = 0.01 if scheme == 3 else 0.0
+    mine = orc.deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt)
+    ref = orc.ref_deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt, threads=1)
+    assert np.array_equal(mine, ref)
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 3])
+def test_openmp_deposit_matches_reference_openmp(orc, scheme):
+    # TSC is excluded: the reference OpenMP TSC indexes rho_local[-1] for
+    # x < 0.5*dx (deposition_parallel.cpp:108-111); the oracle wraps instead.
+    _need_ref(orc)
+    dom, p = _maxwellian(orc, 4000, 32, seed=7)
+    dt = 0.01 if scheme == 3 else 0.0
+    mine = orc.deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt, threads=4)
+    ref = orc.ref_deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt, threads=4)
+    np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+def test_openmp_equals_serial(orc, scheme):
+    dom, p = _maxwellian(orc, 5000, 16, seed=9)
+    dt = 0.01 if scheme == 3 else 0.0
+    a = orc.deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt)
+    b = orc.deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt, threads=3)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * np.abs(a).max())
+
+
+def test_cell_keys_bit_exact_vs_reference(orc):
+    _need_ref(orc)
+    rng = np.random.default_rng(1)
+    for (L, n) in [(1.0, 64), (128.0, 128), (1.0, 100), (3.7, 50)]:
+        dom = orc.domain(L, L, n, n)
+        x = rng.uniform(-0.5 * L, 1.5 * L, 20000)
+        y = rng.uniform(-0.5 * L, 1.5 * L, 20000)
+        # boundary cases: cell faces, 0, L, just below L
+        faces = np.arange(n + 1) * (L / n)
+        x[: n + 1] = faces
+        y[: n + 1] = faces[::-1]
+        x[n + 1] = np.nextafter(L, 0)
+        assert np.array_equal(orc.cell_keys(x, y, dom), orc.ref_cell_index(x, y, dom))
+
+
+def test_sort_matches_reference_up_to_ties(orc):
+    _need_ref(orc)
+    dom, p = _maxwellian(orc, 20000, 32, seed=5)
+    keys = orc.cell_keys(p["x"], p["y"], dom)
+    perm = orc.sort_perm(keys)
+    refp = orc.ref_sort(p, dom)
+    ref_keys = orc.cell_keys(refp["x"], refp["y"], dom)
+    assert np.array_equal(keys[perm], ref_keys)  # identical key sequence
+    # same particles in each cell (the reference's in-cell order is unstable)
+    for k in ("x", "y", "q"):
+        mine = np.lexsort((p[k][perm], keys[perm]))
+        theirs = np.lexsort((refp[k], ref_keys))
+        assert np.array_equal(p[k][perm][mine], refp[k][theirs])
+    # the oracle permutation is the stable one
+    assert np.array_equal(perm, np.argsort(keys, kind="stable"))
+
+
+def test_cell_offsets_closed_form(orc):
+    rng = np.random.default_rng(3)
+    keys = rng.integers(0, 50, 1000).astype(np.int32)
+    off = orc.cell_offsets(keys, 64)
+    expect = np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=64))])
+    assert np.array_equal(off, expect)
+
+
+def test_gather_push_bit_exact_vs_reference(orc):
+    _need_ref(orc)
+    dom, p = _maxwellian(orc, 5000, 32, seed=11)
+    rng = np.random.default_rng(2)
+    ex = rng.normal(size=(32, 32))
+    ey = rng.normal(size=(32, 32))
+    vx, vy = orc.gather(p["x"], p["y"], p["vx"], p["vy"], p["q"], p["m"], ex, ey, dom, 0.01)
+    x, y = orc.push(p["x"], p["y"], vx, vy, dom, 0.01)
+    ref = orc.ref_gather_push(p, ex, ey, dom, 0.01)
+    for a, b in ((vx, ref["vx"]), (vy, ref["vy"]), (x, ref["x"]), (y, ref["y"])):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_initialisers_bit_exact_vs_reference(orc, kind):
+    _need_ref(orc)
+    dom = orc.domain(1.0, 1.0, 32, 32)
+    n = 1001
+    if kind == 0:
+        mine = orc.init("maxwellian", n, dom, seed=42, T=1.0)
+        ref = orc.ref_init(0, n, dom, p1=1.0, seed=42)
+    elif kind == 1:
+        mine = orc.init("langmuir", n, dom, amplitude=0.05, mode=2)
+        ref = orc.ref_init(1, n, dom, p1=0.05, mode=2)
+    else:
+        q1 = kind == 3
+        mine = orc.init("two_stream", n, dom, seed=42, beam_velocity=0.2, perturbation=0.01, quasi1d=q1)
+        ref = orc.ref_init(kind, n, dom, p1=0.2, p2=0.01, seed=42)
+    for k in mine:
+        assert np.array_equal(mine[k], ref[k]), k
+
+
+def test_tsc_and_esirkepov_weight_helpers_vs_reference(orc):
+    _need_ref(orc)
+    for xi in [0.0, 0.25, 0.5, 0.75, 1.0, 3.49, 3.5, 7.999]:
+        i0, w = orc.ref_tsc_weights(xi)
+        assert i0 == int(np.floor(xi - 0.5))
+        assert abs(w.sum() - 1.0) < 1e-15
+        assert abs((w * (i0 + np.arange(3))).sum() - xi) < 1e-14  # first moment
+    n, dx = 16, 1.0 / 16
+    for x0, x1 in [(0.30, 0.30), (0.30, 0.33), (0.30, 0.26), (0.05, 0.01), (0.31, 0.45)]:
+        w = orc.ref_esirkepov_weights_1d(x0, x1, dx, n)
+        assert abs(w.sum() - 1.0) < 1e-14
+
+
+# ---- closed forms -------------------------------------------------------------
+
+def test_single_particle_kats(orc):
+    dom = orc.domain(1.0, 1.0, 8, 8)
+    inv_vol = 64.0
+    x, y, q = np.array([0.3]), np.array([0.55]), np.array([2.0])
+    fx, fy = 0.3 * 8, 0.55 * 8  # 2.4, 4.4
+    # CIC: nodes (2,4),(3,4),(2,5),(3,5)
+    rho = orc.deposit(1, x, y, q, dom)
+    wx, wy = fx - 2, fy - 4
+    exp = {(4, 2): (1 - wx) * (1 - wy), (4, 3): wx * (1 - wy), (5, 2): (1 - wx) * wy, (5, 3): wx * wy}
+    for (j, i), w in exp.items():
+        assert rho[j, i] == pytest.approx(2.0 * inv_vol * w, rel=1e-14)
+    assert rho.sum() == pytest.approx(2.0 * inv_vol, rel=1e-14)
+    # NGP: wx = 0.4 < 0.5 -> ix = 2, wy = 0.4 -> iy = 4
+    rho = orc.deposit(0, x, y, q, dom)
+    assert rho[4, 2] == pytest.approx(2.0 * inv_vol) and np.count_nonzero(rho) == 1
+    # TSC: 3x3 around nearest node (2, 4), quadratic spline weights
+    rho = orc.deposit(2, x, y, q, dom)
+
+    def spl(d):
+        d = abs(d)
+        return 0.75 - d * d if d <= 0.5 else 0.5 * (1.5 - d) ** 2
+
+    for j in range(3, 6):
+        for i in range(1, 4):
+            assert rho[j, i] == pytest.approx(2.0 * inv_vol * spl(fx - i) * spl(fy - j), rel=1e-13)
+    assert rho.sum() == pytest.approx(2.0 * inv_vol, rel=1e-14)
+    # first moments of CIC and TSC reproduce the position
+    for s in (1, 2):
+        rho = orc.deposit(s, x, y, q, dom)
+        ii, jj = np.meshgrid(np.arange(8), np.arange(8))
+        assert (rho * ii).sum() / rho.sum() == pytest.approx(fx, rel=1e-13)
+        assert (rho * jj).sum() / rho.sum() == pytest.approx(fy, rel=1e-13)
+
+
+def test_periodic_wrap_cases(orc):
+    dom = orc.domain(1.0, 1.0, 8, 8)
+    # x in the last cell: CIC wraps to column 0; TSC with x < 0.5 dx wraps to column 7
+    rho = orc.deposit(1, np.array([0.99]), np.array([0.5]), np.array([1.0]), dom)
+    assert rho[:, 0].sum() > 0 and rho[:, 7].sum() > 0
+    rho = orc.deposit(2, np.array([0.01]), np.array([0.5]), np.array([1.0]), dom)
+    assert rho[:, 7].sum() > 0 and rho[:, 1].sum() > 0
+    assert rho.sum() == pytest.approx(64.0, rel=1e-14)
+    # x == L lands in cell 0 (cellCoords wraps)
+    k = orc.cell_keys(np.array([1.0]), np.array([0.0]), dom)
+    assert k[0] == 0
+
+
+def test_esirkepov_density_closed_forms(orc):
+    dom = orc.domain(1.0, 1.0, 16, 16)
+    p = {"x": np.array([0.30]), "y": np.array([0.50]), "q": np.array([1.0])}
+    static = orc.deposit(3, p["x"], p["y"], p["q"], dom, np.array([0.0]), np.array([0.0]), 0.01)
+    cic = orc.deposit(1, p["x"], p["y"], p["q"], dom)
+    assert np.array_equal(static, cic)  # zero motion == CIC exactly
+    # move from 0.30 (cell 4.8) to 0.33 (cell 5.28): fractions 0.2/0.48 and 0.28/0.48 along x,
+    # y static at a node -> all y weight on row 8
+    rho = orc.deposit(3, p["x"], p["y"], p["q"], dom, np.array([3.0]), np.array([0.0]), 0.01)
+    vol = 1.0 / 256
+    assert rho[8, 4] * vol == pytest.approx(0.2 / 0.48, rel=1e-12)
+    assert rho[8, 5] * vol == pytest.approx(0.28 / 0.48, rel=1e-12)
+
+
+def test_charge_conservation_like_reference_tests(orc):
+    """main_test.cpp:25-72: 32x32, 1000 Maxwellian particles, seed 123."""
+    dom, p = _maxwellian(orc, 1000, 32, seed=123)
+    for scheme, dt, tol in ((1, 0.0, 1e-10), (2, 0.0, 1e-10), (3, 0.0, 1e-10), (3, 0.01, 1e-10), (0, 0.0, 0.05)):
+        rho = orc.deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt)
+        err = orc.charge_error(orc.integrated_rho(rho, dom), orc.total_charge(p["q"]), orc.total_abs_charge(p["q"]))
+        assert err <= tol
+
+
+def test_uniform_lattice_gives_uniform_rho(orc):
+    dom = orc.domain(1.0, 1.0, 16, 16)
+    g = (np.arange(64) + 0.5) / 64
+    X, Y = np.meshgrid(g, g)
+    q = np.full(X.size, 1.0 / X.size)
+    for s in (1, 2):
+        rho = orc.deposit(s, X.ravel(), Y.ravel(), q, dom)
+        np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
+
+
+def test_poisson_single_mode(orc):
+    nx = ny = 32
+    dom = orc.domain(1.0, 1.0, nx, ny)
+    xs = np.arange(nx) / nx
+    k = 2 * np.pi * 3
+    rho = np.tile(np.cos(k * xs), (ny, 1))
+    phi, ex, ey = orc.poisson(rho, dom)
+    cv = 1.0 / (nx * ny)
+    np.testing.assert_allclose(phi, cv * rho / k ** 2, atol=1e-15)
+    exp_ex = -(np.roll(phi, -1, axis=1) - phi) * nx
+    np.testing.assert_allclose(ex, exp_ex, atol=1e-15)
+    np.testing.assert_allclose(ey, 0.0, atol=1e-15)
+    # mean removal: a constant rho gives phi = 0
+    phi, _, _ = orc.poisson(np.full((ny, nx), 5.0), dom)
+    np.testing.assert_allclose(phi, 0.0, atol=1e-15)
+
+
+def test_poisson_matches_numpy_spectral_solve(orc):
+    rng = np.random.default_rng(0)
+    for nx, ny, L in ((16, 32, 1.0), (12, 10, 2.0)):
+        dom = orc.domain(L, L * 0.5, nx, ny)
+        rho = rng.normal(size=(ny, nx))
+        phi, _, _ = orc.poisson(rho, dom)
+        kx = 2 * np.pi * np.fft.fftfreq(nx, d=1.0 / nx) / L
+        ky = 2 * np.pi * np.fft.fftfreq(ny, d=1.0 / ny) / (L * 0.5)
+        K2 = kx[None, :] ** 2 + ky[:, None] ** 2
+        K2[0, 0] = 1.0
+        cv = (L / nx) * (L * 0.5 / ny)
+        F = np.fft.fft2(rho * cv) / K2
+        F[0, 0] = 0
+        np.testing.assert_allclose(phi, np.fft.ifft2(F).real, atol=1e-13 * np.abs(phi).max())
+
+
+def test_estimators_closed_forms(orc):
+    dt = 0.1
+    t = np.arange(400)
+    s = np.cos(2 * np.pi * 12 * t / 400) + 3.0
+    assert orc.dominant_frequency(s, dt) == pytest.approx(12 / (400 * dt))
+    g = 0.37
+    amp = 1e-3 * np.exp(g * (t + 100) * dt)
+    assert orc.growth_rate(amp, dt, 100) == pytest.approx(g, rel=1e-10)
+    dom = orc.domain(1.0, 1.0, 32, 8)
+    xs = (np.arange(32) + 0.5) / 32
+    ex = np.tile(0.7 * np.cos(2 * np.pi * xs) + 0.2 * np.sin(2 * np.pi * xs), (8, 1))
+    # (re, im) = (0.35, 0.1)
+    assert orc.fourier_mode_amplitude(ex, dom, 1) == pytest.approx(np.hypot(0.35, 0.1), rel=1e-12)
+    assert orc.plasma_frequency(orc.domain(1, 1, 64, 64), 5000) == pytest.approx(1 / np.sqrt(5000))
+
+
+def test_esirkepov_current_continuity(orc):
+    dom = orc.domain(1.0, 1.0, 16, 16)
+    rng = np.random.default_rng(4)
+    n, dt = 500, 0.01
+    x, y = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    vx, vy = rng.uniform(-0.5, 0.5, n) / 16 / dt, rng.uniform(-0.5, 0.5, n) / 16 / dt
+    q = rng.uniform(0.5, 1.5, n)
+    r0, r1, jx, jy = orc.esirkepov_current(x, y, vx, vy, q, dom, dt)
+    div = (jx - np.roll(jx, 1, axis=1)) * 16 + (jy - np.roll(jy, 1, axis=0)) * 16
+    resid = (r1 - r0) / dt + div
+    assert np.abs(resid).max() <= 1e-9 * np.abs((r1 - r0) / dt).max()
+    np.testing.assert_allclose(r0, orc.deposit(1, x, y, q, dom), rtol=1e-12, atol=1e-9)
+    # static particles: no current
+    _, _, jx, jy = orc.esirkepov_current(x, y, 0 * vx, 0 * vy, q, dom, dt)
+    assert np.abs(jx).max() == 0 and np.abs(jy).max() == 0
+
+
+# ---- reference-held scalars ----------------------------------------------------
+
+def test_golden_spectral_noise_vs_grid(orc):
+    """Oracle deposit + R2HC noise restatement reproduce noise_vs_grid.csv."""
+    gold = json.load(open(os.path.join(HERE, "golden", "reference_scalars.json")))["noise_vs_grid"]
+    names = {"NGP": 0, "CIC": 1, "TSC": 2, "Esirkepov": 3}
+    cache = {}
+    for g, name, val in gold["rows"]:
+        dom = orc.domain(1.0, 1.0, g, g)
+        if g not in cache:
+            cache[g] = orc.init("maxwellian", gold["num_particles"], dom, seed=gold["seed"], T=1.0)
+        p = cache[g]
+        rho = orc.deposit(names[name], p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], 0.0)
+        assert orc.spectral_noise_level(rho) == pytest.approx(val, rel=2e-9, abs=1e-10), (g, name)
