@@ -167,6 +167,52 @@ pic_status pic_hip_gather_push(pic_hip_ctx* ctx, double dt, int do_gather, int d
 /* FP64 sums over rho, Ex/Ey and the particles (synchronous). */
 pic_status pic_hip_diagnostics(pic_hip_ctx* ctx, pic_diagnostics* out);
 
+/* ---- timestep driver ------------------------------------------------------
+ * Runs nsteps of the electrostatic loop on the device.  Step orders follow
+ * the reference loops:
+ *   PIC_ORDER_PUSH_FIRST          push, deposit, +background, Poisson, gather
+ *       (Simulation::run for NGP/CIC/TSC, simulation.cpp:85-96,112-122;
+ *        validation.cpp:225-235 and :375-385)
+ *   PIC_ORDER_DEPOSIT_SOLVE_PUSH  deposit, +background, Poisson, push, gather
+ *       (Esirkepov in Simulation::run and the Langmuir loop, validation.cpp:226-233)
+ *   PIC_ORDER_DEPOSIT_PUSH_SOLVE  deposit, push, +background, Poisson, gather
+ *       (Esirkepov in the two-stream loop, validation.cpp:375-379)
+ * After each step the FP64 diagnostics are recorded for steps >= record_from:
+ * field energy (FieldGrid::fieldEnergy), |mode-1 Fourier amplitude of Ex|
+ * (extractFourierModeAmplitude, validation.cpp:99-119) and, when
+ * record_particles is set, kinetic energy and charge error
+ * (computeDiagnostics, diagnostics.cpp:42-51).  Output arrays hold
+ * nsteps - record_from values; NULL skips.  With GPU_PRIV the particles are
+ * re-sorted by cell before the deposit of every sort_interval-th step (first
+ * step included); between sorts the deposit stays exact through its overflow
+ * path.  use_graph captures one step (with and without sort) as hipGraphs and
+ * replays them. */
+enum { PIC_ORDER_PUSH_FIRST = 0, PIC_ORDER_DEPOSIT_SOLVE_PUSH = 1, PIC_ORDER_DEPOSIT_PUSH_SOLVE = 2 };
+
+typedef struct {
+    int32_t scheme;
+    int32_t backend;
+    int32_t sort_interval;
+    int32_t order;
+    double dt;
+    double background;
+    int32_t record_particles;
+    int32_t use_graph;
+} pic_step_config;
+
+pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nsteps, int32_t record_from,
+                       double* field_energy, double* mode1_amplitude, double* kinetic_energy, double* charge_error);
+
+/* Physics estimators on recorded series (host FP64; reference
+ * src/pic/validation.cpp).  Dominant frequency: DFT peak bin of the
+ * mean-removed series, k / (n dt) (:19-54).  Growth rate: least-squares slope
+ * of ln(max(s, 1e-30)) over the middle half, times t = (record_start + i) dt
+ * (:121-152).  Plasma frequency sqrt(n0 q^2 / m) with |q| = 1/N, n0 = N/V
+ * (:66-71). */
+double pic_hip_dominant_frequency(const double* signal, int64_t n, double dt);
+double pic_hip_growth_rate(const double* signal, int64_t n, double dt, int32_t record_start_step);
+double pic_hip_plasma_frequency(const pic_domain* domain, int64_t num_particles);
+
 /* ---- kernel timing (build-defined; no reference counterpart) -------------
  * Between begin and end, every deposit records a pair of HIP events on the
  * context's stream around its main scatter kernel (the tiled kernel for

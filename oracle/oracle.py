@@ -90,6 +90,8 @@ _OR_SIGS = {
     "or_fourier_mode_amplitude": (ctypes.c_double, [_D, _DOM, ctypes.c_int]),
     "or_growth_rate": (ctypes.c_double, [_D, _L, ctypes.c_double, ctypes.c_int]),
     "or_esirkepov_current": (ctypes.c_int, [_D, _D, _D, _D, _D, _L, _DOM, ctypes.c_double, _D, _D, _D, _D]),
+    "or_run_loop": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _L, _DOM, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
 }
 
 
@@ -303,6 +305,21 @@ def fourier_mode_amplitude(Ex, dom, mode=1):
 def growth_rate(signal, dt, record_start):
     s = _d(signal)
     return lib().or_growth_rate(_pd(s), len(s), float(dt), int(record_start))
+
+
+def run_loop(p: dict, dom, scheme, order, dt, background, steps, record_from):
+    """FP64 timestep loop (simulation.cpp / validation.cpp order); returns the
+    advanced particles and the recorded series."""
+    arrs = {k: _d(p[k]).copy() for k in ("x", "y", "vx", "vy", "q", "m")}
+    cap = max(steps - record_from, 0)
+    rec = {k: np.zeros(cap) for k in ("field_energy", "mode1", "kinetic", "charge_error")}
+    rc = lib().or_run_loop(*[_pd(arrs[k]) for k in ("x", "y", "vx", "vy", "q", "m")], len(arrs["x"]),
+                           ctypes.byref(dom), int(scheme), int(order), float(dt), float(background), int(steps),
+                           int(record_from), *[_pd(rec[k]) for k in ("field_energy", "mode1", "kinetic",
+                                                                      "charge_error")])
+    if rc:
+        raise ValueError("or_run_loop failed")
+    return arrs, rec
 
 
 def esirkepov_current(x, y, vx, vy, q, dom, dt):

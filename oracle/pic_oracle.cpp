@@ -475,7 +475,17 @@ void or_mt19937_words(unsigned seed, int64_t n, uint32_t* out) {
 // followed by E = -forward difference of phi.  This oracle computes that with
 // its own FP64 radix-2 complex FFT (power-of-two grids) or a direct DFT.
 
-static void fft1d(std::vector<std::complex<double>>& a, bool inverse) {
+// Twiddles w_k = exp(-+2 pi i k / n), k < n, evaluated directly (no recurrence).
+static std::vector<std::complex<double>> twiddles(size_t n, bool inverse) {
+    std::vector<std::complex<double>> w(n);
+    for (size_t k = 0; k < n; ++k) {
+        const double ang = 2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n) * (inverse ? 1.0 : -1.0);
+        w[k] = std::complex<double>(std::cos(ang), std::sin(ang));
+    }
+    return w;
+}
+
+static void fft1d(std::vector<std::complex<double>>& a, const std::vector<std::complex<double>>& tw) {
     const size_t n = a.size();
     if (n <= 1) return;
     if ((n & (n - 1)) == 0) {
@@ -486,12 +496,11 @@ static void fft1d(std::vector<std::complex<double>>& a, bool inverse) {
             if (i < j) std::swap(a[i], a[j]);
         }
         for (size_t len = 2; len <= n; len <<= 1) {
-            const double ang = 2.0 * M_PI / static_cast<double>(len) * (inverse ? 1.0 : -1.0);
+            const size_t step = n / len;
             for (size_t i = 0; i < n; i += len) {
                 for (size_t k = 0; k < len / 2; ++k) {
-                    const std::complex<double> w(std::cos(ang * k), std::sin(ang * k));
                     const std::complex<double> u = a[i + k];
-                    const std::complex<double> v = a[i + k + len / 2] * w;
+                    const std::complex<double> v = a[i + k + len / 2] * tw[k * step];
                     a[i + k] = u + v;
                     a[i + k + len / 2] = u - v;
                 }
@@ -502,26 +511,24 @@ static void fft1d(std::vector<std::complex<double>>& a, bool inverse) {
     std::vector<std::complex<double>> out(n);
     for (size_t k = 0; k < n; ++k) {
         std::complex<double> s(0.0, 0.0);
-        for (size_t t = 0; t < n; ++t) {
-            const double ang = 2.0 * M_PI * static_cast<double>((k * t) % n) / static_cast<double>(n) *
-                               (inverse ? 1.0 : -1.0);
-            s += a[t] * std::complex<double>(std::cos(ang), std::sin(ang));
-        }
+        for (size_t t = 0; t < n; ++t) s += a[t] * tw[(k * t) % n];
         out[k] = s;
     }
     a.swap(out);
 }
 
 static void fft2d(std::vector<std::complex<double>>& a, int nx, int ny, bool inverse) {
+    const auto twx = twiddles(static_cast<size_t>(nx), inverse);
+    const auto twy = twiddles(static_cast<size_t>(ny), inverse);
     std::vector<std::complex<double>> row(static_cast<size_t>(nx)), col(static_cast<size_t>(ny));
     for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) row[static_cast<size_t>(i)] = a[static_cast<size_t>(j) * nx + i];
-        fft1d(row, inverse);
+        fft1d(row, twx);
         for (int i = 0; i < nx; ++i) a[static_cast<size_t>(j) * nx + i] = row[static_cast<size_t>(i)];
     }
     for (int i = 0; i < nx; ++i) {
         for (int j = 0; j < ny; ++j) col[static_cast<size_t>(j)] = a[static_cast<size_t>(j) * nx + i];
-        fft1d(col, inverse);
+        fft1d(col, twy);
         for (int j = 0; j < ny; ++j) a[static_cast<size_t>(j) * nx + i] = col[static_cast<size_t>(j)];
     }
 }
@@ -648,6 +655,54 @@ double or_growth_rate(const double* signal, int64_t n, double dt, int record_sta
     const double denom = static_cast<double>(count) * stt - st * st;
     if (std::abs(denom) < 1e-30) return 0.0;
     return (static_cast<double>(count) * stl - st * sl) / denom;
+}
+
+// ---------------------------------------------------------------------------
+// Timestep loop (src/pic/simulation.cpp:69-145, src/pic/validation.cpp:223-241
+// and :373-387).  order 0: push, deposit, +bg, solve, gather; order 1:
+// deposit, +bg, solve, push, gather; order 2: deposit, push, +bg, solve,
+// gather.  The deposit is unsorted (the validation loops set sorted=false).
+// After each step with index >= record_from it records field energy,
+// |mode-1 amplitude of Ex|, kinetic energy and charge error.  Particles are
+// advanced in place.
+// ---------------------------------------------------------------------------
+int or_run_loop(double* x, double* y, double* vx, double* vy, const double* q, const double* m, int64_t n,
+                const or_domain* dom, int scheme, int order, double dt, double background, int steps,
+                int record_from, double* field_energy, double* mode1, double* kinetic, double* charge_err) {
+    Dom d(dom);
+    const size_t G = static_cast<size_t>(d.Nx) * d.Ny;
+    std::vector<double> rho(G), phi(G), Ex(G, 0.0), Ey(G, 0.0);
+    const double abs_q = or_total_abs_charge(q, n);
+    const double tot_q = or_total_charge(q, n);
+    for (int step = 0; step < steps; ++step) {
+        auto deposit = [&]() { return or_deposit(scheme, x, y, vx, vy, q, n, dom, dt, rho.data()); };
+        auto solve = [&]() {
+            for (double& v : rho) v += background;
+            or_poisson(rho.data(), dom, phi.data(), Ex.data(), Ey.data());
+        };
+        if (order == 0) {
+            or_push(x, y, vx, vy, n, dom, dt);
+            if (deposit()) return 1;
+            solve();
+        } else if (order == 1) {
+            if (deposit()) return 1;
+            solve();
+            or_push(x, y, vx, vy, n, dom, dt);
+        } else {
+            if (deposit()) return 1;
+            or_push(x, y, vx, vy, n, dom, dt);
+            solve();
+        }
+        or_gather(x, y, vx, vy, q, m, n, Ex.data(), Ey.data(), dom, dt);
+        if (step >= record_from) {
+            const int s = step - record_from;
+            if (field_energy) field_energy[s] = or_field_energy(Ex.data(), Ey.data(), dom);
+            if (mode1) mode1[s] = or_fourier_mode_amplitude(Ex.data(), dom, 1);
+            if (kinetic) kinetic[s] = or_kinetic_energy(vx, vy, m, n);
+            if (charge_err) charge_err[s] = or_charge_error(or_integrated_rho(rho.data(), dom), tot_q, abs_q);
+        }
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

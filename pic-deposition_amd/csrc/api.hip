@@ -66,6 +66,8 @@ DevDomain make_dev_domain(const pic_domain& d) {
     dd.inv_dx = (float)(1.0 / dd.dx);
     dd.inv_dy = (float)(1.0 / dd.dy);
     dd.inv_vol = (float)(1.0 / (dd.dx * dd.dy));
+    int ex = 0, ey = 0;
+    dd.exact = (std::frexp(dd.dx, &ex) == 0.5 && std::frexp(dd.dy, &ey) == 0.5) ? 1 : 0;
     return dd;
 }
 
@@ -84,6 +86,7 @@ struct pic_hip_ctx {
     bool order_valid = false;
     float* rho = nullptr;
     float* overflow = nullptr;
+    float* comp = nullptr;
     int* ovf_count = nullptr;
     int* status = nullptr;
     float* partial = nullptr;
@@ -94,7 +97,10 @@ struct pic_hip_ctx {
     float* Ex = nullptr;
     float* Ey = nullptr;
     float* fft_in = nullptr;
-    float2* spec = nullptr;
+    float2* spec = nullptr;     // phi spectrum [Ny][Nx/2+1]
+    float2* spec_ex = nullptr;  // Ex, Ey spectra
+    float2* spec_ey = nullptr;
+    PoissonTables ptab{};
     hipfftHandle plan_r2c = 0, plan_c2r = 0;
     bool fft_ready = false;
     double* diag_partials = nullptr;
@@ -102,6 +108,10 @@ struct pic_hip_ctx {
     int diag_blocks = 0;
     int* h_status = nullptr;     // pinned
     double* h_diag = nullptr;    // pinned
+    double* cosk = nullptr;      // cos/sin(2 pi (i + 1/2) dx / Lx), mode-1 diagnostic tables
+    double* sink = nullptr;
+    int* step_counter = nullptr;
+    double* step_partials = nullptr;
     std::vector<hipEvent_t> prof_events;  // pairs (begin, end)
     int prof_max = 0, prof_used = 0;
     bool profiling = false;
@@ -124,6 +134,7 @@ struct pic_hip_ctx {
         (void)hipFree(sb.cell_off);
         (void)hipFree(rho);
         (void)hipFree(overflow);
+        (void)hipFree(comp);
         (void)hipFree(ovf_count);
         (void)hipFree(status);
         (void)hipFree(partial);
@@ -132,8 +143,18 @@ struct pic_hip_ctx {
         (void)hipFree(Ey);
         (void)hipFree(fft_in);
         (void)hipFree(spec);
+        (void)hipFree(spec_ex);
+        (void)hipFree(spec_ey);
+        (void)hipFree(ptab.k2x);
+        (void)hipFree(ptab.k2y);
+        (void)hipFree(ptab.dx_sym);
+        (void)hipFree(ptab.dy_sym);
         (void)hipFree(diag_partials);
         (void)hipFree(diag_out);
+        (void)hipFree(cosk);
+        (void)hipFree(sink);
+        (void)hipFree(step_counter);
+        (void)hipFree(step_partials);
         (void)hipHostFree(h_status);
         (void)hipHostFree(h_diag);
         if (stream) (void)hipStreamDestroy(stream);
@@ -176,6 +197,35 @@ pic_status ensure_fft(pic_hip_ctx* c) {
     PIC_HIP_TRY(dmalloc(&c->phi, (size_t)c->G));
     PIC_HIP_TRY(dmalloc(&c->fft_in, (size_t)c->G));
     PIC_HIP_TRY(dmalloc(&c->spec, (size_t)c->dom.Ny * hx));
+    PIC_HIP_TRY(dmalloc(&c->spec_ex, (size_t)c->dom.Ny * hx));
+    PIC_HIP_TRY(dmalloc(&c->spec_ey, (size_t)c->dom.Ny * hx));
+    {
+        // k^2 per axis (poisson_fft.cpp:23-35) and the forward-difference
+        // symbols (e^{i k d} - 1)/d, tabulated in FP64.
+        const int nx = c->dom.Nx, ny = c->dom.Ny;
+        std::vector<double> k2x((size_t)hx), k2y((size_t)ny);
+        std::vector<float2> sx((size_t)hx), sy((size_t)ny);
+        for (int i = 0; i < hx; ++i) {
+            const double kx = 2.0 * M_PI * ((i <= nx / 2) ? i : i - nx) / c->dom.Lx;
+            k2x[(size_t)i] = kx * kx;
+            const double th = 2.0 * M_PI * (double)i / nx;
+            sx[(size_t)i] = make_float2((float)((std::cos(th) - 1.0) / c->dev.dx), (float)(std::sin(th) / c->dev.dx));
+        }
+        for (int j = 0; j < ny; ++j) {
+            const double ky = 2.0 * M_PI * ((j <= ny / 2) ? j : j - ny) / c->dom.Ly;
+            k2y[(size_t)j] = ky * ky;
+            const double th = 2.0 * M_PI * (double)j / ny;
+            sy[(size_t)j] = make_float2((float)((std::cos(th) - 1.0) / c->dev.dy), (float)(std::sin(th) / c->dev.dy));
+        }
+        PIC_HIP_TRY(dmalloc(&c->ptab.k2x, (size_t)hx));
+        PIC_HIP_TRY(dmalloc(&c->ptab.k2y, (size_t)ny));
+        PIC_HIP_TRY(dmalloc(&c->ptab.dx_sym, (size_t)hx));
+        PIC_HIP_TRY(dmalloc(&c->ptab.dy_sym, (size_t)ny));
+        PIC_HIP_TRY(hipMemcpy(c->ptab.k2x, k2x.data(), hx * sizeof(double), hipMemcpyHostToDevice));
+        PIC_HIP_TRY(hipMemcpy(c->ptab.k2y, k2y.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+        PIC_HIP_TRY(hipMemcpy(c->ptab.dx_sym, sx.data(), hx * sizeof(float2), hipMemcpyHostToDevice));
+        PIC_HIP_TRY(hipMemcpy(c->ptab.dy_sym, sy.data(), ny * sizeof(float2), hipMemcpyHostToDevice));
+    }
     PIC_FFT_TRY(hipfftPlan2d(&c->plan_r2c, c->dom.Ny, c->dom.Nx, HIPFFT_R2C));
     PIC_FFT_TRY(hipfftPlan2d(&c->plan_c2r, c->dom.Ny, c->dom.Nx, HIPFFT_C2R));
     PIC_FFT_TRY(hipfftSetStream(c->plan_r2c, c->stream));
@@ -226,6 +276,8 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(dmalloc(&c->overflow, (size_t)c->G));
     PIC_HIP_TRY(hipMemsetAsync(c->rho, 0, (size_t)c->G * sizeof(float), c->stream));
     PIC_HIP_TRY(hipMemsetAsync(c->overflow, 0, (size_t)c->G * sizeof(float), c->stream));
+    PIC_HIP_TRY(dmalloc(&c->comp, (size_t)c->G));
+    PIC_HIP_TRY(hipMemsetAsync(c->comp, 0, (size_t)c->G * sizeof(float), c->stream));
     PIC_HIP_TRY(dmalloc(&c->ovf_count, 1));
     PIC_HIP_TRY(dmalloc(&c->status, 1));
     PIC_HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
@@ -338,16 +390,33 @@ pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* p, si
     return PIC_OK;
 }
 
-pic_status pic_hip_sort(pic_hip_ctx* ctx) {
-    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+namespace {
+
+// Sort by cell and permute the six streams.  copy_back keeps the stream
+// pointers fixed (needed inside captured graphs, which bake pointers in).
+pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
     PIC_HIP_TRY(launch_sort_keys_perm(ctx->s[0], ctx->s[1], (int64_t)ctx->n, ctx->dev, ctx->sb, ctx->stream));
     for (int i = 0; i < kStreams; ++i) {
         launch_gather_f32(ctx->s[i], ctx->alt[i], ctx->sb.perm, (int64_t)ctx->n, ctx->stream);
-        std::swap(ctx->s[i], ctx->alt[i]);
+        if (copy_back)
+            launch_copy_f32(ctx->alt[i], ctx->s[i], (int64_t)ctx->n, ctx->stream);
+        else
+            std::swap(ctx->s[i], ctx->alt[i]);
     }
     PIC_HIP_TRY(hipGetLastError());
     ctx->order_valid = true;
     return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pic_status pic_hip_sort(pic_hip_ctx* ctx) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    return sort_particles(ctx, false);
 }
 
 pic_status pic_hip_sort_perm(pic_hip_ctx* ctx, int32_t* keys_sorted, int32_t* perm, int32_t* cell_offsets) {
@@ -402,6 +471,7 @@ pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt)
     a.rho = ctx->rho;
     a.partial = ctx->partial;
     a.overflow = ctx->overflow;
+    a.comp = ctx->comp;
     a.ovf_count = ctx->ovf_count;
     a.status = ctx->status;
     a.stream = ctx->stream;
@@ -485,9 +555,10 @@ pic_status pic_hip_poisson(pic_hip_ctx* ctx) {
                        ctx->diag_blocks, ctx->diag_out, ctx->stream);
     launch_poisson_prep(ctx->rho, ctx->fft_in, ctx->G, 0.0, cv, ctx->diag_out, ctx->stream);
     PIC_FFT_TRY(hipfftExecR2C(ctx->plan_r2c, ctx->fft_in, reinterpret_cast<hipfftComplex*>(ctx->spec)));
-    launch_poisson_scale(ctx->spec, ctx->dev, ctx->stream);
+    launch_poisson_spectral(ctx->spec, ctx->spec_ex, ctx->spec_ey, ctx->ptab, ctx->dev, ctx->stream);
     PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec), ctx->phi));
-    launch_efield(ctx->phi, ctx->Ex, ctx->Ey, ctx->dev, ctx->stream);
+    PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec_ex), ctx->Ex));
+    PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec_ey), ctx->Ey));
     PIC_HIP_TRY(hipGetLastError());
     return PIC_OK;
 }
@@ -541,6 +612,202 @@ pic_status pic_hip_diagnostics(pic_hip_ctx* ctx, pic_diagnostics* out) {
     out->charge_error =
         std::fabs(out->integrated_rho - out->total_charge) / std::max(out->total_abs_charge, 1e-12);
     return PIC_OK;
+}
+
+// ---- timestep driver ----------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+pic_status ensure_step_buffers(pic_hip_ctx* ctx) {
+    if (ctx->cosk) return PIC_OK;
+    const int nx = ctx->dom.Nx;
+    std::vector<double> c((size_t)nx), s((size_t)nx);
+    const double k = 2.0 * M_PI / ctx->dom.Lx;
+    for (int i = 0; i < nx; ++i) {
+        const double x = (static_cast<double>(i) + 0.5) * ctx->dev.dx;
+        c[(size_t)i] = std::cos(k * x);
+        s[(size_t)i] = std::sin(k * x);
+    }
+    PIC_HIP_TRY(dmalloc(&ctx->cosk, (size_t)nx));
+    PIC_HIP_TRY(dmalloc(&ctx->sink, (size_t)nx));
+    PIC_HIP_TRY(dmalloc(&ctx->step_counter, 1));
+    PIC_HIP_TRY(dmalloc(&ctx->step_partials, (size_t)7 * ctx->diag_blocks));
+    PIC_HIP_TRY(hipMemcpyAsync(ctx->cosk, c.data(), nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    PIC_HIP_TRY(hipMemcpyAsync(ctx->sink, s.data(), nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PIC_OK;
+}
+
+// Enqueues one step (see pic_hip_run).
+pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, bool copy_back,
+                        int record_from, int cap, const StepRecords& rec) {
+    pic_status st = PIC_OK;
+    const float dt = (float)cfg.dt;
+    auto push = [&]() {
+        launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
+                           ctx->Ey, ctx->dev, dt, 0, 1, ctx->stream);
+    };
+    auto deposit = [&]() -> pic_status {
+        if (with_sort) {
+            pic_status s2 = sort_particles(ctx, copy_back);
+            if (s2 != PIC_OK) return s2;
+        }
+        return pic_hip_deposit(ctx, cfg.scheme, cfg.backend, cfg.dt);
+    };
+    auto solve = [&]() -> pic_status {
+        if (cfg.background != 0.0) launch_add_scalar(ctx->rho, ctx->G, (float)cfg.background, ctx->stream);
+        return pic_hip_poisson(ctx);
+    };
+    if (cfg.order == PIC_ORDER_PUSH_FIRST) {
+        push();
+        if ((st = deposit()) != PIC_OK) return st;
+        if ((st = solve()) != PIC_OK) return st;
+    } else if (cfg.order == PIC_ORDER_DEPOSIT_SOLVE_PUSH) {
+        if ((st = deposit()) != PIC_OK) return st;
+        if ((st = solve()) != PIC_OK) return st;
+        push();
+    } else {
+        if ((st = deposit()) != PIC_OK) return st;
+        push();
+        if ((st = solve()) != PIC_OK) return st;
+    }
+    launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
+                       ctx->Ey, ctx->dev, dt, 1, 0, ctx->stream);
+    const bool parts = cfg.record_particles != 0;
+    launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3], ctx->s[4],
+                     ctx->s[5], parts ? (int64_t)ctx->n : 0, ctx->step_partials, ctx->diag_blocks, ctx->step_counter,
+                     record_from, cap, rec, ctx->stream);
+    PIC_HIP_TRY(hipGetLastError());
+    return PIC_OK;
+}
+
+struct GraphPair {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    ~GraphPair() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+pic_status capture_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, int record_from, int cap,
+                        const StepRecords& rec, GraphPair& g) {
+    PIC_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    pic_status st = enqueue_step(ctx, cfg, with_sort, true, record_from, cap, rec);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != PIC_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    PIC_HIP_TRY(e);
+    g.graph = graph;
+    PIC_HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nsteps, int32_t record_from,
+                       double* field_energy, double* mode1_amplitude, double* kinetic_energy, double* charge_error) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!cfg) return fail(PIC_ERR_INVALID_ARGUMENT, "null step config");
+    if (nsteps < 0 || record_from < 0) return fail(PIC_ERR_INVALID_ARGUMENT, "negative step counts");
+    if (cfg->order < 0 || cfg->order > 2) return fail(PIC_ERR_INVALID_ARGUMENT, "unknown step order");
+    if (cfg->backend == PIC_BACKEND_GPU_PRIV && cfg->sort_interval < 1)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "GPU_Priv steps need sort_interval >= 1");
+    if (cfg->scheme < 0 || cfg->scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "unknown scheme");
+    if (cfg->backend != PIC_BACKEND_GPU_ATOMICS && cfg->backend != PIC_BACKEND_GPU_PRIV)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "unknown backend");
+    if (!std::isfinite(cfg->dt)) return fail(PIC_ERR_INVALID_ARGUMENT, "dt must be finite");
+    pic_status st = ensure_step_buffers(ctx);
+    if (st != PIC_OK) return st;
+    if ((st = ensure_fft(ctx)) != PIC_OK) return st;
+    const int cap = nsteps > record_from ? nsteps - record_from : 0;
+    double* dev_rec = nullptr;
+    PIC_HIP_TRY(dmalloc(&dev_rec, (size_t)4 * std::max(cap, 1)));
+    std::unique_ptr<double, hipError_t (*)(void*)> rec_guard(dev_rec, hipFree);
+    StepRecords rec{field_energy ? dev_rec : nullptr, mode1_amplitude ? dev_rec + cap : nullptr,
+                    kinetic_energy ? dev_rec + 2 * cap : nullptr, charge_error ? dev_rec + 3 * cap : nullptr};
+    launch_set_int(ctx->step_counter, 0, ctx->stream);
+    const bool priv = cfg->backend == PIC_BACKEND_GPU_PRIV;
+    if (cfg->use_graph && nsteps > 0) {
+        GraphPair g_sort, g_plain;
+        if ((st = capture_step(ctx, *cfg, priv, record_from, cap, rec, g_sort)) != PIC_OK) return st;
+        if (priv && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, g_plain)) != PIC_OK) return st;
+        for (int step = 0; step < nsteps; ++step) {
+            const bool sort_now = priv && step % cfg->sort_interval == 0;
+            PIC_HIP_TRY(hipGraphLaunch(sort_now || !priv ? g_sort.exec : g_plain.exec, ctx->stream));
+        }
+        PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    } else {
+        for (int step = 0; step < nsteps; ++step) {
+            const bool sort_now = priv && step % cfg->sort_interval == 0;
+            if ((st = enqueue_step(ctx, *cfg, sort_now, false, record_from, cap, rec)) != PIC_OK) return st;
+        }
+    }
+    double* outs[4] = {field_energy, mode1_amplitude, kinetic_energy, charge_error};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k] && cap)
+            PIC_HIP_TRY(hipMemcpyAsync(outs[k], dev_rec + (size_t)k * cap, cap * sizeof(double),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+    return pic_hip_sync(ctx);
+}
+
+double pic_hip_dominant_frequency(const double* signal, int64_t n, double dt) {
+    if (!signal || n < 8) return 0.0;
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; ++i) mean += signal[i];
+    mean /= static_cast<double>(n);
+    const int nn = static_cast<int>(n);
+    double best_power = 0.0;
+    int best_k = 1;
+    for (int k = 1; k < nn / 2; ++k) {
+        double re = 0.0, im = 0.0;
+        for (int t = 0; t < nn; ++t) {
+            const double phase = 2.0 * M_PI * static_cast<double>(k * t) / static_cast<double>(nn);
+            const double c = signal[t] - mean;
+            re += c * std::cos(phase);
+            im -= c * std::sin(phase);
+        }
+        const double power = re * re + im * im;
+        if (power > best_power) {
+            best_power = power;
+            best_k = k;
+        }
+    }
+    return static_cast<double>(best_k) / (static_cast<double>(nn) * dt);
+}
+
+double pic_hip_growth_rate(const double* signal, int64_t n, double dt, int32_t record_start_step) {
+    if (!signal || n < 8) return 0.0;
+    const int64_t start = n / 4, end = (3 * n) / 4;
+    if (end <= start + 2) return 0.0;
+    double st = 0.0, sl = 0.0, stt = 0.0, stl = 0.0;
+    int count = 0;
+    for (int64_t i = start; i < end; ++i) {
+        const double t = static_cast<double>(record_start_step + static_cast<int>(i)) * dt;
+        const double l = std::log(std::max(signal[i], 1e-30));
+        st += t;
+        sl += l;
+        stt += t * t;
+        stl += t * l;
+        ++count;
+    }
+    const double denom = static_cast<double>(count) * stt - st * st;
+    if (std::abs(denom) < 1e-30) return 0.0;
+    return (static_cast<double>(count) * stl - st * sl) / denom;
+}
+
+double pic_hip_plasma_frequency(const pic_domain* domain, int64_t num_particles) {
+    if (!domain || num_particles <= 0) return 0.0;
+    const double n0 = static_cast<double>(num_particles) / (domain->Lx * domain->Ly);
+    const double q = 1.0 / static_cast<double>(num_particles);
+    return std::sqrt(n0 * q * q);
 }
 
 // ---- drop-in seam -----------------------------------------------------------

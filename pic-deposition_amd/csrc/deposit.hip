@@ -92,31 +92,33 @@ __device__ __forceinline__ void emit_cell(float* __restrict__ ws, float* __restr
 }
 
 // General reference-Esirkepov walk (esirkepovWeights1DSparse,
-// src/deposition/ngp.cpp:36-84) for displacements of any length, in cell units.
-// Returns the number of support cells (<= kEskMaxCells) or -1 if longer.
+// src/deposition/ngp.cpp:36-84) for moves of any length, measured in cells
+// from the home cell's lower face.  Returns the number of support cells
+// (<= kEskMaxCells) or -1 if the move is longer.
 constexpr int kEskMaxCells = 16;
-__device__ int esk_walk_1d(float x0, float v, float dt, float inv_dx, int* cell, float* w) {
+__device__ __noinline__ int esk_walk_1d(float x0, float v, float dt, float inv_d, double dd, int exact, int* cell,
+                                        float* w) {
+    int c0;
+    float fr0;
+    cell_coord(x0, inv_d, dd, exact, c0, fr0);
     const float disp = v * dt;
-    const float f0 = x0 * inv_dx;
-    const int c0 = (int)floorf(f0);
     if (fabsf(disp) < 1e-15f) {
-        const float xi = f0 - (float)c0;
         cell[0] = c0;
-        w[0] = 1.0f - xi;
+        w[0] = 1.0f - fr0;
         cell[1] = c0 + 1;
-        w[1] = xi;
+        w[1] = fr0;
         return 2;
     }
-    const float f1 = (x0 + disp) * inv_dx;
+    const float end = fr0 + cells(disp, inv_d, dd, exact);  // relative to the home cell's lower face
     int n = 0;
     float total = 0.0f;
-    float cur = f0;
-    int c = c0;
+    float cur = fr0;
+    int c = 0;
     if (disp > 0.0f) {
-        while (cur < f1) {
+        while (cur < end) {
             if (n == kEskMaxCells) return -1;
-            const float nxt = fminf(f1, (float)(c + 1));
-            cell[n] = c;
+            const float nxt = fminf(end, (float)(c + 1));
+            cell[n] = c0 + c;
             w[n] = nxt - cur;
             total += w[n];
             ++n;
@@ -124,10 +126,10 @@ __device__ int esk_walk_1d(float x0, float v, float dt, float inv_dx, int* cell,
             ++c;
         }
     } else {
-        while (cur > f1) {
+        while (cur > end) {
             if (n == kEskMaxCells) return -1;
-            const float nxt = fmaxf(f1, (float)c);
-            cell[n] = c;
+            const float nxt = fmaxf(end, (float)c);
+            cell[n] = c0 + c;
             w[n] = cur - nxt;
             total += w[n];
             ++n;
@@ -147,11 +149,40 @@ __device__ int esk_walk_1d(float x0, float v, float dt, float inv_dx, int* cell,
     return n;
 }
 
-// Scatter one particle with global FP32 atomics (used by GPU_Atomics and by the
-// tile kernel's overflow path).
+// Compensated global add: grid + comp == exact running sum (TwoSum of each
+// atomic's old value and the addend).
+__device__ __forceinline__ void comp_atomic_add(float* __restrict__ grid, float* __restrict__ comp, size_t i, float v) {
+    const float old = unsafeAtomicAdd(grid + i, v);
+    unsafeAtomicAdd(comp + i, two_sum_err(old, v));
+}
+
+// Rare path of scatter_global: Esirkepov moves longer than one cell.
+__device__ __noinline__ void scatter_esk_long(float x, float y, float vx, float vy, float qv, float dt,
+                                              const DevDomain& d, float* __restrict__ grid,
+                                              float* __restrict__ comp, int* __restrict__ status) {
+    int cxs[kEskMaxCells], cys[kEskMaxCells];
+    float wxs[kEskMaxCells], wys[kEskMaxCells];
+    const int nx = esk_walk_1d(x, vx, dt, d.inv_dx, d.dx, d.exact, cxs, wxs);
+    const int ny = esk_walk_1d(y, vy, dt, d.inv_dy, d.dy, d.exact, cys, wys);
+    if (nx < 0 || ny < 0) {
+        atomicOr(status, 1);
+        return;
+    }
+    for (int i = 0; i < nx; ++i) {
+        const int gx = wrap_index(cxs[i], d.Nx);
+        for (int j = 0; j < ny; ++j) {
+            const int gy = wrap_index(cys[j], d.Ny);
+            comp_atomic_add(grid, comp, (size_t)gy * d.Nx + gx, qv * wxs[i] * wys[j]);
+        }
+    }
+}
+
+// Scatter one particle with compensated global FP32 atomics (used by
+// GPU_Atomics and by the tile kernel's overflow path).
 template <int S>
 __device__ __forceinline__ void scatter_global(float x, float y, float vx, float vy, float qv, float dt,
-                                               const DevDomain& d, float* __restrict__ grid, int* __restrict__ status) {
+                                               const DevDomain& d, float* __restrict__ grid,
+                                               float* __restrict__ comp, int* __restrict__ status) {
     using St = Stencil<S>;
     float w[St::V];
     int cx, cy;
@@ -162,27 +193,13 @@ __device__ __forceinline__ void scatter_global(float x, float y, float vx, float
             if (w[k] != 0.0f) {
                 const int gx = wrap_index(cx + St::OX + k % St::SX, d.Nx);
                 const int gy = wrap_index(cy + St::OY + k / St::SX, d.Ny);
-                unsafeAtomicAdd(grid + (size_t)gy * d.Nx + gx, w[k]);
+                comp_atomic_add(grid, comp, (size_t)gy * d.Nx + gx, w[k]);
             }
         }
         return;
     }
     // Only the Esirkepov density can fail the fixed stencil (long moves).
-    int cxs[kEskMaxCells], cys[kEskMaxCells];
-    float wxs[kEskMaxCells], wys[kEskMaxCells];
-    const int nx = esk_walk_1d(x, vx, dt, d.inv_dx, cxs, wxs);
-    const int ny = esk_walk_1d(y, vy, dt, d.inv_dy, cys, wys);
-    if (nx < 0 || ny < 0) {
-        atomicOr(status, 1);
-        return;
-    }
-    for (int i = 0; i < nx; ++i) {
-        const int gx = wrap_index(cxs[i], d.Nx);
-        for (int j = 0; j < ny; ++j) {
-            const int gy = wrap_index(cys[j], d.Ny);
-            unsafeAtomicAdd(grid + (size_t)gy * d.Nx + gx, qv * wxs[i] * wys[j]);
-        }
-    }
+    scatter_esk_long(x, y, vx, vy, qv, dt, d, grid, comp, status);
 }
 
 template <int S>
@@ -191,12 +208,21 @@ __global__ __launch_bounds__(256) void deposit_atomic_kernel(const float* __rest
                                                              const float* __restrict__ pvy,
                                                              const float* __restrict__ pq, int64_t n, DevDomain d,
                                                              float dt, float* __restrict__ rho,
-                                                             int* __restrict__ status) {
+                                                             float* __restrict__ comp, int* __restrict__ status) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float vx = Stencil<S>::kNeedsVelocity ? pvx[i] : 0.0f;
     const float vy = Stencil<S>::kNeedsVelocity ? pvy[i] : 0.0f;
-    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * d.inv_vol, dt, d, rho, status);
+    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * d.inv_vol, dt, d, rho, comp, status);
+}
+
+// rho += comp; comp = 0 (closes the compensated atomic sums).
+__global__ __launch_bounds__(256) void fold_comp_kernel(float* __restrict__ rho, float* __restrict__ comp, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        rho[i] += comp[i];
+        comp[i] = 0.0f;
+    }
 }
 
 template <int S, int K>
@@ -206,8 +232,8 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                                                            const float* __restrict__ pq,
                                                            const int* __restrict__ cell_off, DevDomain d, TileGeom g,
                                                            float dt, float* __restrict__ partial,
-                                                           float* __restrict__ overflow, int* __restrict__ ovf_count,
-                                                           int* __restrict__ status) {
+                                                           float* __restrict__ overflow, float* __restrict__ comp,
+                                                           int* __restrict__ ovf_count, int* __restrict__ status) {
     using St = Stencil<S>;
     constexpr int V = St::V;
     constexpr int Q = K / 4;
@@ -317,7 +343,7 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                     const int lx = wrap_index(cx, d.Nx) - x0;
                     const int ly = wrap_index(cy, d.Ny) - y0;
                     if (!ok || (unsigned)lx >= (unsigned)tw || (unsigned)ly >= (unsigned)th) {
-                        scatter_global<S>(xx, yy, vxx, vyy, qv, dt, d, overflow, status);
+                        scatter_global<S>(xx, yy, vxx, vyy, qv, dt, d, overflow, comp, status);
                         atomicAdd(ovf_count, 1);
                     } else {
                         const int lk = (ly << 12) | lx;
@@ -392,7 +418,7 @@ __device__ __forceinline__ int window_candidates(int gnode, int T, int nt, int n
 }
 
 __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
-                                                            float* __restrict__ overflow,
+                                                            float* __restrict__ overflow, float* __restrict__ comp,
                                                             const int* __restrict__ ovf_count, DevDomain d,
                                                             TileGeom g, float* __restrict__ rho) {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,8 +457,9 @@ __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restr
     float out = s + c;
     const size_t node = (size_t)gy * d.Nx + gx;
     if (*ovf_count > 0) {
-        out += overflow[node];
+        out += overflow[node] + comp[node];
         overflow[node] = 0.0f;
+        comp[node] = 0.0f;
     }
     rho[node] = out;
 }
@@ -450,8 +477,10 @@ void launch_atomic(const DepositArgs& a) {
     if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
     if (a.n > 0)
         hipLaunchKernelGGL(deposit_atomic_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, a.stream, a.x, a.y, a.vx,
-                           a.vy, a.q, a.n, a.dom, a.dt, a.rho, a.status);
+                           a.vy, a.q, a.n, a.dom, a.dt, a.rho, a.comp, a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
+    const size_t G = (size_t)a.dom.Nx * a.dom.Ny;
+    hipLaunchKernelGGL(fold_comp_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.comp, G);
 }
 
 template <int S, int K>
@@ -463,10 +492,11 @@ void launch_tile(const DepositArgs& a) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, a.stream, a.ovf_count);
     if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
     hipLaunchKernelGGL((deposit_tile_kernel<S, K>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream, a.x,
-                       a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.overflow, a.ovf_count, a.status);
+                       a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.overflow, a.comp, a.ovf_count,
+                       a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
     hipLaunchKernelGGL(deposit_merge_kernel, dim3((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny), dim3(256), 0,
-                       a.stream, a.partial, a.overflow, a.ovf_count, a.dom, g, a.rho);
+                       a.stream, a.partial, a.overflow, a.comp, a.ovf_count, a.dom, g, a.rho);
 }
 
 template <int S>

@@ -18,7 +18,30 @@ struct DevDomain {
     double Lx, Ly;
     float inv_dx, inv_dy;  // FP32 reciprocals for shape-function weights
     float inv_vol;         // 1 / (dx*dy) rounded to FP32
+    int exact;             // 1: dx and dy are powers of two, so x * inv_dx is exact in FP32
 };
+
+// Cell index (unwrapped floor) and in-cell fraction of a coordinate.  With a
+// power-of-two cell size the FP32 product is exact; otherwise x / dx is formed
+// in FP64 (an FP32 product would carry an error of ~Nx ulps in the fraction).
+__device__ __forceinline__ void cell_coord(float x, float inv_d, double d, int exact, int& c, float& f) {
+    if (exact) {
+        const float fx = x * inv_d;
+        const float fl = floorf(fx);
+        c = (int)fl;
+        f = fx - fl;
+    } else {
+        const double fx = (double)x / d;
+        const double fl = floor(fx);
+        c = (int)fl;
+        f = (float)(fx - fl);
+    }
+}
+
+// A displacement in cell units.
+__device__ __forceinline__ float cells(float disp, float inv_d, double d, int exact) {
+    return exact ? disp * inv_d : (float)((double)disp / d);
+}
 
 // Reference wrapIndex (src/deposition/ngp.cpp:9): ((i % n) + n) % n, with the
 // common in-range case taken without integer division.
@@ -64,11 +87,10 @@ struct Stencil<NGP> {
     static constexpr bool kNeedsVelocity = false;
     __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
                                                    int& cx, int& cy, float (&w)[V]) {
-        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
-        const float flx = floorf(fx), fly = floorf(fy);
-        cx = (int)flx;
-        cy = (int)fly;
-        const bool rx = (fx - flx) >= 0.5f, ry = (fy - fly) >= 0.5f;
+        float fx, fy;
+        cell_coord(x, d.inv_dx, d.dx, d.exact, cx, fx);
+        cell_coord(y, d.inv_dy, d.dy, d.exact, cy, fy);
+        const bool rx = fx >= 0.5f, ry = fy >= 0.5f;
         w[0] = (!rx && !ry) ? qv : 0.0f;
         w[1] = (rx && !ry) ? qv : 0.0f;
         w[2] = (!rx && ry) ? qv : 0.0f;
@@ -84,11 +106,9 @@ struct Stencil<CIC> {
     static constexpr bool kNeedsVelocity = false;
     __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
                                                    int& cx, int& cy, float (&w)[V]) {
-        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
-        const float flx = floorf(fx), fly = floorf(fy);
-        cx = (int)flx;
-        cy = (int)fly;
-        const float wx = fx - flx, wy = fy - fly;
+        float wx, wy;
+        cell_coord(x, d.inv_dx, d.dx, d.exact, cx, wx);
+        cell_coord(y, d.inv_dy, d.dy, d.exact, cy, wy);
         const float ux = 1.0f - wx, uy = 1.0f - wy;
         w[0] = qv * ux * uy;
         w[1] = qv * wx * uy;
@@ -125,13 +145,12 @@ struct Stencil<TSC> {
     static constexpr bool kNeedsVelocity = false;
     __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
                                                    int& cx, int& cy, float (&w)[V]) {
-        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
-        const float flx = floorf(fx), fly = floorf(fy);
-        cx = (int)flx;
-        cy = (int)fly;
+        float fx, fy;
+        cell_coord(x, d.inv_dx, d.dx, d.exact, cx, fx);
+        cell_coord(y, d.inv_dy, d.dy, d.exact, cy, fy);
         float wx[4], wy[4];
-        tsc_1d(fx - flx, wx);
-        tsc_1d(fy - fly, wy);
+        tsc_1d(fx, wx);
+        tsc_1d(fy, wy);
 #pragma unroll
         for (int a = 0; a < 4; ++a) wx[a] *= qv;
 #pragma unroll
@@ -145,45 +164,46 @@ struct Stencil<TSC> {
 // Reference "Esirkepov" density (esirkepovWeights1DSparse, src/deposition/ngp.cpp:36-84):
 // per axis, path-length fractions of the straight move x0 -> x0 + v*dt over
 // the floor cells it crosses, renormalised to sum 1; a static particle
-// (|dx| < 1e-15) gets the CIC pair.  For |v*dt| < 1 cell the support is
-// within offsets -1..1 of the home cell.  Returns false when a move crosses
-// more than one cell boundary (handled by the general per-particle path).
-__device__ __forceinline__ bool esk_1d(float x0, float v, float dt, float inv_dx, int& c0, float (&w)[3]) {
+// (|v*dt| < 1e-15) gets the CIC pair.  Everything is measured from the home
+// cell's faces (fr0 = in-cell fraction, dc = displacement in cells), so no
+// FP32 rounding of x0 + v*dt enters the fractions.  For moves shorter than a
+// cell the support is within offsets -1..1 of the home cell.  Returns false
+// when a move crosses more than one face (handled by the general walk).
+__device__ __forceinline__ bool esk_1d(float x0, float v, float dt, float inv_d, double dd, int exact, int& c0,
+                                       float (&w)[3]) {
+    float fr0;
+    cell_coord(x0, inv_d, dd, exact, c0, fr0);
     const float disp = v * dt;
-    const float x1 = x0 + disp;
-    const float f0 = x0 * inv_dx;
-    const float fl = floorf(f0);
-    c0 = (int)fl;
     w[0] = w[1] = w[2] = 0.0f;
-    // The reference tests |x1 - x0| < 1e-15 in FP64; in FP32 x1 == x0 already
-    // for displacements far above 1e-15, so the displacement itself is tested.
+    // The reference tests |x1 - x0| < 1e-15 in FP64; the displacement itself
+    // is the same quantity without the cancellation.
     if (fabsf(disp) < 1e-15f) {
-        const float xi = f0 - fl;
-        w[1] = 1.0f - xi;
-        w[2] = xi;
+        w[1] = 1.0f - fr0;
+        w[2] = fr0;
         return true;
     }
-    const float f1 = x1 * inv_dx;
-    if (disp > 0.0f) {
-        const float b = fl + 1.0f;  // upper face of the home cell, in cell units
-        if (f1 <= b) {
+    const float dc = cells(disp, inv_d, dd, exact);
+    if (dc > 0.0f) {
+        const float wa = 1.0f - fr0;  // path inside the home cell up to its upper face
+        if (dc <= wa) {
             w[1] = 1.0f;
             return true;
         }
-        if (f1 > b + 1.0f) return false;
-        const float wa = b - f0, wb = f1 - b;
+        const float wb = dc - wa;
+        if (wb > 1.0f) return false;
         const float inv = 1.0f / (wa + wb);
         w[1] = wa * inv;
         w[2] = wb * inv;
         return true;
     }
-    // Moving left: the home cell's lower face is at fl.
-    if (f1 >= fl) {
+    const float adc = -dc;
+    const float wa = fr0;  // path inside the home cell down to its lower face
+    if (adc <= wa) {
         w[1] = 1.0f;
         return true;
     }
-    if (f1 < fl - 1.0f) return false;
-    const float wa = f0 - fl, wb = fl - f1;
+    const float wb = adc - wa;
+    if (wb > 1.0f) return false;
     const float inv = 1.0f / (wa + wb);
     w[1] = wa * inv;
     w[0] = wb * inv;
@@ -197,8 +217,8 @@ struct Stencil<ESK> {
     __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
                                                    const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
         float wx[3], wy[3];
-        const bool okx = esk_1d(x, vx, dt, d.inv_dx, cx, wx);
-        const bool oky = esk_1d(y, vy, dt, d.inv_dy, cy, wy);
+        const bool okx = esk_1d(x, vx, dt, d.inv_dx, d.dx, d.exact, cx, wx);
+        const bool oky = esk_1d(y, vy, dt, d.inv_dy, d.dy, d.exact, cy, wy);
 #pragma unroll
         for (int a = 0; a < 3; ++a) wx[a] *= qv;
 #pragma unroll

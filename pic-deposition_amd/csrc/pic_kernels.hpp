@@ -35,6 +35,7 @@ struct DepositArgs {
     float* rho;       // [Ny][Nx] FP32 out
     float* partial;   // ntiles * wsize
     float* overflow;  // [Ny][Nx], kept zero between calls
+    float* comp;      // [Ny][Nx] compensation of the global atomic sums, kept zero between calls
     int* ovf_count;
     int* status;      // bit 0: Esirkepov move longer than the general walk supports
     hipStream_t stream;
@@ -100,10 +101,36 @@ void launch_init(float* x, float* y, float* vx, float* vy, float* q, float* m, i
 // in[i] = (rho[i] + bg) * cv - mean, mean from the FP64 charge sum.
 void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, double cv, const double* rho_sum,
                          hipStream_t s);
-// Divide the half spectrum [Ny][Nx/2+1] by |k|^2 and by Nx*Ny; zero k = 0.
-void launch_poisson_scale(float2* spec, const DevDomain& d, hipStream_t s);
+struct PoissonTables {
+    double* k2x;     // [Nx/2+1] kx^2
+    double* k2y;     // [Ny]     ky^2
+    float2* dx_sym;  // [Nx/2+1] (e^{2 pi i i/Nx} - 1) / dx: forward difference along x
+    float2* dy_sym;  // [Ny]     (e^{2 pi i j/Ny} - 1) / dy
+};
+// On the half spectrum [Ny][Nx/2+1]: phi_hat = rho_hat / (|k|^2 Nx Ny) with
+// k = 0 zeroed; Ex_hat = -Dx phi_hat, Ey_hat = -Dy phi_hat (the spectral form
+// of E = -forward difference of phi, identical in exact arithmetic and free of
+// the cancellation of differencing an FP32 phi).
+void launch_poisson_spectral(float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
+                             const DevDomain& d, hipStream_t s);
 // Ex = -(phi(i+1,j) - phi(i,j))/dx, Ey = -(phi(i,j+1) - phi(i,j))/dy.
 void launch_efield(const float* phi, float* Ex, float* Ey, const DevDomain& d, hipStream_t s);
 void launch_add_scalar(float* p, int64_t n, float v, hipStream_t s);
+
+// Per-step FP64 records (any pointer may be null).
+struct StepRecords {
+    double* field_energy;  // 0.5 * sum(Ex^2+Ey^2) * cellVolume
+    double* mode1;         // |mode-1 Fourier amplitude of Ex| (extractFourierModeAmplitude)
+    double* kinetic;       // sum 0.5 m v^2
+    double* charge_error;  // |integrated rho - sum q| / sum |q|
+};
+// Sums over the grid (and particles when n > 0), appends one record at
+// *counter - record_from (if inside [0, cap)) and increments *counter.
+void launch_step_diag(const float* rho, const float* Ex, const float* Ey, const double* cosk, const double* sink,
+                      const DevDomain& d, const float* vx, const float* vy, const float* q, const float* m, int64_t n,
+                      double* partials, int nblocks, int* counter, int record_from, int cap, const StepRecords& rec,
+                      hipStream_t s);
+void launch_set_int(int* p, int v, hipStream_t s);
+void launch_copy_f32(const float* src, float* dst, int64_t n, hipStream_t s);
 
 }  // namespace pichip

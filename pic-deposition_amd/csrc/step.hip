@@ -24,10 +24,11 @@ __global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px
     float x = px[i], y = py[i];
     float vx = pvx[i], vy = pvy[i];
     if (do_gather) {
-        const float fx = x * d.inv_dx, fy = y * d.inv_dy;
-        const float flx = floorf(fx), fly = floorf(fy);
-        const float wx = fx - flx, wy = fy - fly;
-        const int ix = wrap_index((int)flx, d.Nx), iy = wrap_index((int)fly, d.Ny);
+        int cx, cy;
+        float wx, wy;
+        cell_coord(x, d.inv_dx, d.dx, d.exact, cx, wx);
+        cell_coord(y, d.inv_dy, d.dy, d.exact, cy, wy);
+        const int ix = wrap_index(cx, d.Nx), iy = wrap_index(cy, d.Ny);
         const int ixp = ix + 1 == d.Nx ? 0 : ix + 1;
         const int iyp = iy + 1 == d.Ny ? 0 : iy + 1;
         const size_t r0 = (size_t)iy * d.Nx, r1 = (size_t)iyp * d.Nx;
@@ -212,23 +213,25 @@ __global__ __launch_bounds__(256) void poisson_prep_kernel(const float* __restri
     in[i] = (float)(((double)rho[i] + bg) * cv - mean);
 }
 
-// k^2 table of poisson_fft.cpp:23-35 evaluated on the fly; the 1/(Nx*Ny) of
-// poisson_fft.cpp:66 is folded in.
-__global__ __launch_bounds__(256) void poisson_scale_kernel(float2* __restrict__ spec, DevDomain d) {
+// phi_hat = rho_hat / (k^2 Nx Ny) (poisson_fft.cpp:59-66, k^2 table :23-35),
+// then Ex_hat, Ey_hat from the forward-difference symbols.
+__global__ __launch_bounds__(256) void poisson_spectral_kernel(float2* __restrict__ spec, float2* __restrict__ sex,
+                                                               float2* __restrict__ sey, PoissonTables t, DevDomain d) {
     const int hx = d.Nx / 2 + 1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= hx) return;
     const size_t id = (size_t)j * hx + i;
-    if (i == 0 && j == 0) {
-        spec[id] = make_float2(0.0f, 0.0f);
-        return;
+    float2 p = make_float2(0.0f, 0.0f);
+    if (i != 0 || j != 0) {
+        const float f = (float)(1.0 / ((t.k2x[i] + t.k2y[j]) * (double)d.Nx * (double)d.Ny));
+        const float2 v = spec[id];
+        p = make_float2(v.x * f, v.y * f);
     }
-    const double kx = 2.0 * M_PI * (double)((i <= d.Nx / 2) ? i : i - d.Nx) / d.Lx;
-    const double ky = 2.0 * M_PI * (double)((j <= d.Ny / 2) ? j : j - d.Ny) / d.Ly;
-    const float f = (float)(1.0 / ((kx * kx + ky * ky) * (double)d.Nx * (double)d.Ny));
-    const float2 v = spec[id];
-    spec[id] = make_float2(v.x * f, v.y * f);
+    spec[id] = p;
+    const float2 a = t.dx_sym[i], b = t.dy_sym[j];
+    sex[id] = make_float2(-(a.x * p.x - a.y * p.y), -(a.x * p.y + a.y * p.x));
+    sey[id] = make_float2(-(b.x * p.x - b.y * p.y), -(b.x * p.y + b.y * p.x));
 }
 
 __global__ __launch_bounds__(256) void efield_kernel(const float* __restrict__ phi, float* __restrict__ Ex,
@@ -249,6 +252,94 @@ __global__ __launch_bounds__(256) void add_scalar_kernel(float* __restrict__ p, 
     if (i < n) p[i] += v;
 }
 
+
+// ---- per-step diagnostics (recorded into device arrays) ---------------------
+// Partial sums per block: 0 sum rho, 1 sum(Ex^2+Ey^2), 2 sum Ex cos(k x_i),
+// 3 sum Ex sin(k x_i) (x_i = (i + 1/2) dx, extractFourierModeAmplitude,
+// validation.cpp:99-119), 4 sum 0.5 m v^2, 5 sum q, 6 sum |q|.
+constexpr int kStepSums = 7;
+
+__global__ __launch_bounds__(256) void step_diag_partial_kernel(const float* __restrict__ rho,
+                                                                const float* __restrict__ Ex,
+                                                                const float* __restrict__ Ey,
+                                                                const double* __restrict__ cosk,
+                                                                const double* __restrict__ sink, int Nx, int64_t G,
+                                                                const float* __restrict__ vx,
+                                                                const float* __restrict__ vy,
+                                                                const float* __restrict__ q,
+                                                                const float* __restrict__ m, int64_t n,
+                                                                double* __restrict__ partials) {
+    double s[kStepSums] = {0, 0, 0, 0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += stride) {
+        const double ex = Ex[i], ey = Ey[i];
+        s[0] += (double)rho[i];
+        s[1] += ex * ex + ey * ey;
+        const int ix = (int)(i % Nx);
+        s[2] += ex * cosk[ix];
+        s[3] += ex * sink[ix];
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double a = vx[i], b = vy[i], qq = q[i];
+        s[4] += 0.5 * (double)m[i] * (a * a + b * b);
+        s[5] += qq;
+        s[6] += fabs(qq);
+    }
+    __shared__ double red[kStepSums][256];
+#pragma unroll
+    for (int k = 0; k < kStepSums; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+#pragma unroll
+            for (int k = 0; k < kStepSums; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < kStepSums) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+}
+
+// Adds the block partials in index order and appends one record at the
+// device-side step counter; then advances the counter.
+__global__ __launch_bounds__(256) void step_diag_final_kernel(const double* __restrict__ partials, int nblocks,
+                                                              double cv, int64_t G, int* __restrict__ counter,
+                                                              int record_from, int cap, StepRecords rec) {
+    __shared__ double red[256];
+    __shared__ double tot[kStepSums];
+    for (int k = 0; k < kStepSums; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(size_t)k * nblocks + b];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tot[k] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int step = *counter;
+        const int slot = step - record_from;
+        if (slot >= 0 && slot < cap) {
+            const double re = tot[2] / (double)G, im = tot[3] / (double)G;
+            const double irho = tot[0] * cv;
+            if (rec.field_energy) rec.field_energy[slot] = 0.5 * tot[1] * cv;
+            if (rec.mode1) rec.mode1[slot] = sqrt(re * re + im * im);
+            if (rec.kinetic) rec.kinetic[slot] = tot[4];
+            if (rec.charge_error) rec.charge_error[slot] = fabs(irho - tot[5]) / fmax(tot[6], 1e-12);
+        }
+        *counter = step + 1;
+    }
+}
+
+__global__ void set_int_kernel(int* __restrict__ p, int v) { *p = v; }
+
+__global__ __launch_bounds__(256) void copy_f32_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                       int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
 
 void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, double cv, const double* rho_sum,
@@ -257,10 +348,11 @@ void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, doub
                        rho_sum);
 }
 
-void launch_poisson_scale(float2* spec, const DevDomain& d, hipStream_t s) {
+void launch_poisson_spectral(float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
+                             const DevDomain& d, hipStream_t s) {
     const int hx = d.Nx / 2 + 1;
-    hipLaunchKernelGGL(poisson_scale_kernel, dim3((unsigned)((hx + 255) / 256), (unsigned)d.Ny), dim3(256), 0, s, spec,
-                       d);
+    hipLaunchKernelGGL(poisson_spectral_kernel, dim3((unsigned)((hx + 255) / 256), (unsigned)d.Ny), dim3(256), 0, s,
+                       spec, spec_ex, spec_ey, t, d);
 }
 
 void launch_efield(const float* phi, float* Ex, float* Ey, const DevDomain& d, hipStream_t s) {
@@ -294,6 +386,24 @@ void launch_init(float* x, float* y, float* vx, float* vy, float* q, float* m, i
     if (n > 0)
         hipLaunchKernelGGL(init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, vx, vy, q, m, n, d,
                            p);
+}
+
+void launch_step_diag(const float* rho, const float* Ex, const float* Ey, const double* cosk, const double* sink,
+                      const DevDomain& d, const float* vx, const float* vy, const float* q, const float* m, int64_t n,
+                      double* partials, int nblocks, int* counter, int record_from, int cap, const StepRecords& rec,
+                      hipStream_t s) {
+    const int64_t G = (int64_t)d.Nx * d.Ny;
+    hipLaunchKernelGGL(step_diag_partial_kernel, dim3(nblocks), dim3(256), 0, s, rho, Ex, Ey, cosk, sink, d.Nx, G, vx,
+                       vy, q, m, n, partials);
+    hipLaunchKernelGGL(step_diag_final_kernel, dim3(1), dim3(256), 0, s, partials, nblocks, d.dx * d.dy, G, counter,
+                       record_from, cap, rec);
+}
+
+void launch_set_int(int* p, int v, hipStream_t s) { hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v); }
+
+void launch_copy_f32(const float* src, float* dst, int64_t n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(copy_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
 }
 
 }  // namespace pichip

@@ -86,6 +86,17 @@ class Domain(ctypes.Structure):
         return self.dx * self.dy
 
 
+class StepConfig(ctypes.Structure):
+    _fields_ = [("scheme", ctypes.c_int32), ("backend", ctypes.c_int32), ("sort_interval", ctypes.c_int32),
+                ("order", ctypes.c_int32), ("dt", ctypes.c_double), ("background", ctypes.c_double),
+                ("record_particles", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+
+ORDER_PUSH_FIRST = 0
+ORDER_DEPOSIT_SOLVE_PUSH = 1
+ORDER_DEPOSIT_PUSH_SOLVE = 2
+
+
 class Diagnostics(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("rho_sum", "integrated_rho", "field_energy", "kinetic_energy",
                                                 "total_charge", "total_abs_charge", "charge_error")]
@@ -140,6 +151,11 @@ _SIGS = {
     "pic_hip_set_fields": (ctypes.c_int, [_P, _f32p, _f32p]),
     "pic_hip_gather_push": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "pic_hip_diagnostics": (ctypes.c_int, [_P, ctypes.POINTER(Diagnostics)]),
+    "pic_hip_run": (ctypes.c_int, [_P, ctypes.POINTER(StepConfig), ctypes.c_int32, ctypes.c_int32, _f64p, _f64p,
+                                   _f64p, _f64p]),
+    "pic_hip_dominant_frequency": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double]),
+    "pic_hip_growth_rate": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int32]),
+    "pic_hip_plasma_frequency": (ctypes.c_double, [ctypes.POINTER(Domain), ctypes.c_int64]),
     "pic_hip_profile_begin": (ctypes.c_int, [_P, ctypes.c_int]),
     "pic_hip_profile_end": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
 }
@@ -323,10 +339,39 @@ class Context:
         _check(_lib.pic_hip_profile_end(self._h, ctypes.byref(ms), ctypes.byref(k)))
         return ms.value, k.value
 
+    def run(self, steps: int, scheme=DepositionScheme.CIC, backend=DepositionBackend.GPUPrivatized, dt=0.01,
+            background=0.0, order=ORDER_PUSH_FIRST, sort_interval=1, record_from=0, record_particles=False,
+            use_graph=False):
+        """pic_hip_run: returns dict of recorded FP64 series (length steps - record_from)."""
+        cfg = StepConfig(int(scheme), int(backend), int(sort_interval), int(order), float(dt), float(background),
+                         int(bool(record_particles)), int(bool(use_graph)))
+        cap = max(steps - record_from, 0)
+        out = {k: np.zeros(cap) for k in ("field_energy", "mode1", "kinetic", "charge_error")}
+        ptrs = [_ptr(out[k], ctypes.c_double) for k in ("field_energy", "mode1")]
+        ptrs += [_ptr(out[k], ctypes.c_double) if record_particles else None for k in ("kinetic", "charge_error")]
+        _check(_lib.pic_hip_run(self._h, ctypes.byref(cfg), int(steps), int(record_from), *ptrs))
+        if not record_particles:
+            del out["kinetic"], out["charge_error"]
+        return out
+
     def diagnostics(self) -> Diagnostics:
         d = Diagnostics()
         _check(_lib.pic_hip_diagnostics(self._h, ctypes.byref(d)))
         return d
+
+
+def dominant_frequency(signal, dt: float) -> float:
+    s = np.ascontiguousarray(signal, dtype=np.float64)
+    return _lib.pic_hip_dominant_frequency(_ptr(s, ctypes.c_double), len(s), float(dt))
+
+
+def growth_rate(signal, dt: float, record_start: int) -> float:
+    s = np.ascontiguousarray(signal, dtype=np.float64)
+    return _lib.pic_hip_growth_rate(_ptr(s, ctypes.c_double), len(s), float(dt), int(record_start))
+
+
+def plasma_frequency(domain: Domain, n: int) -> float:
+    return _lib.pic_hip_plasma_frequency(ctypes.byref(domain), int(n))
 
 
 def deposit_charge(particles: dict, domain: Domain, scheme="CIC", backend="GPU_Priv", sorted_=False,

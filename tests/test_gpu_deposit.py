@@ -214,4 +214,7 @@ def test_golden_noise_vs_grid_on_gpu(hip, orc):
         p = orc.init("maxwellian", gold["num_particles"], od, seed=gold["seed"], T=1.0)
         dom = hip.make_domain(1.0, 1.0, g, g)
         rho = hip.deposit_host(p["x"], p["y"], p["q"], dom, names[name], 2, True, p["vx"], p["vy"], 0.0)
-        assert orc.spectral_noise_level(rho) == pytest.approx(val, rel=1e-5), (g, name)
+        # NGP: rounding the FP64 positions to FP32 moves the few particles that lie
+        # within half an FP32 ulp of a half-cell onto the other node (~1 in 1e5).
+        tol = 1e-4 if name == "NGP" else 1e-5
+        assert orc.spectral_noise_level(rho) == pytest.approx(val, rel=tol), (g, name)

@@ -299,3 +299,41 @@ def test_golden_spectral_noise_vs_grid(orc):
         p = cache[g]
         rho = orc.deposit(names[name], p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], 0.0)
         assert orc.spectral_noise_level(rho) == pytest.approx(val, rel=2e-9, abs=1e-10), (g, name)
+
+
+# ---- full timestep loop against reference-held physics scalars -------------------
+
+def _gold(name):
+    return json.load(open(os.path.join(HERE, "golden", "reference_scalars.json")))[name]
+
+
+@pytest.mark.parametrize("scheme,name,order", [(0, "NGP", 0), (1, "CIC", 0), (2, "TSC", 0), (3, "Esirkepov", 2)])
+def test_golden_two_stream_growth(orc, scheme, name, order):
+    """FP64 loop (push, deposit, background, FFT solve, gather) reproduces
+    two_stream_validation.csv: the oracle chain is pinned end to end."""
+    g = _gold("two_stream_validation")
+    dom = orc.domain(1.0, 1.0, g["grid"], g["grid"])
+    n = g["num_particles"]
+    v = orc.resonant_beam_velocity(dom, n, 1)
+    p = orc.init("two_stream", n, dom, seed=g["seed"], beam_velocity=v, perturbation=g["perturbation"])
+    rs = g["steps"] // 4
+    _, rec = orc.run_loop(p, dom, scheme, order, g["dt"], 1.0, g["steps"], rs)
+    gamma = orc.growth_rate(rec["mode1"], g["dt"], rs) * orc.plasma_frequency(dom, n)
+    assert gamma == pytest.approx(g["growth_rate_measured"][name], rel=5e-9)
+
+
+def test_golden_langmuir_frequency_and_noise(orc):
+    g = _gold("langmuir_validation")
+    dom = orc.domain(1.0, 1.0, g["grid"], g["grid"])
+    n = g["num_particles"]
+    p = orc.init("langmuir", n, dom, amplitude=g["amplitude"], mode=g["mode"])
+    rs = g["steps"] // 4
+    _, rec = orc.run_loop(p, dom, 1, 0, g["dt"], 1.0, g["steps"], rs)
+    assert orc.dominant_frequency(rec["field_energy"], g["dt"]) == g["omega_measured"]
+    # spectral noise and charge error of the N=200 Maxwellian deposit (validation.cpp:275-290)
+    m = orc.init("maxwellian", n, dom, seed=42, T=1.0)
+    for name, s in (("NGP", 0), ("CIC", 1), ("TSC", 2), ("Esirkepov", 3)):
+        rho = orc.deposit(s, m["x"], m["y"], m["q"], dom, m["vx"], m["vy"], 0.0)
+        assert orc.spectral_noise_level(rho) == pytest.approx(g["spectral_noise"][name], rel=2e-9)
+        err = orc.charge_error(orc.integrated_rho(rho, dom), orc.total_charge(m["q"]), orc.total_abs_charge(m["q"]))
+        assert err < 1e-16
