@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline CIC deposit only (for profiling runs)")
+    ap.add_argument("--sweep", action="store_true", help="time every tiling candidate per scheme and exit")
+    ap.add_argument("--tiling", type=str, default="", help="tile_x,tile_y,particles_per_lane for the run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design "
@@ -115,6 +117,23 @@ def main():
     ctx.init_particles(N, kind=1, seed=42, amp=0.01, mode=4, q=1.0 / PPC, vmax=0.5 / NX / DT)
     ctx.sort()
     ctx.sync()
+    if args.sweep:
+        for scheme in (1, 2, 0, 3):
+            for tx, ty in ((16, 8), (32, 4), (32, 8), (64, 4), (32, 16), (64, 8), (128, 4), (16, 16)):
+                for ppl in (4, 8):
+                    for pipe in (0, 1):
+                        try:
+                            ctx.set_tiling(tx, ty, ppl, pipe, scheme=scheme)
+                        except pichip.PicError:
+                            continue
+                        _, k_ms = time_deposits(ctx, scheme, 2, 10, 2)
+                        frac = deposit_bytes(scheme) / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBS
+                        print(json.dumps({"scheme": scheme, "tiling": [tx, ty, ppl, pipe], "kernel_ms": k_ms,
+                                          "hbm_frac": frac}), flush=True)
+        ctx.close()
+        return
+    if args.tiling:
+        ctx.set_tiling(*[int(v) for v in args.tiling.split(",")], scheme=1)
 
     ms, kern_ms = time_deposits(ctx, 1, 2, args.steps, args.warmup)
     value = N / (ms / 1000.0)

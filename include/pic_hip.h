@@ -122,10 +122,15 @@ void* pic_hip_stream(pic_hip_ctx* ctx);
 pic_status pic_hip_sync(pic_hip_ctx* ctx);
 size_t pic_hip_count(pic_hip_ctx* ctx);
 
-/* LDS tiling of the GPU_PRIV deposit: tiles of tile_x by tile_y cells,
- * particles_per_lane in {4, 8, 16}.  Defaults are chosen at create time. */
-pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int tile_x, int tile_y, int particles_per_lane);
-pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int* tile_x, int* tile_y, int* particles_per_lane);
+/* LDS tiling of the GPU_PRIV deposit for one scheme (or all with scheme -1):
+ * tiles of tile_x by tile_y cells, particles_per_lane in {4, 8}, pipelined =
+ * overlap the next chunk's loads with the current chunk (two register sets).
+ * Build-defined tuning knobs (no reference counterpart); defaults are chosen
+ * at create time and pic_hip_autotune searches them. */
+pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int scheme, int tile_x, int tile_y, int particles_per_lane,
+                              int pipelined);
+pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int scheme, int* tile_x, int* tile_y, int* particles_per_lane,
+                              int* pipelined);
 
 /* Particles: FP32 SoA x, y, vx, vy, q, m.  NULL vx / vy upload zeros, NULL m
  * uploads ones; x, y, q are required.  Upload clears the cell order. */

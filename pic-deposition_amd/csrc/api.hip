@@ -91,8 +91,11 @@ struct pic_hip_ctx {
     int* status = nullptr;
     float* partial = nullptr;
     size_t partial_cap = 0;
-    TileGeom geom{};
-    int ppl = 8;
+    // LDS tiling per scheme (NGP, CIC, TSC, Esirkepov); the partial buffer is
+    // sized for the largest.
+    TileGeom geom[4]{};
+    int ppl[4] = {8, 8, 8, 4};
+    int pipe[4] = {1, 1, 0, 0};
     float* phi = nullptr;
     float* Ex = nullptr;
     float* Ey = nullptr;
@@ -170,7 +173,8 @@ hipError_t dmalloc(T** p, size_t elems) {
 }
 
 pic_status ensure_partial(pic_hip_ctx* c) {
-    const size_t need = (size_t)c->geom.ntiles * c->geom.wsize;
+    size_t need = 0;
+    for (int k = 0; k < 4; ++k) need = std::max(need, (size_t)c->geom[k].ntiles * c->geom[k].wsize);
     if (need <= c->partial_cap) return PIC_OK;
     (void)hipFree(c->partial);
     c->partial = nullptr;
@@ -180,14 +184,19 @@ pic_status ensure_partial(pic_hip_ctx* c) {
     return PIC_OK;
 }
 
-pic_status apply_tiling(pic_hip_ctx* c, int tx, int ty, int ppl) {
+pic_status apply_tiling(pic_hip_ctx* c, int scheme, int tx, int ty, int ppl, int pipe) {
+    if (scheme < -1 || scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "scheme must be -1 (all) or 0..3");
     if (tx < 1 || ty < 1) return fail(PIC_ERR_INVALID_ARGUMENT, "tile sizes must be >= 1");
-    if (ppl != 4 && ppl != 8 && ppl != 16) return fail(PIC_ERR_INVALID_ARGUMENT, "particles_per_lane must be 4, 8 or 16");
+    if (ppl != 4 && ppl != 8) return fail(PIC_ERR_INVALID_ARGUMENT, "particles_per_lane must be 4 or 8");
     TileGeom g = make_tile_geom(c->dom.Nx, c->dom.Ny, tx, ty);
     const size_t lds = (size_t)4 * g.lds_per_wave * sizeof(float);
     if (lds > 64 * 1024) return fail(PIC_ERR_INVALID_ARGUMENT, "tile too large for LDS (4 windows must fit 64 KiB)");
-    c->geom = g;
-    c->ppl = ppl;
+    for (int k = 0; k < 4; ++k) {
+        if (scheme >= 0 && k != scheme) continue;
+        c->geom[k] = g;
+        c->ppl[k] = ppl;
+        c->pipe[k] = pipe ? 1 : 0;
+    }
     return ensure_partial(c);
 }
 
@@ -278,7 +287,8 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(hipMemsetAsync(c->overflow, 0, (size_t)c->G * sizeof(float), c->stream));
     PIC_HIP_TRY(dmalloc(&c->comp, (size_t)c->G));
     PIC_HIP_TRY(hipMemsetAsync(c->comp, 0, (size_t)c->G * sizeof(float), c->stream));
-    PIC_HIP_TRY(dmalloc(&c->ovf_count, 1));
+    PIC_HIP_TRY(dmalloc(&c->ovf_count, 2));
+    PIC_HIP_TRY(hipMemsetAsync(c->ovf_count, 0, 2 * sizeof(int), c->stream));
     PIC_HIP_TRY(dmalloc(&c->status, 1));
     PIC_HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
     PIC_HIP_TRY(dmalloc(&c->Ex, (size_t)c->G));
@@ -290,7 +300,11 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(dmalloc(&c->diag_out, 8));
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(int) * 4, hipHostMallocDefault));
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_diag), sizeof(double) * 8, hipHostMallocDefault));
-    pic_status st = apply_tiling(c.get(), 32, 8, 8);
+    // Defaults from the MI355X tiling sweep at 1024^2 / 64 ppc.
+    pic_status st = apply_tiling(c.get(), PIC_SCHEME_NGP, 64, 4, 8, 0);
+    if (st == PIC_OK) st = apply_tiling(c.get(), PIC_SCHEME_CIC, 64, 4, 8, 0);
+    if (st == PIC_OK) st = apply_tiling(c.get(), PIC_SCHEME_TSC, 32, 4, 8, 0);
+    if (st == PIC_OK) st = apply_tiling(c.get(), PIC_SCHEME_ESIRKEPOV, 32, 4, 4, 0);
     if (st != PIC_OK) return st;
     PIC_HIP_TRY(hipStreamSynchronize(c->stream));
     *out = c.release();
@@ -322,16 +336,19 @@ pic_status pic_hip_sync(pic_hip_ctx* ctx) {
     return PIC_OK;
 }
 
-pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int tile_x, int tile_y, int particles_per_lane) {
+pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int scheme, int tile_x, int tile_y, int particles_per_lane,
+                              int pipelined) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
-    return apply_tiling(ctx, tile_x, tile_y, particles_per_lane);
+    return apply_tiling(ctx, scheme, tile_x, tile_y, particles_per_lane, pipelined);
 }
 
-pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int* tile_x, int* tile_y, int* ppl) {
+pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int scheme, int* tile_x, int* tile_y, int* ppl, int* pipelined) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
-    if (tile_x) *tile_x = ctx->geom.T;
-    if (tile_y) *tile_y = ctx->geom.R;
-    if (ppl) *ppl = ctx->ppl;
+    if (scheme < 0 || scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "scheme must be 0..3");
+    if (tile_x) *tile_x = ctx->geom[scheme].T;
+    if (tile_y) *tile_y = ctx->geom[scheme].R;
+    if (ppl) *ppl = ctx->ppl[scheme];
+    if (pipelined) *pipelined = ctx->pipe[scheme];
     return PIC_OK;
 }
 
@@ -465,8 +482,9 @@ pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt)
     a.dom = ctx->dev;
     a.scheme = scheme;
     a.backend = backend;
-    a.ppl = ctx->ppl;
-    a.geom = ctx->geom;
+    a.ppl = ctx->ppl[scheme];
+    a.pipe = ctx->pipe[scheme];
+    a.geom = ctx->geom[scheme];
     a.dt = (float)dt;
     a.rho = ctx->rho;
     a.partial = ctx->partial;

@@ -76,19 +76,17 @@ __device__ __forceinline__ void wave_segmented_scan(float (&B)[V], float A) {
     lin_scan_step<kRowBcast31, 0xc>(B, A);
 }
 
-// Compensated add of a cell's V stencil values into the wave's LDS window.
+// Compensated add of a cell's V stencil values into the wave's LDS window:
+// all V returning adds are issued first, then the TwoSum errors go to the
+// compensation window.  (Adding 0 leaves both windows unchanged.)
 template <int SX, int V>
 __device__ __forceinline__ void emit_cell(float* __restrict__ ws, float* __restrict__ wc, int wbase, int WX,
                                           const float (&z)[V]) {
+    float old[V];
 #pragma unroll
-    for (int k = 0; k < V; ++k) {
-        const float v = z[k];
-        if (v != 0.0f) {
-            const int a = wbase + (k / SX) * WX + (k % SX);
-            const float old = atomicAdd(ws + a, v);
-            atomicAdd(wc + a, two_sum_err(old, v));
-        }
-    }
+    for (int k = 0; k < V; ++k) old[k] = atomicAdd(ws + wbase + (k / SX) * WX + (k % SX), z[k]);
+#pragma unroll
+    for (int k = 0; k < V; ++k) atomicAdd(wc + wbase + (k / SX) * WX + (k % SX), two_sum_err(old[k], z[k]));
 }
 
 // General reference-Esirkepov walk (esirkepovWeights1DSparse,
@@ -225,7 +223,183 @@ __global__ __launch_bounds__(256) void fold_comp_kernel(float* __restrict__ rho,
     }
 }
 
-template <int S, int K>
+// Q consecutive particle quads of one lane: physical start index of each quad
+// and the valid particle range [lo, hi) of the row it belongs to.
+template <int Q>
+struct Chunk {
+    float4 X[Q], Y[Q], QQ[Q], VX[Q], VY[Q];
+    int p0[Q], lo[Q], hi[Q];
+};
+
+// Maps the lane's virtual quads vq0 .. vq0+Q-1 (rows of the tile laid end to
+// end) to physical quads with the wave's row table and issues the loads.
+template <int Q, bool VEL>
+__device__ __forceinline__ void load_chunk(int vq0, int total_q, int th, const int* __restrict__ t_qcum,
+                                           const int* __restrict__ t_qstart, const int* __restrict__ t_rs,
+                                           const int* __restrict__ t_re, const float* __restrict__ px,
+                                           const float* __restrict__ py, const float* __restrict__ pq,
+                                           const float* __restrict__ pvx, const float* __restrict__ pvy,
+                                           Chunk<Q>& c) {
+    int r = 0;
+    while (r + 1 < th && vq0 >= t_qcum[r + 1]) ++r;
+    int next = t_qcum[r + 1];
+    int delta = t_qstart[r] - t_qcum[r];
+    int lo = t_rs[r], hi = t_re[r];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+        const int vq = vq0 + j;
+        if (vq < total_q) {
+            if (vq >= next) {  // lane straddles a row boundary (rare)
+                while (vq >= t_qcum[r + 1]) ++r;
+                next = t_qcum[r + 1];
+                delta = t_qstart[r] - t_qcum[r];
+                lo = t_rs[r];
+                hi = t_re[r];
+            }
+            const int p0 = (vq + delta) * 4;
+            c.p0[j] = p0;
+            c.lo[j] = lo;
+            c.hi[j] = hi;
+            c.X[j] = *reinterpret_cast<const float4*>(px + p0);
+            c.Y[j] = *reinterpret_cast<const float4*>(py + p0);
+            c.QQ[j] = *reinterpret_cast<const float4*>(pq + p0);
+            if (VEL) {
+                c.VX[j] = *reinterpret_cast<const float4*>(pvx + p0);
+                c.VY[j] = *reinterpret_cast<const float4*>(pvy + p0);
+            }
+        } else {
+            c.p0[j] = 0;
+            c.lo[j] = 1;  // empty range: no particle of this quad is valid
+            c.hi[j] = 0;
+        }
+    }
+}
+
+// A particle whose current cell is outside its tile (drift since the last
+// sort): its already computed stencil goes to the overflow grid with
+// compensated global atomics.  `ok` false (an Esirkepov move longer than a
+// cell) takes the general walk.
+template <int S>
+__device__ __forceinline__ void overflow_scatter(bool ok, int cx, int cy, const float (&w)[Stencil<S>::V], float x,
+                                                 float y, float vx, float vy, float qv, float dt, const DevDomain& d,
+                                                 float* __restrict__ overflow, float* __restrict__ comp,
+                                                 int* __restrict__ status) {
+    using St = Stencil<S>;
+    if (ok) {
+#pragma unroll
+        for (int k = 0; k < St::V; ++k) {
+            const int gx = wrap_index(cx + St::OX + k % St::SX, d.Nx);
+            const int gy = wrap_index(cy + St::OY + k / St::SX, d.Ny);
+            comp_atomic_add(overflow, comp, (size_t)gy * d.Nx + gx, w[k]);
+        }
+    } else {
+        scatter_esk_long(x, y, vx, vy, qv, dt, d, overflow, comp, status);
+    }
+}
+
+// Per-wave state of the tiled deposit (one tile per wavefront).
+struct TileCtx {
+    float* win_s;
+    float* win_c;
+    int WX;
+    int x0, y0, tw, th;
+};
+
+// Processes one chunk: K consecutive particles per lane accumulated as runs of
+// equal cell in registers, then one segmented wave scan, then one
+// compensated LDS add per (cell, stencil slot).
+template <int S, int Q, int EX>
+__device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx& t, const DevDomain& d, float dt,
+                                              float* __restrict__ overflow, float* __restrict__ comp,
+                                              int* __restrict__ ovf_count, int* __restrict__ status) {
+    using St = Stencil<S>;
+    constexpr int V = St::V;
+    float acc[V], H[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        acc[k] = 0.0f;
+        H[k] = 0.0f;
+    }
+    int hk = -1, ck = -1;
+    bool multi = false;
+    bool any_ovf = false;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int p = cur.p0[j] + e;
+            // Particles of the quad outside this row's range belong to other
+            // tiles: they contribute zero and leave the current run open.
+            const bool valid = p >= cur.lo[j] && p < cur.hi[j];
+            const float xx = e == 0 ? cur.X[j].x : e == 1 ? cur.X[j].y : e == 2 ? cur.X[j].z : cur.X[j].w;
+            const float yy = e == 0 ? cur.Y[j].x : e == 1 ? cur.Y[j].y : e == 2 ? cur.Y[j].z : cur.Y[j].w;
+            const float qq = e == 0 ? cur.QQ[j].x : e == 1 ? cur.QQ[j].y : e == 2 ? cur.QQ[j].z : cur.QQ[j].w;
+            float vxx = 0.0f, vyy = 0.0f;
+            if (St::kNeedsVelocity) {
+                vxx = e == 0 ? cur.VX[j].x : e == 1 ? cur.VX[j].y : e == 2 ? cur.VX[j].z : cur.VX[j].w;
+                vyy = e == 0 ? cur.VY[j].x : e == 1 ? cur.VY[j].y : e == 2 ? cur.VY[j].z : cur.VY[j].w;
+            }
+            const float qv = valid ? qq * d.inv_vol : 0.0f;
+            float w[V];
+            int cx, cy;
+            const bool ok = St::template weights<EX>(xx, yy, vxx, vyy, qv, dt, d, cx, cy, w);
+            // Unwrapped cell: positions outside [0, L) fall outside every tile
+            // and take the (wrapping) overflow path.
+            const int lx = cx - t.x0, ly = cy - t.y0;
+            const bool inside = ok && (unsigned)lx < (unsigned)t.tw && (unsigned)ly < (unsigned)t.th;
+            if (valid && !inside) {
+                overflow_scatter<S>(ok, cx, cy, w, xx, yy, vxx, vyy, qv, dt, d, overflow, comp, status);
+                any_ovf = true;
+            }
+            const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
+            const bool nseg = lk != ck;
+            if (nseg && ck >= 0) {
+                if (!multi) {
+#pragma unroll
+                    for (int k = 0; k < V; ++k) H[k] = acc[k];
+                    multi = true;
+                } else {
+                    const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
+                    emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, acc);
+                }
+            }
+            const float keep = nseg ? 0.0f : 1.0f;
+            const bool take = valid && inside;
+#pragma unroll
+            for (int k = 0; k < V; ++k) acc[k] = fmaf(acc[k], keep, take ? w[k] : 0.0f);
+            if (nseg) {
+                ck = lk;
+                if (hk < 0) hk = lk;
+            }
+        }
+    }
+    if (__any(any_ovf) && __lane_id() == 0) atomicAdd(ovf_count, 1);
+    // Combine the runs that straddle lanes.  Lane i's outgoing run is acc
+    // with key ck; it continues lane i-1's outgoing run when hk == ck(i-1).
+    const int ck_prev = dpp_i<kWaveShr1, 0xf>(-2, ck);
+    const bool cont = hk >= 0 && hk == ck_prev;
+    const float a = (cont && !multi) ? 1.0f : 0.0f;
+    float Z[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) Z[k] = acc[k];
+    wave_segmented_scan<V>(Z, a);
+    float Zp[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) Zp[k] = dpp_f<kWaveShr1, 0xf>(0.0f, Z[k]);
+    if (multi) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) H[k] = cont ? H[k] + Zp[k] : H[k];
+        const int wb = ((hk >> 12) + 1 + St::OY) * t.WX + (hk & 4095) + 1 + St::OX;
+        emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, H);
+    }
+    const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
+    if (ck >= 0 && !cont_next) {
+        const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
+        emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, Z);
+    }
+}
+
+template <int S, int K, int EX, int PIPE>
 __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restrict__ px, const float* __restrict__ py,
                                                            const float* __restrict__ pvx,
                                                            const float* __restrict__ pvy,
@@ -235,7 +409,6 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                                                            float* __restrict__ overflow, float* __restrict__ comp,
                                                            int* __restrict__ ovf_count, int* __restrict__ status) {
     using St = Stencil<S>;
-    constexpr int V = St::V;
     constexpr int Q = K / 4;
     static_assert(K % 4 == 0, "K must be a multiple of 4 (float4 loads)");
     extern __shared__ float smem[];
@@ -245,27 +418,32 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
     const int tile = blockIdx.x * (blockDim.x >> 6) + wv;
     if (tile >= g.ntiles) return;  // wave-uniform
 
-    float* win_s = smem + (size_t)wv * g.lds_per_wave;
-    float* win_c = win_s + g.wsize;
-    int* tab = reinterpret_cast<int*>(win_c + g.wsize);
-    int* t_qstart = tab;              // [R]   first aligned quad of row r
-    int* t_rs = tab + g.R;            // [R]   first particle of row r
-    int* t_re = tab + 2 * g.R;        // [R]   one past the last particle of row r
-    int* t_qcum = tab + 3 * g.R;      // [R+1] cumulative quad counts
+    TileCtx t;
+    t.win_s = smem + (size_t)wv * g.lds_per_wave;
+    t.win_c = t.win_s + g.wsize;
+    int* tab = reinterpret_cast<int*>(t.win_c + g.wsize);
+    int* t_qstart = tab;          // [R]   first aligned quad of row r
+    int* t_rs = tab + g.R;        // [R]   first particle of row r
+    int* t_re = tab + 2 * g.R;    // [R]   one past the last particle of row r
+    int* t_qcum = tab + 3 * g.R;  // [R+1] cumulative quad counts
 
     const int tx = tile % g.ntx, ty = tile / g.ntx;
-    const int x0 = tx * g.T, y0 = ty * g.R;
-    const int tw = min(g.T, d.Nx - x0), th = min(g.R, d.Ny - y0);
+    t.x0 = tx * g.T;
+    t.y0 = ty * g.R;
+    t.tw = min(g.T, d.Nx - t.x0);
+    t.th = min(g.R, d.Ny - t.y0);
+    t.WX = g.WX;
+    const int th = t.th;
 
     for (int i = lane; i < g.wsize; i += 64) {
-        win_s[i] = 0.0f;
-        win_c[i] = 0.0f;
+        t.win_s[i] = 0.0f;
+        t.win_c[i] = 0.0f;
     }
     // Row table: lane r < th describes row y0 + r of the tile.
     int nq = 0;
     if (lane < th) {
-        const int c = (y0 + lane) * d.Nx + x0;
-        const int s = cell_off[c], e = cell_off[c + tw];
+        const int c = (t.y0 + lane) * d.Nx + t.x0;
+        const int s = cell_off[c], e = cell_off[c + t.tw];
         const int qs = s >> 2, qe = (e + 3) >> 2;
         nq = e > s ? qe - qs : 0;
         t_qstart[lane] = qs;
@@ -276,128 +454,49 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
     int incl = nq;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
+        const int u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
     }
     if (lane < th) t_qcum[lane + 1] = incl;
     if (lane == 0) t_qcum[0] = 0;
     wave_sync();
     const int total_q = t_qcum[th];
+    const int step = 64 * Q;
 
-    const int WX = g.WX;
-    for (int base = 0; base < total_q; base += 64 * Q) {
-        const int vq0 = base + lane * Q;
-        float4 X[Q], Y[Q], QQ[Q], VX[Q], VY[Q];
-        int p0[Q], lo[Q], hi[Q];
-        int r = 0;
-        while (r + 1 < th && vq0 >= t_qcum[r + 1]) ++r;
-#pragma unroll
-        for (int j = 0; j < Q; ++j) {
-            const int vq = vq0 + j;
-            p0[j] = -1;
-            lo[j] = 0;
-            hi[j] = 0;
-            if (vq < total_q) {
-                while (vq >= t_qcum[r + 1]) ++r;
-                const int pq4 = t_qstart[r] + (vq - t_qcum[r]);
-                p0[j] = pq4 * 4;
-                lo[j] = t_rs[r];
-                hi[j] = t_re[r];
-                X[j] = *reinterpret_cast<const float4*>(px + p0[j]);
-                Y[j] = *reinterpret_cast<const float4*>(py + p0[j]);
-                QQ[j] = *reinterpret_cast<const float4*>(pq + p0[j]);
-                if (St::kNeedsVelocity) {
-                    VX[j] = *reinterpret_cast<const float4*>(pvx + p0[j]);
-                    VY[j] = *reinterpret_cast<const float4*>(pvy + p0[j]);
-                }
-            }
+    if (PIPE) {
+        // Two register sets: the loads of one chunk are in flight while the
+        // other is processed.
+        Chunk<Q> A, B;
+        int base = 0;
+        if (base < total_q)
+            load_chunk<Q, St::kNeedsVelocity>(lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq, pvx,
+                                              pvy, A);
+        while (base < total_q) {
+            const int b1 = base + step;
+            if (b1 < total_q)
+                load_chunk<Q, St::kNeedsVelocity>(b1 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
+                                                  pq, pvx, pvy, B);
+            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, ovf_count, status);
+            if (b1 >= total_q) break;
+            const int b2 = b1 + step;
+            if (b2 < total_q)
+                load_chunk<Q, St::kNeedsVelocity>(b2 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
+                                                  pq, pvx, pvy, A);
+            process_chunk<S, Q, EX>(B, t, d, dt, overflow, comp, ovf_count, status);
+            base = b2;
         }
-
-        float acc[V], H[V];
-#pragma unroll
-        for (int k = 0; k < V; ++k) {
-            acc[k] = 0.0f;
-            H[k] = 0.0f;
-        }
-        int hk = -1, ck = -1;
-        bool multi = false;
-
-#pragma unroll
-        for (int j = 0; j < Q; ++j) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int p = p0[j] + e;
-                if (p0[j] >= 0 && p >= lo[j] && p < hi[j]) {
-                    const float xx = e == 0 ? X[j].x : e == 1 ? X[j].y : e == 2 ? X[j].z : X[j].w;
-                    const float yy = e == 0 ? Y[j].x : e == 1 ? Y[j].y : e == 2 ? Y[j].z : Y[j].w;
-                    const float qq = e == 0 ? QQ[j].x : e == 1 ? QQ[j].y : e == 2 ? QQ[j].z : QQ[j].w;
-                    float vxx = 0.0f, vyy = 0.0f;
-                    if (St::kNeedsVelocity) {
-                        vxx = e == 0 ? VX[j].x : e == 1 ? VX[j].y : e == 2 ? VX[j].z : VX[j].w;
-                        vyy = e == 0 ? VY[j].x : e == 1 ? VY[j].y : e == 2 ? VY[j].z : VY[j].w;
-                    }
-                    const float qv = qq * d.inv_vol;
-                    float w[V];
-                    int cx, cy;
-                    const bool ok = St::weights(xx, yy, vxx, vyy, qv, dt, d, cx, cy, w);
-                    const int lx = wrap_index(cx, d.Nx) - x0;
-                    const int ly = wrap_index(cy, d.Ny) - y0;
-                    if (!ok || (unsigned)lx >= (unsigned)tw || (unsigned)ly >= (unsigned)th) {
-                        scatter_global<S>(xx, yy, vxx, vyy, qv, dt, d, overflow, comp, status);
-                        atomicAdd(ovf_count, 1);
-                    } else {
-                        const int lk = (ly << 12) | lx;
-                        const bool nseg = lk != ck;
-                        if (nseg && ck >= 0) {
-                            if (!multi) {
-#pragma unroll
-                                for (int k = 0; k < V; ++k) H[k] = acc[k];
-                                multi = true;
-                            } else {
-                                const int wb = ((ck >> 12) + 1 + St::OY) * WX + (ck & 4095) + 1 + St::OX;
-                                emit_cell<St::SX, V>(win_s, win_c, wb, WX, acc);
-                            }
-                        }
-                        const float keep = nseg ? 0.0f : 1.0f;
-#pragma unroll
-                        for (int k = 0; k < V; ++k) acc[k] = fmaf(acc[k], keep, w[k]);
-                        if (nseg) {
-                            ck = lk;
-                            if (hk < 0) hk = lk;
-                        }
-                    }
-                }
-            }
-        }
-
-        // Combine the runs that straddle lanes.  Lane i's outgoing run is acc
-        // with key ck; it continues lane i-1's outgoing run when hk == ck(i-1).
-        const int ck_prev = dpp_i<kWaveShr1, 0xf>(-2, ck);
-        const bool cont = hk >= 0 && hk == ck_prev;
-        const float a = (cont && !multi) ? 1.0f : 0.0f;
-        float Z[V];
-#pragma unroll
-        for (int k = 0; k < V; ++k) Z[k] = acc[k];
-        wave_segmented_scan<V>(Z, a);
-        float Zp[V];
-#pragma unroll
-        for (int k = 0; k < V; ++k) Zp[k] = dpp_f<kWaveShr1, 0xf>(0.0f, Z[k]);
-        if (multi) {
-#pragma unroll
-            for (int k = 0; k < V; ++k) H[k] = cont ? H[k] + Zp[k] : H[k];
-            const int wb = ((hk >> 12) + 1 + St::OY) * WX + (hk & 4095) + 1 + St::OX;
-            emit_cell<St::SX, V>(win_s, win_c, wb, WX, H);
-        }
-        const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
-        if (ck >= 0 && !cont_next) {
-            const int wb = ((ck >> 12) + 1 + St::OY) * WX + (ck & 4095) + 1 + St::OX;
-            emit_cell<St::SX, V>(win_s, win_c, wb, WX, Z);
+    } else {
+        for (int base = 0; base < total_q; base += step) {
+            Chunk<Q> A;
+            load_chunk<Q, St::kNeedsVelocity>(base + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq,
+                                              pvx, pvy, A);
+            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, ovf_count, status);
         }
     }
 
     wave_sync();
     float* out = partial + (size_t)tile * g.wsize;
-    for (int i = lane; i < g.wsize; i += 64) out[i] = win_s[i] + win_c[i];
+    for (int i = lane; i < g.wsize; i += 64) out[i] = t.win_s[i] + t.win_c[i];
 }
 
 // Tiles (along one axis) whose window can hold node g: those owning cells
@@ -417,54 +516,106 @@ __device__ __forceinline__ int window_candidates(int gnode, int T, int nt, int n
     return cnt;
 }
 
+// Fast path (every tile at least 3 cells wide and tall, at least 3 tiles per
+// axis): along each axis a node lies in its own tile's window and, near the
+// tile's edge, in one neighbour's.  Writes (tile, window index) pairs.
+__device__ __forceinline__ int axis_pairs(int gn, int T, int nt, int n, int (&t)[2], int (&w)[2]) {
+    const int t0 = gn / T;
+    const int x0 = t0 * T;
+    const int tw0 = min(T, n - x0);
+    const int l = gn - x0;
+    t[0] = t0;
+    w[0] = l + 1;
+    if (l < 2) {  // the left neighbour's window reaches two nodes past its end
+        const int tl = t0 == 0 ? nt - 1 : t0 - 1;
+        const int twl = min(T, n - tl * T);
+        t[1] = tl;
+        w[1] = l + twl + 1;
+        return 2;
+    }
+    if (l >= tw0 - 1) {  // the right neighbour's window starts one node before it
+        t[1] = t0 == nt - 1 ? 0 : t0 + 1;
+        w[1] = l - tw0 + 1;
+        return 2;
+    }
+    return 1;
+}
+
 __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
                                                             float* __restrict__ overflow, float* __restrict__ comp,
-                                                            const int* __restrict__ ovf_count, DevDomain d,
-                                                            TileGeom g, float* __restrict__ rho) {
+                                                            int* __restrict__ ovf, DevDomain d, TileGeom g,
+                                                            float* __restrict__ rho) {
+    __shared__ int s_ovf;
+    if (threadIdx.x == 0) s_ovf = __hip_atomic_load(ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
-    if (gx >= d.Nx) return;
-    int txs[4], tys[4];
-    const int ncx = window_candidates(gx, g.T, g.ntx, d.Nx, txs);
-    const int ncy = window_candidates(gy, g.R, g.nty, d.Ny, tys);
-    float s = 0.0f, c = 0.0f;
-    for (int iy = 0; iy < ncy; ++iy) {
-        const int ty = tys[iy];
-        const int y0 = ty * g.R;
-        const int th = min(g.R, d.Ny - y0);
-        // images gy' = gy + m*Ny inside [y0 - 1, y0 + th + 2)
-        int gyi = gy;
-        while (gyi - d.Ny >= y0 - 1) gyi -= d.Ny;
-        while (gyi < y0 - 1) gyi += d.Ny;
-        for (; gyi < y0 + th + 2; gyi += d.Ny) {
-            const int wy = gyi - (y0 - 1);
-            for (int ix = 0; ix < ncx; ++ix) {
-                const int tx = txs[ix];
-                const int x0 = tx * g.T;
-                const int tw = min(g.T, d.Nx - x0);
-                int gxi = gx;
-                while (gxi - d.Nx >= x0 - 1) gxi -= d.Nx;
-                while (gxi < x0 - 1) gxi += d.Nx;
-                for (; gxi < x0 + tw + 2; gxi += d.Nx) {
-                    const int wx = gxi - (x0 - 1);
-                    const float v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
+    if (gx < d.Nx) {
+        float s = 0.0f, c = 0.0f;
+        if (g.fast) {
+            int txs[2], wxs[2], tys[2], wys[2];
+            const int ncx = axis_pairs(gx, g.T, g.ntx, d.Nx, txs, wxs);
+            const int ncy = axis_pairs(gy, g.R, g.nty, d.Ny, tys, wys);
+            for (int iy = 0; iy < ncy; ++iy) {
+                for (int ix = 0; ix < ncx; ++ix) {
+                    const float v =
+                        partial[(size_t)(tys[iy] * g.ntx + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
                     c += two_sum_err(s, v);
                     s += v;
                 }
             }
+        } else {
+            int txs[4], tys[4];
+            const int ncx = window_candidates(gx, g.T, g.ntx, d.Nx, txs);
+            const int ncy = window_candidates(gy, g.R, g.nty, d.Ny, tys);
+            for (int iy = 0; iy < ncy; ++iy) {
+                const int ty = tys[iy];
+                const int y0 = ty * g.R;
+                const int th = min(g.R, d.Ny - y0);
+                // images gy' = gy + m*Ny inside [y0 - 1, y0 + th + 2)
+                int gyi = gy;
+                while (gyi - d.Ny >= y0 - 1) gyi -= d.Ny;
+                while (gyi < y0 - 1) gyi += d.Ny;
+                for (; gyi < y0 + th + 2; gyi += d.Ny) {
+                    const int wy = gyi - (y0 - 1);
+                    for (int ix = 0; ix < ncx; ++ix) {
+                        const int tx = txs[ix];
+                        const int x0 = tx * g.T;
+                        const int tw = min(g.T, d.Nx - x0);
+                        int gxi = gx;
+                        while (gxi - d.Nx >= x0 - 1) gxi -= d.Nx;
+                        while (gxi < x0 - 1) gxi += d.Nx;
+                        for (; gxi < x0 + tw + 2; gxi += d.Nx) {
+                            const int wx = gxi - (x0 - 1);
+                            const float v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
+                            c += two_sum_err(s, v);
+                            s += v;
+                        }
+                    }
+                }
+            }
+        }
+        float out = s + c;
+        const size_t node = (size_t)gy * d.Nx + gx;
+        if (s_ovf > 0) {
+            out += overflow[node] + comp[node];
+            overflow[node] = 0.0f;
+            comp[node] = 0.0f;
+        }
+        rho[node] = out;
+    }
+    // The last block to finish resets the overflow counter for the next
+    // deposit (every block has read it by then).
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int nblocks = (int)(gridDim.x * gridDim.y);
+        if (atomicAdd(ovf + 1, 1) == nblocks - 1) {
+            __hip_atomic_store(ovf, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ovf + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    float out = s + c;
-    const size_t node = (size_t)gy * d.Nx + gx;
-    if (*ovf_count > 0) {
-        out += overflow[node] + comp[node];
-        overflow[node] = 0.0f;
-        comp[node] = 0.0f;
-    }
-    rho[node] = out;
 }
-
-__global__ void reset_counter_kernel(int* __restrict__ c) { *c = 0; }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, size_t n, float v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -483,29 +634,36 @@ void launch_atomic(const DepositArgs& a) {
     hipLaunchKernelGGL(fold_comp_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.comp, G);
 }
 
-template <int S, int K>
+template <int S, int K, int EX, int PIPE>
 void launch_tile(const DepositArgs& a) {
     const TileGeom& g = a.geom;
     const int waves_per_block = 4;
     const unsigned blocks = (unsigned)((g.ntiles + waves_per_block - 1) / waves_per_block);
     const size_t lds = (size_t)waves_per_block * g.lds_per_wave * sizeof(float);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, a.stream, a.ovf_count);
     if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
-    hipLaunchKernelGGL((deposit_tile_kernel<S, K>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream, a.x,
-                       a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.overflow, a.comp, a.ovf_count,
-                       a.status);
+    hipLaunchKernelGGL((deposit_tile_kernel<S, K, EX, PIPE>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream,
+                       a.x, a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.overflow, a.comp,
+                       a.ovf_count, a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
     hipLaunchKernelGGL(deposit_merge_kernel, dim3((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny), dim3(256), 0,
                        a.stream, a.partial, a.overflow, a.comp, a.ovf_count, a.dom, g, a.rho);
 }
 
+template <int S, int K>
+void launch_tile_kp(const DepositArgs& a) {
+    if (a.dom.exact) {
+        if (a.pipe) launch_tile<S, K, 1, 1>(a);
+        else launch_tile<S, K, 1, 0>(a);
+    } else {
+        if (a.pipe) launch_tile<S, K, 0, 1>(a);
+        else launch_tile<S, K, 0, 0>(a);
+    }
+}
+
 template <int S>
 void launch_tile_k(const DepositArgs& a) {
-    switch (a.ppl) {
-        case 4: launch_tile<S, 4>(a); break;
-        case 16: launch_tile<S, 16>(a); break;
-        default: launch_tile<S, 8>(a); break;
-    }
+    if (a.ppl == 4) launch_tile_kp<S, 4>(a);
+    else launch_tile_kp<S, 8>(a);
 }
 
 }  // namespace
@@ -523,6 +681,8 @@ TileGeom make_tile_geom(int Nx, int Ny, int T, int R) {
     g.WY = g.R + 3;
     g.wsize = g.WX * g.WY;
     g.lds_per_wave = 2 * g.wsize + 4 * g.R + 1;
+    const int rx = Nx % g.T, ry = Ny % g.R;
+    g.fast = g.T >= 3 && g.R >= 3 && g.ntx >= 3 && g.nty >= 3 && (rx == 0 || rx >= 3) && (ry == 0 || ry >= 3);
     return g;
 }
 

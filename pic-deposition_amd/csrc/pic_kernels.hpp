@@ -18,6 +18,7 @@ struct TileGeom {
     int ntx, nty, ntiles;
     int WX, WY, wsize;
     int lds_per_wave;  // floats: 2 windows + row tables
+    int fast;          // merge fast path: tiles >= 3 cells on each axis, >= 3 tiles per axis
 };
 
 TileGeom make_tile_geom(int Nx, int Ny, int T, int R);
@@ -29,14 +30,15 @@ struct DepositArgs {
     DevDomain dom;
     int scheme;   // Scheme
     int backend;  // 1 = global atomics, 2 = LDS tiles
-    int ppl;      // particles per lane per chunk (4, 8, 16)
+    int ppl;      // particles per lane per chunk (4 or 8)
+    int pipe;     // 1: two register sets, next chunk's loads overlap processing
     TileGeom geom;
     float dt;
     float* rho;       // [Ny][Nx] FP32 out
     float* partial;   // ntiles * wsize
     float* overflow;  // [Ny][Nx], kept zero between calls
     float* comp;      // [Ny][Nx] compensation of the global atomic sums, kept zero between calls
-    int* ovf_count;
+    int* ovf_count;   // [2]: overflow particle count, merge blocks done (both zero between calls)
     int* status;      // bit 0: Esirkepov move longer than the general walk supports
     hipStream_t stream;
     hipEvent_t ev_begin = nullptr;  // if set, recorded around the main scatter kernel

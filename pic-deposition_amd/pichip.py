@@ -132,9 +132,9 @@ _SIGS = {
     "pic_hip_stream": (_P, [_P]),
     "pic_hip_sync": (ctypes.c_int, [_P]),
     "pic_hip_count": (ctypes.c_size_t, [_P]),
-    "pic_hip_set_tiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
-    "pic_hip_get_tiling": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
-                                          ctypes.POINTER(ctypes.c_int)]),
+    "pic_hip_set_tiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pic_hip_get_tiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pic_hip_upload": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_size_t]),
     "pic_hip_download": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
     "pic_hip_init_particles": (ctypes.c_int, [_P, ctypes.POINTER(InitParams), ctypes.c_size_t]),
@@ -213,6 +213,7 @@ class Tiling:
     tile_x: int
     tile_y: int
     particles_per_lane: int
+    pipelined: int
 
 
 class Context:
@@ -253,13 +254,15 @@ class Context:
     def sync(self) -> None:
         _check(_lib.pic_hip_sync(self._h))
 
-    def set_tiling(self, tile_x: int, tile_y: int, particles_per_lane: int = 8) -> None:
-        _check(_lib.pic_hip_set_tiling(self._h, tile_x, tile_y, particles_per_lane))
+    def set_tiling(self, tile_x: int, tile_y: int, particles_per_lane: int = 8, pipelined: int = 0,
+                   scheme: int = -1) -> None:
+        _check(_lib.pic_hip_set_tiling(self._h, int(scheme), tile_x, tile_y, particles_per_lane, int(pipelined)))
 
-    def tiling(self) -> Tiling:
-        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(_lib.pic_hip_get_tiling(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return Tiling(a.value, b.value, c.value)
+    def tiling(self, scheme: int = 1) -> Tiling:
+        a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.pic_hip_get_tiling(self._h, int(scheme), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                       ctypes.byref(d)))
+        return Tiling(a.value, b.value, c.value, d.value)
 
     def upload(self, x, y, q, vx=None, vy=None, m=None) -> None:
         arrs = [_f32(a) for a in (x, y, vx, vy, q, m)]

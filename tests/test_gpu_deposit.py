@@ -72,7 +72,8 @@ def test_parity_64x64_64ppc(hip, orc, scheme, backend):
     assert np.abs(g - r).max() <= 1e-5 * np.abs(r).max()
 
 
-@pytest.mark.parametrize("tiling", [(32, 8, 8), (64, 4, 16), (16, 16, 4), (3, 3, 4), (7, 5, 8), (128, 1, 8)])
+@pytest.mark.parametrize("tiling", [(32, 8, 8, 1), (64, 4, 8, 0), (16, 16, 4, 1), (3, 3, 4, 0), (7, 5, 8, 1),
+                                    (128, 1, 8, 0), (64, 4, 4, 1)])
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_tile_shapes(hip, orc, scheme, tiling):
     say("tiling", scheme, tiling)
