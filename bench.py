@@ -53,6 +53,20 @@ def time_deposits(ctx, scheme, backend, steps, warmup):
     return (t1 - t0) * 1000.0 / steps, kern_ms / max(k, 1)
 
 
+def measured_traffic():
+    """HBM bytes per launch of the CIC tile kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB
+    copy kernel); None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "deposit_cic_1024x1024_64ppc_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("TRAFFIC_BYTES_PER_LAUNCH="):
+                return int(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(ctx, dom_args, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -97,7 +111,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline CIC deposit only (for profiling runs)")
     ap.add_argument("--sweep", action="store_true", help="time every tiling candidate per scheme and exit")
-    ap.add_argument("--tiling", type=str, default="", help="tile_x,tile_y,particles_per_lane for the run")
+    ap.add_argument("--tiling", type=str, default="", help="tile_x,tile_y,particles_per_lane,pipelined for CIC")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design "
@@ -135,6 +151,8 @@ def main():
     if args.tiling:
         ctx.set_tiling(*[int(v) for v in args.tiling.split(",")], scheme=1)
 
+    if args.copy_probe:
+        ctx.copy_probe(1 << 30)
     ms, kern_ms = time_deposits(ctx, 1, 2, args.steps, args.warmup)
     value = N / (ms / 1000.0)
     achieved = deposit_bytes(1) / (kern_ms / 1000.0) / 1e9
@@ -156,7 +174,8 @@ def main():
                    "particles": N, "grid": [NX, NY], "scheme": "CIC", "backend": "GPU_Priv",
                    "tiling": vars(ctx.tiling())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if not args.tiling else None,
                      "kernel": "deposit_tile_kernel<CIC>", "kernel_ms": kern_ms,
                      "algorithmic_bytes": deposit_bytes(1)},
     }

@@ -226,6 +226,10 @@ double pic_hip_plasma_frequency(const pic_domain* domain, int64_t num_particles)
  * (at most max_launches). */
 pic_status pic_hip_profile_begin(pic_hip_ctx* ctx, int max_launches);
 pic_status pic_hip_profile_end(pic_hip_ctx* ctx, double* total_ms, int* launches);
+/* Copies `bytes` (rounded down to 4) between two scratch device buffers with
+ * a plain streaming kernel (copy_f32_kernel): a known-size reference for
+ * calibrating hardware byte counters.  Synchronous. */
+pic_status pic_hip_copy_probe(pic_hip_ctx* ctx, size_t bytes);
 
 #ifdef __cplusplus
 }

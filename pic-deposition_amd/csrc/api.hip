@@ -87,7 +87,6 @@ struct pic_hip_ctx {
     float* rho = nullptr;
     float* overflow = nullptr;
     float* comp = nullptr;
-    int* ovf_count = nullptr;
     int* status = nullptr;
     float* partial = nullptr;
     size_t partial_cap = 0;
@@ -138,7 +137,6 @@ struct pic_hip_ctx {
         (void)hipFree(rho);
         (void)hipFree(overflow);
         (void)hipFree(comp);
-        (void)hipFree(ovf_count);
         (void)hipFree(status);
         (void)hipFree(partial);
         (void)hipFree(phi);
@@ -287,8 +285,6 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(hipMemsetAsync(c->overflow, 0, (size_t)c->G * sizeof(float), c->stream));
     PIC_HIP_TRY(dmalloc(&c->comp, (size_t)c->G));
     PIC_HIP_TRY(hipMemsetAsync(c->comp, 0, (size_t)c->G * sizeof(float), c->stream));
-    PIC_HIP_TRY(dmalloc(&c->ovf_count, 2));
-    PIC_HIP_TRY(hipMemsetAsync(c->ovf_count, 0, 2 * sizeof(int), c->stream));
     PIC_HIP_TRY(dmalloc(&c->status, 1));
     PIC_HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
     PIC_HIP_TRY(dmalloc(&c->Ex, (size_t)c->G));
@@ -490,7 +486,6 @@ pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt)
     a.partial = ctx->partial;
     a.overflow = ctx->overflow;
     a.comp = ctx->comp;
-    a.ovf_count = ctx->ovf_count;
     a.status = ctx->status;
     a.stream = ctx->stream;
     if (ctx->profiling && ctx->prof_used < ctx->prof_max) {
@@ -826,6 +821,21 @@ double pic_hip_plasma_frequency(const pic_domain* domain, int64_t num_particles)
     const double n0 = static_cast<double>(num_particles) / (domain->Lx * domain->Ly);
     const double q = 1.0 / static_cast<double>(num_particles);
     return std::sqrt(n0 * q * q);
+}
+
+pic_status pic_hip_copy_probe(pic_hip_ctx* ctx, size_t bytes) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    const size_t n = bytes / sizeof(float);
+    float *a = nullptr, *b = nullptr;
+    PIC_HIP_TRY(dmalloc(&a, n));
+    std::unique_ptr<float, hipError_t (*)(void*)> ga(a, hipFree);
+    PIC_HIP_TRY(dmalloc(&b, n));
+    std::unique_ptr<float, hipError_t (*)(void*)> gb(b, hipFree);
+    fill_async(a, n, 1.0f, ctx->stream);
+    launch_copy_f32(a, b, (int64_t)n, ctx->stream);
+    PIC_HIP_TRY(hipGetLastError());
+    PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PIC_OK;
 }
 
 // ---- drop-in seam -----------------------------------------------------------

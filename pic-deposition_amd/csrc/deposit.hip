@@ -311,7 +311,7 @@ struct TileCtx {
 template <int S, int Q, int EX>
 __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx& t, const DevDomain& d, float dt,
                                               float* __restrict__ overflow, float* __restrict__ comp,
-                                              int* __restrict__ ovf_count, int* __restrict__ status) {
+                                              int* __restrict__ status) {
     using St = Stencil<S>;
     constexpr int V = St::V;
     float acc[V], H[V];
@@ -322,7 +322,6 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
     }
     int hk = -1, ck = -1;
     bool multi = false;
-    bool any_ovf = false;
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
 #pragma unroll
@@ -349,7 +348,6 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             const bool inside = ok && (unsigned)lx < (unsigned)t.tw && (unsigned)ly < (unsigned)t.th;
             if (valid && !inside) {
                 overflow_scatter<S>(ok, cx, cy, w, xx, yy, vxx, vyy, qv, dt, d, overflow, comp, status);
-                any_ovf = true;
             }
             const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
             const bool nseg = lk != ck;
@@ -373,7 +371,6 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             }
         }
     }
-    if (__any(any_ovf) && __lane_id() == 0) atomicAdd(ovf_count, 1);
     // Combine the runs that straddle lanes.  Lane i's outgoing run is acc
     // with key ck; it continues lane i-1's outgoing run when hk == ck(i-1).
     const int ck_prev = dpp_i<kWaveShr1, 0xf>(-2, ck);
@@ -407,7 +404,7 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                                                            const int* __restrict__ cell_off, DevDomain d, TileGeom g,
                                                            float dt, float* __restrict__ partial,
                                                            float* __restrict__ overflow, float* __restrict__ comp,
-                                                           int* __restrict__ ovf_count, int* __restrict__ status) {
+                                                           int* __restrict__ status) {
     using St = Stencil<S>;
     constexpr int Q = K / 4;
     static_assert(K % 4 == 0, "K must be a multiple of 4 (float4 loads)");
@@ -476,13 +473,13 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
             if (b1 < total_q)
                 load_chunk<Q, St::kNeedsVelocity>(b1 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
                                                   pq, pvx, pvy, B);
-            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, ovf_count, status);
+            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, status);
             if (b1 >= total_q) break;
             const int b2 = b1 + step;
             if (b2 < total_q)
                 load_chunk<Q, St::kNeedsVelocity>(b2 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
                                                   pq, pvx, pvy, A);
-            process_chunk<S, Q, EX>(B, t, d, dt, overflow, comp, ovf_count, status);
+            process_chunk<S, Q, EX>(B, t, d, dt, overflow, comp, status);
             base = b2;
         }
     } else {
@@ -490,7 +487,7 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
             Chunk<Q> A;
             load_chunk<Q, St::kNeedsVelocity>(base + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq,
                                               pvx, pvy, A);
-            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, ovf_count, status);
+            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, status);
         }
     }
 
@@ -543,11 +540,7 @@ __device__ __forceinline__ int axis_pairs(int gn, int T, int nt, int n, int (&t)
 
 __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
                                                             float* __restrict__ overflow, float* __restrict__ comp,
-                                                            int* __restrict__ ovf, DevDomain d, TileGeom g,
-                                                            float* __restrict__ rho) {
-    __shared__ int s_ovf;
-    if (threadIdx.x == 0) s_ovf = __hip_atomic_load(ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
+                                                            DevDomain d, TileGeom g, float* __restrict__ rho) {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
     if (gx < d.Nx) {
@@ -596,26 +589,19 @@ __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restr
             }
         }
         float out = s + c;
+        // Overflow grid and its compensation are zero except where drifted
+        // particles were scattered; fold those nodes in and clear them.
         const size_t node = (size_t)gy * d.Nx + gx;
-        if (s_ovf > 0) {
-            out += overflow[node] + comp[node];
+        const float o = overflow[node], oc = comp[node];
+        if (o != 0.0f || oc != 0.0f) {
+            out += o + oc;
             overflow[node] = 0.0f;
             comp[node] = 0.0f;
         }
         rho[node] = out;
     }
-    // The last block to finish resets the overflow counter for the next
-    // deposit (every block has read it by then).
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int nblocks = (int)(gridDim.x * gridDim.y);
-        if (atomicAdd(ovf + 1, 1) == nblocks - 1) {
-            __hip_atomic_store(ovf, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(ovf + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 }
+
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, size_t n, float v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -643,10 +629,10 @@ void launch_tile(const DepositArgs& a) {
     if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
     hipLaunchKernelGGL((deposit_tile_kernel<S, K, EX, PIPE>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream,
                        a.x, a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.overflow, a.comp,
-                       a.ovf_count, a.status);
+                       a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
     hipLaunchKernelGGL(deposit_merge_kernel, dim3((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny), dim3(256), 0,
-                       a.stream, a.partial, a.overflow, a.comp, a.ovf_count, a.dom, g, a.rho);
+                       a.stream, a.partial, a.overflow, a.comp, a.dom, g, a.rho);
 }
 
 template <int S, int K>

@@ -38,7 +38,6 @@ struct DepositArgs {
     float* partial;   // ntiles * wsize
     float* overflow;  // [Ny][Nx], kept zero between calls
     float* comp;      // [Ny][Nx] compensation of the global atomic sums, kept zero between calls
-    int* ovf_count;   // [2]: overflow particle count, merge blocks done (both zero between calls)
     int* status;      // bit 0: Esirkepov move longer than the general walk supports
     hipStream_t stream;
     hipEvent_t ev_begin = nullptr;  // if set, recorded around the main scatter kernel

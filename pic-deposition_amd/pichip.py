@@ -156,6 +156,7 @@ _SIGS = {
     "pic_hip_dominant_frequency": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double]),
     "pic_hip_growth_rate": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int32]),
     "pic_hip_plasma_frequency": (ctypes.c_double, [ctypes.POINTER(Domain), ctypes.c_int64]),
+    "pic_hip_copy_probe": (ctypes.c_int, [_P, ctypes.c_size_t]),
     "pic_hip_profile_begin": (ctypes.c_int, [_P, ctypes.c_int]),
     "pic_hip_profile_end": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
 }
@@ -332,6 +333,9 @@ class Context:
 
     def gather_push(self, dt: float, gather: bool = True, push: bool = True) -> None:
         _check(_lib.pic_hip_gather_push(self._h, float(dt), int(gather), int(push)))
+
+    def copy_probe(self, nbytes: int) -> None:
+        _check(_lib.pic_hip_copy_probe(self._h, int(nbytes)))
 
     def profile_begin(self, max_launches: int) -> None:
         _check(_lib.pic_hip_profile_begin(self._h, int(max_launches)))
