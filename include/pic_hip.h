@@ -172,6 +172,18 @@ pic_status pic_hip_gather_push(pic_hip_ctx* ctx, double dt, int do_gather, int d
 /* FP64 sums over rho, Ex/Ey and the particles (synchronous). */
 pic_status pic_hip_diagnostics(pic_hip_ctx* ctx, pic_diagnostics* out);
 
+/* ---- charge-conserving current (build-defined; no reference counterpart) --
+ * Esirkepov (Comput. Phys. Commun. 135 (2001) 144) current of the move
+ * x -> x + v dt with CIC shape factors: Jx on x faces (i + 1/2, j), Jy on y
+ * faces (i, j + 1/2), plus the CIC densities at both ends, so that
+ * (rho_new - rho_old)/dt + Dx Jx + Dy Jy = 0 with backward differences.
+ * Moves must be shorter than one cell (PIC_ERR_UNSUPPORTED otherwise, at the
+ * next pic_hip_sync).  Any particle order. */
+pic_status pic_hip_esirkepov_current(pic_hip_ctx* ctx, double dt);
+pic_status pic_hip_get_current(pic_hip_ctx* ctx, float* Jx, float* Jy, float* rho_old, float* rho_new);
+/* FP64 max over cells of the continuity residual and of |d rho / dt|. */
+pic_status pic_hip_continuity_residual(pic_hip_ctx* ctx, double dt, double* max_residual, double* max_drho_dt);
+
 /* ---- timestep driver ------------------------------------------------------
  * Runs nsteps of the electrostatic loop on the device.  Step orders follow
  * the reference loops:

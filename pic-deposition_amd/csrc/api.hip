@@ -110,6 +110,7 @@ struct pic_hip_ctx {
     int diag_blocks = 0;
     int* h_status = nullptr;     // pinned
     double* h_diag = nullptr;    // pinned
+    CurrentGrids cur{};           // Esirkepov current buffers (allocated on first use)
     double* cosk = nullptr;      // cos/sin(2 pi (i + 1/2) dx / Lx), mode-1 diagnostic tables
     double* sink = nullptr;
     int* step_counter = nullptr;
@@ -152,6 +153,7 @@ struct pic_hip_ctx {
         (void)hipFree(ptab.dy_sym);
         (void)hipFree(diag_partials);
         (void)hipFree(diag_out);
+        (void)hipFree(cur.Jx);  // one allocation holds all eight grids
         (void)hipFree(cosk);
         (void)hipFree(sink);
         (void)hipFree(step_counter);
@@ -327,6 +329,8 @@ pic_status pic_hip_sync(pic_hip_ctx* ctx) {
         if (st & 1)
             return fail(PIC_ERR_UNSUPPORTED,
                         "Esirkepov deposit: a particle moved more than 15 cells in one step (not supported)");
+        if (st & 2)
+            return fail(PIC_ERR_UNSUPPORTED, "Esirkepov current: a particle moved one cell or more in one step");
         return fail(PIC_ERR_HIP, "device status flag set");
     }
     return PIC_OK;
@@ -821,6 +825,50 @@ double pic_hip_plasma_frequency(const pic_domain* domain, int64_t num_particles)
     const double n0 = static_cast<double>(num_particles) / (domain->Lx * domain->Ly);
     const double q = 1.0 / static_cast<double>(num_particles);
     return std::sqrt(n0 * q * q);
+}
+
+pic_status pic_hip_esirkepov_current(pic_hip_ctx* ctx, double dt) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!std::isfinite(dt) || dt <= 0.0) return fail(PIC_ERR_INVALID_ARGUMENT, "dt must be finite and positive");
+    if (!ctx->cur.Jx) {
+        float* base = nullptr;
+        PIC_HIP_TRY(dmalloc(&base, (size_t)8 * ctx->G));
+        PIC_HIP_TRY(hipMemsetAsync(base, 0, (size_t)8 * ctx->G * sizeof(float), ctx->stream));
+        float** f[8] = {&ctx->cur.Jx, &ctx->cur.Jy, &ctx->cur.rho_old, &ctx->cur.rho_new,
+                        &ctx->cur.Jx_c, &ctx->cur.Jy_c, &ctx->cur.rho_old_c, &ctx->cur.rho_new_c};
+        for (int k = 0; k < 8; ++k) *f[k] = base + (size_t)k * ctx->G;
+    }
+    launch_esirkepov_current(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], (int64_t)ctx->n, ctx->dev,
+                             (float)dt, ctx->cur, ctx->status, ctx->stream);
+    PIC_HIP_TRY(hipGetLastError());
+    return PIC_OK;
+}
+
+pic_status pic_hip_get_current(pic_hip_ctx* ctx, float* Jx, float* Jy, float* rho_old, float* rho_new) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!ctx->cur.Jx) return fail(PIC_ERR_INVALID_ARGUMENT, "no current computed yet (call pic_hip_esirkepov_current)");
+    const size_t bytes = (size_t)ctx->G * sizeof(float);
+    float* dst[4] = {Jx, Jy, rho_old, rho_new};
+    float* src[4] = {ctx->cur.Jx, ctx->cur.Jy, ctx->cur.rho_old, ctx->cur.rho_new};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) PIC_HIP_TRY(hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return pic_hip_sync(ctx);
+}
+
+pic_status pic_hip_continuity_residual(pic_hip_ctx* ctx, double dt, double* max_residual, double* max_drho_dt) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!ctx->cur.Jx) return fail(PIC_ERR_INVALID_ARGUMENT, "no current computed yet (call pic_hip_esirkepov_current)");
+    const int nb = 512;
+    double* part = nullptr;
+    PIC_HIP_TRY(dmalloc(&part, (size_t)2 * nb + 2));
+    std::unique_ptr<double, hipError_t (*)(void*)> guard(part, hipFree);
+    launch_continuity_residual(ctx->cur, ctx->dev, dt, part, nb, part + 2 * nb, ctx->stream);
+    PIC_HIP_TRY(hipMemcpyAsync(ctx->h_diag, part + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    const pic_status st = pic_hip_sync(ctx);
+    if (st != PIC_OK) return st;
+    if (max_residual) *max_residual = ctx->h_diag[0];
+    if (max_drho_dt) *max_drho_dt = ctx->h_diag[1];
+    return PIC_OK;
 }
 
 pic_status pic_hip_copy_probe(pic_hip_ctx* ctx, size_t bytes) {

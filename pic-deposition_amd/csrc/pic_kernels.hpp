@@ -132,6 +132,20 @@ void launch_step_diag(const float* rho, const float* Ex, const float* Ey, const 
                       double* partials, int nblocks, int* counter, int record_from, int cap, const StepRecords& rec,
                       hipStream_t s);
 void launch_set_int(int* p, int v, hipStream_t s);
+
+// ---- current.hip -------------------------------------------------------------
+// Charge-conserving current and the CIC densities at both ends of the move;
+// the *_c grids are TwoSum compensation terms (zero between calls).
+struct CurrentGrids {
+    float *Jx, *Jy, *rho_old, *rho_new;
+    float *Jx_c, *Jy_c, *rho_old_c, *rho_new_c;
+};
+void launch_esirkepov_current(const float* x, const float* y, const float* vx, const float* vy, const float* q,
+                              int64_t n, const DevDomain& d, float dt, const CurrentGrids& g, int* status,
+                              hipStream_t s);
+// out[0] = max |(rho_new - rho_old)/dt + Dx Jx + Dy Jy|, out[1] = max |(rho_new - rho_old)/dt| (FP64).
+void launch_continuity_residual(const CurrentGrids& g, const DevDomain& d, double dt, double* partial, int nblocks,
+                                double* out, hipStream_t s);
 void launch_copy_f32(const float* src, float* dst, int64_t n, hipStream_t s);
 
 }  // namespace pichip

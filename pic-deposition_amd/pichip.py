@@ -157,6 +157,9 @@ _SIGS = {
     "pic_hip_growth_rate": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int32]),
     "pic_hip_plasma_frequency": (ctypes.c_double, [ctypes.POINTER(Domain), ctypes.c_int64]),
     "pic_hip_copy_probe": (ctypes.c_int, [_P, ctypes.c_size_t]),
+    "pic_hip_esirkepov_current": (ctypes.c_int, [_P, ctypes.c_double]),
+    "pic_hip_get_current": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p]),
+    "pic_hip_continuity_residual": (ctypes.c_int, [_P, ctypes.c_double, _f64p, _f64p]),
     "pic_hip_profile_begin": (ctypes.c_int, [_P, ctypes.c_int]),
     "pic_hip_profile_end": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
 }
@@ -333,6 +336,21 @@ class Context:
 
     def gather_push(self, dt: float, gather: bool = True, push: bool = True) -> None:
         _check(_lib.pic_hip_gather_push(self._h, float(dt), int(gather), int(push)))
+
+    def esirkepov_current(self, dt: float) -> None:
+        _check(_lib.pic_hip_esirkepov_current(self._h, float(dt)))
+
+    def current(self):
+        """(Jx, Jy, rho_old, rho_new) as [Ny, Nx] FP32 arrays."""
+        G = self.domain.Nx * self.domain.Ny
+        out = [np.empty(G, dtype=np.float32) for _ in range(4)]
+        _check(_lib.pic_hip_get_current(self._h, *[_ptr(a, ctypes.c_float) for a in out]))
+        return tuple(a.reshape(self.domain.Ny, self.domain.Nx) for a in out)
+
+    def continuity_residual(self, dt: float):
+        r, m = np.zeros(1), np.zeros(1)
+        _check(_lib.pic_hip_continuity_residual(self._h, float(dt), _ptr(r, ctypes.c_double), _ptr(m, ctypes.c_double)))
+        return float(r[0]), float(m[0])
 
     def copy_probe(self, nbytes: int) -> None:
         _check(_lib.pic_hip_copy_probe(self._h, int(nbytes)))
