@@ -243,6 +243,97 @@ pic_status pic_hip_profile_end(pic_hip_ctx* ctx, double* total_ms, int* launches
  * calibrating hardware byte counters.  Synchronous. */
 pic_status pic_hip_copy_probe(pic_hip_ctx* ctx, size_t bytes);
 
+/* ---- tiling autotuner (build-defined; no reference counterpart) ----------
+ * Times every LDS tiling candidate (tile shapes from 16x8 to 128x4 cells,
+ * 4 or 8 particles per lane) for `scheme` on the context's current, sorted
+ * particles (tiled kernel + merge, HIP events, median of `reps` deposits) and
+ * keeps the fastest for that scheme.  With a cache_path, a JSON file keyed by
+ * device architecture, scheme, grid and particles per cell (rounded to a power
+ * of two) is consulted first and updated after a measurement; NULL disables
+ * the cache.  Outputs (any may be NULL): the chosen tiling and its time per
+ * deposit (0 when it came from the cache). */
+pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path, int reps, int* tile_x,
+                            int* tile_y, int* particles_per_lane, double* ms_per_deposit);
+
+/* ---- batched ensemble (build-defined; no reference counterpart) ----------
+ * Many independent periodic PIC instances ("members") of uneven grid size,
+ * particle count, scheme and seed, held in one set of device buffers:
+ *   - particles of all members in shared FP32 SoA streams, member by member,
+ *     each member's range padded to a multiple of 4096 slots;
+ *   - one cell sort over all members (keys offset per member, so members
+ *     stay contiguous);
+ *   - the tiled deposit of every member in persistent kernels (one per
+ *     scheme / particles-per-lane / coordinate-path group) that pull
+ *     (member, tile) work items from a device counter, largest tiles first,
+ *     then one merge over all members' nodes;
+ *   - per-member FP64 mean removal and batched hipFFT plans (one per grid
+ *     size) for the spectral Poisson solve, fused gather + push.
+ * pic_hip_ensemble_run captures a step in hipGraphs and replays them.  Each
+ * member's results equal those of a pic_hip_ctx with the same particles and
+ * tiling (deposit: bit for bit). */
+typedef struct {
+    pic_domain domain;
+    int32_t scheme;        /* PIC_SCHEME_*: the member's deposit scheme */
+    int64_t num_particles;
+    pic_init_params init;  /* used by pic_hip_ensemble_init */
+} pic_ensemble_member;
+
+typedef struct pic_hip_ensemble pic_hip_ensemble;
+
+pic_status pic_hip_ensemble_create(const pic_ensemble_member* members, int32_t count, pic_hip_ensemble** out);
+pic_status pic_hip_ensemble_destroy(pic_hip_ensemble* ens);
+int32_t pic_hip_ensemble_count(pic_hip_ensemble* ens);
+/* Waits for the ensemble's stream; reports device-side errors as pic_hip_sync. */
+pic_status pic_hip_ensemble_sync(pic_hip_ensemble* ens);
+/* Initialises every member's particles on the device from its init params. */
+pic_status pic_hip_ensemble_init(pic_hip_ensemble* ens);
+/* Copies one member's particles in / out (num_particles FP32 values per
+ * stream, synchronous; NULL vx/vy upload zeros, NULL m uploads ones, NULL
+ * outputs are skipped). */
+pic_status pic_hip_ensemble_upload(pic_hip_ensemble* ens, int32_t member, const float* x, const float* y,
+                                   const float* vx, const float* vy, const float* q, const float* m);
+pic_status pic_hip_ensemble_download(pic_hip_ensemble* ens, int32_t member, float* x, float* y, float* vx, float* vy,
+                                     float* q, float* m);
+/* Cell sort of all members. */
+pic_status pic_hip_ensemble_sort(pic_hip_ensemble* ens);
+/* Tiled deposit of every member with its own scheme (needs a sort). */
+pic_status pic_hip_ensemble_deposit(pic_hip_ensemble* ens, double dt);
+pic_status pic_hip_ensemble_get_rho(pic_hip_ensemble* ens, int32_t member, float* rho);
+/* Ex, Ey of every member from its rho (k = 0 removed per member in FP64). */
+pic_status pic_hip_ensemble_poisson(pic_hip_ensemble* ens);
+pic_status pic_hip_ensemble_get_fields(pic_hip_ensemble* ens, int32_t member, float* Ex, float* Ey);
+/* Tiling of one member (member -1: all members). */
+pic_status pic_hip_ensemble_set_tiling(pic_hip_ensemble* ens, int32_t member, int tile_x, int tile_y,
+                                       int particles_per_lane);
+pic_status pic_hip_ensemble_get_tiling(pic_hip_ensemble* ens, int32_t member, int* tile_x, int* tile_y,
+                                       int* particles_per_lane);
+/* Autotunes the members by class (grid, scheme, particles per cell rounded
+ * to a power of two): each class's candidates are timed with only that
+ * class's work items in the persistent kernels.  Needs a sort.  cache_path
+ * as pic_hip_autotune. */
+pic_status pic_hip_ensemble_autotune(pic_hip_ensemble* ens, const char* cache_path, int reps);
+
+/* Step loop of every member (PIC_ORDER_PUSH_FIRST: push, [sort], deposit,
+ * Poisson, gather), no background (the solve removes each member's mean).
+ * Records per member for steps >= record_from, laid out
+ * [member][nsteps - record_from]: field energy, |mode-1 amplitude of Ex|,
+ * kinetic energy, charge error (as pic_hip_run with record_particles).
+ * Sort before the deposit of every sort_interval-th step (first included). */
+typedef struct {
+    double dt;
+    int32_t sort_interval;
+    int32_t use_graph;
+} pic_ensemble_run_config;
+
+pic_status pic_hip_ensemble_run(pic_hip_ensemble* ens, const pic_ensemble_run_config* cfg, int32_t nsteps,
+                                int32_t record_from, double* field_energy, double* mode1_amplitude,
+                                double* kinetic_energy, double* charge_error);
+/* Mean device time (HIP events on the ensemble's stream, `reps` runs after
+ * one warm-up) of: what = 0 the deposit (persistent kernels + merge),
+ * 1 the sort, 2 one step replayed from its hipGraph without sort, 3 the same
+ * with sort.  Particles are left as the runs leave them. */
+pic_status pic_hip_ensemble_time(pic_hip_ensemble* ens, int what, int reps, double dt, double* mean_ms);
+
 #ifdef __cplusplus
 }
 #endif

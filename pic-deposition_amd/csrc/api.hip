@@ -18,59 +18,11 @@
 #include "../../include/pic_hip.h"
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
+#include "api_util.hpp"
 
 using namespace pichip;
 
 namespace {
-
-thread_local std::string g_last_error;
-
-pic_status fail(pic_status s, const std::string& msg) {
-    g_last_error = msg;
-    return s;
-}
-
-#define PIC_HIP_TRY(call)                                                                                   \
-    do {                                                                                                    \
-        hipError_t e_ = (call);                                                                             \
-        if (e_ != hipSuccess) {                                                                             \
-            return fail(e_ == hipErrorOutOfMemory ? PIC_ERR_OUT_OF_MEMORY : PIC_ERR_HIP,                     \
-                        std::string("HIP error: ") + hipGetErrorString(e_) + " (" #call ")");                \
-        }                                                                                                   \
-    } while (0)
-
-#define PIC_FFT_TRY(call)                                                                        \
-    do {                                                                                         \
-        hipfftResult r_ = (call);                                                                \
-        if (r_ != HIPFFT_SUCCESS)                                                                \
-            return fail(PIC_ERR_FFT, std::string("hipFFT error ") + std::to_string((int)r_) + " (" #call ")"); \
-    } while (0)
-
-constexpr int kStreams = 6;  // x, y, vx, vy, q, m
-
-size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-bool valid_domain(const pic_domain* d) {
-    return d && d->Nx > 0 && d->Ny > 0 && std::isfinite(d->Lx) && std::isfinite(d->Ly) && d->Lx > 0.0 &&
-           d->Ly > 0.0 && (int64_t)d->Nx * d->Ny < (1ll << 30);
-}
-
-DevDomain make_dev_domain(const pic_domain& d) {
-    DevDomain dd;
-    dd.Nx = d.Nx;
-    dd.Ny = d.Ny;
-    dd.Lx = d.Lx;
-    dd.Ly = d.Ly;
-    dd.dx = d.Lx / d.Nx;  // Domain::updateDerived, domain.hpp:17-20
-    dd.dy = d.Ly / d.Ny;
-    dd.inv_dx = (float)(1.0 / dd.dx);
-    dd.inv_dy = (float)(1.0 / dd.dy);
-    dd.inv_vol = (float)(1.0 / (dd.dx * dd.dy));
-    int ex = 0, ey = 0;
-    dd.exact = (std::frexp(dd.dx, &ex) == 0.5 && std::frexp(dd.dy, &ey) == 0.5) ? 1 : 0;
-    return dd;
-}
-
 }  // namespace
 
 struct pic_hip_ctx {
@@ -166,12 +118,6 @@ struct pic_hip_ctx {
 
 namespace {
 
-template <typename T>
-hipError_t dmalloc(T** p, size_t elems) {
-    *p = nullptr;
-    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(elems, 1) * sizeof(T));
-}
-
 pic_status ensure_partial(pic_hip_ctx* c) {
     size_t need = 0;
     for (int k = 0; k < 4; ++k) need = std::max(need, (size_t)c->geom[k].ntiles * c->geom[k].wsize);
@@ -190,7 +136,7 @@ pic_status apply_tiling(pic_hip_ctx* c, int scheme, int tx, int ty, int ppl, int
     if (ppl != 4 && ppl != 8) return fail(PIC_ERR_INVALID_ARGUMENT, "particles_per_lane must be 4 or 8");
     TileGeom g = make_tile_geom(c->dom.Nx, c->dom.Ny, tx, ty);
     const size_t lds = (size_t)4 * g.lds_per_wave * sizeof(float);
-    if (lds > 64 * 1024) return fail(PIC_ERR_INVALID_ARGUMENT, "tile too large for LDS (4 windows must fit 64 KiB)");
+    if (lds > kMaxDepositLds) return fail(PIC_ERR_INVALID_ARGUMENT, "tile too large for LDS (4 windows must fit 64 KiB)");
     for (int k = 0; k < 4; ++k) {
         if (scheme >= 0 && k != scheme) continue;
         c->geom[k] = g;
@@ -252,7 +198,7 @@ pic_status check_ctx(pic_hip_ctx* c) {
 
 extern "C" {
 
-const char* pic_hip_last_error(void) { return g_last_error.c_str(); }
+const char* pic_hip_last_error(void) { return last_error_slot().c_str(); }
 const char* pic_hip_version(void) { return "pic_hip 0.1 gfx950"; }
 
 pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx** out) {
@@ -299,11 +245,11 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(int) * 4, hipHostMallocDefault));
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_diag), sizeof(double) * 8, hipHostMallocDefault));
     // Defaults from the MI355X tiling sweep at 1024^2 / 64 ppc.
-    pic_status st = apply_tiling(c.get(), PIC_SCHEME_NGP, 64, 4, 8, 0);
-    if (st == PIC_OK) st = apply_tiling(c.get(), PIC_SCHEME_CIC, 64, 4, 8, 0);
-    if (st == PIC_OK) st = apply_tiling(c.get(), PIC_SCHEME_TSC, 32, 4, 8, 0);
-    if (st == PIC_OK) st = apply_tiling(c.get(), PIC_SCHEME_ESIRKEPOV, 32, 4, 4, 0);
-    if (st != PIC_OK) return st;
+    for (int k = 0; k < 4; ++k) {
+        const pic_status st = apply_tiling(c.get(), k, kDefaultTiling[k][0], kDefaultTiling[k][1],
+                                           kDefaultTiling[k][2], 0);
+        if (st != PIC_OK) return st;
+    }
     PIC_HIP_TRY(hipStreamSynchronize(c->stream));
     *out = c.release();
     return PIC_OK;
@@ -415,13 +361,19 @@ namespace {
 // pointers fixed (needed inside captured graphs, which bake pointers in).
 pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
     PIC_HIP_TRY(launch_sort_keys_perm(ctx->s[0], ctx->s[1], (int64_t)ctx->n, ctx->dev, ctx->sb, ctx->stream));
+    StreamSet fwd{}, back{};
+    fwd.count = back.count = kStreams;
     for (int i = 0; i < kStreams; ++i) {
-        launch_gather_f32(ctx->s[i], ctx->alt[i], ctx->sb.perm, (int64_t)ctx->n, ctx->stream);
-        if (copy_back)
-            launch_copy_f32(ctx->alt[i], ctx->s[i], (int64_t)ctx->n, ctx->stream);
-        else
-            std::swap(ctx->s[i], ctx->alt[i]);
+        fwd.src[i] = ctx->s[i];
+        fwd.dst[i] = ctx->alt[i];
+        back.src[i] = ctx->alt[i];
+        back.dst[i] = ctx->s[i];
     }
+    launch_permute_streams(fwd, ctx->sb.perm, (int64_t)ctx->n, ctx->stream);
+    if (copy_back)
+        launch_permute_streams(back, nullptr, (int64_t)ctx->n, ctx->stream);
+    else
+        for (int i = 0; i < kStreams; ++i) std::swap(ctx->s[i], ctx->alt[i]);
     PIC_HIP_TRY(hipGetLastError());
     ctx->order_valid = true;
     return PIC_OK;
@@ -869,6 +821,59 @@ pic_status pic_hip_continuity_residual(pic_hip_ctx* ctx, double dt, double* max_
     if (max_residual) *max_residual = ctx->h_diag[0];
     if (max_drho_dt) *max_drho_dt = ctx->h_diag[1];
     return PIC_OK;
+}
+
+pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path, int reps, int* tile_x,
+                            int* tile_y, int* particles_per_lane, double* ms_per_deposit) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (scheme < 0 || scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "scheme must be 0..3");
+    if (!ctx->order_valid) return fail(PIC_ERR_NOT_SORTED, "autotune times the tiled deposit: call pic_hip_sort first");
+    if (reps < 1) reps = 5;
+    const std::string key = tuning_key(scheme, ctx->dom.Nx, ctx->dom.Ny, ppc_bucket((int64_t)ctx->n, ctx->G));
+    TilingChoice best{ctx->geom[scheme].T, ctx->geom[scheme].R, ctx->ppl[scheme]};
+    double best_ms = 0.0;
+    const auto cached = tuning_cache_read(cache_path);
+    auto hit = cached.find(key);
+    if (hit != cached.end() && apply_tiling(ctx, scheme, hit->second.tx, hit->second.ty, hit->second.ppl, 0) == PIC_OK) {
+        best = hit->second;
+    } else {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        PIC_HIP_TRY(hipEventCreate(&e0));
+        std::unique_ptr<ihipEvent_t, hipError_t (*)(hipEvent_t)> g0(e0, hipEventDestroy);
+        PIC_HIP_TRY(hipEventCreate(&e1));
+        std::unique_ptr<ihipEvent_t, hipError_t (*)(hipEvent_t)> g1(e1, hipEventDestroy);
+        best_ms = -1.0;
+        for (const TilingChoice& c : tiling_candidates(ctx->dom.Nx, ctx->dom.Ny, false)) {
+            if (apply_tiling(ctx, scheme, c.tx, c.ty, c.ppl, 0) != PIC_OK) continue;
+            pic_status st = pic_hip_deposit(ctx, scheme, PIC_BACKEND_GPU_PRIV, 0.0);  // warm-up
+            if (st != PIC_OK) return st;
+            std::vector<double> t;
+            for (int r = 0; r < reps; ++r) {
+                PIC_HIP_TRY(hipEventRecord(e0, ctx->stream));
+                if ((st = pic_hip_deposit(ctx, scheme, PIC_BACKEND_GPU_PRIV, 0.0)) != PIC_OK) return st;
+                PIC_HIP_TRY(hipEventRecord(e1, ctx->stream));
+                PIC_HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                PIC_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                t.push_back(ms);
+            }
+            const double m = median_of(t);
+            if (best_ms < 0.0 || m < best_ms) {
+                best_ms = m;
+                best = c;
+            }
+        }
+        if (best_ms < 0.0) return fail(PIC_ERR_INVALID_ARGUMENT, "no tiling candidate fits this grid");
+        if (!tuning_cache_store(cache_path, key, best))
+            return fail(PIC_ERR_INVALID_ARGUMENT, std::string("cannot write tuning cache ") + cache_path);
+    }
+    pic_status st = apply_tiling(ctx, scheme, best.tx, best.ty, best.ppl, 0);
+    if (st != PIC_OK) return st;
+    if (tile_x) *tile_x = best.tx;
+    if (tile_y) *tile_y = best.ty;
+    if (particles_per_lane) *particles_per_lane = best.ppl;
+    if (ms_per_deposit) *ms_per_deposit = best_ms;
+    return pic_hip_sync(ctx);
 }
 
 pic_status pic_hip_copy_probe(pic_hip_ctx* ctx, size_t bytes) {

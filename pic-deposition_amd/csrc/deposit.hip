@@ -21,184 +21,11 @@
 // The reference's own privatised kernel (deposition_cuda.cu:212-241) scans all
 // particles per tile; this design reads each particle once.
 
-#include "pic_common.hpp"
-#include "pic_kernels.hpp"
+#include "deposit_device.hpp"
 
 namespace pichip {
 
 namespace {
-
-// LDS visibility between lanes of one wavefront (in-order LDS queue + no
-// compiler reordering across this point).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// DPP controls (GFX9 encoding).
-constexpr int kRowShr = 0x110;
-constexpr int kRowBcast15 = 0x142;
-constexpr int kRowBcast31 = 0x143;
-constexpr int kWaveShr1 = 0x138;
-constexpr int kWaveShl1 = 0x130;
-
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_f(float old, float v) {
-    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
-}
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ int dpp_i(int old, int v) {
-    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
-}
-
-// One Hillis-Steele step of the linear recurrence B_i = b_i + a_i * B_{i-1}.
-template <int CTRL, int ROW_MASK, int V>
-__device__ __forceinline__ void lin_scan_step(float (&B)[V], float& A) {
-    const float As = dpp_f<CTRL, ROW_MASK>(1.0f, A);
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        const float Bs = dpp_f<CTRL, ROW_MASK>(0.0f, B[k]);
-        B[k] = fmaf(A, Bs, B[k]);
-    }
-    A *= As;
-}
-
-// Inclusive wave-wide scan of B_i = b_i + a_i * B_{i-1} with a_i in {0, 1}:
-// a segmented sum where a_i = 0 starts a new segment at lane i.
-template <int V>
-__device__ __forceinline__ void wave_segmented_scan(float (&B)[V], float A) {
-    lin_scan_step<kRowShr + 1, 0xf>(B, A);
-    lin_scan_step<kRowShr + 2, 0xf>(B, A);
-    lin_scan_step<kRowShr + 4, 0xf>(B, A);
-    lin_scan_step<kRowShr + 8, 0xf>(B, A);
-    lin_scan_step<kRowBcast15, 0xa>(B, A);
-    lin_scan_step<kRowBcast31, 0xc>(B, A);
-}
-
-// Compensated add of a cell's V stencil values into the wave's LDS window:
-// all V returning adds are issued first, then the TwoSum errors go to the
-// compensation window.  (Adding 0 leaves both windows unchanged.)
-template <int SX, int V>
-__device__ __forceinline__ void emit_cell(float* __restrict__ ws, float* __restrict__ wc, int wbase, int WX,
-                                          const float (&z)[V]) {
-    float old[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) old[k] = atomicAdd(ws + wbase + (k / SX) * WX + (k % SX), z[k]);
-#pragma unroll
-    for (int k = 0; k < V; ++k) atomicAdd(wc + wbase + (k / SX) * WX + (k % SX), two_sum_err(old[k], z[k]));
-}
-
-// General reference-Esirkepov walk (esirkepovWeights1DSparse,
-// src/deposition/ngp.cpp:36-84) for moves of any length, measured in cells
-// from the home cell's lower face.  Returns the number of support cells
-// (<= kEskMaxCells) or -1 if the move is longer.
-constexpr int kEskMaxCells = 16;
-__device__ __noinline__ int esk_walk_1d(float x0, float v, float dt, float inv_d, double dd, int exact, int* cell,
-                                        float* w) {
-    int c0;
-    float fr0;
-    cell_coord(x0, inv_d, dd, exact, c0, fr0);
-    const float disp = v * dt;
-    if (fabsf(disp) < 1e-15f) {
-        cell[0] = c0;
-        w[0] = 1.0f - fr0;
-        cell[1] = c0 + 1;
-        w[1] = fr0;
-        return 2;
-    }
-    const float end = fr0 + cells(disp, inv_d, dd, exact);  // relative to the home cell's lower face
-    int n = 0;
-    float total = 0.0f;
-    float cur = fr0;
-    int c = 0;
-    if (disp > 0.0f) {
-        while (cur < end) {
-            if (n == kEskMaxCells) return -1;
-            const float nxt = fminf(end, (float)(c + 1));
-            cell[n] = c0 + c;
-            w[n] = nxt - cur;
-            total += w[n];
-            ++n;
-            cur = nxt;
-            ++c;
-        }
-    } else {
-        while (cur > end) {
-            if (n == kEskMaxCells) return -1;
-            const float nxt = fmaxf(end, (float)c);
-            cell[n] = c0 + c;
-            w[n] = cur - nxt;
-            total += w[n];
-            ++n;
-            cur = nxt;
-            --c;
-        }
-    }
-    if (n == 0) {  // displacement below FP32 resolution: all weight on the home cell
-        cell[0] = c0;
-        w[0] = 1.0f;
-        return 1;
-    }
-    if (total > 1e-30f) {
-        const float inv = 1.0f / total;
-        for (int i = 0; i < n; ++i) w[i] *= inv;
-    }
-    return n;
-}
-
-// Compensated global add: grid + comp == exact running sum (TwoSum of each
-// atomic's old value and the addend).
-__device__ __forceinline__ void comp_atomic_add(float* __restrict__ grid, float* __restrict__ comp, size_t i, float v) {
-    const float old = unsafeAtomicAdd(grid + i, v);
-    unsafeAtomicAdd(comp + i, two_sum_err(old, v));
-}
-
-// Rare path of scatter_global: Esirkepov moves longer than one cell.
-__device__ __noinline__ void scatter_esk_long(float x, float y, float vx, float vy, float qv, float dt,
-                                              const DevDomain& d, float* __restrict__ grid,
-                                              float* __restrict__ comp, int* __restrict__ status) {
-    int cxs[kEskMaxCells], cys[kEskMaxCells];
-    float wxs[kEskMaxCells], wys[kEskMaxCells];
-    const int nx = esk_walk_1d(x, vx, dt, d.inv_dx, d.dx, d.exact, cxs, wxs);
-    const int ny = esk_walk_1d(y, vy, dt, d.inv_dy, d.dy, d.exact, cys, wys);
-    if (nx < 0 || ny < 0) {
-        atomicOr(status, 1);
-        return;
-    }
-    for (int i = 0; i < nx; ++i) {
-        const int gx = wrap_index(cxs[i], d.Nx);
-        for (int j = 0; j < ny; ++j) {
-            const int gy = wrap_index(cys[j], d.Ny);
-            comp_atomic_add(grid, comp, (size_t)gy * d.Nx + gx, qv * wxs[i] * wys[j]);
-        }
-    }
-}
-
-// Scatter one particle with compensated global FP32 atomics (used by
-// GPU_Atomics and by the tile kernel's overflow path).
-template <int S>
-__device__ __forceinline__ void scatter_global(float x, float y, float vx, float vy, float qv, float dt,
-                                               const DevDomain& d, float* __restrict__ grid,
-                                               float* __restrict__ comp, int* __restrict__ status) {
-    using St = Stencil<S>;
-    float w[St::V];
-    int cx, cy;
-    const bool ok = St::weights(x, y, vx, vy, qv, dt, d, cx, cy, w);
-    if (ok) {
-#pragma unroll
-        for (int k = 0; k < St::V; ++k) {
-            if (w[k] != 0.0f) {
-                const int gx = wrap_index(cx + St::OX + k % St::SX, d.Nx);
-                const int gy = wrap_index(cy + St::OY + k / St::SX, d.Ny);
-                comp_atomic_add(grid, comp, (size_t)gy * d.Nx + gx, w[k]);
-            }
-        }
-        return;
-    }
-    // Only the Esirkepov density can fail the fixed stencil (long moves).
-    scatter_esk_long(x, y, vx, vy, qv, dt, d, grid, comp, status);
-}
 
 template <int S>
 __global__ __launch_bounds__(256) void deposit_atomic_kernel(const float* __restrict__ px, const float* __restrict__ py,
@@ -223,179 +50,6 @@ __global__ __launch_bounds__(256) void fold_comp_kernel(float* __restrict__ rho,
     }
 }
 
-// Q consecutive particle quads of one lane: physical start index of each quad
-// and the valid particle range [lo, hi) of the row it belongs to.
-template <int Q>
-struct Chunk {
-    float4 X[Q], Y[Q], QQ[Q], VX[Q], VY[Q];
-    int p0[Q], lo[Q], hi[Q];
-};
-
-// Maps the lane's virtual quads vq0 .. vq0+Q-1 (rows of the tile laid end to
-// end) to physical quads with the wave's row table and issues the loads.
-template <int Q, bool VEL>
-__device__ __forceinline__ void load_chunk(int vq0, int total_q, int th, const int* __restrict__ t_qcum,
-                                           const int* __restrict__ t_qstart, const int* __restrict__ t_rs,
-                                           const int* __restrict__ t_re, const float* __restrict__ px,
-                                           const float* __restrict__ py, const float* __restrict__ pq,
-                                           const float* __restrict__ pvx, const float* __restrict__ pvy,
-                                           Chunk<Q>& c) {
-    int r = 0;
-    while (r + 1 < th && vq0 >= t_qcum[r + 1]) ++r;
-    int next = t_qcum[r + 1];
-    int delta = t_qstart[r] - t_qcum[r];
-    int lo = t_rs[r], hi = t_re[r];
-#pragma unroll
-    for (int j = 0; j < Q; ++j) {
-        const int vq = vq0 + j;
-        if (vq < total_q) {
-            if (vq >= next) {  // lane straddles a row boundary (rare)
-                while (vq >= t_qcum[r + 1]) ++r;
-                next = t_qcum[r + 1];
-                delta = t_qstart[r] - t_qcum[r];
-                lo = t_rs[r];
-                hi = t_re[r];
-            }
-            const int p0 = (vq + delta) * 4;
-            c.p0[j] = p0;
-            c.lo[j] = lo;
-            c.hi[j] = hi;
-            c.X[j] = *reinterpret_cast<const float4*>(px + p0);
-            c.Y[j] = *reinterpret_cast<const float4*>(py + p0);
-            c.QQ[j] = *reinterpret_cast<const float4*>(pq + p0);
-            if (VEL) {
-                c.VX[j] = *reinterpret_cast<const float4*>(pvx + p0);
-                c.VY[j] = *reinterpret_cast<const float4*>(pvy + p0);
-            }
-        } else {
-            c.p0[j] = 0;
-            c.lo[j] = 1;  // empty range: no particle of this quad is valid
-            c.hi[j] = 0;
-        }
-    }
-}
-
-// A particle whose current cell is outside its tile (drift since the last
-// sort): its already computed stencil goes to the overflow grid with
-// compensated global atomics.  `ok` false (an Esirkepov move longer than a
-// cell) takes the general walk.
-template <int S>
-__device__ __forceinline__ void overflow_scatter(bool ok, int cx, int cy, const float (&w)[Stencil<S>::V], float x,
-                                                 float y, float vx, float vy, float qv, float dt, const DevDomain& d,
-                                                 float* __restrict__ overflow, float* __restrict__ comp,
-                                                 int* __restrict__ status) {
-    using St = Stencil<S>;
-    if (ok) {
-#pragma unroll
-        for (int k = 0; k < St::V; ++k) {
-            const int gx = wrap_index(cx + St::OX + k % St::SX, d.Nx);
-            const int gy = wrap_index(cy + St::OY + k / St::SX, d.Ny);
-            comp_atomic_add(overflow, comp, (size_t)gy * d.Nx + gx, w[k]);
-        }
-    } else {
-        scatter_esk_long(x, y, vx, vy, qv, dt, d, overflow, comp, status);
-    }
-}
-
-// Per-wave state of the tiled deposit (one tile per wavefront).
-struct TileCtx {
-    float* win_s;
-    float* win_c;
-    int WX;
-    int x0, y0, tw, th;
-};
-
-// Processes one chunk: K consecutive particles per lane accumulated as runs of
-// equal cell in registers, then one segmented wave scan, then one
-// compensated LDS add per (cell, stencil slot).
-template <int S, int Q, int EX>
-__device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx& t, const DevDomain& d, float dt,
-                                              float* __restrict__ overflow, float* __restrict__ comp,
-                                              int* __restrict__ status) {
-    using St = Stencil<S>;
-    constexpr int V = St::V;
-    float acc[V], H[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        acc[k] = 0.0f;
-        H[k] = 0.0f;
-    }
-    int hk = -1, ck = -1;
-    bool multi = false;
-#pragma unroll
-    for (int j = 0; j < Q; ++j) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int p = cur.p0[j] + e;
-            // Particles of the quad outside this row's range belong to other
-            // tiles: they contribute zero and leave the current run open.
-            const bool valid = p >= cur.lo[j] && p < cur.hi[j];
-            const float xx = e == 0 ? cur.X[j].x : e == 1 ? cur.X[j].y : e == 2 ? cur.X[j].z : cur.X[j].w;
-            const float yy = e == 0 ? cur.Y[j].x : e == 1 ? cur.Y[j].y : e == 2 ? cur.Y[j].z : cur.Y[j].w;
-            const float qq = e == 0 ? cur.QQ[j].x : e == 1 ? cur.QQ[j].y : e == 2 ? cur.QQ[j].z : cur.QQ[j].w;
-            float vxx = 0.0f, vyy = 0.0f;
-            if (St::kNeedsVelocity) {
-                vxx = e == 0 ? cur.VX[j].x : e == 1 ? cur.VX[j].y : e == 2 ? cur.VX[j].z : cur.VX[j].w;
-                vyy = e == 0 ? cur.VY[j].x : e == 1 ? cur.VY[j].y : e == 2 ? cur.VY[j].z : cur.VY[j].w;
-            }
-            const float qv = valid ? qq * d.inv_vol : 0.0f;
-            float w[V];
-            int cx, cy;
-            const bool ok = St::template weights<EX>(xx, yy, vxx, vyy, qv, dt, d, cx, cy, w);
-            // Unwrapped cell: positions outside [0, L) fall outside every tile
-            // and take the (wrapping) overflow path.
-            const int lx = cx - t.x0, ly = cy - t.y0;
-            const bool inside = ok && (unsigned)lx < (unsigned)t.tw && (unsigned)ly < (unsigned)t.th;
-            if (valid && !inside) {
-                overflow_scatter<S>(ok, cx, cy, w, xx, yy, vxx, vyy, qv, dt, d, overflow, comp, status);
-            }
-            const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
-            const bool nseg = lk != ck;
-            if (nseg && ck >= 0) {
-                if (!multi) {
-#pragma unroll
-                    for (int k = 0; k < V; ++k) H[k] = acc[k];
-                    multi = true;
-                } else {
-                    const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
-                    emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, acc);
-                }
-            }
-            const float keep = nseg ? 0.0f : 1.0f;
-            const bool take = valid && inside;
-#pragma unroll
-            for (int k = 0; k < V; ++k) acc[k] = fmaf(acc[k], keep, take ? w[k] : 0.0f);
-            if (nseg) {
-                ck = lk;
-                if (hk < 0) hk = lk;
-            }
-        }
-    }
-    // Combine the runs that straddle lanes.  Lane i's outgoing run is acc
-    // with key ck; it continues lane i-1's outgoing run when hk == ck(i-1).
-    const int ck_prev = dpp_i<kWaveShr1, 0xf>(-2, ck);
-    const bool cont = hk >= 0 && hk == ck_prev;
-    const float a = (cont && !multi) ? 1.0f : 0.0f;
-    float Z[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) Z[k] = acc[k];
-    wave_segmented_scan<V>(Z, a);
-    float Zp[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) Zp[k] = dpp_f<kWaveShr1, 0xf>(0.0f, Z[k]);
-    if (multi) {
-#pragma unroll
-        for (int k = 0; k < V; ++k) H[k] = cont ? H[k] + Zp[k] : H[k];
-        const int wb = ((hk >> 12) + 1 + St::OY) * t.WX + (hk & 4095) + 1 + St::OX;
-        emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, H);
-    }
-    const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
-    if (ck >= 0 && !cont_next) {
-        const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
-        emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, Z);
-    }
-}
-
 template <int S, int K, int EX, int PIPE>
 __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restrict__ px, const float* __restrict__ py,
                                                            const float* __restrict__ pvx,
@@ -405,137 +59,12 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                                                            float dt, float* __restrict__ partial,
                                                            float* __restrict__ overflow, float* __restrict__ comp,
                                                            int* __restrict__ status) {
-    using St = Stencil<S>;
-    constexpr int Q = K / 4;
-    static_assert(K % 4 == 0, "K must be a multiple of 4 (float4 loads)");
     extern __shared__ float smem[];
-
-    const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int tile = blockIdx.x * (blockDim.x >> 6) + wv;
     if (tile >= g.ntiles) return;  // wave-uniform
-
-    TileCtx t;
-    t.win_s = smem + (size_t)wv * g.lds_per_wave;
-    t.win_c = t.win_s + g.wsize;
-    int* tab = reinterpret_cast<int*>(t.win_c + g.wsize);
-    int* t_qstart = tab;          // [R]   first aligned quad of row r
-    int* t_rs = tab + g.R;        // [R]   first particle of row r
-    int* t_re = tab + 2 * g.R;    // [R]   one past the last particle of row r
-    int* t_qcum = tab + 3 * g.R;  // [R+1] cumulative quad counts
-
-    const int tx = tile % g.ntx, ty = tile / g.ntx;
-    t.x0 = tx * g.T;
-    t.y0 = ty * g.R;
-    t.tw = min(g.T, d.Nx - t.x0);
-    t.th = min(g.R, d.Ny - t.y0);
-    t.WX = g.WX;
-    const int th = t.th;
-
-    for (int i = lane; i < g.wsize; i += 64) {
-        t.win_s[i] = 0.0f;
-        t.win_c[i] = 0.0f;
-    }
-    // Row table: lane r < th describes row y0 + r of the tile.
-    int nq = 0;
-    if (lane < th) {
-        const int c = (t.y0 + lane) * d.Nx + t.x0;
-        const int s = cell_off[c], e = cell_off[c + t.tw];
-        const int qs = s >> 2, qe = (e + 3) >> 2;
-        nq = e > s ? qe - qs : 0;
-        t_qstart[lane] = qs;
-        t_rs[lane] = s;
-        t_re[lane] = e;
-    }
-    // Inclusive prefix of nq over the (at most 64) row lanes.
-    int incl = nq;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += u;
-    }
-    if (lane < th) t_qcum[lane + 1] = incl;
-    if (lane == 0) t_qcum[0] = 0;
-    wave_sync();
-    const int total_q = t_qcum[th];
-    const int step = 64 * Q;
-
-    if (PIPE) {
-        // Two register sets: the loads of one chunk are in flight while the
-        // other is processed.
-        Chunk<Q> A, B;
-        int base = 0;
-        if (base < total_q)
-            load_chunk<Q, St::kNeedsVelocity>(lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq, pvx,
-                                              pvy, A);
-        while (base < total_q) {
-            const int b1 = base + step;
-            if (b1 < total_q)
-                load_chunk<Q, St::kNeedsVelocity>(b1 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
-                                                  pq, pvx, pvy, B);
-            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, status);
-            if (b1 >= total_q) break;
-            const int b2 = b1 + step;
-            if (b2 < total_q)
-                load_chunk<Q, St::kNeedsVelocity>(b2 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
-                                                  pq, pvx, pvy, A);
-            process_chunk<S, Q, EX>(B, t, d, dt, overflow, comp, status);
-            base = b2;
-        }
-    } else {
-        for (int base = 0; base < total_q; base += step) {
-            Chunk<Q> A;
-            load_chunk<Q, St::kNeedsVelocity>(base + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq,
-                                              pvx, pvy, A);
-            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, status);
-        }
-    }
-
-    wave_sync();
-    float* out = partial + (size_t)tile * g.wsize;
-    for (int i = lane; i < g.wsize; i += 64) out[i] = t.win_s[i] + t.win_c[i];
-}
-
-// Tiles (along one axis) whose window can hold node g: those owning cells
-// g-2 .. g+1 (window halo is 1 before and 2 after the tile).  Writes up to 4
-// distinct tile indices in a fixed order.
-__device__ __forceinline__ int window_candidates(int gnode, int T, int nt, int n, int (&t)[4]) {
-    int cnt = 0;
-#pragma unroll
-    for (int o = -2; o <= 1; ++o) {
-        const int c = wrap_index(gnode + o, n);
-        const int ti = c / T;
-        bool dup = false;
-        for (int i = 0; i < cnt; ++i) dup |= (t[i] == ti);
-        if (!dup) t[cnt++] = ti;
-    }
-    (void)nt;
-    return cnt;
-}
-
-// Fast path (every tile at least 3 cells wide and tall, at least 3 tiles per
-// axis): along each axis a node lies in its own tile's window and, near the
-// tile's edge, in one neighbour's.  Writes (tile, window index) pairs.
-__device__ __forceinline__ int axis_pairs(int gn, int T, int nt, int n, int (&t)[2], int (&w)[2]) {
-    const int t0 = gn / T;
-    const int x0 = t0 * T;
-    const int tw0 = min(T, n - x0);
-    const int l = gn - x0;
-    t[0] = t0;
-    w[0] = l + 1;
-    if (l < 2) {  // the left neighbour's window reaches two nodes past its end
-        const int tl = t0 == 0 ? nt - 1 : t0 - 1;
-        const int twl = min(T, n - tl * T);
-        t[1] = tl;
-        w[1] = l + twl + 1;
-        return 2;
-    }
-    if (l >= tw0 - 1) {  // the right neighbour's window starts one node before it
-        t[1] = t0 == nt - 1 ? 0 : t0 + 1;
-        w[1] = l - tw0 + 1;
-        return 2;
-    }
-    return 1;
+    deposit_one_tile<S, K, EX, PIPE>(tile, smem + (size_t)wv * g.lds_per_wave, px, py, pvx, pvy, pq, cell_off, d, g,
+                                     dt, partial, overflow, comp, status);
 }
 
 __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
@@ -543,65 +72,8 @@ __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restr
                                                             DevDomain d, TileGeom g, float* __restrict__ rho) {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
-    if (gx < d.Nx) {
-        float s = 0.0f, c = 0.0f;
-        if (g.fast) {
-            int txs[2], wxs[2], tys[2], wys[2];
-            const int ncx = axis_pairs(gx, g.T, g.ntx, d.Nx, txs, wxs);
-            const int ncy = axis_pairs(gy, g.R, g.nty, d.Ny, tys, wys);
-            for (int iy = 0; iy < ncy; ++iy) {
-                for (int ix = 0; ix < ncx; ++ix) {
-                    const float v =
-                        partial[(size_t)(tys[iy] * g.ntx + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
-                    c += two_sum_err(s, v);
-                    s += v;
-                }
-            }
-        } else {
-            int txs[4], tys[4];
-            const int ncx = window_candidates(gx, g.T, g.ntx, d.Nx, txs);
-            const int ncy = window_candidates(gy, g.R, g.nty, d.Ny, tys);
-            for (int iy = 0; iy < ncy; ++iy) {
-                const int ty = tys[iy];
-                const int y0 = ty * g.R;
-                const int th = min(g.R, d.Ny - y0);
-                // images gy' = gy + m*Ny inside [y0 - 1, y0 + th + 2)
-                int gyi = gy;
-                while (gyi - d.Ny >= y0 - 1) gyi -= d.Ny;
-                while (gyi < y0 - 1) gyi += d.Ny;
-                for (; gyi < y0 + th + 2; gyi += d.Ny) {
-                    const int wy = gyi - (y0 - 1);
-                    for (int ix = 0; ix < ncx; ++ix) {
-                        const int tx = txs[ix];
-                        const int x0 = tx * g.T;
-                        const int tw = min(g.T, d.Nx - x0);
-                        int gxi = gx;
-                        while (gxi - d.Nx >= x0 - 1) gxi -= d.Nx;
-                        while (gxi < x0 - 1) gxi += d.Nx;
-                        for (; gxi < x0 + tw + 2; gxi += d.Nx) {
-                            const int wx = gxi - (x0 - 1);
-                            const float v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
-                            c += two_sum_err(s, v);
-                            s += v;
-                        }
-                    }
-                }
-            }
-        }
-        float out = s + c;
-        // Overflow grid and its compensation are zero except where drifted
-        // particles were scattered; fold those nodes in and clear them.
-        const size_t node = (size_t)gy * d.Nx + gx;
-        const float o = overflow[node], oc = comp[node];
-        if (o != 0.0f || oc != 0.0f) {
-            out += o + oc;
-            overflow[node] = 0.0f;
-            comp[node] = 0.0f;
-        }
-        rho[node] = out;
-    }
+    if (gx < d.Nx) rho[(size_t)gy * d.Nx + gx] = merge_node(gx, gy, partial, overflow, comp, d, g);
 }
-
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, size_t n, float v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

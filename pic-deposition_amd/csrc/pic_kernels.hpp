@@ -1,5 +1,6 @@
 // Host-side launch interfaces of the HIP kernels (one translation unit each:
-// deposit.hip, sort.hip, step.hip).  Used only by api.hip.
+// deposit.hip, sort.hip, step.hip, current.hip, ensemble.hip).  Used by the
+// C ABI files (api.hip, ensemble_api.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -69,8 +70,19 @@ size_t sort_scan_tmp_elems(size_t m);
 // keys_sorted (== sb.keys) the sorted keys, and cell_off the G+1 offsets.
 hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, const DevDomain& d, SortBuffers& sb,
                                  hipStream_t s);
+// Stable sort of precomputed keys in sb.keys (each in [0, key_range)):
+// same outputs as launch_sort_keys_perm, cell_off with key_range + 1 entries.
+hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s);
 // dst[i] = src[perm[i]] for one float stream.
 void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n, hipStream_t s);
+// dst_k[i] = src_k[perm[i]] for ss.count streams in one pass (perm == nullptr: copy).
+constexpr int kMaxStreams = 6;
+struct StreamSet {
+    const float* src[kMaxStreams];
+    float* dst[kMaxStreams];
+    int count;
+};
+void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s);
 void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s);
 
 // ---- step.hip -------------------------------------------------------------
@@ -147,5 +159,85 @@ void launch_esirkepov_current(const float* x, const float* y, const float* vx, c
 void launch_continuity_residual(const CurrentGrids& g, const DevDomain& d, double dt, double* partial, int nblocks,
                                 double* out, hipStream_t s);
 void launch_copy_f32(const float* src, float* dst, int64_t n, hipStream_t s);
+
+// ---- ensemble.hip -----------------------------------------------------------
+// Particle slots per chunk; each member's slot range is chunk aligned, so a
+// chunk (one workgroup of the per-particle kernels) belongs to one member.
+constexpr int kEnsChunk = 4096;
+
+struct EnsMember {
+    DevDomain d;
+    TileGeom g;
+    InitParams init;
+    int scheme, ppl, user;  // deposit scheme, particles per lane, index in the caller's member list
+    int64_t n;              // particles
+    int64_t pbase;          // first particle slot (multiple of kEnsChunk)
+    int64_t cbase;          // first cell key; key cbase + G collects the member's padding slots
+    int64_t gbase;          // first node of the member's rho / Ex / Ey / overflow grids
+    int64_t G;              // Nx * Ny
+    int64_t partial_base;   // first float of the member's tile windows
+    int64_t sbase;          // first element of its half spectrum [Ny][Nx/2+1]
+    int64_t tbase;          // first entry of its k^2 and difference-symbol tables [Nx/2+1 + Ny]
+    int64_t xbase;          // first entry of its mode-1 cos / sin tables [Nx]
+};
+
+// Member m owns blocks [start[m], start[m+1]) of each launch family.
+struct EnsTables {
+    const EnsMember* mem;
+    int M;
+    const int64_t* pchunk;  // particle chunks
+    const int64_t* gblk;    // grid blocks of 256 nodes
+    const int64_t* sblk;    // spectrum blocks of 256 elements
+};
+
+struct EnsParticles {
+    float *x, *y, *vx, *vy, *q, *m;
+};
+
+// One persistent deposit launch: the (member, tile) items of the members
+// with this scheme, particles per lane and coordinate path.
+struct EnsWork {
+    int scheme, ppl, exact;
+    const int2* items;
+    int nitems;
+    int lds_per_wave;  // floats, the largest of the group's members
+    int grid;          // workgroups of the persistent launch (set by ens_deposit_grid)
+};
+
+// Workgroups for a persistent launch of group w: the resident workgroups per
+// CU of its kernel at its LDS size times ncu, at most one per four items.
+// Queries the runtime, so it is called outside stream capture.
+int ens_deposit_grid(const EnsWork& w, int ncu);
+
+void launch_ens_init(const EnsParticles& p, const EnsTables& t, int64_t nchunks, hipStream_t s);
+void launch_ens_keys(const EnsParticles& p, const EnsTables& t, int64_t nchunks, int* keys, hipStream_t s);
+// counters: one int per group, reset by the launch itself.
+hipError_t launch_ens_deposit(const EnsWork* groups, int ngroups, const EnsParticles& p, const int* cell_off,
+                              const EnsTables& t, float dt, float* partial, float* overflow, float* comp,
+                              int* status, int* counters, int ncu, hipStream_t s);
+// rho of grid blocks [blk0, blk0 + nblk) and their FP64 block sums rho_part[b].
+void launch_ens_merge(const float* partial, float* overflow, float* comp, const EnsTables& t, int64_t blk0,
+                      int64_t nblk, float* rho, double* rho_part, hipStream_t s);
+// out[m] = sum of part[b * stride + k] over member m's grid blocks (k fixed).
+void launch_ens_member_grid_sum(const double* part, int stride, int k, const EnsTables& t, double* out,
+                                hipStream_t s);
+void launch_ens_prep(const float* rho, float* in, const EnsTables& t, int64_t nblk, const double* rho_sum,
+                     hipStream_t s);
+void launch_ens_spectral(const float2* spec, float2* spec_ex, float2* spec_ey, const double* k2, const float2* sym,
+                         const EnsTables& t, int64_t nblk, hipStream_t s);
+// Per grid block: sum(Ex^2 + Ey^2), sum Ex cos(k x_i), sum Ex sin(k x_i).
+void launch_ens_field_sums(const float* Ex, const float* Ey, const double* cosk, const double* sink,
+                           const EnsTables& t, int64_t nblk, double* field_part, hipStream_t s);
+// Gather (v += -(q/m) E dt) then, if *push_flag, push; per chunk FP64 sums of
+// 0.5 m v^2, q and |q| into part_p[3 c + k].
+void launch_ens_gather_push(const EnsParticles& p, const float* Ex, const float* Ey, const EnsTables& t,
+                            int64_t nchunks, float dt, int do_gather, const int* push_flag, double* part_p,
+                            hipStream_t s);
+// One block per member: records field energy, mode-1 amplitude, kinetic
+// energy and charge error at [user][counter[m] - record_from] if inside
+// [0, cap), then counter[m] += 1.
+void launch_ens_diag(const double* rho_part, const double* field_part, const double* part_p, const EnsTables& t,
+                     int* counters, int record_from, int cap, const StepRecords& rec, hipStream_t s);
+void launch_set_ints(int* p, int n, int v, hipStream_t s);
 
 }  // namespace pichip

@@ -195,6 +195,21 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ s
     if (i < n) dst[i] = src[perm[i]];
 }
 
+// dst_k[i] = src_k[perm[i]] for up to kMaxStreams streams: the permutation is
+// read once for all of them.  With perm == nullptr it is a plain copy.
+__global__ __launch_bounds__(256) void permute_streams_kernel(StreamSet ss, const int* __restrict__ perm, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = perm ? (int64_t)perm[i] : i;
+    float v[kMaxStreams];
+#pragma unroll
+    for (int k = 0; k < kMaxStreams; ++k)
+        if (k < ss.count) v[k] = ss.src[k][p];
+#pragma unroll
+    for (int k = 0; k < kMaxStreams; ++k)
+        if (k < ss.count) ss.dst[k][i] = v[k];
+}
+
 int key_bits(int G) {
     int bits = 0;
     while (bits < 31 && (1ll << bits) < (long long)G) ++bits;
@@ -215,14 +230,11 @@ void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, 
         hipLaunchKernelGGL(keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n, d, keys);
 }
 
-hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, const DevDomain& d, SortBuffers& sb,
-                                 hipStream_t s) {
-    const int G = d.Nx * d.Ny;
+hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s) {
     if (n > 0) {
-        launch_keys(x, y, n, d, sb.keys, s);
         const int nblocks = (int)((n + kBlockItems - 1) / kBlockItems);
         const size_t m = (size_t)nblocks * kDigits;
-        const int bits = key_bits(G);
+        const int bits = key_bits(key_range);
         const int passes = bits == 0 ? 1 : (bits + 7) / 8;
         int* kin = sb.keys;
         int* kout = sb.keys_alt;
@@ -254,9 +266,20 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
             sb.perm_alt = t;
         }
     }
-    hipLaunchKernelGGL(cell_offsets_kernel, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys, n, G,
-                       sb.cell_off);
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys, n,
+                       key_range, sb.cell_off);
     return hipGetLastError();
+}
+
+hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, const DevDomain& d, SortBuffers& sb,
+                                 hipStream_t s) {
+    launch_keys(x, y, n, d, sb.keys, s);
+    return launch_sort_precomputed(n, d.Nx * d.Ny, sb, s);
+}
+
+void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(permute_streams_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, perm, n);
 }
 
 void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n, hipStream_t s) {

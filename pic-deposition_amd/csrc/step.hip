@@ -4,15 +4,13 @@
 
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
+#include "particle_device.hpp"
 
 namespace pichip {
 
 namespace {
 
-// gatherFieldsSoA (reference src/deposition/deposition.cpp:335-356) followed by
-// pushParticlesSoA (deposition.cpp:367-374) in FP32.  The push reduces the
-// displacement modulo L before adding it, so that |v dt| >> L does not cost
-// FP32 position bits; in exact arithmetic this equals fmod(x + v dt, L).
+// Gather (v += q/m E dt) then push (x += v dt) of one particle per lane.
 __global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px, float* __restrict__ py,
                                                           float* __restrict__ pvx, float* __restrict__ pvy,
                                                           const float* __restrict__ pq, const float* __restrict__ pm,
@@ -24,17 +22,8 @@ __global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px
     float x = px[i], y = py[i];
     float vx = pvx[i], vy = pvy[i];
     if (do_gather) {
-        int cx, cy;
-        float wx, wy;
-        cell_coord(x, d.inv_dx, d.dx, d.exact, cx, wx);
-        cell_coord(y, d.inv_dy, d.dy, d.exact, cy, wy);
-        const int ix = wrap_index(cx, d.Nx), iy = wrap_index(cy, d.Ny);
-        const int ixp = ix + 1 == d.Nx ? 0 : ix + 1;
-        const int iyp = iy + 1 == d.Ny ? 0 : iy + 1;
-        const size_t r0 = (size_t)iy * d.Nx, r1 = (size_t)iyp * d.Nx;
-        const float ux = 1.0f - wx, uy = 1.0f - wy;
-        const float ex = Ex[r0 + ix] * ux * uy + Ex[r0 + ixp] * wx * uy + Ex[r1 + ix] * ux * wy + Ex[r1 + ixp] * wx * wy;
-        const float ey = Ey[r0 + ix] * ux * uy + Ey[r0 + ixp] * wx * uy + Ey[r1 + ix] * ux * wy + Ey[r1 + ixp] * wx * wy;
+        float ex, ey;
+        gather_cic(x, y, Ex, Ey, d, ex, ey);
         const float qm = -pq[i] / pm[i];
         vx += qm * ex * dt;
         vy += qm * ey * dt;
@@ -42,15 +31,8 @@ __global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px
         pvy[i] = vy;
     }
     if (do_push) {
-        const float Lx = (float)d.Lx, Ly = (float)d.Ly;
-        x += fmodf(vx * dt, Lx);
-        y += fmodf(vy * dt, Ly);
-        x = fmodf(x, Lx);
-        if (x < 0.0f) x += Lx;
-        y = fmodf(y, Ly);
-        if (y < 0.0f) y += Ly;
-        px[i] = x;
-        py[i] = y;
+        px[i] = push_coord(x, vx, dt, (float)d.Lx);
+        py[i] = push_coord(y, vy, dt, (float)d.Ly);
     }
 }
 
@@ -109,94 +91,19 @@ __global__ __launch_bounds__(256) void diag_final_kernel(const double* __restric
     }
 }
 
-// ---- Philox4x32-10 (Salmon et al., SC'11) ----------------------------------
-struct U4 {
-    uint32_t v[4];
-};
-
-__device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)M0 * c0;
-        const uint64_t p1 = (uint64_t)M1 * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n1 = (uint32_t)p1;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
-        const uint32_t n3 = (uint32_t)p0;
-        c0 = n0;
-        c1 = n1;
-        c2 = n2;
-        c3 = n3;
-        k0 += W0;
-        k1 += W1;
-    }
-    U4 o;
-    o.v[0] = c0;
-    o.v[1] = c1;
-    o.v[2] = c2;
-    o.v[3] = c3;
-    return o;
-}
-
-// 53-bit uniform in [0, 1) from two words.
-__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
-    const uint64_t r = (((uint64_t)a << 32) | b) >> 11;
-    return (double)r * (1.0 / 9007199254740992.0);
-}
-
-__device__ __forceinline__ float wrap_pos(double x, double L) {
-    float f = (float)x;
-    const float Lf = (float)L;
-    if (f >= Lf) f -= Lf;
-    if (f < 0.0f) f += Lf;
-    if (f >= Lf) f = 0.0f;
-    return f;
-}
-
 __global__ __launch_bounds__(256) void init_kernel(float* __restrict__ px, float* __restrict__ py,
                                                    float* __restrict__ pvx, float* __restrict__ pvy,
                                                    float* __restrict__ pq, float* __restrict__ pm, int64_t n,
                                                    DevDomain d, InitParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t k0 = (uint32_t)p.seed, k1 = (uint32_t)(p.seed >> 32);
-    const U4 a = philox((uint32_t)i, (uint32_t)((uint64_t)i >> 32), 0u, 0u, k0, k1);
-    const U4 b = philox((uint32_t)i, (uint32_t)((uint64_t)i >> 32), 1u, 0u, k0, k1);
-    const double ux = u01(a.v[0], a.v[1]);
-    const double uy = u01(a.v[2], a.v[3]);
-    const double kx = 2.0 * M_PI * (double)p.mode / d.Lx;
-    double x = ux * d.Lx;
-    if (p.kind == 1 && p.amp != 0.0) {
-        // Inverse CDF of n(x) ~ 1 + amp cos(kx) on [0, Lx) by Newton iteration.
-        const double target = ux * d.Lx;
-        for (int it = 0; it < 8; ++it) {
-            const double F = x + p.amp / kx * sin(kx * x) - target;
-            const double dF = 1.0 + p.amp * cos(kx * x);
-            x -= F / dF;
-        }
-    }
-    const double y = uy * d.Ly;
-    double vx = 0.0, vy = 0.0;
-    if (p.vmax > 0.0) {
-        vx = (2.0 * u01(b.v[0], b.v[1]) - 1.0) * p.vmax;
-        vy = (2.0 * u01(b.v[2], b.v[3]) - 1.0) * p.vmax;
-    }
-    if (p.vth > 0.0) {
-        // Box-Muller on (1 - u) in (0, 1].
-        const double r = sqrt(-2.0 * log(1.0 - u01(b.v[0], b.v[1])));
-        const double th = 2.0 * M_PI * u01(b.v[2], b.v[3]);
-        vx += p.vth * r * cos(th);
-        vy += p.vth * r * sin(th);
-    }
-    if (p.kind == 2) vx += (i < n / 2 ? 1.0 : -1.0) * p.amp;
-    if (p.pert != 0.0) vx += p.pert * sin(kx * x);
-    px[i] = wrap_pos(x, d.Lx);
-    py[i] = wrap_pos(y, d.Ly);
-    pvx[i] = (float)vx;
-    pvy[i] = (float)vy;
-    pq[i] = (float)p.qp;
-    pm[i] = (float)p.mp;
+    const ParticleInit o = init_particle(i, n, d, p);
+    px[i] = o.x;
+    py[i] = o.y;
+    pvx[i] = o.vx;
+    pvy[i] = o.vy;
+    pq[i] = o.q;
+    pm[i] = o.m;
 }
 
 

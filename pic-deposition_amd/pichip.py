@@ -108,6 +108,15 @@ class InitParams(ctypes.Structure):
                 ("mode", ctypes.c_int32), ("q", ctypes.c_double), ("m", ctypes.c_double)]
 
 
+class EnsembleMember(ctypes.Structure):
+    _fields_ = [("domain", Domain), ("scheme", ctypes.c_int32), ("num_particles", ctypes.c_int64),
+                ("init", InitParams)]
+
+
+class EnsembleRunConfig(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("sort_interval", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `make -C pic-deposition_amd` (or __graft_entry__.build())")
@@ -162,6 +171,25 @@ _SIGS = {
     "pic_hip_continuity_residual": (ctypes.c_int, [_P, ctypes.c_double, _f64p, _f64p]),
     "pic_hip_profile_begin": (ctypes.c_int, [_P, ctypes.c_int]),
     "pic_hip_profile_end": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "pic_hip_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _f64p]),
+    "pic_hip_ensemble_create": (ctypes.c_int, [ctypes.POINTER(EnsembleMember), ctypes.c_int32, ctypes.POINTER(_P)]),
+    "pic_hip_ensemble_destroy": (ctypes.c_int, [_P]),
+    "pic_hip_ensemble_count": (ctypes.c_int32, [_P]),
+    "pic_hip_ensemble_sync": (ctypes.c_int, [_P]),
+    "pic_hip_ensemble_init": (ctypes.c_int, [_P]),
+    "pic_hip_ensemble_upload": (ctypes.c_int, [_P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    "pic_hip_ensemble_download": (ctypes.c_int, [_P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    "pic_hip_ensemble_sort": (ctypes.c_int, [_P]),
+    "pic_hip_ensemble_deposit": (ctypes.c_int, [_P, ctypes.c_double]),
+    "pic_hip_ensemble_get_rho": (ctypes.c_int, [_P, ctypes.c_int32, _f32p]),
+    "pic_hip_ensemble_poisson": (ctypes.c_int, [_P]),
+    "pic_hip_ensemble_get_fields": (ctypes.c_int, [_P, ctypes.c_int32, _f32p, _f32p]),
+    "pic_hip_ensemble_set_tiling": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pic_hip_ensemble_get_tiling": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "pic_hip_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "pic_hip_ensemble_run": (ctypes.c_int, [_P, ctypes.POINTER(EnsembleRunConfig), ctypes.c_int32, ctypes.c_int32,
+                                            _f64p, _f64p, _f64p, _f64p]),
+    "pic_hip_ensemble_time": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _f64p]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -383,6 +411,159 @@ class Context:
         d = Diagnostics()
         _check(_lib.pic_hip_diagnostics(self._h, ctypes.byref(d)))
         return d
+
+    def autotune(self, scheme=DepositionScheme.CIC, cache_path=None, reps: int = 5):
+        """pic_hip_autotune: (Tiling chosen, ms per deposit or 0.0 when read from the cache)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(1)
+        path = None if cache_path is None else os.fsencode(cache_path)
+        _check(_lib.pic_hip_autotune(self._h, int(scheme), path, int(reps), ctypes.byref(a), ctypes.byref(b),
+                                     ctypes.byref(c), _ptr(ms, ctypes.c_double)))
+        return Tiling(a.value, b.value, c.value, 0), float(ms[0])
+
+
+@dataclass
+class Member:
+    """One ensemble member (pic_ensemble_member)."""
+    Nx: int
+    Ny: int
+    num_particles: int
+    scheme: int = DepositionScheme.CIC
+    Lx: float = 1.0
+    Ly: float = 1.0
+    kind: int = 1
+    seed: int = 42
+    amp: float = 0.0
+    pert: float = 0.0
+    vth: float = 0.0
+    vmax: float = 0.0
+    mode: int = 1
+    q: float = 1.0
+    m: float = 1.0
+
+    def domain(self) -> Domain:
+        return make_domain(self.Lx, self.Ly, self.Nx, self.Ny)
+
+    def _c(self) -> EnsembleMember:
+        return EnsembleMember(self.domain(), int(self.scheme), int(self.num_particles),
+                              InitParams(self.kind, self.seed, self.amp, self.pert, self.vth, self.vmax, self.mode,
+                                         self.q, self.m))
+
+
+def langmuir_ensemble(count: int = 64, grids=(64, 128, 256, 512), ppc=(16, 256), amp=(0.001, 0.05),
+                      schemes=(DepositionScheme.CIC, DepositionScheme.TSC), seed: int = 2024, mode: int = 1):
+    """Members of a Langmuir ensemble: square grids drawn from `grids`, particles
+    per cell uniform in [ppc[0], ppc[1]], density perturbation amplitude
+    uniform in [amp[0], amp[1]] on mode `mode`, scheme drawn from `schemes`,
+    seeds from one generator; unit box, cold electrons with q = -1/sqrt(N),
+    m = 1 (plasma frequency 1 in every member)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        g = int(rng.choice(grids))
+        n = g * g * int(rng.integers(ppc[0], ppc[1] + 1))
+        out.append(Member(Nx=g, Ny=g, num_particles=n, scheme=int(rng.choice(schemes)), kind=1,
+                          seed=int(rng.integers(1, 2**62)), amp=float(rng.uniform(*amp)), mode=mode,
+                          q=-1.0 / float(np.sqrt(n)), m=1.0))
+    return out
+
+
+class Ensemble:
+    """A batch of independent PIC instances on one device (pic_hip_ensemble)."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (EnsembleMember * len(self.members))(*[m._c() for m in self.members])
+        h = _P()
+        _check(_lib.pic_hip_ensemble_create(arr, len(self.members), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.pic_hip_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self) -> int:
+        return _lib.pic_hip_ensemble_count(self._h)
+
+    def sync(self) -> None:
+        _check(_lib.pic_hip_ensemble_sync(self._h))
+
+    def init(self) -> None:
+        _check(_lib.pic_hip_ensemble_init(self._h))
+
+    def upload(self, i: int, x, y, q, vx=None, vy=None, m=None) -> None:
+        n = self.members[i].num_particles
+        arrs = [_f32(a) for a in (x, y, vx, vy, q, m)]
+        for a in arrs:
+            if a is not None and len(a) != n:
+                raise ValueError(f"member {i} holds {n} particles")
+        _check(_lib.pic_hip_ensemble_upload(self._h, int(i), *[_ptr(a, ctypes.c_float) for a in arrs]))
+
+    def download(self, i: int):
+        n = self.members[i].num_particles
+        out = [np.empty(n, dtype=np.float32) for _ in range(6)]
+        _check(_lib.pic_hip_ensemble_download(self._h, int(i), *[_ptr(a, ctypes.c_float) for a in out]))
+        return dict(zip(("x", "y", "vx", "vy", "q", "m"), out))
+
+    def sort(self) -> None:
+        _check(_lib.pic_hip_ensemble_sort(self._h))
+
+    def deposit(self, dt: float = 0.0) -> None:
+        _check(_lib.pic_hip_ensemble_deposit(self._h, float(dt)))
+
+    def rho(self, i: int) -> np.ndarray:
+        mb = self.members[i]
+        r = np.empty(mb.Nx * mb.Ny, dtype=np.float32)
+        _check(_lib.pic_hip_ensemble_get_rho(self._h, int(i), _ptr(r, ctypes.c_float)))
+        return r.reshape(mb.Ny, mb.Nx)
+
+    def poisson(self) -> None:
+        _check(_lib.pic_hip_ensemble_poisson(self._h))
+
+    def fields(self, i: int):
+        mb = self.members[i]
+        ex, ey = (np.empty(mb.Nx * mb.Ny, dtype=np.float32) for _ in range(2))
+        _check(_lib.pic_hip_ensemble_get_fields(self._h, int(i), _ptr(ex, ctypes.c_float), _ptr(ey, ctypes.c_float)))
+        return ex.reshape(mb.Ny, mb.Nx), ey.reshape(mb.Ny, mb.Nx)
+
+    def set_tiling(self, tile_x: int, tile_y: int, particles_per_lane: int = 8, member: int = -1) -> None:
+        _check(_lib.pic_hip_ensemble_set_tiling(self._h, int(member), tile_x, tile_y, particles_per_lane))
+
+    def tiling(self, i: int) -> Tiling:
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.pic_hip_ensemble_get_tiling(self._h, int(i), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return Tiling(a.value, b.value, c.value, 0)
+
+    def autotune(self, cache_path=None, reps: int = 5) -> None:
+        path = None if cache_path is None else os.fsencode(cache_path)
+        _check(_lib.pic_hip_ensemble_autotune(self._h, path, int(reps)))
+
+    def run(self, steps: int, dt: float, sort_interval: int = 1, record_from: int = 0, use_graph: bool = True):
+        """pic_hip_ensemble_run: dict of [members, steps - record_from] FP64 records."""
+        cfg = EnsembleRunConfig(float(dt), int(sort_interval), int(bool(use_graph)))
+        cap = max(steps - record_from, 0)
+        M = len(self.members)
+        out = {k: np.zeros((M, cap)) for k in ("field_energy", "mode1", "kinetic", "charge_error")}
+        _check(_lib.pic_hip_ensemble_run(self._h, ctypes.byref(cfg), int(steps), int(record_from),
+                                         *[_ptr(out[k], ctypes.c_double) for k in out]))
+        return out
+
+    def time(self, what: str, reps: int = 10, dt: float = 0.0) -> float:
+        """Mean milliseconds of 'deposit', 'sort', 'step' or 'step_sort' (HIP events)."""
+        code = {"deposit": 0, "sort": 1, "step": 2, "step_sort": 3}[what]
+        ms = np.zeros(1)
+        _check(_lib.pic_hip_ensemble_time(self._h, code, int(reps), float(dt), _ptr(ms, ctypes.c_double)))
+        return float(ms[0])
 
 
 def dominant_frequency(signal, dt: float) -> float:
