@@ -334,6 +334,67 @@ pic_status pic_hip_ensemble_run(pic_hip_ensemble* ens, const pic_ensemble_run_co
  * with sort.  Particles are left as the runs leave them. */
 pic_status pic_hip_ensemble_time(pic_hip_ensemble* ens, int what, int reps, double dt, double* mean_ms);
 
+/* ---- learned density denoiser (build-defined; no reference counterpart) --
+ * A periodic convolution network maps a low-particles-per-cell density to
+ * the density a high-particles-per-cell deposit would give.  It works on the
+ * relative fluctuation delta = rho / mean(rho) - 1: four 3x3 convolution
+ * layers 1 -> 32 -> 32 -> 32 -> 1 with ReLU between them and per-layer
+ * dilation (default 1, 2, 4, 8: a 31 x 31 receptive field), 19,105
+ * parameters, BF16 weights and activations with FP32 accumulation on the
+ * matrix cores, FP32 master weights trained with Adam on mean-squared error.
+ * Training pairs are deposits made by this library (pic_hip_denoiser_add_sample
+ * normalises a context's current rho into the input or the target set).
+ *
+ * pic_hip_denoise replaces a context's rho by the network's estimate,
+ * renormalised in FP64 so that sum(rho) * cellVolume equals the particles'
+ * total charge to FP64 round-off, unless the gate rejects it.  The gate is
+ * the Gauss-law residual of the learned field against the deposited charge,
+ * ||div E_learned - rho_plain||^2 = ||rho_learned - rho_plain||^2, divided by
+ * its shot-noise expectation kappa * sum(q^2) / cellVolume^2 (kappa = sum of
+ * squared node weights per particle: 1 NGP, 4/9 CIC and Esirkepov, 0.3025
+ * TSC); above `tolerance`, or for a non-finite result or zero mean charge,
+ * the plain deposit is kept.  The grid must have Nx a multiple of 32. */
+enum { PIC_DENOISER_PARAMS = 19105 };
+typedef struct pic_hip_denoiser pic_hip_denoiser;
+
+/* dilation: 4 values (NULL: 1, 2, 4, 8), each below min(Nx, Ny).  Weights
+ * get He-normal initialisation from seed (the last layer scaled by 0.1). */
+pic_status pic_hip_denoiser_create(int32_t Nx, int32_t Ny, int32_t max_batch, int32_t max_pairs,
+                                   const int32_t* dilation, uint64_t seed, pic_hip_denoiser** out);
+pic_status pic_hip_denoiser_destroy(pic_hip_denoiser* net);
+/* FP32 master parameters: W1[32][9], b1[32], W2[9][32][32] (tap, out, in),
+ * b2[32], W3[9][32][32], b3[32], W4[9][32], b4[1]; tap = 3 * dy + dx. */
+pic_status pic_hip_denoiser_get_params(pic_hip_denoiser* net, float* params);
+pic_status pic_hip_denoiser_set_params(pic_hip_denoiser* net, const float* params);
+/* Network on host images [batch][Ny][Nx] (delta in, delta out), synchronous. */
+pic_status pic_hip_denoiser_forward_host(pic_hip_denoiser* net, const float* input, float* output, int32_t batch);
+/* Mean-squared-error loss and its gradient with respect to the parameters
+ * (PIC_DENOISER_PARAMS values) for one host batch, synchronous. */
+pic_status pic_hip_denoiser_gradient_host(pic_hip_denoiser* net, const float* input, const float* target,
+                                          int32_t batch, double* loss, float* grad);
+/* Appends the context's current rho (as delta) to the input (is_target = 0)
+ * or target (1) set; the k-th input pairs with the k-th target. */
+pic_status pic_hip_denoiser_add_sample(pic_hip_denoiser* net, pic_hip_ctx* ctx, int32_t is_target);
+int32_t pic_hip_denoiser_pairs(pic_hip_denoiser* net);
+pic_status pic_hip_denoiser_clear_samples(pic_hip_denoiser* net);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on batches of random pairs, each rolled
+ * by a random periodic shift; learning rate cosine-decayed from lr to
+ * lr / 20.  loss_history (NULL to skip) receives one loss per iteration. */
+pic_status pic_hip_denoiser_train(pic_hip_denoiser* net, int32_t iterations, int32_t batch, double lr,
+                                  uint64_t seed, double* loss_history);
+/* Learned density for the context (see above).  fallback / residual_ratio
+ * (NULL to skip; non-NULL makes the call synchronous) report the gate. */
+pic_status pic_hip_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double tolerance, int32_t* fallback,
+                           double* residual_ratio);
+/* FP64 density of the last pic_hip_denoise (renormalised, or the plain
+ * deposit when the gate fell back), Nx * Ny values. */
+pic_status pic_hip_denoiser_get_output_f64(pic_hip_denoiser* net, double* rho);
+/* Deposit (global atomics) of a hash-selected Bernoulli(1/stride) subset of
+ * the context's particles with charges multiplied by stride: a density with
+ * stride times fewer particles per cell of the same state, for training
+ * pairs.  Any particle order. */
+pic_status pic_hip_deposit_subsample(pic_hip_ctx* ctx, int scheme, uint32_t stride, uint64_t salt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,12 @@ or_fourier_mode_amplitude    pure cos/sin field.
 or_esirkepov_current         discrete continuity identity (rho_new-rho_old)/dt
                              + div J = 0 to round-off; zero motion gives
                              J = 0 and rho_old = rho_new = CIC.
+or_cnn_forward               an independent numpy restatement with np.roll
+                             (periodic correlation) to 1e-12; a centred delta
+                             kernel is the identity and a shifted one a roll.
+or_cnn_backward              central finite differences of or_cnn_forward's
+                             loss (FP64, no BF16 rounding) on sampled
+                             parameters of every layer, weights and biases.
 ===========================  =================================================
 """
 from __future__ import annotations
@@ -90,6 +96,9 @@ _OR_SIGS = {
     "or_fourier_mode_amplitude": (ctypes.c_double, [_D, _DOM, ctypes.c_int]),
     "or_growth_rate": (ctypes.c_double, [_D, _L, ctypes.c_double, ctypes.c_int]),
     "or_esirkepov_current": (ctypes.c_int, [_D, _D, _D, _D, _D, _L, _DOM, ctypes.c_double, _D, _D, _D, _D]),
+    "or_cnn_forward": (ctypes.c_int, [_D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _D]),
+    "or_cnn_backward": (ctypes.c_int, [_D, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _D,
+                                       _D]),
     "or_run_loop": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _L, _DOM, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
 }
@@ -438,3 +447,33 @@ def spectral_noise_level(rho):
     if total <= 1e-30:
         return 0.0
     return float(power[high].sum() / total)
+
+
+CNN_PARAMS = 19105
+
+
+def cnn_forward(params, images, dilation=(1, 2, 4, 8), emulate_bf16=False) -> np.ndarray:
+    """FP64 network output for images [B, H, W] (delta in, delta out)."""
+    x = np.ascontiguousarray(images, dtype=np.float64)
+    B, H, W = x.shape
+    p = _d(params)
+    dil = np.ascontiguousarray(dilation, dtype=np.int32)
+    out = np.zeros_like(x)
+    if lib().or_cnn_forward(_pd(p), _pd(x), B, H, W, _pi(dil), int(bool(emulate_bf16)), _pd(out)):
+        raise ValueError("or_cnn_forward failed")
+    return out
+
+
+def cnn_backward(params, images, targets, dilation=(1, 2, 4, 8), emulate_bf16=False):
+    """(loss, gradient[CNN_PARAMS]) of the mean-squared error."""
+    x = np.ascontiguousarray(images, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.float64)
+    B, H, W = x.shape
+    p = _d(params)
+    dil = np.ascontiguousarray(dilation, dtype=np.int32)
+    loss = np.zeros(1)
+    grad = np.zeros(CNN_PARAMS)
+    if lib().or_cnn_backward(_pd(p), _pd(x), _pd(t), B, H, W, _pi(dil), int(bool(emulate_bf16)), _pd(loss),
+                             _pd(grad)):
+        raise ValueError("or_cnn_backward failed")
+    return float(loss[0]), grad

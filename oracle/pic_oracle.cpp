@@ -794,3 +794,186 @@ int or_esirkepov_current(const double* x, const double* y, const double* vx, con
 }
 
 }  // extern "C"
+
+// ---- denoiser network (build-defined; no reference counterpart) -------------
+// FP64 restatement of the 1 -> 32 -> 32 -> 32 -> 1 periodic 3x3 convolution
+// network: z_l(p, co) = b_l(co) + sum_t sum_ci W_l(t, co, ci) a_{l-1}(p + d_l off(t), ci),
+// a_l = relu(z_l) for l < 4, out = z_4; tap t = 3 ty + tx, off(t) = (ty - 1, tx - 1);
+// loss = mean((out - target)^2).  Parameter layout as the device library:
+// W1[co][t], b1, W2[t][co][ci], b2, W3[t][co][ci], b3, W4[t][ci], b4.
+// With emulate_bf16 the values the device keeps in BF16 are rounded the same
+// way (round to nearest even through FP32): weights, the input image,
+// hidden activations, the output gradient where it feeds a product, and the
+// masked hidden gradients; sums stay in FP64.
+
+namespace {
+
+constexpr int kCW1 = 0, kCB1 = 288, kCW2 = 320, kCB2 = 9536, kCW3 = 9568, kCB3 = 18784, kCW4 = 18816, kCB4 = 19104;
+constexpr int kCParams = 19105;
+
+double bf16_round(double v) {
+    float f = static_cast<float>(v);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct Net {
+    int B, H, W;
+    int dil[4];
+    bool emu;
+    std::vector<double> w;  // possibly BF16-rounded weights
+    size_t npix() const { return static_cast<size_t>(B) * H * W; }
+    size_t shifted(size_t p, int t, int d) const {
+        const int x = static_cast<int>(p % W), y = static_cast<int>((p / W) % H), b = static_cast<int>(p / (static_cast<size_t>(W) * H));
+        int yy = (y + (t / 3 - 1) * d) % H, xx = (x + (t % 3 - 1) * d) % W;
+        if (yy < 0) yy += H;
+        if (xx < 0) xx += W;
+        return (static_cast<size_t>(b) * H + yy) * W + xx;
+    }
+    double r(double v) const { return emu ? bf16_round(v) : v; }
+    // layer l (1..4) forward: in has cin channels, out cout channels (NHWC)
+    void conv(int l, const std::vector<double>& in, int cin, std::vector<double>& out, int cout) const {
+        const size_t P = npix();
+        out.assign(P * cout, 0.0);
+        for (size_t p = 0; p < P; ++p)
+            for (int t = 0; t < 9; ++t) {
+                const size_t q = shifted(p, t, dil[l - 1]);
+                for (int co = 0; co < cout; ++co) {
+                    double s = 0.0;
+                    for (int ci = 0; ci < cin; ++ci) s += weight(l, t, co, ci) * in[q * cin + ci];
+                    out[p * cout + co] += s;
+                }
+            }
+        for (size_t p = 0; p < P; ++p)
+            for (int co = 0; co < cout; ++co) out[p * cout + co] += bias(l, co);
+    }
+    double weight(int l, int t, int co, int ci) const {
+        switch (l) {
+            case 1: return w[kCW1 + co * 9 + t];
+            case 2: return w[kCW2 + (t * 32 + co) * 32 + ci];
+            case 3: return w[kCW3 + (t * 32 + co) * 32 + ci];
+            default: return w[kCW4 + t * 32 + ci];
+        }
+    }
+    static int weight_index(int l, int t, int co, int ci) {
+        switch (l) {
+            case 1: return kCW1 + co * 9 + t;
+            case 2: return kCW2 + (t * 32 + co) * 32 + ci;
+            case 3: return kCW3 + (t * 32 + co) * 32 + ci;
+            default: return kCW4 + t * 32 + ci;
+        }
+    }
+    double bias(int l, int co) const {
+        switch (l) {
+            case 1: return w[kCB1 + co];
+            case 2: return w[kCB2 + co];
+            case 3: return w[kCB3 + co];
+            default: return w[kCB4];
+        }
+    }
+    static int bias_index(int l, int co) {
+        switch (l) {
+            case 1: return kCB1 + co;
+            case 2: return kCB2 + co;
+            case 3: return kCB3 + co;
+            default: return kCB4;
+        }
+    }
+};
+
+Net make_net(const double* params, int B, int H, int W, const int* dil, int emu) {
+    Net n{B, H, W, {1, 1, 1, 1}, emu != 0, std::vector<double>(params, params + kCParams)};
+    for (int k = 0; k < 4; ++k) n.dil[k] = dil ? dil[k] : 1;
+    if (n.emu)
+        for (int i = 0; i < kCParams; ++i)
+            if (!(i >= kCB1 && i < kCW2) && !(i >= kCB2 && i < kCW3) && !(i >= kCB3 && i < kCW4) && i != kCB4)
+                n.w[static_cast<size_t>(i)] = bf16_round(n.w[static_cast<size_t>(i)]);
+    return n;
+}
+
+// a[0] = input (1 channel), a[1..3] = hidden activations, out = output.
+void net_forward(const Net& n, const double* in, std::vector<double> (&a)[4], std::vector<double>& out) {
+    const size_t P = n.npix();
+    a[0].resize(P);
+    for (size_t p = 0; p < P; ++p) a[0][p] = n.r(in[p]);
+    std::vector<double> z;
+    for (int l = 1; l <= 3; ++l) {
+        n.conv(l, a[l - 1], l == 1 ? 1 : 32, z, 32);
+        a[l].resize(z.size());
+        for (size_t i = 0; i < z.size(); ++i) a[l][i] = n.r(std::max(z[i], 0.0));
+    }
+    n.conv(4, a[3], 32, out, 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int or_cnn_forward(const double* params, const double* in, int B, int H, int W, const int* dil, int emulate_bf16,
+                   double* out) {
+    if (B < 1 || H < 1 || W < 1) return 1;
+    const Net n = make_net(params, B, H, W, dil, emulate_bf16);
+    std::vector<double> a[4], o;
+    net_forward(n, in, a, o);
+    std::copy(o.begin(), o.end(), out);
+    return 0;
+}
+
+// Backpropagation through net_forward: loss and d loss / d params.
+int or_cnn_backward(const double* params, const double* in, const double* target, int B, int H, int W,
+                    const int* dil, int emulate_bf16, double* loss, double* grad) {
+    if (B < 1 || H < 1 || W < 1) return 1;
+    const Net n = make_net(params, B, H, W, dil, emulate_bf16);
+    const size_t P = n.npix();
+    std::vector<double> a[4], out;
+    net_forward(n, in, a, out);
+    double L = 0.0;
+    std::vector<double> dout(P), dz(P);  // dz: the output gradient as it enters products
+    for (size_t p = 0; p < P; ++p) {
+        const double d = out[p] - target[p];
+        L += d * d;
+        dout[p] = 2.0 * d / static_cast<double>(P);
+        dz[p] = n.r(dout[p]);
+    }
+    *loss = L / static_cast<double>(P);
+    std::fill(grad, grad + kCParams, 0.0);
+    std::vector<double> g = dz;  // gradient at the current layer's output (cout channels)
+    std::vector<double> gb = dout;  // the same, as the bias gradient reads it
+    for (int l = 4; l >= 1; --l) {
+        const int cout = l == 4 ? 1 : 32, cin = l == 1 ? 1 : 32;
+        const int d = n.dil[l - 1];
+        for (size_t p = 0; p < P; ++p)
+            for (int co = 0; co < cout; ++co) grad[Net::bias_index(l, co)] += gb[p * cout + co];
+        for (size_t p = 0; p < P; ++p)
+            for (int t = 0; t < 9; ++t) {
+                const size_t q = n.shifted(p, t, d);
+                for (int co = 0; co < cout; ++co) {
+                    const double gv = g[p * cout + co];
+                    if (gv == 0.0) continue;
+                    for (int ci = 0; ci < cin; ++ci) grad[Net::weight_index(l, t, co, ci)] += gv * a[l - 1][q * cin + ci];
+                }
+            }
+        if (l == 1) break;
+        // gradient at the input activations, masked by ReLU'
+        std::vector<double> ga(P * 32, 0.0);
+        for (size_t p = 0; p < P; ++p)
+            for (int t = 0; t < 9; ++t) {
+                const size_t q = n.shifted(p, t, d);
+                for (int co = 0; co < cout; ++co) {
+                    const double gv = g[p * cout + co];
+                    if (gv == 0.0) continue;
+                    for (int ci = 0; ci < 32; ++ci) ga[q * 32 + ci] += n.weight(l, t, co, ci) * gv;
+                }
+            }
+        for (size_t i = 0; i < ga.size(); ++i) ga[i] = a[l - 1][i] > 0.0 ? n.r(ga[i]) : 0.0;
+        g = ga;
+        gb = ga;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// Internal layout of pic_hip_ctx (the opaque handle of include/pic_hip.h),
+// shared by the C ABI translation units.
+//
+// The context owns every device buffer of one PIC instance: the FP32 SoA
+// particle streams (and their sort double-buffers), the cell-sort scratch, the
+// FP32 rho / phi / Ex / Ey grids, the tile partials of the LDS deposit and the
+// hipFFT plans.  All work is enqueued on the context's stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <vector>
+
+#include "../../include/pic_hip.h"
+#include "api_util.hpp"
+#include "pic_common.hpp"
+#include "pic_kernels.hpp"
+
+using pichip::CurrentGrids;
+using pichip::DevDomain;
+using pichip::kStreams;
+using pichip::PoissonTables;
+using pichip::SortBuffers;
+using pichip::TileGeom;
+
+struct pic_hip_ctx {
+    pic_domain dom{};
+    DevDomain dev{};
+    int64_t G = 0;
+    size_t cap = 0;  // particle capacity (rounded, plus float4 over-read pad)
+    size_t n = 0;
+    hipStream_t stream = nullptr;
+    float* s[kStreams] = {};
+    float* alt[kStreams] = {};
+    SortBuffers sb{};
+    bool order_valid = false;
+    float* rho = nullptr;
+    float* overflow = nullptr;
+    float* comp = nullptr;
+    int* status = nullptr;
+    float* partial = nullptr;
+    size_t partial_cap = 0;
+    // LDS tiling per scheme (NGP, CIC, TSC, Esirkepov); the partial buffer is
+    // sized for the largest.
+    TileGeom geom[4]{};
+    int ppl[4] = {8, 8, 8, 4};
+    int pipe[4] = {1, 1, 0, 0};
+    float* phi = nullptr;
+    float* Ex = nullptr;
+    float* Ey = nullptr;
+    float* fft_in = nullptr;
+    float2* spec = nullptr;     // phi spectrum [Ny][Nx/2+1]
+    float2* spec_ex = nullptr;  // Ex, Ey spectra
+    float2* spec_ey = nullptr;
+    PoissonTables ptab{};
+    hipfftHandle plan_r2c = 0, plan_c2r = 0;
+    bool fft_ready = false;
+    double* diag_partials = nullptr;
+    double* diag_out = nullptr;
+    int diag_blocks = 0;
+    int* h_status = nullptr;     // pinned
+    double* h_diag = nullptr;    // pinned
+    CurrentGrids cur{};           // Esirkepov current buffers (allocated on first use)
+    double* cosk = nullptr;      // cos/sin(2 pi (i + 1/2) dx / Lx), mode-1 diagnostic tables
+    double* sink = nullptr;
+    int* step_counter = nullptr;
+    double* step_partials = nullptr;
+    std::vector<hipEvent_t> prof_events;  // pairs (begin, end)
+    int prof_max = 0, prof_used = 0;
+    bool profiling = false;
+
+    ~pic_hip_ctx() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : prof_events) (void)hipEventDestroy(e);
+        if (plan_r2c) (void)hipfftDestroy(plan_r2c);
+        if (plan_c2r) (void)hipfftDestroy(plan_c2r);
+        for (int i = 0; i < kStreams; ++i) {
+            (void)hipFree(s[i]);
+            (void)hipFree(alt[i]);
+        }
+        (void)hipFree(sb.keys);
+        (void)hipFree(sb.keys_alt);
+        (void)hipFree(sb.perm);
+        (void)hipFree(sb.perm_alt);
+        (void)hipFree(sb.hist);
+        (void)hipFree(sb.scan_tmp);
+        (void)hipFree(sb.cell_off);
+        (void)hipFree(rho);
+        (void)hipFree(overflow);
+        (void)hipFree(comp);
+        (void)hipFree(status);
+        (void)hipFree(partial);
+        (void)hipFree(phi);
+        (void)hipFree(Ex);
+        (void)hipFree(Ey);
+        (void)hipFree(fft_in);
+        (void)hipFree(spec);
+        (void)hipFree(spec_ex);
+        (void)hipFree(spec_ey);
+        (void)hipFree(ptab.k2x);
+        (void)hipFree(ptab.k2y);
+        (void)hipFree(ptab.dx_sym);
+        (void)hipFree(ptab.dy_sym);
+        (void)hipFree(diag_partials);
+        (void)hipFree(diag_out);
+        (void)hipFree(cur.Jx);  // one allocation holds all eight grids
+        (void)hipFree(cosk);
+        (void)hipFree(sink);
+        (void)hipFree(step_counter);
+        (void)hipFree(step_partials);
+        (void)hipHostFree(h_status);
+        (void)hipHostFree(h_diag);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};

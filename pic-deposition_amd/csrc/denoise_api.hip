@@ -1,0 +1,380 @@
+// C ABI of the learned density denoiser (pic_hip_denoiser_*, pic_hip_denoise,
+// pic_hip_deposit_subsample; declared in include/pic_hip.h).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/pic_hip.h"
+#include "api_util.hpp"
+#include "context.hpp"
+#include "pic_common.hpp"
+#include "pic_kernels.hpp"
+
+using namespace pichip;
+
+struct pic_hip_denoiser {
+    int H = 0, W = 0, max_batch = 0, max_pairs = 0;
+    int dil[4] = {1, 2, 4, 8};
+    int ncu = 1;
+    int step = 0;  // Adam steps taken
+    hipStream_t stream = nullptr;
+    CnnBuffers c{};
+    float* inputs = nullptr;   // [max_pairs][H][W]
+    float* targets = nullptr;  // [max_pairs][H][W]
+    int n_in = 0, n_tg = 0;
+    double* losses = nullptr;  // per-iteration loss of the last train call
+    size_t losses_cap = 0;
+    int4* items = nullptr;     // batch items of the last train call
+    size_t items_cap = 0;
+    double* diag_part = nullptr;  // rho sum scratch (5 * 1024)
+    double* diag_out = nullptr;   // [8]
+    double* scratch = nullptr;
+    double* result = nullptr;     // [4]
+    double* out64 = nullptr;      // [H][W]
+    double* h_result = nullptr;   // pinned [4]
+    // Work on the denoiser's buffers is enqueued on its own stream (training,
+    // host calls) or on a context's stream (pic_hip_denoise); each hand-over
+    // waits for the other side through these events.
+    hipEvent_t ev_net = nullptr, ev_ctx = nullptr;
+
+    ~pic_hip_denoiser() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (ev_net) (void)hipEventDestroy(ev_net);
+        if (ev_ctx) (void)hipEventDestroy(ev_ctx);
+        void* bufs[] = {c.w,       c.grad,      c.m,         c.v,       c.packed, c.x0,     c.out,     c.target,
+                        c.dout,    c.a1,        c.a2,        c.a3,      c.d2,     c.d3,     c.wpart,   c.bias_part,
+                        c.loss_part, inputs,    targets,     losses,    items,    diag_part, diag_out, scratch,
+                        result,    out64};
+        for (void* b : bufs) (void)hipFree(b);
+        (void)hipHostFree(h_result);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t plane() const { return (size_t)H * W; }
+};
+
+namespace {
+
+pic_status check_net(pic_hip_denoiser* n) {
+    if (!n) return fail(PIC_ERR_INVALID_ARGUMENT, "null denoiser");
+    return PIC_OK;
+}
+
+pic_status check_pair(pic_hip_denoiser* n, pic_hip_ctx* ctx) {
+    if (check_net(n) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!ctx) return fail(PIC_ERR_INVALID_ARGUMENT, "null context");
+    if (ctx->dom.Nx != n->W || ctx->dom.Ny != n->H)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "context grid differs from the denoiser's");
+    return PIC_OK;
+}
+
+CnnGeom geom(const pic_hip_denoiser* n, int batch) { return CnnGeom{batch, n->H, n->W}; }
+
+// He-normal weights (std sqrt(2 / fan_in)), zero biases, last layer x 0.1.
+std::vector<float> initial_params(uint64_t seed) {
+    std::vector<float> w(cnn::kParams, 0.0f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int off, int count, int fan_in, double gain) {
+        std::normal_distribution<double> nd(0.0, gain * std::sqrt(2.0 / fan_in));
+        for (int i = 0; i < count; ++i) w[(size_t)(off + i)] = (float)nd(rng);
+    };
+    fill(cnn::kW1, 288, 9, 1.0);
+    fill(cnn::kW2, 9216, 288, 1.0);
+    fill(cnn::kW3, 9216, 288, 1.0);
+    fill(cnn::kW4, 288, 288, 0.1);
+    return w;
+}
+
+pic_status upload_params(pic_hip_denoiser* n, const float* p) {
+    PIC_HIP_TRY(hipMemcpyAsync(n->c.w, p, cnn::kParams * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    launch_cnn_pack(n->c, n->stream);
+    PIC_HIP_TRY(hipGetLastError());
+    PIC_HIP_TRY(hipStreamSynchronize(n->stream));
+    return PIC_OK;
+}
+
+// FP64 rho sum of the context into n->diag_out[0], then delta into dst.
+void enqueue_normalise(pic_hip_denoiser* n, pic_hip_ctx* ctx, float* dst) {
+    launch_diagnostics(ctx->rho, nullptr, nullptr, ctx->G, nullptr, nullptr, nullptr, nullptr, 0, n->diag_part, 1024,
+                       n->diag_out, ctx->stream);
+    launch_normalize(ctx->rho, ctx->G, n->diag_out, dst, ctx->stream);
+}
+
+double kappa_of(int scheme) {
+    switch (scheme) {
+        case PIC_SCHEME_NGP: return 1.0;
+        case PIC_SCHEME_TSC: return 0.55 * 0.55;
+        default: return 4.0 / 9.0;  // CIC; the Esirkepov density of a short move is CIC-like
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+pic_status pic_hip_denoiser_create(int32_t Nx, int32_t Ny, int32_t max_batch, int32_t max_pairs,
+                                   const int32_t* dilation, uint64_t seed, pic_hip_denoiser** out) {
+    if (!out) return fail(PIC_ERR_INVALID_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    if (Nx < 32 || Ny < 4 || Nx % 32 != 0) return fail(PIC_ERR_INVALID_ARGUMENT, "need Nx a multiple of 32 and Ny >= 4");
+    if ((int64_t)Nx * Ny > (1ll << 26)) return fail(PIC_ERR_INVALID_ARGUMENT, "grid above 2^26 cells");
+    if (max_batch < 1 || max_batch > 256 || max_pairs < 0)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "max_batch must be 1..256 and max_pairs >= 0");
+    std::unique_ptr<pic_hip_denoiser> n(new pic_hip_denoiser());
+    n->H = Ny;
+    n->W = Nx;
+    n->max_batch = max_batch;
+    n->max_pairs = max_pairs;
+    for (int k = 0; k < 4; ++k) {
+        if (dilation) n->dil[k] = dilation[k];
+        if (n->dil[k] < 1 || n->dil[k] >= std::min(Nx, Ny))
+            return fail(PIC_ERR_INVALID_ARGUMENT, "dilation must be >= 1 and below the grid size");
+    }
+    int dev = 0;
+    hipDeviceProp_t prop;
+    PIC_HIP_TRY(hipGetDevice(&dev));
+    PIC_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    n->ncu = std::max(1, prop.multiProcessorCount);
+    PIC_HIP_TRY(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+    PIC_HIP_TRY(hipEventCreateWithFlags(&n->ev_net, hipEventDisableTiming));
+    PIC_HIP_TRY(hipEventCreateWithFlags(&n->ev_ctx, hipEventDisableTiming));
+    const size_t P = n->plane(), NB = P * (size_t)max_batch;
+    CnnBuffers& c = n->c;
+    PIC_HIP_TRY(dmalloc(&c.w, cnn::kParams));
+    PIC_HIP_TRY(dmalloc(&c.grad, cnn::kParams));
+    PIC_HIP_TRY(dmalloc(&c.m, cnn::kParams));
+    PIC_HIP_TRY(dmalloc(&c.v, cnn::kParams));
+    PIC_HIP_TRY(hipMemsetAsync(c.m, 0, cnn::kParams * sizeof(float), n->stream));
+    PIC_HIP_TRY(hipMemsetAsync(c.v, 0, cnn::kParams * sizeof(float), n->stream));
+    uint16_t* pk = nullptr;
+    PIC_HIP_TRY(dmalloc(&pk, (size_t)cnn::kPackVecs * 8));
+    c.packed = pk;
+    PIC_HIP_TRY(dmalloc(&c.x0, NB));
+    PIC_HIP_TRY(dmalloc(&c.out, NB));
+    PIC_HIP_TRY(dmalloc(&c.target, NB));
+    PIC_HIP_TRY(dmalloc(&c.dout, NB));
+    uint16_t* acts[5] = {};
+    for (auto& a : acts) PIC_HIP_TRY(dmalloc(&a, NB * 32));
+    c.a1 = acts[0];
+    c.a2 = acts[1];
+    c.a3 = acts[2];
+    c.d2 = acts[3];
+    c.d3 = acts[4];
+    PIC_HIP_TRY(dmalloc(&c.wpart, cnn_wpart_floats(n->ncu)));
+    PIC_HIP_TRY(dmalloc(&c.bias_part, (size_t)256 * 32));
+    PIC_HIP_TRY(dmalloc(&c.loss_part, 256));
+    PIC_HIP_TRY(dmalloc(&n->inputs, P * std::max(max_pairs, 1)));
+    PIC_HIP_TRY(dmalloc(&n->targets, P * std::max(max_pairs, 1)));
+    PIC_HIP_TRY(dmalloc(&n->diag_part, (size_t)5 * 1024));
+    PIC_HIP_TRY(dmalloc(&n->diag_out, 8));
+    PIC_HIP_TRY(dmalloc(&n->scratch, denoise_scratch_doubles()));
+    PIC_HIP_TRY(dmalloc(&n->result, 4));
+    PIC_HIP_TRY(dmalloc(&n->out64, P));
+    PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&n->h_result), 4 * sizeof(double), hipHostMallocDefault));
+    const std::vector<float> w0 = initial_params(seed);
+    pic_status st = upload_params(n.get(), w0.data());
+    if (st != PIC_OK) return st;
+    *out = n.release();
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_destroy(pic_hip_denoiser* net) {
+    delete net;
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_get_params(pic_hip_denoiser* net, float* params) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!params) return fail(PIC_ERR_INVALID_ARGUMENT, "null params");
+    PIC_HIP_TRY(hipMemcpyAsync(params, net->c.w, cnn::kParams * sizeof(float), hipMemcpyDeviceToHost, net->stream));
+    PIC_HIP_TRY(hipStreamSynchronize(net->stream));
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_set_params(pic_hip_denoiser* net, const float* params) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!params) return fail(PIC_ERR_INVALID_ARGUMENT, "null params");
+    PIC_HIP_TRY(hipMemsetAsync(net->c.m, 0, cnn::kParams * sizeof(float), net->stream));
+    PIC_HIP_TRY(hipMemsetAsync(net->c.v, 0, cnn::kParams * sizeof(float), net->stream));
+    net->step = 0;
+    return upload_params(net, params);
+}
+
+pic_status pic_hip_denoiser_forward_host(pic_hip_denoiser* net, const float* input, float* output, int32_t batch) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!input || !output) return fail(PIC_ERR_INVALID_ARGUMENT, "null image");
+    if (batch < 1 || batch > net->max_batch) return fail(PIC_ERR_INVALID_ARGUMENT, "batch outside 1..max_batch");
+    const size_t bytes = net->plane() * batch * sizeof(float);
+    PIC_HIP_TRY(hipMemcpyAsync(net->c.x0, input, bytes, hipMemcpyHostToDevice, net->stream));
+    launch_cnn_forward(net->c, geom(net, batch), net->dil, net->ncu, net->stream);
+    PIC_HIP_TRY(hipGetLastError());
+    PIC_HIP_TRY(hipMemcpyAsync(output, net->c.out, bytes, hipMemcpyDeviceToHost, net->stream));
+    PIC_HIP_TRY(hipStreamSynchronize(net->stream));
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_gradient_host(pic_hip_denoiser* net, const float* input, const float* target,
+                                          int32_t batch, double* loss, float* grad) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!input || !target) return fail(PIC_ERR_INVALID_ARGUMENT, "null image");
+    if (batch < 1 || batch > net->max_batch) return fail(PIC_ERR_INVALID_ARGUMENT, "batch outside 1..max_batch");
+    const size_t bytes = net->plane() * batch * sizeof(float);
+    const CnnGeom g = geom(net, batch);
+    PIC_HIP_TRY(hipMemcpyAsync(net->c.x0, input, bytes, hipMemcpyHostToDevice, net->stream));
+    PIC_HIP_TRY(hipMemcpyAsync(net->c.target, target, bytes, hipMemcpyHostToDevice, net->stream));
+    launch_cnn_forward(net->c, g, net->dil, net->ncu, net->stream);
+    launch_cnn_loss(net->c, g, net->diag_out, net->stream);
+    launch_cnn_backward(net->c, g, net->dil, net->ncu, net->stream);
+    PIC_HIP_TRY(hipGetLastError());
+    if (loss) PIC_HIP_TRY(hipMemcpyAsync(loss, net->diag_out, sizeof(double), hipMemcpyDeviceToHost, net->stream));
+    if (grad)
+        PIC_HIP_TRY(hipMemcpyAsync(grad, net->c.grad, cnn::kParams * sizeof(float), hipMemcpyDeviceToHost,
+                                   net->stream));
+    PIC_HIP_TRY(hipStreamSynchronize(net->stream));
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_add_sample(pic_hip_denoiser* net, pic_hip_ctx* ctx, int32_t is_target) {
+    if (check_pair(net, ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    int& count = is_target ? net->n_tg : net->n_in;
+    if (count >= net->max_pairs) return fail(PIC_ERR_INVALID_ARGUMENT, "sample set is full (max_pairs)");
+    float* dst = (is_target ? net->targets : net->inputs) + (size_t)count * net->plane();
+    PIC_HIP_TRY(hipEventRecord(net->ev_net, net->stream));
+    PIC_HIP_TRY(hipStreamWaitEvent(ctx->stream, net->ev_net, 0));
+    enqueue_normalise(net, ctx, dst);
+    PIC_HIP_TRY(hipGetLastError());
+    PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ++count;
+    return PIC_OK;
+}
+
+int32_t pic_hip_denoiser_pairs(pic_hip_denoiser* net) { return net ? std::min(net->n_in, net->n_tg) : 0; }
+
+pic_status pic_hip_denoiser_clear_samples(pic_hip_denoiser* net) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    net->n_in = net->n_tg = 0;
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_train(pic_hip_denoiser* net, int32_t iterations, int32_t batch, double lr, uint64_t seed,
+                                  double* loss_history) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (iterations < 1) return fail(PIC_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
+    if (batch < 1 || batch > net->max_batch) return fail(PIC_ERR_INVALID_ARGUMENT, "batch outside 1..max_batch");
+    if (!(lr > 0.0) || !std::isfinite(lr)) return fail(PIC_ERR_INVALID_ARGUMENT, "learning rate must be positive");
+    const int pairs = pic_hip_denoiser_pairs(net);
+    if (pairs < 1) return fail(PIC_ERR_INVALID_ARGUMENT, "no training pairs (pic_hip_denoiser_add_sample)");
+    const size_t nitems = (size_t)iterations * batch;
+    if (nitems > net->items_cap) {
+        (void)hipFree(net->items);
+        net->items = nullptr;
+        net->items_cap = 0;
+        PIC_HIP_TRY(dmalloc(&net->items, nitems));
+        net->items_cap = nitems;
+    }
+    if ((size_t)iterations > net->losses_cap) {
+        (void)hipFree(net->losses);
+        net->losses = nullptr;
+        net->losses_cap = 0;
+        PIC_HIP_TRY(dmalloc(&net->losses, (size_t)iterations));
+        net->losses_cap = (size_t)iterations;
+    }
+    std::mt19937_64 rng(seed);
+    std::vector<int4> h(nitems);
+    for (auto& it : h) {
+        it.x = (int)(rng() % (uint64_t)pairs);
+        it.y = (int)(rng() % (uint64_t)net->H);
+        it.z = (int)(rng() % (uint64_t)net->W);
+        it.w = 0;
+    }
+    PIC_HIP_TRY(hipMemcpyAsync(net->items, h.data(), nitems * sizeof(int4), hipMemcpyHostToDevice, net->stream));
+    const CnnGeom g = geom(net, batch);
+    for (int it = 0; it < iterations; ++it) {
+        const double frac = iterations > 1 ? (double)it / (iterations - 1) : 0.0;
+        const double lr_it = lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * frac)));
+        launch_cnn_batch(net->inputs, net->targets, net->items + (size_t)it * batch, g, net->c.x0, net->c.target,
+                         net->stream);
+        launch_cnn_forward(net->c, g, net->dil, net->ncu, net->stream);
+        launch_cnn_loss(net->c, g, net->losses + it, net->stream);
+        launch_cnn_backward(net->c, g, net->dil, net->ncu, net->stream);
+        launch_cnn_adam(net->c, (float)lr_it, 0.9f, 0.999f, 1e-8f, ++net->step, net->stream);
+        launch_cnn_pack(net->c, net->stream);
+    }
+    PIC_HIP_TRY(hipGetLastError());
+    if (loss_history)
+        PIC_HIP_TRY(hipMemcpyAsync(loss_history, net->losses, (size_t)iterations * sizeof(double),
+                                   hipMemcpyDeviceToHost, net->stream));
+    PIC_HIP_TRY(hipStreamSynchronize(net->stream));  // h (host items) stays alive until here
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double tolerance, int32_t* fallback,
+                           double* residual_ratio) {
+    if (check_pair(net, ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (scheme < 0 || scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "unknown scheme");
+    if (!(tolerance >= 0.0)) return fail(PIC_ERR_INVALID_ARGUMENT, "tolerance must be >= 0");
+    hipStream_t s = ctx->stream;
+    PIC_HIP_TRY(hipEventRecord(net->ev_net, net->stream));
+    PIC_HIP_TRY(hipStreamWaitEvent(s, net->ev_net, 0));
+    enqueue_normalise(net, ctx, net->c.x0);
+    launch_cnn_forward(net->c, geom(net, 1), net->dil, net->ncu, s);
+    const double cv = ctx->dev.dx * ctx->dev.dy;
+    launch_denoise_finish(net->c.out, ctx->rho, net->out64, ctx->G, net->diag_out, ctx->s[4], (int64_t)ctx->n, cv,
+                          kappa_of(scheme), tolerance, net->scratch, net->result, s);
+    PIC_HIP_TRY(hipGetLastError());
+    PIC_HIP_TRY(hipEventRecord(net->ev_ctx, s));
+    PIC_HIP_TRY(hipStreamWaitEvent(net->stream, net->ev_ctx, 0));
+    if (fallback || residual_ratio) {
+        PIC_HIP_TRY(hipMemcpyAsync(net->h_result, net->result, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+        PIC_HIP_TRY(hipStreamSynchronize(s));
+        if (fallback) *fallback = net->h_result[1] != 0.0 ? 1 : 0;
+        if (residual_ratio) *residual_ratio = net->h_result[0];
+    }
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_get_output_f64(pic_hip_denoiser* net, double* rho) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!rho) return fail(PIC_ERR_INVALID_ARGUMENT, "null rho");
+    // The net stream waits for the last pic_hip_denoise (ev_ctx).
+    PIC_HIP_TRY(hipMemcpyAsync(rho, net->out64, net->plane() * sizeof(double), hipMemcpyDeviceToHost, net->stream));
+    PIC_HIP_TRY(hipStreamSynchronize(net->stream));
+    return PIC_OK;
+}
+
+pic_status pic_hip_deposit_subsample(pic_hip_ctx* ctx, int scheme, uint32_t stride, uint64_t salt) {
+    if (!ctx) return fail(PIC_ERR_INVALID_ARGUMENT, "null context");
+    if (scheme < 0 || scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "unknown scheme");
+    if (stride < 1) return fail(PIC_ERR_INVALID_ARGUMENT, "stride must be >= 1");
+    DepositArgs a;
+    a.x = ctx->s[0];
+    a.y = ctx->s[1];
+    a.vx = ctx->s[2];
+    a.vy = ctx->s[3];
+    a.q = ctx->s[4];
+    a.n = (int64_t)ctx->n;
+    a.cell_off = nullptr;
+    a.dom = ctx->dev;
+    a.scheme = scheme;
+    a.backend = PIC_BACKEND_GPU_ATOMICS;
+    a.ppl = 4;
+    a.pipe = 0;
+    a.geom = ctx->geom[scheme];
+    a.dt = 0.0f;
+    a.rho = ctx->rho;
+    a.partial = nullptr;
+    a.overflow = ctx->overflow;
+    a.comp = ctx->comp;
+    a.status = ctx->status;
+    a.stream = ctx->stream;
+    PIC_HIP_TRY(launch_deposit_subsample(a, stride, salt, (float)stride));
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -124,7 +124,56 @@ void launch_tile_k(const DepositArgs& a) {
     else launch_tile_kp<S, 8>(a);
 }
 
+// Bernoulli thinning by hash: particle i (in the current order) is kept when
+// the top 32 bits of splitmix64(i, salt) fall below 2^32 / stride, i.e. with
+// probability 1/stride, independently of where the particle sits.
+__device__ __forceinline__ bool keep_particle(int64_t i, uint64_t salt, uint32_t threshold) {
+    uint64_t z = (uint64_t)i + salt * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (uint32_t)(z >> 32) < threshold;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void deposit_subsample_kernel(const float* __restrict__ px,
+                                                                const float* __restrict__ py,
+                                                                const float* __restrict__ pvx,
+                                                                const float* __restrict__ pvy,
+                                                                const float* __restrict__ pq, int64_t n, DevDomain d,
+                                                                float dt, uint32_t threshold, uint64_t salt,
+                                                                float scale, float* __restrict__ rho,
+                                                                float* __restrict__ comp, int* __restrict__ status) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep_particle(i, salt, threshold)) return;
+    const float vx = Stencil<S>::kNeedsVelocity ? pvx[i] : 0.0f;
+    const float vy = Stencil<S>::kNeedsVelocity ? pvy[i] : 0.0f;
+    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * scale * d.inv_vol, dt, d, rho, comp, status);
+}
+
+template <int S>
+void launch_subsample(const DepositArgs& a, uint32_t threshold, uint64_t salt, float scale) {
+    if (a.n > 0)
+        hipLaunchKernelGGL(deposit_subsample_kernel<S>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, a.stream,
+                           a.x, a.y, a.vx, a.vy, a.q, a.n, a.dom, a.dt, threshold, salt, scale, a.rho, a.comp,
+                           a.status);
+    const size_t G = (size_t)a.dom.Nx * a.dom.Ny;
+    hipLaunchKernelGGL(fold_comp_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.comp, G);
+}
+
 }  // namespace
+
+hipError_t launch_deposit_subsample(const DepositArgs& a, uint32_t stride, uint64_t salt, float scale) {
+    const uint32_t threshold = (uint32_t)((1ull << 32) / stride);
+    fill_async(a.rho, (size_t)a.dom.Nx * a.dom.Ny, 0.0f, a.stream);
+    switch (a.scheme) {
+        case NGP: launch_subsample<NGP>(a, threshold, salt, scale); break;
+        case CIC: launch_subsample<CIC>(a, threshold, salt, scale); break;
+        case TSC: launch_subsample<TSC>(a, threshold, salt, scale); break;
+        default: launch_subsample<ESK>(a, threshold, salt, scale); break;
+    }
+    return hipGetLastError();
+}
 
 TileGeom make_tile_geom(int Nx, int Ny, int T, int R) {
     TileGeom g;

@@ -46,6 +46,9 @@ struct DepositArgs {
 };
 
 hipError_t launch_deposit(const DepositArgs& a);
+// Global-atomic deposit of a Bernoulli(1/stride) subset of the particles
+// (selected by a hash of index and salt) with charges scaled by `scale`.
+hipError_t launch_deposit_subsample(const DepositArgs& a, uint32_t stride, uint64_t salt, float scale);
 void fill_async(float* p, size_t n, float v, hipStream_t s);
 
 // ---- sort.hip -------------------------------------------------------------
@@ -239,5 +242,56 @@ void launch_ens_gather_push(const EnsParticles& p, const float* Ex, const float*
 void launch_ens_diag(const double* rho_part, const double* field_part, const double* part_p, const EnsTables& t,
                      int* counters, int record_from, int cap, const StepRecords& rec, hipStream_t s);
 void launch_set_ints(int* p, int n, int v, hipStream_t s);
+
+// ---- denoise.hip --------------------------------------------------------------
+// Parameter vector of the 1 -> 32 -> 32 -> 32 -> 1 network (FP32 master copy):
+// W1[co][tap], b1, W2[tap][co][ci], b2, W3[tap][co][ci], b3, W4[tap][ci], b4.
+namespace cnn {
+constexpr int kW1 = 0, kB1 = 288, kW2 = 320, kB2 = 9536, kW3 = 9568, kB3 = 18784, kW4 = 18816, kB4 = 19104;
+constexpr int kParams = 19105;
+// Packed BF16 B operands (units of 8 BF16 = one lane's operand, 64 per block):
+// forward layers 1-4, then the input-gradient operands of layers 4, 3, 2.
+constexpr int kPackF2 = 128, kPackF3 = 1280, kPackF4 = 2432, kPackB4 = 3008, kPackB3 = 3136, kPackB2 = 4288;
+constexpr int kPackVecs = 5440;
+}  // namespace cnn
+
+struct CnnGeom {
+    int B, H, W;  // images, rows, columns (W a multiple of 32)
+};
+
+struct CnnBuffers {
+    float *w, *grad, *m, *v;  // [kParams]
+    void* packed;             // [kPackVecs * 8] BF16
+    float* x0;                // input  [B][H][W]
+    float* out;               // output [B][H][W]
+    float* target;            // [B][H][W]
+    float* dout;              // d loss / d out [B][H][W]
+    void *a1, *a2, *a3;       // BF16 [B][H][W][32] activations
+    void *d2, *d3;            // BF16 [B][H][W][32] gradients
+    float* wpart;             // cnn_wpart_floats(ncu)
+    double* bias_part;        // 256 * 32
+    double* loss_part;        // 256
+};
+
+void launch_cnn_forward(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, hipStream_t s);
+// Gradients of the loss into c.grad from c.dout (needs the forward activations).
+void launch_cnn_backward(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, hipStream_t s);
+size_t cnn_wpart_floats(int ncu);
+// Mean-squared error of c.out against c.target into *loss, and c.dout.
+void launch_cnn_loss(const CnnBuffers& c, const CnnGeom& g, double* loss, hipStream_t s);
+void launch_cnn_adam(const CnnBuffers& c, float lr, float b1, float b2, float eps, int step, hipStream_t s);
+void launch_cnn_pack(const CnnBuffers& c, hipStream_t s);
+// Training batch: image b = pair items[b].x of the sets, rolled by (items[b].y, items[b].z).
+void launch_cnn_batch(const float* inputs, const float* targets, const int4* items, const CnnGeom& g, float* in_b,
+                      float* tg_b, hipStream_t s);
+// delta = rho / (rho_sum / n) - 1.
+void launch_normalize(const float* rho, int64_t n, const double* rho_sum, float* delta, hipStream_t s);
+// Renormalisation and gate: result = {ratio, fallback, scale, sum q}; when the
+// gate passes rho (FP32) and out64 hold the renormalised learned density,
+// otherwise rho is left as deposited and out64 is its FP64 copy.
+void launch_denoise_finish(const float* delta_out, float* rho, double* out64, int64_t G, const double* rho_sum,
+                           const float* q, int64_t n, double cv, double kappa, double tol, double* scratch,
+                           double* result, hipStream_t s);
+size_t denoise_scratch_doubles();
 
 }  // namespace pichip

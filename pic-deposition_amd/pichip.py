@@ -190,6 +190,21 @@ _SIGS = {
     "pic_hip_ensemble_run": (ctypes.c_int, [_P, ctypes.POINTER(EnsembleRunConfig), ctypes.c_int32, ctypes.c_int32,
                                             _f64p, _f64p, _f64p, _f64p]),
     "pic_hip_ensemble_time": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _f64p]),
+    "pic_hip_denoiser_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i32p,
+                                               ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "pic_hip_denoiser_destroy": (ctypes.c_int, [_P]),
+    "pic_hip_denoiser_get_params": (ctypes.c_int, [_P, _f32p]),
+    "pic_hip_denoiser_set_params": (ctypes.c_int, [_P, _f32p]),
+    "pic_hip_denoiser_forward_host": (ctypes.c_int, [_P, _f32p, _f32p, ctypes.c_int32]),
+    "pic_hip_denoiser_gradient_host": (ctypes.c_int, [_P, _f32p, _f32p, ctypes.c_int32, _f64p, _f32p]),
+    "pic_hip_denoiser_add_sample": (ctypes.c_int, [_P, _P, ctypes.c_int32]),
+    "pic_hip_denoiser_pairs": (ctypes.c_int32, [_P]),
+    "pic_hip_denoiser_clear_samples": (ctypes.c_int, [_P]),
+    "pic_hip_denoiser_train": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
+                                              _f64p]),
+    "pic_hip_denoise": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, _i32p, _f64p]),
+    "pic_hip_denoiser_get_output_f64": (ctypes.c_int, [_P, _f64p]),
+    "pic_hip_deposit_subsample": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -412,6 +427,10 @@ class Context:
         _check(_lib.pic_hip_diagnostics(self._h, ctypes.byref(d)))
         return d
 
+    def deposit_subsample(self, scheme, stride: int, salt: int) -> None:
+        """rho from a hash-selected 1/stride subset of the particles, charges x stride."""
+        _check(_lib.pic_hip_deposit_subsample(self._h, int(scheme), int(stride), int(salt)))
+
     def autotune(self, scheme=DepositionScheme.CIC, cache_path=None, reps: int = 5):
         """pic_hip_autotune: (Tiling chosen, ms per deposit or 0.0 when read from the cache)."""
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -590,3 +609,105 @@ def deposit_charge(particles: dict, domain: Domain, scheme="CIC", backend="GPU_P
         raise RuntimeError("CPU backend requested from the HIP library; the CPU schemes live in the host code")
     return deposit_host(particles["x"], particles["y"], particles["q"], domain, s, b, sorted_,
                         particles.get("vx"), particles.get("vy"), esirkepov_dt)
+
+
+DENOISER_PARAMS = 19105
+
+
+class Denoiser:
+    """Learned density denoiser (pic_hip_denoiser): delta = rho / mean - 1 in and out."""
+
+    def __init__(self, Nx: int, Ny: int, max_batch: int = 8, max_pairs: int = 64, dilation=(1, 2, 4, 8),
+                 seed: int = 1):
+        self.Nx, self.Ny = int(Nx), int(Ny)
+        d = np.ascontiguousarray(dilation, dtype=np.int32)
+        h = _P()
+        _check(_lib.pic_hip_denoiser_create(self.Nx, self.Ny, int(max_batch), int(max_pairs),
+                                            _ptr(d, ctypes.c_int32), int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.pic_hip_denoiser_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def params(self) -> np.ndarray:
+        p = np.empty(DENOISER_PARAMS, dtype=np.float32)
+        _check(_lib.pic_hip_denoiser_get_params(self._h, _ptr(p, ctypes.c_float)))
+        return p
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != DENOISER_PARAMS:
+            raise ValueError(f"expected {DENOISER_PARAMS} parameters")
+        _check(_lib.pic_hip_denoiser_set_params(self._h, _ptr(a, ctypes.c_float)))
+
+    def forward(self, images) -> np.ndarray:
+        x = np.ascontiguousarray(images, dtype=np.float32)
+        out = np.empty_like(x)
+        _check(_lib.pic_hip_denoiser_forward_host(self._h, _ptr(x, ctypes.c_float), _ptr(out, ctypes.c_float),
+                                                  x.shape[0]))
+        return out
+
+    def gradient(self, images, targets):
+        x = np.ascontiguousarray(images, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        loss = np.zeros(1)
+        g = np.empty(DENOISER_PARAMS, dtype=np.float32)
+        _check(_lib.pic_hip_denoiser_gradient_host(self._h, _ptr(x, ctypes.c_float), _ptr(t, ctypes.c_float),
+                                                   x.shape[0], _ptr(loss, ctypes.c_double), _ptr(g, ctypes.c_float)))
+        return float(loss[0]), g
+
+    def add_sample(self, ctx: Context, is_target: bool) -> None:
+        _check(_lib.pic_hip_denoiser_add_sample(self._h, ctx.handle, int(bool(is_target))))
+
+    @property
+    def pairs(self) -> int:
+        return _lib.pic_hip_denoiser_pairs(self._h)
+
+    def clear(self) -> None:
+        _check(_lib.pic_hip_denoiser_clear_samples(self._h))
+
+    def train(self, iterations: int, batch: int = 8, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        hist = np.zeros(int(iterations))
+        _check(_lib.pic_hip_denoiser_train(self._h, int(iterations), int(batch), float(lr), int(seed),
+                                           _ptr(hist, ctypes.c_double)))
+        return hist
+
+    def denoise(self, ctx: Context, scheme=DepositionScheme.TSC, tolerance: float = 4.0, report: bool = True):
+        """Replaces ctx's rho by the learned density unless the gate falls back.
+        With report, returns (fallback, residual_ratio) (synchronous)."""
+        if not report:
+            _check(_lib.pic_hip_denoise(ctx.handle, self._h, int(scheme), float(tolerance), None, None))
+            return None
+        fb = np.zeros(1, dtype=np.int32)
+        r = np.zeros(1)
+        _check(_lib.pic_hip_denoise(ctx.handle, self._h, int(scheme), float(tolerance), _ptr(fb, ctypes.c_int32),
+                                    _ptr(r, ctypes.c_double)))
+        return bool(fb[0]), float(r[0])
+
+    def output_f64(self) -> np.ndarray:
+        out = np.empty(self.Nx * self.Ny)
+        _check(_lib.pic_hip_denoiser_get_output_f64(self._h, _ptr(out, ctypes.c_double)))
+        return out.reshape(self.Ny, self.Nx)
+
+
+def langmuir_delta_truth(Nx: int, Ny: int, amp: float, mode: int, scheme=DepositionScheme.TSC) -> np.ndarray:
+    """Expected delta = rho / mean - 1 of a CIC or TSC deposit of particles
+    drawn from n(x) ~ 1 + amp cos(2 pi mode x / Lx) (init kind 1): node i sits
+    at x = i dx and the shape function's Fourier factor is sinc(k dx / 2)^2
+    (CIC) or ^3 (TSC)."""
+    order = {DepositionScheme.CIC: 2, DepositionScheme.TSC: 3}[int(scheme)]
+    kdx = 2 * np.pi * mode / Nx
+    form = (np.sin(kdx / 2) / (kdx / 2)) ** order
+    row = amp * form * np.cos(kdx * np.arange(Nx))
+    return np.tile(row, (Ny, 1))
