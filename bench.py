@@ -15,6 +15,12 @@ timed with HIP events on the context's stream.  Peak: 8 TB/s HBM3E.
 CPU baseline: the reference's own OpenMP deposit (oracle/_ref, compiled from
 the reference sources) on the same particles, timed like
 depositChargeTimedSoA, on this host's cores.
+
+Also reported (not the headline): the 64-member Langmuir ensemble (persistent
+deposit kernels, autotuned tilings, graph-replayed steps) and the learned
+density study at 256^2 (8-ppc input with the device-trained network against
+plain deposits of 8-512 ppc: error against the exact density and wall time
+per step).
 """
 import argparse
 import json
@@ -67,13 +73,12 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(ctx, dom_args, budget_s=20.0):
+def cpu_baseline(p, dom_args, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = max(1, min(threads, os.cpu_count() or 1))
-    p = ctx.download()
     od = oracle.domain(*dom_args)
     kind = "reference" if oracle.ref_available() else "port"
     x = p["x"].astype(np.float64)
@@ -103,6 +108,135 @@ def cpu_baseline(ctx, dom_args, budget_s=20.0):
     }
 
 
+def ensemble_bench(pichip):
+    """64-member Langmuir ensemble (grids 64^2..512^2, 16-256 particles per
+    cell, CIC/TSC mixed): deposit by the persistent kernels with default and
+    autotuned tilings, sort, and graph-replayed steps."""
+    import tempfile
+
+    members = pichip.langmuir_ensemble()
+    n = sum(m.num_particles for m in members)
+    g = sum(m.Nx * m.Ny for m in members)
+    with pichip.Ensemble(members) as e:
+        e.init()
+        sort_ms = e.time("sort", 3)
+        e.sort()
+        dep_default = e.time("deposit", 10)
+        with tempfile.TemporaryDirectory() as tmp:
+            t0 = time.perf_counter()
+            e.autotune(os.path.join(tmp, "tiling.json"), 3)
+            tune_s = time.perf_counter() - t0
+        dep = e.time("deposit", 10)
+        step = e.time("step", 5, 0.1)
+        step_sort = e.time("step_sort", 3, 0.1)
+    return {"members": len(members), "particles": n, "cells": g,
+            "deposit_ms": dep, "deposit_particles_per_s": n / (dep / 1000.0),
+            "deposit_hbm_frac": (12 * n + 4 * g) / (dep / 1000.0) / 1e9 / HBM_PEAK_GBS,
+            "deposit_ms_default_tiling": dep_default, "autotune_s": tune_s, "sort_ms": sort_ms,
+            "step_ms": step, "step_with_sort_ms": step_sort,
+            "step_ms_sort_every_10": step + (step_sort - step) / 10.0,
+            "particle_steps_per_s_sort_every_10": n / ((step + (step_sort - step) / 10.0) / 1000.0)}
+
+
+def learned_study(pichip):
+    """Learned density at 256^2: trained on pairs this library deposits
+    (512-ppc TSC target, 1/64 hash subsample as the 8-ppc input) from 48
+    Langmuir states (amplitude 0.001-0.05, modes 1-4) and 24 two-stream
+    states (v0 0.1-0.3, unit plasma frequency, 0-30 steps evolved).  Errors
+    are RMS of delta = rho / mean - 1 against the exact expected density on
+    held-out Langmuir states; wall time per step from graph-replayed loops
+    (sort every 10 steps)."""
+    N, hi, stride = 256, 512, 64
+    dom = pichip.make_domain(1.0, 1.0, N, N)
+    n_hi = N * N * hi
+    rng = np.random.default_rng(2024)
+    out = {"grid": [N, N], "target_ppc": hi, "input_ppc": hi // stride, "dilation": [1, 3, 9, 27]}
+    t0 = time.perf_counter()
+    net = pichip.Denoiser(N, N, max_batch=8, max_pairs=80, seed=1)
+    with pichip.Context(dom, n_hi) as c:
+        def add_pair(seed):
+            c.deposit(2, 2)
+            net.add_sample(c, True)
+            c.deposit_subsample(2, stride, 1 + 1000 * seed)
+            net.add_sample(c, False)
+
+        for k in range(48):
+            c.init_particles(n_hi, kind=1, seed=100 + k, amp=float(rng.uniform(0.001, 0.05)),
+                             mode=int(rng.integers(1, 5)), q=-1.0 / n_hi)
+            c.sort()
+            add_pair(100 + k)
+        for k in range(24):
+            v0 = float(rng.uniform(0.1, 0.3))
+            c.init_particles(n_hi, kind=2, seed=200 + k, amp=v0, vth=0.01, pert=0.01 * v0,
+                             mode=int(rng.integers(1, 3)), q=pichip.unit_plasma_charge(n_hi, N, N), m=1.0)
+            steps = int(rng.integers(0, 31))
+            if steps:
+                c.run(steps, 2, 2, 0.1, 0.0, pichip.ORDER_PUSH_FIRST, sort_interval=10, use_graph=True)
+            c.sort()
+            add_pair(200 + k)
+    out["pairs"] = net.pairs
+    out["data_s"] = time.perf_counter() - t0
+    iters = 4000
+    t0 = time.perf_counter()
+    hist = net.train(iters, batch=8, lr=3e-3, seed=7)
+    out["train_s"] = time.perf_counter() - t0
+    out["train_iterations"] = iters
+    out["train_loss_first_last"] = [float(hist[:20].mean()), float(hist[-50:].mean())]
+
+    errs = {}
+    held = [(900, 0.04, 2), (901, 0.01, 1), (902, 0.03, 3), (903, 0.002, 4), (904, 0.0, 1)]
+    for ppc in (8, 32, 128, 512):
+        n = N * N * ppc
+        with pichip.Context(dom, n) as c:
+            for seed, amp, mode in held:
+                truth = pichip.langmuir_delta_truth(N, N, amp, mode, 2)
+                c.init_particles(n, kind=1, seed=seed, amp=amp, mode=mode, q=-1.0 / n)
+                c.sort()
+                c.deposit(2, 2)
+                rho = c.rho().astype(np.float64)
+                errs.setdefault(f"plain{ppc}", []).append(float(np.sqrt(np.mean((rho / rho.mean() - 1 - truth) ** 2))))
+                if ppc == 8:
+                    fb, _ = net.denoise(c, 2, tolerance=4.0)
+                    r = net.output_f64()
+                    errs.setdefault("learned8", []).append(
+                        float(np.sqrt(np.mean((r / r.mean() - 1 - truth) ** 2))))
+                    errs.setdefault("fallbacks", []).append(int(fb))
+    out["rms_delta_error"] = {k: float(np.mean(v)) for k, v in errs.items() if k != "fallbacks"}
+    out["gate_fallbacks"] = int(sum(errs["fallbacks"]))
+    # plain error ~ c / sqrt(ppc): particles per cell at which plain matches learned
+    pp = np.array([8, 32, 128, 512], float)
+    ee = np.array([out["rms_delta_error"][f"plain{int(p)}"] for p in pp])
+    cfit = float(np.exp(np.mean(np.log(ee) + 0.5 * np.log(pp))))
+    out["plain_ppc_at_learned_error"] = (cfit / out["rms_delta_error"]["learned8"]) ** 2
+
+    def step_ms(ppc, learned):
+        n = N * N * ppc
+        with pichip.Context(dom, n) as c:
+            def go(steps):
+                # reference normalisation (q = -1/N) and a thermal spread moving
+                # particles ~0.2 cells per step: a steady, drifting load
+                c.init_particles(n, kind=1, seed=5, amp=0.02, mode=1, vth=0.2 / N / 0.1, q=-1.0 / n, m=1.0)
+                c.sync()
+                t = time.perf_counter()
+                if learned:
+                    net.run(c, steps, 2, 2, dt=0.1, sort_interval=10, record_from=steps, tolerance=4.0)
+                else:
+                    c.run(steps, 2, 2, 0.1, 0.0, pichip.ORDER_PUSH_FIRST, sort_interval=10, record_from=steps,
+                          use_graph=True)
+                return time.perf_counter() - t
+
+            go(20)
+            # the difference of two run lengths removes graph capture and set-up
+            return (go(220) - go(20)) * 1000.0 / 200
+
+    out["step_ms_plain512"] = step_ms(512, False)
+    out["step_ms_plain8"] = step_ms(8, False)
+    out["step_ms_learned8"] = step_ms(8, True)
+    out["speedup_learned8_vs_plain512"] = out["step_ms_plain512"] / out["step_ms_learned8"]
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +246,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline CIC deposit only (for profiling runs)")
     ap.add_argument("--sweep", action="store_true", help="time every tiling candidate per scheme and exit")
     ap.add_argument("--tiling", type=str, default="", help="tile_x,tile_y,particles_per_lane,pipelined for CIC")
+    ap.add_argument("--skip-ensemble", action="store_true", help="skip the 64-member ensemble measurement")
+    ap.add_argument("--skip-learned", action="store_true", help="skip the learned-density study")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -179,6 +315,8 @@ def main():
                      "kernel": "deposit_tile_kernel<CIC>", "kernel_ms": kern_ms,
                      "algorithmic_bytes": deposit_bytes(1)},
     }
+    # the CPU baseline deposits these same cell-sorted particles
+    base_particles = None if args.no_cpu_baseline else ctx.download()
     if not args.quick:
         others = {}
         for scheme, name in ((0, "NGP"), (2, "TSC"), (3, "Esirkepov")):
@@ -197,9 +335,19 @@ def main():
         sort_ms = (time.perf_counter() - t0) * 1000.0 / 3
         others["cell sort"] = {"particles_per_s": N / (sort_ms / 1000.0), "ms": sort_ms}
         out["other"] = others
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(ctx, dom_args)
     ctx.close()
+    if not args.quick and not args.skip_ensemble:
+        out["ensemble"] = ensemble_bench(pichip)
+    if not args.quick and not args.skip_learned:
+        out["learned"] = learned_study(pichip)
+        lr = out["learned"]
+        out["learned_vs_plain"] = {
+            "learned8_rms_error": lr["rms_delta_error"]["learned8"],
+            "plain512_rms_error": lr["rms_delta_error"]["plain512"],
+            "learned8_step_ms": lr["step_ms_learned8"], "plain512_step_ms": lr["step_ms_plain512"],
+            "wall_time_ratio": lr["speedup_learned8_vs_plain512"]}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(base_particles, dom_args)
     print(json.dumps(out), flush=True)
 
 

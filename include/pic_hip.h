@@ -339,7 +339,8 @@ pic_status pic_hip_ensemble_time(pic_hip_ensemble* ens, int what, int reps, doub
  * the density a high-particles-per-cell deposit would give.  It works on the
  * relative fluctuation delta = rho / mean(rho) - 1: four 3x3 convolution
  * layers 1 -> 32 -> 32 -> 32 -> 1 with ReLU between them and per-layer
- * dilation (default 1, 2, 4, 8: a 31 x 31 receptive field), 19,105
+ * dilation (default 1, 3, 9, 27: an 81 x 81 receptive field in which every
+ * offset is reached by exactly one tap path), 19,105
  * parameters, BF16 weights and activations with FP32 accumulation on the
  * matrix cores, FP32 master weights trained with Adam on mean-squared error.
  * Training pairs are deposits made by this library (pic_hip_denoiser_add_sample
@@ -357,7 +358,7 @@ pic_status pic_hip_ensemble_time(pic_hip_ensemble* ens, int what, int reps, doub
 enum { PIC_DENOISER_PARAMS = 19105 };
 typedef struct pic_hip_denoiser pic_hip_denoiser;
 
-/* dilation: 4 values (NULL: 1, 2, 4, 8), each below min(Nx, Ny).  Weights
+/* dilation: 4 values (NULL: 1, 3, 9, 27), each below min(Nx, Ny).  Weights
  * get He-normal initialisation from seed (the last layer scaled by 0.1). */
 pic_status pic_hip_denoiser_create(int32_t Nx, int32_t Ny, int32_t max_batch, int32_t max_pairs,
                                    const int32_t* dilation, uint64_t seed, pic_hip_denoiser** out);
@@ -366,10 +367,17 @@ pic_status pic_hip_denoiser_destroy(pic_hip_denoiser* net);
  * b2[32], W3[9][32][32], b3[32], W4[9][32], b4[1]; tap = 3 * dy + dx. */
 pic_status pic_hip_denoiser_get_params(pic_hip_denoiser* net, float* params);
 pic_status pic_hip_denoiser_set_params(pic_hip_denoiser* net, const float* params);
-/* Network on host images [batch][Ny][Nx] (delta in, delta out), synchronous. */
+/* Standardisation: the network sees in_scale * delta and predicts
+ * out_scale * delta.  Both are 1 until the first pic_hip_denoiser_train,
+ * which sets them to 1 / RMS of the input and target sets unless
+ * pic_hip_denoiser_set_scales fixed them before. */
+pic_status pic_hip_denoiser_get_scales(pic_hip_denoiser* net, double* in_scale, double* out_scale);
+pic_status pic_hip_denoiser_set_scales(pic_hip_denoiser* net, double in_scale, double out_scale);
+/* The raw network on host images [batch][Ny][Nx] (no scales), synchronous. */
 pic_status pic_hip_denoiser_forward_host(pic_hip_denoiser* net, const float* input, float* output, int32_t batch);
-/* Mean-squared-error loss and its gradient with respect to the parameters
- * (PIC_DENOISER_PARAMS values) for one host batch, synchronous. */
+/* Mean-squared-error loss of the raw network and its gradient with respect
+ * to the parameters (PIC_DENOISER_PARAMS values) for one host batch,
+ * synchronous. */
 pic_status pic_hip_denoiser_gradient_host(pic_hip_denoiser* net, const float* input, const float* target,
                                           int32_t batch, double* loss, float* grad);
 /* Appends the context's current rho (as delta) to the input (is_target = 0)
@@ -386,6 +394,14 @@ pic_status pic_hip_denoiser_train(pic_hip_denoiser* net, int32_t iterations, int
  * (NULL to skip; non-NULL makes the call synchronous) report the gate. */
 pic_status pic_hip_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double tolerance, int32_t* fallback,
                            double* residual_ratio);
+/* pic_hip_run with the learned density: after every deposit the context's
+ * rho goes through pic_hip_denoise(ctx, net, cfg->scheme, tolerance) (gate
+ * decided on the device, no host round trip), before the background and the
+ * Poisson solve; graph capture covers it.  fallbacks (NULL to skip) receives
+ * the number of steps whose gate kept the plain deposit. */
+pic_status pic_hip_run_learned(pic_hip_ctx* ctx, pic_hip_denoiser* net, const pic_step_config* cfg, double tolerance,
+                               int32_t nsteps, int32_t record_from, double* field_energy, double* mode1_amplitude,
+                               double* kinetic_energy, double* charge_error, int32_t* fallbacks);
 /* FP64 density of the last pic_hip_denoise (renormalised, or the plain
  * deposit when the gate fell back), Nx * Ny values. */
 pic_status pic_hip_denoiser_get_output_f64(pic_hip_denoiser* net, double* rho);

@@ -516,7 +516,7 @@ pic_status ensure_step_buffers(pic_hip_ctx* ctx) {
 
 // Enqueues one step (see pic_hip_run).
 pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, bool copy_back,
-                        int record_from, int cap, const StepRecords& rec) {
+                        int record_from, int cap, const StepRecords& rec, const StepHook& post_deposit) {
     pic_status st = PIC_OK;
     const float dt = (float)cfg.dt;
     auto push = [&]() {
@@ -528,7 +528,9 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
             pic_status s2 = sort_particles(ctx, copy_back);
             if (s2 != PIC_OK) return s2;
         }
-        return pic_hip_deposit(ctx, cfg.scheme, cfg.backend, cfg.dt);
+        pic_status s3 = pic_hip_deposit(ctx, cfg.scheme, cfg.backend, cfg.dt);
+        if (s3 == PIC_OK && post_deposit) s3 = post_deposit();
+        return s3;
     };
     auto solve = [&]() -> pic_status {
         if (cfg.background != 0.0) launch_add_scalar(ctx->rho, ctx->G, (float)cfg.background, ctx->stream);
@@ -567,9 +569,9 @@ struct GraphPair {
 };
 
 pic_status capture_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, int record_from, int cap,
-                        const StepRecords& rec, GraphPair& g) {
+                        const StepRecords& rec, GraphPair& g, const StepHook& post_deposit) {
     PIC_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    pic_status st = enqueue_step(ctx, cfg, with_sort, true, record_from, cap, rec);
+    pic_status st = enqueue_step(ctx, cfg, with_sort, true, record_from, cap, rec, post_deposit);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (st != PIC_OK) {
@@ -588,6 +590,15 @@ extern "C" {
 
 pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nsteps, int32_t record_from,
                        double* field_energy, double* mode1_amplitude, double* kinetic_energy, double* charge_error) {
+    return run_steps(ctx, cfg, nsteps, record_from, field_energy, mode1_amplitude, kinetic_energy, charge_error,
+                     StepHook());
+}
+
+}  // extern "C"
+
+pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nsteps, int32_t record_from,
+                     double* field_energy, double* mode1_amplitude, double* kinetic_energy, double* charge_error,
+                     const StepHook& post_deposit) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     if (!cfg) return fail(PIC_ERR_INVALID_ARGUMENT, "null step config");
     if (nsteps < 0 || record_from < 0) return fail(PIC_ERR_INVALID_ARGUMENT, "negative step counts");
@@ -611,8 +622,8 @@ pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nst
     const bool priv = cfg->backend == PIC_BACKEND_GPU_PRIV;
     if (cfg->use_graph && nsteps > 0) {
         GraphPair g_sort, g_plain;
-        if ((st = capture_step(ctx, *cfg, priv, record_from, cap, rec, g_sort)) != PIC_OK) return st;
-        if (priv && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, g_plain)) != PIC_OK) return st;
+        if ((st = capture_step(ctx, *cfg, priv, record_from, cap, rec, g_sort, post_deposit)) != PIC_OK) return st;
+        if (priv && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, g_plain, post_deposit)) != PIC_OK) return st;
         for (int step = 0; step < nsteps; ++step) {
             const bool sort_now = priv && step % cfg->sort_interval == 0;
             PIC_HIP_TRY(hipGraphLaunch(sort_now || !priv ? g_sort.exec : g_plain.exec, ctx->stream));
@@ -621,7 +632,7 @@ pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nst
     } else {
         for (int step = 0; step < nsteps; ++step) {
             const bool sort_now = priv && step % cfg->sort_interval == 0;
-            if ((st = enqueue_step(ctx, *cfg, sort_now, false, record_from, cap, rec)) != PIC_OK) return st;
+            if ((st = enqueue_step(ctx, *cfg, sort_now, false, record_from, cap, rec, post_deposit)) != PIC_OK) return st;
         }
     }
     double* outs[4] = {field_energy, mode1_amplitude, kinetic_energy, charge_error};
@@ -631,6 +642,8 @@ pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nst
                                        hipMemcpyDeviceToHost, ctx->stream));
     return pic_hip_sync(ctx);
 }
+
+extern "C" {
 
 double pic_hip_dominant_frequency(const double* signal, int64_t n, double dt) {
     if (!signal || n < 8) return 0.0;

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <functional>
 #include <vector>
 
 #include "../../include/pic_hip.h"
@@ -114,3 +115,12 @@ struct pic_hip_ctx {
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// Work enqueued on the context's stream right after each step's deposit
+// (kernel launches only, so that it can be captured in a graph).
+using StepHook = std::function<pic_status()>;
+
+// pic_hip_run with an optional hook after the deposit.
+pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nsteps, int32_t record_from,
+                     double* field_energy, double* mode1_amplitude, double* kinetic_energy, double* charge_error,
+                     const StepHook& post_deposit);

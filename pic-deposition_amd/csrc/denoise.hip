@@ -343,20 +343,22 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ w, 
     pk[e] = (__bf16)v;
 }
 
-// delta = rho / mean - 1 with mean = *rho_sum / n (FP64).
+// delta = scale * (rho / mean - 1) with mean = *rho_sum / n (FP64).
 __global__ __launch_bounds__(256) void normalize_kernel(const float* __restrict__ rho, int64_t n,
-                                                        const double* __restrict__ rho_sum,
+                                                        const double* __restrict__ rho_sum, double scale,
                                                         float* __restrict__ delta) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double mean = *rho_sum / (double)n;
-    delta[i] = (float)((double)rho[i] / mean - 1.0);
+    delta[i] = (float)(scale * ((double)rho[i] / mean - 1.0));
 }
 
-// Training batch: image b = pair items[b].x rolled by (items[b].y, items[b].z).
+// Training batch: image b = pair items[b].x rolled by (items[b].y, items[b].z),
+// input and target multiplied by their scales.
 __global__ __launch_bounds__(256) void batch_kernel(const float* __restrict__ inputs,
                                                     const float* __restrict__ targets, const int4* __restrict__ items,
-                                                    CnnGeom g, float* __restrict__ in_b, float* __restrict__ tg_b) {
+                                                    CnnGeom g, float sa, float sb, float* __restrict__ in_b,
+                                                    float* __restrict__ tg_b) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t plane = (int64_t)g.H * g.W;
     if (i >= plane * g.B) return;
@@ -364,8 +366,8 @@ __global__ __launch_bounds__(256) void batch_kernel(const float* __restrict__ in
     const int y = (int)((i % plane) / g.W), x = (int)(i % g.W);
     const int4 it = items[b];
     const int64_t src = (int64_t)it.x * plane + (int64_t)((y + it.y) % g.H) * g.W + (x + it.z) % g.W;
-    in_b[i] = inputs[src];
-    tg_b[i] = targets[src];
+    in_b[i] = sa * inputs[src];
+    tg_b[i] = sb * targets[src];
 }
 
 // Per-block FP64 sums of q and q^2 over particles.
@@ -384,15 +386,16 @@ __global__ __launch_bounds__(256) void charge_moments_kernel(const float* __rest
     }
 }
 
-// Per-block FP64 sums of 1 + delta_out and of (mean (1 + delta_out) - rho_plain)^2.
-__global__ __launch_bounds__(256) void denoise_stats_kernel(const float* __restrict__ delta_out,
+// Per-block FP64 sums of 1 + delta_out and of (mean (1 + delta_out) - rho_plain)^2,
+// delta_out = net_out * inv_scale.
+__global__ __launch_bounds__(256) void denoise_stats_kernel(const float* __restrict__ net_out,
                                                             const float* __restrict__ rho, int64_t n,
-                                                            const double* __restrict__ rho_sum,
+                                                            const double* __restrict__ rho_sum, double inv_scale,
                                                             double* __restrict__ partial) {
     const double mean = *rho_sum / (double)n;
     double v[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double u = 1.0 + (double)delta_out[i];
+        const double u = 1.0 + (double)net_out[i] * inv_scale;
         const double r = mean * u - (double)rho[i];
         v[0] += u;
         v[1] += r * r;
@@ -407,7 +410,7 @@ __global__ __launch_bounds__(256) void denoise_stats_kernel(const float* __restr
 // Gate decision (one thread): result = {residual ratio, fallback flag, scale, Q}.
 __global__ void denoise_decide_kernel(const double* __restrict__ stats, int nbs, const double* __restrict__ moments,
                                       int nbm, const double* __restrict__ rho_sum, int64_t n, double cv, double kappa,
-                                      double tol, double* __restrict__ result) {
+                                      double tol, double* __restrict__ result, int* __restrict__ fallback_count) {
     if (threadIdx.x != 0) return;
     double s1 = 0.0, s2 = 0.0, q1 = 0.0, q2 = 0.0;
     for (int b = 0; b < nbs; ++b) {
@@ -427,12 +430,13 @@ __global__ void denoise_decide_kernel(const double* __restrict__ stats, int nbs,
     result[1] = ok ? 0.0 : 1.0;
     result[2] = scale;
     result[3] = q1;
+    if (!ok && fallback_count) *fallback_count += 1;
 }
 
-__global__ __launch_bounds__(256) void denoise_apply_kernel(const float* __restrict__ delta_out,
+__global__ __launch_bounds__(256) void denoise_apply_kernel(const float* __restrict__ net_out,
                                                             float* __restrict__ rho, double* __restrict__ out64,
                                                             int64_t n, const double* __restrict__ rho_sum,
-                                                            const double* __restrict__ result) {
+                                                            double inv_scale, const double* __restrict__ result) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (result[1] != 0.0) {  // fallback: the plain deposit stays
@@ -440,7 +444,7 @@ __global__ __launch_bounds__(256) void denoise_apply_kernel(const float* __restr
         return;
     }
     const double mean = *rho_sum / (double)n;
-    const double v = mean * (1.0 + (double)delta_out[i]) * result[2];
+    const double v = mean * (1.0 + (double)net_out[i] * inv_scale) * result[2];
     out64[i] = v;
     rho[i] = (float)v;
 }
@@ -531,29 +535,36 @@ void launch_cnn_pack(const CnnBuffers& c, hipStream_t s) {
                        reinterpret_cast<__bf16*>(c.packed));
 }
 
-void launch_cnn_batch(const float* inputs, const float* targets, const int4* items, const CnnGeom& g, float* in_b,
-                      float* tg_b, hipStream_t s) {
+void launch_cnn_batch(const float* inputs, const float* targets, const int4* items, const CnnGeom& g, float sa,
+                      float sb, float* in_b, float* tg_b, hipStream_t s) {
     const int64_t n = (int64_t)g.B * g.H * g.W;
     hipLaunchKernelGGL(batch_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, inputs, targets, items, g,
-                       in_b, tg_b);
+                       sa, sb, in_b, tg_b);
 }
 
-void launch_normalize(const float* rho, int64_t n, const double* rho_sum, float* delta, hipStream_t s) {
-    hipLaunchKernelGGL(normalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rho, n, rho_sum, delta);
+void launch_normalize(const float* rho, int64_t n, const double* rho_sum, double scale, float* delta,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(normalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rho, n, rho_sum, scale,
+                       delta);
 }
 
-void launch_denoise_finish(const float* delta_out, float* rho, double* out64, int64_t G, const double* rho_sum,
-                           const float* q, int64_t n, double cv, double kappa, double tol, double* scratch,
-                           double* result, hipStream_t s) {
+void launch_moments(const float* v, int64_t n, double* partial, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(charge_moments_kernel, dim3(nb), dim3(256), 0, s, v, n, partial);
+}
+
+void launch_denoise_finish(const float* delta_out, double inv_scale, float* rho, double* out64, int64_t G,
+                           const double* rho_sum, const float* q, int64_t n, double cv, double kappa, double tol,
+                           double* scratch, double* result, int* fallback_count, hipStream_t s) {
     const int nbs = 128, nbm = 128;
     double* stats = scratch;
     double* moments = scratch + 2 * nbs;
-    hipLaunchKernelGGL(denoise_stats_kernel, dim3(nbs), dim3(256), 0, s, delta_out, rho, G, rho_sum, stats);
+    hipLaunchKernelGGL(denoise_stats_kernel, dim3(nbs), dim3(256), 0, s, delta_out, rho, G, rho_sum, inv_scale,
+                       stats);
     hipLaunchKernelGGL(charge_moments_kernel, dim3(nbm), dim3(256), 0, s, q, n, moments);
     hipLaunchKernelGGL(denoise_decide_kernel, dim3(1), dim3(64), 0, s, stats, nbs, moments, nbm, rho_sum, G, cv, kappa,
-                       tol, result);
+                       tol, result, fallback_count);
     hipLaunchKernelGGL(denoise_apply_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, delta_out, rho, out64,
-                       G, rho_sum, result);
+                       G, rho_sum, inv_scale, result);
 }
 
 size_t denoise_scratch_doubles() { return 2 * 128 + 2 * 128; }

@@ -20,9 +20,13 @@ using namespace pichip;
 
 struct pic_hip_denoiser {
     int H = 0, W = 0, max_batch = 0, max_pairs = 0;
-    int dil[4] = {1, 2, 4, 8};
+    int dil[4] = {1, 3, 9, 27};  // balanced-ternary offsets: every shift in [-40, 40]^2 exactly once
     int ncu = 1;
     int step = 0;  // Adam steps taken
+    // The network sees in_scale * delta and predicts out_scale * delta; both
+    // are set from the training set's RMS at the first training (1 before).
+    double in_scale = 1.0, out_scale = 1.0;
+    bool scaled = false;
     hipStream_t stream = nullptr;
     CnnBuffers c{};
     float* inputs = nullptr;   // [max_pairs][H][W]
@@ -38,6 +42,7 @@ struct pic_hip_denoiser {
     double* result = nullptr;     // [4]
     double* out64 = nullptr;      // [H][W]
     double* h_result = nullptr;   // pinned [4]
+    int* fallbacks = nullptr;     // gate fallbacks counted during pic_hip_run_learned
     // Work on the denoiser's buffers is enqueued on its own stream (training,
     // host calls) or on a context's stream (pic_hip_denoise); each hand-over
     // waits for the other side through these events.
@@ -50,7 +55,7 @@ struct pic_hip_denoiser {
         void* bufs[] = {c.w,       c.grad,      c.m,         c.v,       c.packed, c.x0,     c.out,     c.target,
                         c.dout,    c.a1,        c.a2,        c.a3,      c.d2,     c.d3,     c.wpart,   c.bias_part,
                         c.loss_part, inputs,    targets,     losses,    items,    diag_part, diag_out, scratch,
-                        result,    out64};
+                        result,    out64,       fallbacks};
         for (void* b : bufs) (void)hipFree(b);
         (void)hipHostFree(h_result);
         if (stream) (void)hipStreamDestroy(stream);
@@ -99,11 +104,24 @@ pic_status upload_params(pic_hip_denoiser* n, const float* p) {
     return PIC_OK;
 }
 
-// FP64 rho sum of the context into n->diag_out[0], then delta into dst.
-void enqueue_normalise(pic_hip_denoiser* n, pic_hip_ctx* ctx, float* dst) {
+// FP64 rho sum of the context into n->diag_out[0], then scale * delta into dst.
+void enqueue_normalise(pic_hip_denoiser* n, pic_hip_ctx* ctx, float* dst, double scale) {
     launch_diagnostics(ctx->rho, nullptr, nullptr, ctx->G, nullptr, nullptr, nullptr, nullptr, 0, n->diag_part, 1024,
                        n->diag_out, ctx->stream);
-    launch_normalize(ctx->rho, ctx->G, n->diag_out, dst, ctx->stream);
+    launch_normalize(ctx->rho, ctx->G, n->diag_out, scale, dst, ctx->stream);
+}
+
+double kappa_of(int scheme);
+
+// Normalise, network, renormalise and gate on the context's stream (kernel
+// launches only).
+void enqueue_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double tolerance, int* fallback_count) {
+    enqueue_normalise(net, ctx, net->c.x0, net->in_scale);
+    launch_cnn_forward(net->c, geom(net, 1), net->dil, net->ncu, ctx->stream);
+    const double cv = ctx->dev.dx * ctx->dev.dy;
+    launch_denoise_finish(net->c.out, 1.0 / net->out_scale, ctx->rho, net->out64, ctx->G, net->diag_out, ctx->s[4],
+                          (int64_t)ctx->n, cv, kappa_of(scheme), tolerance, net->scratch, net->result, fallback_count,
+                          ctx->stream);
 }
 
 double kappa_of(int scheme) {
@@ -176,6 +194,7 @@ pic_status pic_hip_denoiser_create(int32_t Nx, int32_t Ny, int32_t max_batch, in
     PIC_HIP_TRY(dmalloc(&n->scratch, denoise_scratch_doubles()));
     PIC_HIP_TRY(dmalloc(&n->result, 4));
     PIC_HIP_TRY(dmalloc(&n->out64, P));
+    PIC_HIP_TRY(dmalloc(&n->fallbacks, 1));
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&n->h_result), 4 * sizeof(double), hipHostMallocDefault));
     const std::vector<float> w0 = initial_params(seed);
     pic_status st = upload_params(n.get(), w0.data());
@@ -247,10 +266,27 @@ pic_status pic_hip_denoiser_add_sample(pic_hip_denoiser* net, pic_hip_ctx* ctx, 
     float* dst = (is_target ? net->targets : net->inputs) + (size_t)count * net->plane();
     PIC_HIP_TRY(hipEventRecord(net->ev_net, net->stream));
     PIC_HIP_TRY(hipStreamWaitEvent(ctx->stream, net->ev_net, 0));
-    enqueue_normalise(net, ctx, dst);
+    enqueue_normalise(net, ctx, dst, 1.0);
     PIC_HIP_TRY(hipGetLastError());
     PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
     ++count;
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_get_scales(pic_hip_denoiser* net, double* in_scale, double* out_scale) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (in_scale) *in_scale = net->in_scale;
+    if (out_scale) *out_scale = net->out_scale;
+    return PIC_OK;
+}
+
+pic_status pic_hip_denoiser_set_scales(pic_hip_denoiser* net, double in_scale, double out_scale) {
+    if (check_net(net) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!(in_scale > 0.0) || !(out_scale > 0.0) || !std::isfinite(in_scale) || !std::isfinite(out_scale))
+        return fail(PIC_ERR_INVALID_ARGUMENT, "scales must be positive and finite");
+    net->in_scale = in_scale;
+    net->out_scale = out_scale;
+    net->scaled = true;
     return PIC_OK;
 }
 
@@ -285,6 +321,26 @@ pic_status pic_hip_denoiser_train(pic_hip_denoiser* net, int32_t iterations, int
         PIC_HIP_TRY(dmalloc(&net->losses, (size_t)iterations));
         net->losses_cap = (size_t)iterations;
     }
+    if (!net->scaled) {
+        // RMS of the input and target sets (FP64 sums on the device).
+        const int nb = 128;
+        const size_t len = (size_t)pairs * net->plane();
+        double* part = net->scratch;  // 2 * 128 + 2 * 128 doubles
+        launch_moments(net->inputs, (int64_t)len, part, nb, net->stream);
+        launch_moments(net->targets, (int64_t)len, part + 2 * nb, nb, net->stream);
+        std::vector<double> h(4 * nb);
+        PIC_HIP_TRY(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, net->stream));
+        PIC_HIP_TRY(hipStreamSynchronize(net->stream));
+        double s2[2] = {0.0, 0.0};
+        for (int k = 0; k < 2; ++k)
+            for (int b = 0; b < nb; ++b) s2[k] += h[(size_t)(2 * nb * k + 2 * b + 1)];
+        const double rin = std::sqrt(s2[0] / len), rtg = std::sqrt(s2[1] / len);
+        if (rin > 0.0 && rtg > 0.0) {
+            net->in_scale = 1.0 / rin;
+            net->out_scale = 1.0 / rtg;
+        }
+        net->scaled = true;
+    }
     std::mt19937_64 rng(seed);
     std::vector<int4> h(nitems);
     for (auto& it : h) {
@@ -298,8 +354,8 @@ pic_status pic_hip_denoiser_train(pic_hip_denoiser* net, int32_t iterations, int
     for (int it = 0; it < iterations; ++it) {
         const double frac = iterations > 1 ? (double)it / (iterations - 1) : 0.0;
         const double lr_it = lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * frac)));
-        launch_cnn_batch(net->inputs, net->targets, net->items + (size_t)it * batch, g, net->c.x0, net->c.target,
-                         net->stream);
+        launch_cnn_batch(net->inputs, net->targets, net->items + (size_t)it * batch, g, (float)net->in_scale,
+                         (float)net->out_scale, net->c.x0, net->c.target, net->stream);
         launch_cnn_forward(net->c, g, net->dil, net->ncu, net->stream);
         launch_cnn_loss(net->c, g, net->losses + it, net->stream);
         launch_cnn_backward(net->c, g, net->dil, net->ncu, net->stream);
@@ -322,11 +378,7 @@ pic_status pic_hip_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, 
     hipStream_t s = ctx->stream;
     PIC_HIP_TRY(hipEventRecord(net->ev_net, net->stream));
     PIC_HIP_TRY(hipStreamWaitEvent(s, net->ev_net, 0));
-    enqueue_normalise(net, ctx, net->c.x0);
-    launch_cnn_forward(net->c, geom(net, 1), net->dil, net->ncu, s);
-    const double cv = ctx->dev.dx * ctx->dev.dy;
-    launch_denoise_finish(net->c.out, ctx->rho, net->out64, ctx->G, net->diag_out, ctx->s[4], (int64_t)ctx->n, cv,
-                          kappa_of(scheme), tolerance, net->scratch, net->result, s);
+    enqueue_denoise(ctx, net, scheme, tolerance, nullptr);
     PIC_HIP_TRY(hipGetLastError());
     PIC_HIP_TRY(hipEventRecord(net->ev_ctx, s));
     PIC_HIP_TRY(hipStreamWaitEvent(net->stream, net->ev_ctx, 0));
@@ -336,6 +388,33 @@ pic_status pic_hip_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, 
         if (fallback) *fallback = net->h_result[1] != 0.0 ? 1 : 0;
         if (residual_ratio) *residual_ratio = net->h_result[0];
     }
+    return PIC_OK;
+}
+
+pic_status pic_hip_run_learned(pic_hip_ctx* ctx, pic_hip_denoiser* net, const pic_step_config* cfg, double tolerance,
+                               int32_t nsteps, int32_t record_from, double* field_energy, double* mode1_amplitude,
+                               double* kinetic_energy, double* charge_error, int32_t* fallbacks) {
+    if (check_pair(net, ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!cfg) return fail(PIC_ERR_INVALID_ARGUMENT, "null step config");
+    if (!(tolerance >= 0.0)) return fail(PIC_ERR_INVALID_ARGUMENT, "tolerance must be >= 0");
+    PIC_HIP_TRY(hipEventRecord(net->ev_net, net->stream));
+    PIC_HIP_TRY(hipStreamWaitEvent(ctx->stream, net->ev_net, 0));
+    launch_set_int(net->fallbacks, 0, ctx->stream);
+    const int scheme = cfg->scheme;
+    const StepHook hook = [ctx, net, scheme, tolerance]() -> pic_status {
+        enqueue_denoise(ctx, net, scheme, tolerance, net->fallbacks);
+        PIC_HIP_TRY(hipGetLastError());
+        return PIC_OK;
+    };
+    pic_status st = run_steps(ctx, cfg, nsteps, record_from, field_energy, mode1_amplitude, kinetic_energy,
+                              charge_error, hook);
+    if (st != PIC_OK) return st;
+    if (fallbacks) {
+        PIC_HIP_TRY(hipMemcpyAsync(fallbacks, net->fallbacks, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    PIC_HIP_TRY(hipEventRecord(net->ev_ctx, ctx->stream));
+    PIC_HIP_TRY(hipStreamWaitEvent(net->stream, net->ev_ctx, 0));
     return PIC_OK;
 }
 

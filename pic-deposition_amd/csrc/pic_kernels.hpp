@@ -281,17 +281,23 @@ size_t cnn_wpart_floats(int ncu);
 void launch_cnn_loss(const CnnBuffers& c, const CnnGeom& g, double* loss, hipStream_t s);
 void launch_cnn_adam(const CnnBuffers& c, float lr, float b1, float b2, float eps, int step, hipStream_t s);
 void launch_cnn_pack(const CnnBuffers& c, hipStream_t s);
-// Training batch: image b = pair items[b].x of the sets, rolled by (items[b].y, items[b].z).
-void launch_cnn_batch(const float* inputs, const float* targets, const int4* items, const CnnGeom& g, float* in_b,
-                      float* tg_b, hipStream_t s);
-// delta = rho / (rho_sum / n) - 1.
-void launch_normalize(const float* rho, int64_t n, const double* rho_sum, float* delta, hipStream_t s);
+// Training batch: image b = pair items[b].x of the sets, rolled by (items[b].y,
+// items[b].z); inputs times sa, targets times sb.
+void launch_cnn_batch(const float* inputs, const float* targets, const int4* items, const CnnGeom& g, float sa,
+                      float sb, float* in_b, float* tg_b, hipStream_t s);
+// delta = scale * (rho / (rho_sum / n) - 1).
+void launch_normalize(const float* rho, int64_t n, const double* rho_sum, double scale, float* delta,
+                      hipStream_t s);
+// Per-block FP64 sums of v and v^2 into partial[2 * block + {0, 1}].
+void launch_moments(const float* v, int64_t n, double* partial, int nb, hipStream_t s);
 // Renormalisation and gate: result = {ratio, fallback, scale, sum q}; when the
 // gate passes rho (FP32) and out64 hold the renormalised learned density,
-// otherwise rho is left as deposited and out64 is its FP64 copy.
-void launch_denoise_finish(const float* delta_out, float* rho, double* out64, int64_t G, const double* rho_sum,
-                           const float* q, int64_t n, double cv, double kappa, double tol, double* scratch,
-                           double* result, hipStream_t s);
+// otherwise rho is left as deposited, out64 is its FP64 copy and
+// *fallback_count (if not null) is incremented.
+// The network output is delta / inv_scale.
+void launch_denoise_finish(const float* delta_out, double inv_scale, float* rho, double* out64, int64_t G,
+                           const double* rho_sum, const float* q, int64_t n, double cv, double kappa, double tol,
+                           double* scratch, double* result, int* fallback_count, hipStream_t s);
 size_t denoise_scratch_doubles();
 
 }  // namespace pichip

@@ -199,11 +199,15 @@ _SIGS = {
     "pic_hip_denoiser_gradient_host": (ctypes.c_int, [_P, _f32p, _f32p, ctypes.c_int32, _f64p, _f32p]),
     "pic_hip_denoiser_add_sample": (ctypes.c_int, [_P, _P, ctypes.c_int32]),
     "pic_hip_denoiser_pairs": (ctypes.c_int32, [_P]),
+    "pic_hip_denoiser_get_scales": (ctypes.c_int, [_P, _f64p, _f64p]),
+    "pic_hip_denoiser_set_scales": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
     "pic_hip_denoiser_clear_samples": (ctypes.c_int, [_P]),
     "pic_hip_denoiser_train": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
                                               _f64p]),
     "pic_hip_denoise": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, _i32p, _f64p]),
     "pic_hip_denoiser_get_output_f64": (ctypes.c_int, [_P, _f64p]),
+    "pic_hip_run_learned": (ctypes.c_int, [_P, _P, ctypes.POINTER(StepConfig), ctypes.c_double, ctypes.c_int32,
+                                           ctypes.c_int32, _f64p, _f64p, _f64p, _f64p, _i32p]),
     "pic_hip_deposit_subsample": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64]),
 }
 
@@ -469,13 +473,22 @@ class Member:
                                          self.q, self.m))
 
 
+def unit_plasma_charge(n: int, Nx: int, Ny: int) -> float:
+    """Electron charge giving plasma frequency 1 with m = 1: the Poisson solve
+    follows the reference's normalisation (phi = cellVolume * rho / k^2 per
+    mode), so omega_p^2 = n0 q^2 cellVolume / m = n q^2 / (Nx Ny m)."""
+    return -float(np.sqrt(Nx * Ny / n))
+
+
 def langmuir_ensemble(count: int = 64, grids=(64, 128, 256, 512), ppc=(16, 256), amp=(0.001, 0.05),
-                      schemes=(DepositionScheme.CIC, DepositionScheme.TSC), seed: int = 2024, mode: int = 1):
+                      schemes=(DepositionScheme.CIC, DepositionScheme.TSC), seed: int = 2024, mode: int = 1,
+                      unit_plasma_frequency: bool = False):
     """Members of a Langmuir ensemble: square grids drawn from `grids`, particles
     per cell uniform in [ppc[0], ppc[1]], density perturbation amplitude
     uniform in [amp[0], amp[1]] on mode `mode`, scheme drawn from `schemes`,
-    seeds from one generator; unit box, cold electrons with q = -1/sqrt(N),
-    m = 1 (plasma frequency 1 in every member)."""
+    seeds from one generator; unit box, cold electrons with m = 1 and
+    q = -1/N (the reference's Langmuir normalisation, validation.cpp:200-261)
+    or, with unit_plasma_frequency, the charge of unit_plasma_charge."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -483,7 +496,7 @@ def langmuir_ensemble(count: int = 64, grids=(64, 128, 256, 512), ppc=(16, 256),
         n = g * g * int(rng.integers(ppc[0], ppc[1] + 1))
         out.append(Member(Nx=g, Ny=g, num_particles=n, scheme=int(rng.choice(schemes)), kind=1,
                           seed=int(rng.integers(1, 2**62)), amp=float(rng.uniform(*amp)), mode=mode,
-                          q=-1.0 / float(np.sqrt(n)), m=1.0))
+                          q=unit_plasma_charge(n, g, g) if unit_plasma_frequency else -1.0 / n, m=1.0))
     return out
 
 
@@ -617,7 +630,7 @@ DENOISER_PARAMS = 19105
 class Denoiser:
     """Learned density denoiser (pic_hip_denoiser): delta = rho / mean - 1 in and out."""
 
-    def __init__(self, Nx: int, Ny: int, max_batch: int = 8, max_pairs: int = 64, dilation=(1, 2, 4, 8),
+    def __init__(self, Nx: int, Ny: int, max_batch: int = 8, max_pairs: int = 64, dilation=(1, 3, 9, 27),
                  seed: int = 1):
         self.Nx, self.Ny = int(Nx), int(Ny)
         d = np.ascontiguousarray(dilation, dtype=np.int32)
@@ -670,6 +683,14 @@ class Denoiser:
     def add_sample(self, ctx: Context, is_target: bool) -> None:
         _check(_lib.pic_hip_denoiser_add_sample(self._h, ctx.handle, int(bool(is_target))))
 
+    def scales(self):
+        a, b = np.zeros(1), np.zeros(1)
+        _check(_lib.pic_hip_denoiser_get_scales(self._h, _ptr(a, ctypes.c_double), _ptr(b, ctypes.c_double)))
+        return float(a[0]), float(b[0])
+
+    def set_scales(self, in_scale: float, out_scale: float) -> None:
+        _check(_lib.pic_hip_denoiser_set_scales(self._h, float(in_scale), float(out_scale)))
+
     @property
     def pairs(self) -> int:
         return _lib.pic_hip_denoiser_pairs(self._h)
@@ -694,6 +715,21 @@ class Denoiser:
         _check(_lib.pic_hip_denoise(ctx.handle, self._h, int(scheme), float(tolerance), _ptr(fb, ctypes.c_int32),
                                     _ptr(r, ctypes.c_double)))
         return bool(fb[0]), float(r[0])
+
+    def run(self, ctx: Context, steps: int, scheme=DepositionScheme.TSC, backend=DepositionBackend.GPUPrivatized,
+            dt=0.1, background=0.0, order=ORDER_PUSH_FIRST, sort_interval=10, record_from=0, use_graph=True,
+            tolerance: float = 4.0):
+        """pic_hip_run_learned: the step loop with the learned density after each
+        deposit.  Returns (records dict as Context.run with particles, fallbacks)."""
+        cfg = StepConfig(int(scheme), int(backend), int(sort_interval), int(order), float(dt), float(background), 1,
+                         int(bool(use_graph)))
+        cap = max(steps - record_from, 0)
+        out = {k: np.zeros(cap) for k in ("field_energy", "mode1", "kinetic", "charge_error")}
+        fb = np.zeros(1, dtype=np.int32)
+        _check(_lib.pic_hip_run_learned(ctx.handle, self._h, ctypes.byref(cfg), float(tolerance), int(steps),
+                                        int(record_from), *[_ptr(out[k], ctypes.c_double) for k in out],
+                                        _ptr(fb, ctypes.c_int32)))
+        return out, int(fb[0])
 
     def output_f64(self) -> np.ndarray:
         out = np.empty(self.Nx * self.Ny)
