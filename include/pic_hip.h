@@ -129,6 +129,12 @@ size_t pic_hip_count(pic_hip_ctx* ctx);
  * at create time and pic_hip_autotune searches them. */
 pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int scheme, int tile_x, int tile_y, int particles_per_lane,
                               int pipelined);
+/* Drift halo (0..4 cells, default 0) of every scheme's tile windows: a tile
+ * deposits into LDS the particles sorted into it whose current cell is within
+ * `cells` of the tile, so that particles that moved since the last sort stay
+ * off the global-atomic overflow path.  Costs window LDS and merge reads. */
+pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells);
+int pic_hip_get_tile_halo(pic_hip_ctx* ctx);
 pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int scheme, int* tile_x, int* tile_y, int* particles_per_lane,
                               int* pipelined);
 
@@ -307,6 +313,8 @@ pic_status pic_hip_ensemble_set_tiling(pic_hip_ensemble* ens, int32_t member, in
                                        int particles_per_lane);
 pic_status pic_hip_ensemble_get_tiling(pic_hip_ensemble* ens, int32_t member, int* tile_x, int* tile_y,
                                        int* particles_per_lane);
+/* Drift halo of every member's tile windows (as pic_hip_set_tile_halo). */
+pic_status pic_hip_ensemble_set_tile_halo(pic_hip_ensemble* ens, int cells);
 /* Autotunes the members by class (grid, scheme, particles per cell rounded
  * to a power of two): each class's candidates are timed with only that
  * class's work items in the persistent kernels.  Needs a sort.  cache_path

@@ -39,7 +39,7 @@ pic_status apply_tiling(pic_hip_ctx* c, int scheme, int tx, int ty, int ppl, int
     if (scheme < -1 || scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "scheme must be -1 (all) or 0..3");
     if (tx < 1 || ty < 1) return fail(PIC_ERR_INVALID_ARGUMENT, "tile sizes must be >= 1");
     if (ppl != 4 && ppl != 8) return fail(PIC_ERR_INVALID_ARGUMENT, "particles_per_lane must be 4 or 8");
-    TileGeom g = make_tile_geom(c->dom.Nx, c->dom.Ny, tx, ty);
+    TileGeom g = make_tile_geom(c->dom.Nx, c->dom.Ny, tx, ty, c->halo);
     const size_t lds = (size_t)4 * g.lds_per_wave * sizeof(float);
     if (lds > kMaxDepositLds) return fail(PIC_ERR_INVALID_ARGUMENT, "tile too large for LDS (4 windows must fit 64 KiB)");
     for (int k = 0; k < 4; ++k) {
@@ -192,6 +192,25 @@ pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int scheme, int tile_x, int tile
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     return apply_tiling(ctx, scheme, tile_x, tile_y, particles_per_lane, pipelined);
 }
+
+pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (cells < 0 || cells > kMaxDriftHalo)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "tile halo must be 0.." + std::to_string(kMaxDriftHalo) + " cells");
+    const int old = ctx->halo;
+    ctx->halo = cells;
+    for (int k = 0; k < 4; ++k) {
+        const pic_status st = apply_tiling(ctx, k, ctx->geom[k].T, ctx->geom[k].R, ctx->ppl[k], ctx->pipe[k]);
+        if (st != PIC_OK) {
+            ctx->halo = old;
+            for (int j = 0; j < 4; ++j) (void)apply_tiling(ctx, j, ctx->geom[j].T, ctx->geom[j].R, ctx->ppl[j], ctx->pipe[j]);
+            return st;
+        }
+    }
+    return PIC_OK;
+}
+
+int pic_hip_get_tile_halo(pic_hip_ctx* ctx) { return ctx ? ctx->halo : 0; }
 
 pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int scheme, int* tile_x, int* tile_y, int* ppl, int* pipelined) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
@@ -747,7 +766,7 @@ pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path
     if (scheme < 0 || scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "scheme must be 0..3");
     if (!ctx->order_valid) return fail(PIC_ERR_NOT_SORTED, "autotune times the tiled deposit: call pic_hip_sort first");
     if (reps < 1) reps = 5;
-    const std::string key = tuning_key(scheme, ctx->dom.Nx, ctx->dom.Ny, ppc_bucket((int64_t)ctx->n, ctx->G));
+    const std::string key = tuning_key(scheme, ctx->dom.Nx, ctx->dom.Ny, ppc_bucket((int64_t)ctx->n, ctx->G), ctx->halo);
     TilingChoice best{ctx->geom[scheme].T, ctx->geom[scheme].R, ctx->ppl[scheme]};
     double best_ms = 0.0;
     const auto cached = tuning_cache_read(cache_path);
@@ -761,7 +780,7 @@ pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path
         PIC_HIP_TRY(hipEventCreate(&e1));
         std::unique_ptr<ihipEvent_t, hipError_t (*)(hipEvent_t)> g1(e1, hipEventDestroy);
         best_ms = -1.0;
-        for (const TilingChoice& c : tiling_candidates(ctx->dom.Nx, ctx->dom.Ny, false)) {
+        for (const TilingChoice& c : tiling_candidates(ctx->dom.Nx, ctx->dom.Ny, false, ctx->halo)) {
             if (apply_tiling(ctx, scheme, c.tx, c.ty, c.ppl, 0) != PIC_OK) continue;
             pic_status st = pic_hip_deposit(ctx, scheme, PIC_BACKEND_GPU_PRIV, 0.0);  // warm-up
             if (st != PIC_OK) return st;

@@ -97,12 +97,12 @@ struct TilingChoice {
 // Candidate tilings for a grid: tile shapes that fit the grid (a tile larger
 // than the grid only adds halo), 4 or 8 particles per lane (4 only when
 // `ppl4_only`), windows within the workgroup LDS budget.
-inline std::vector<TilingChoice> tiling_candidates(int Nx, int Ny, bool ppl4_only) {
+inline std::vector<TilingChoice> tiling_candidates(int Nx, int Ny, bool ppl4_only, int halo) {
     static const int shapes[][2] = {{16, 8}, {32, 4}, {32, 8}, {64, 4}, {64, 8}, {128, 4}, {16, 16}, {32, 16}, {128, 2}};
     std::vector<TilingChoice> out;
     for (const auto& sh : shapes) {
         if (sh[0] > std::max(Nx, 16) || sh[1] > std::max(Ny, 2)) continue;
-        const TileGeom g = make_tile_geom(Nx, Ny, sh[0], sh[1]);
+        const TileGeom g = make_tile_geom(Nx, Ny, sh[0], sh[1], halo);
         if ((size_t)4 * g.lds_per_wave * sizeof(float) > kMaxDepositLds) continue;
         for (int ppl : {4, 8}) {
             if (ppl4_only && ppl != 4) continue;
@@ -133,9 +133,10 @@ inline std::string device_arch() {
     return c == std::string::npos ? a : a.substr(0, c);
 }
 
-inline std::string tuning_key(int scheme, int Nx, int Ny, int64_t ppc) {
+inline std::string tuning_key(int scheme, int Nx, int Ny, int64_t ppc, int halo) {
     std::ostringstream k;
     k << device_arch() << "/" << scheme_tag(scheme) << "/" << Nx << "x" << Ny << "/ppc" << ppc;
+    if (halo) k << "/halo" << halo;
     return k.str();
 }
 

@@ -46,6 +46,7 @@ struct pic_hip_ctx {
     // sized for the largest.
     TileGeom geom[4]{};
     int ppl[4] = {8, 8, 8, 4};
+    int halo = 0;  // drift halo of every scheme's tile windows (cells)
     int pipe[4] = {1, 1, 0, 0};
     float* phi = nullptr;
     float* Ex = nullptr;

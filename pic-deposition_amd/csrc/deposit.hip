@@ -175,21 +175,23 @@ hipError_t launch_deposit_subsample(const DepositArgs& a, uint32_t stride, uint6
     return hipGetLastError();
 }
 
-TileGeom make_tile_geom(int Nx, int Ny, int T, int R) {
+TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D) {
     TileGeom g;
+    g.D = D < 0 ? 0 : (D > kMaxDriftHalo ? kMaxDriftHalo : D);
     g.T = T < 1 ? 1 : (T > Nx ? Nx : T);
     g.R = R < 1 ? 1 : (R > Ny ? Ny : R);
-    if (g.T > 4096) g.T = 4096;
+    if (g.T > 4096 - 2 * kMaxDriftHalo) g.T = 4096 - 2 * kMaxDriftHalo;  // window column fits 12 key bits
     if (g.R > 64) g.R = 64;
     g.ntx = (Nx + g.T - 1) / g.T;
     g.nty = (Ny + g.R - 1) / g.R;
     g.ntiles = g.ntx * g.nty;
-    g.WX = g.T + 3;
-    g.WY = g.R + 3;
+    g.WX = g.T + 3 + 2 * g.D;
+    g.WY = g.R + 3 + 2 * g.D;
     g.wsize = g.WX * g.WY;
     g.lds_per_wave = 2 * g.wsize + 4 * g.R + 1;
     const int rx = Nx % g.T, ry = Ny % g.R;
-    g.fast = g.T >= 3 && g.R >= 3 && g.ntx >= 3 && g.nty >= 3 && (rx == 0 || rx >= 3) && (ry == 0 || ry >= 3);
+    g.fast = g.D == 0 && g.T >= 3 && g.R >= 3 && g.ntx >= 3 && g.nty >= 3 && (rx == 0 || rx >= 3) &&
+             (ry == 0 || ry >= 3);
     return g;
 }
 

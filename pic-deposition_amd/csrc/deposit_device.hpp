@@ -261,7 +261,8 @@ struct TileCtx {
     float* win_s;
     float* win_c;
     int WX;
-    int x0, y0, tw, th;
+    int x0, y0, tw, th;  // the tile's own cells
+    int ax0, ay0, aw, ah;  // cells deposited into the window: the tile widened by the drift halo
 };
 
 // Processes one chunk: K consecutive particles per lane accumulated as runs of
@@ -303,8 +304,8 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             const bool ok = St::template weights<EX>(xx, yy, vxx, vyy, qv, dt, d, cx, cy, w);
             // Unwrapped cell: positions outside [0, L) fall outside every tile
             // and take the (wrapping) overflow path.
-            const int lx = cx - t.x0, ly = cy - t.y0;
-            const bool inside = ok && (unsigned)lx < (unsigned)t.tw && (unsigned)ly < (unsigned)t.th;
+            const int lx = cx - t.ax0, ly = cy - t.ay0;
+            const bool inside = ok && (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
             if (valid && !inside) {
                 overflow_scatter<S>(ok, cx, cy, w, xx, yy, vxx, vyy, qv, dt, d, overflow, comp, status);
             }
@@ -385,6 +386,10 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
     t.y0 = ty * g.R;
     t.tw = min(g.T, d.Nx - t.x0);
     t.th = min(g.R, d.Ny - t.y0);
+    t.ax0 = t.x0 - g.D;
+    t.ay0 = t.y0 - g.D;
+    t.aw = t.tw + 2 * g.D;
+    t.ah = t.th + 2 * g.D;
     t.WX = g.WX;
     const int th = t.th;
 
@@ -456,10 +461,10 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
 // Tiles (along one axis) whose window can hold node g: those owning cells
 // g-2 .. g+1 (window halo is 1 before and 2 after the tile).  Writes up to 4
 // distinct tile indices in a fixed order.
-__device__ __forceinline__ int window_candidates(int gnode, int T, int nt, int n, int (&t)[4]) {
+__device__ __forceinline__ int window_candidates(int gnode, int T, int nt, int n, int D,
+                                                 int (&t)[4 + 2 * kMaxDriftHalo]) {
     int cnt = 0;
-#pragma unroll
-    for (int o = -2; o <= 1; ++o) {
+    for (int o = -2 - D; o <= 1 + D; ++o) {
         const int c = wrap_index(gnode + o, n);
         const int ti = c / T;
         bool dup = false;
@@ -514,28 +519,29 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
             }
         }
     } else {
-        int txs[4], tys[4];
-        const int ncx = window_candidates(gx, g.T, g.ntx, d.Nx, txs);
-        const int ncy = window_candidates(gy, g.R, g.nty, d.Ny, tys);
+        int txs[4 + 2 * kMaxDriftHalo], tys[4 + 2 * kMaxDriftHalo];
+        const int D = g.D;
+        const int ncx = window_candidates(gx, g.T, g.ntx, d.Nx, D, txs);
+        const int ncy = window_candidates(gy, g.R, g.nty, d.Ny, D, tys);
         for (int iy = 0; iy < ncy; ++iy) {
             const int ty = tys[iy];
             const int y0 = ty * g.R;
             const int th = min(g.R, d.Ny - y0);
-            // images gy' = gy + m*Ny inside [y0 - 1, y0 + th + 2)
+            // images gy' = gy + m*Ny inside [y0 - 1 - D, y0 + th + 2 + D)
             int gyi = gy;
-            while (gyi - d.Ny >= y0 - 1) gyi -= d.Ny;
-            while (gyi < y0 - 1) gyi += d.Ny;
-            for (; gyi < y0 + th + 2; gyi += d.Ny) {
-                const int wy = gyi - (y0 - 1);
+            while (gyi - d.Ny >= y0 - 1 - D) gyi -= d.Ny;
+            while (gyi < y0 - 1 - D) gyi += d.Ny;
+            for (; gyi < y0 + th + 2 + D; gyi += d.Ny) {
+                const int wy = gyi - (y0 - 1 - D);
                 for (int ix = 0; ix < ncx; ++ix) {
                     const int tx = txs[ix];
                     const int x0 = tx * g.T;
                     const int tw = min(g.T, d.Nx - x0);
                     int gxi = gx;
-                    while (gxi - d.Nx >= x0 - 1) gxi -= d.Nx;
-                    while (gxi < x0 - 1) gxi += d.Nx;
-                    for (; gxi < x0 + tw + 2; gxi += d.Nx) {
-                        const int wx = gxi - (x0 - 1);
+                    while (gxi - d.Nx >= x0 - 1 - D) gxi -= d.Nx;
+                    while (gxi < x0 - 1 - D) gxi += d.Nx;
+                    for (; gxi < x0 + tw + 2 + D; gxi += d.Nx) {
+                        const int wx = gxi - (x0 - 1 - D);
                         const float v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
                         c += two_sum_err(s, v);
                         s += v;

@@ -69,6 +69,7 @@ struct pic_hip_ensemble {
     bool dirty = true;  // member table / work lists must be rebuilt before the next deposit
     std::vector<FftClass> classes;
     bool fft_ready = false;
+    int halo = 0;  // drift halo of every member's tile windows
 
     ~pic_hip_ensemble() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -113,7 +114,7 @@ pic_status set_member_tiling(pic_hip_ensemble* e, int i, int tx, int ty, int ppl
     if (ppl != 4 && ppl != 8) return fail(PIC_ERR_INVALID_ARGUMENT, "particles_per_lane must be 4 or 8");
     if (m.scheme == PIC_SCHEME_ESIRKEPOV && ppl != 4)
         return fail(PIC_ERR_INVALID_ARGUMENT, "ensemble Esirkepov members run 4 particles per lane");
-    const TileGeom g = make_tile_geom(m.d.Nx, m.d.Ny, tx, ty);
+    const TileGeom g = make_tile_geom(m.d.Nx, m.d.Ny, tx, ty, e->halo);
     if ((size_t)4 * g.lds_per_wave * sizeof(float) > kMaxDepositLds)
         return fail(PIC_ERR_INVALID_ARGUMENT, "tile too large for LDS (4 windows must fit 64 KiB)");
     m.g = g;
@@ -630,6 +631,28 @@ pic_status pic_hip_ensemble_set_tiling(pic_hip_ensemble* ens, int32_t member, in
     return PIC_OK;
 }
 
+pic_status pic_hip_ensemble_set_tile_halo(pic_hip_ensemble* ens, int cells) {
+    if (check_ens(ens) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (cells < 0 || cells > kMaxDriftHalo)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "tile halo must be 0.." + std::to_string(kMaxDriftHalo) + " cells");
+    PIC_HIP_TRY(hipStreamSynchronize(ens->stream));
+    const int old = ens->halo;
+    ens->halo = cells;
+    for (int i = 0; i < ens->M; ++i) {
+        const EnsMember& m = ens->mem[(size_t)i];
+        const pic_status st = set_member_tiling(ens, i, m.g.T, m.g.R, m.ppl);
+        if (st != PIC_OK) {
+            ens->halo = old;
+            for (int j = 0; j < ens->M; ++j) {
+                const EnsMember& mj = ens->mem[(size_t)j];
+                (void)set_member_tiling(ens, j, mj.g.T, mj.g.R, mj.ppl);
+            }
+            return st;
+        }
+    }
+    return PIC_OK;
+}
+
 pic_status pic_hip_ensemble_get_tiling(pic_hip_ensemble* ens, int32_t member, int* tile_x, int* tile_y,
                                        int* particles_per_lane) {
     if (check_member(ens, member) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
@@ -662,7 +685,7 @@ pic_status pic_hip_ensemble_autotune(pic_hip_ensemble* ens, const char* cache_pa
                 break;
             ++i1;
         }
-        const std::string key = tuning_key(a.scheme, a.d.Nx, a.d.Ny, bucket);
+        const std::string key = tuning_key(a.scheme, a.d.Nx, a.d.Ny, bucket, ens->halo);
         auto apply = [&](const TilingChoice& c) -> pic_status {
             for (int i = i0; i < i1; ++i) {
                 const pic_status st = set_member_tiling(ens, i, c.tx, c.ty, c.ppl);
@@ -677,7 +700,7 @@ pic_status pic_hip_ensemble_autotune(pic_hip_ensemble* ens, const char* cache_pa
         }
         TilingChoice best{};
         double best_ms = -1.0;
-        for (const TilingChoice& c : tiling_candidates(a.d.Nx, a.d.Ny, a.scheme == PIC_SCHEME_ESIRKEPOV)) {
+        for (const TilingChoice& c : tiling_candidates(a.d.Nx, a.d.Ny, a.scheme == PIC_SCHEME_ESIRKEPOV, ens->halo)) {
             if (apply(c) != PIC_OK) continue;
             pic_status st = rebuild_work(ens, i0, i1);
             if (st != PIC_OK) return st;

@@ -11,18 +11,24 @@
 
 namespace pichip {
 
+// Largest drift halo (cells) of the tile windows.
+constexpr int kMaxDriftHalo = 4;
+
 // Tiling of the grid for the LDS-privatised deposit: tiles of T cells along x
-// by R cells along y; each tile's LDS window spans nodes (x0-1 .. x0+T+1) x
-// (y0-1 .. y0+R+1), i.e. WX = T+3 by WY = R+3.
+// by R cells along y.  A tile's window takes the particles sorted into it whose
+// current cell lies within D cells of the tile (the drift halo: particles move
+// between sorts), so it spans nodes (x0-1-D .. x0+T+1+D) x (y0-1-D .. y0+R+1+D),
+// i.e. WX = T+3+2D by WY = R+3+2D; particles further out take the overflow path.
 struct TileGeom {
     int T, R;
+    int D;  // drift halo, 0 .. kMaxDriftHalo
     int ntx, nty, ntiles;
     int WX, WY, wsize;
     int lds_per_wave;  // floats: 2 windows + row tables
-    int fast;          // merge fast path: tiles >= 3 cells on each axis, >= 3 tiles per axis
+    int fast;          // merge fast path: no halo, tiles >= 3 cells on each axis, >= 3 tiles per axis
 };
 
-TileGeom make_tile_geom(int Nx, int Ny, int T, int R);
+TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D = 0);
 
 struct DepositArgs {
     const float *x, *y, *vx, *vy, *q;

@@ -142,6 +142,9 @@ _SIGS = {
     "pic_hip_sync": (ctypes.c_int, [_P]),
     "pic_hip_count": (ctypes.c_size_t, [_P]),
     "pic_hip_set_tiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pic_hip_set_tile_halo": (ctypes.c_int, [_P, ctypes.c_int]),
+    "pic_hip_get_tile_halo": (ctypes.c_int, [_P]),
+    "pic_hip_ensemble_set_tile_halo": (ctypes.c_int, [_P, ctypes.c_int]),
     "pic_hip_get_tiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pic_hip_upload": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_size_t]),
@@ -308,6 +311,13 @@ class Context:
     def set_tiling(self, tile_x: int, tile_y: int, particles_per_lane: int = 8, pipelined: int = 0,
                    scheme: int = -1) -> None:
         _check(_lib.pic_hip_set_tiling(self._h, int(scheme), tile_x, tile_y, particles_per_lane, int(pipelined)))
+
+    def set_tile_halo(self, cells: int) -> None:
+        _check(_lib.pic_hip_set_tile_halo(self._h, int(cells)))
+
+    @property
+    def tile_halo(self) -> int:
+        return _lib.pic_hip_get_tile_halo(self._h)
 
     def tiling(self, scheme: int = 1) -> Tiling:
         a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -570,6 +580,9 @@ class Ensemble:
 
     def set_tiling(self, tile_x: int, tile_y: int, particles_per_lane: int = 8, member: int = -1) -> None:
         _check(_lib.pic_hip_ensemble_set_tiling(self._h, int(member), tile_x, tile_y, particles_per_lane))
+
+    def set_tile_halo(self, cells: int) -> None:
+        _check(_lib.pic_hip_ensemble_set_tile_halo(self._h, int(cells)))
 
     def tiling(self, i: int) -> Tiling:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -81,23 +81,26 @@ def test_deposit_matches_oracle_and_single_context(hip, orc):
             assert np.array_equal(rho, single), i
 
 
-def test_drifted_particles_between_sorts(hip, orc):
+@pytest.mark.parametrize("halo", [0, 2])
+def test_drifted_particles_between_sorts(hip, orc, halo):
     """Two steps with one sort: the second deposit sees particles up to three
     cells away from their sorted tiles (overflow path), and stays exact
     against the oracle on the positions it deposited; a repeated deposit on
-    the same stale order (overflow grids cleared by the merge) is identical."""
-    say("ensemble drift")
+    the same stale order (overflow grids cleared by the merge) agrees to the
+    last bits (overflow atomics land in scheduling order)."""
+    say("ensemble drift", halo)
     members = mixed_members(hip)[:5]
     dt = 0.01
     for mb in members:
         mb.vmax = 3.0 * min(mb.Lx / mb.Nx, mb.Ly / mb.Ny) / dt
     with hip.Ensemble(members) as e:
+        e.set_tile_halo(halo)
         e.init()
         e.run(2, dt, sort_interval=5)  # step 1 deposits without re-sorting; the last step does not push
         first = [e.rho(i) for i in range(len(members))]
         e.deposit(dt)
         for i, mb in enumerate(members):
-            assert np.array_equal(first[i], e.rho(i)), i
+            assert rel_l2(e.rho(i), first[i]) <= 1e-7, i
             p = {k: v.astype(np.float64) for k, v in e.download(i).items()}
             ref = orc.deposit(mb.scheme, p["x"], p["y"], p["q"], orc.domain(mb.Lx, mb.Ly, mb.Nx, mb.Ny))
             err = rel_l2(first[i], ref)
