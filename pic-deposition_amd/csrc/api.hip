@@ -151,8 +151,9 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_diag), sizeof(double) * 8, hipHostMallocDefault));
     // Defaults from the MI355X tiling sweep at 1024^2 / 64 ppc.
     for (int k = 0; k < 4; ++k) {
-        const pic_status st = apply_tiling(c.get(), k, kDefaultTiling[k][0], kDefaultTiling[k][1],
-                                           kDefaultTiling[k][2], 0);
+        int tx = kDefaultTiling[k][0], ty = kDefaultTiling[k][1];
+        default_tile_shape(k, domain->Nx, domain->Ny, 0, 4096, 0, tx, ty);
+        const pic_status st = apply_tiling(c.get(), k, tx, ty, kDefaultTiling[k][2], 0);
         if (st != PIC_OK) return st;
     }
     PIC_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -526,6 +527,7 @@ pic_status ensure_step_buffers(pic_hip_ctx* ctx) {
     PIC_HIP_TRY(dmalloc(&ctx->cosk, (size_t)nx));
     PIC_HIP_TRY(dmalloc(&ctx->sink, (size_t)nx));
     PIC_HIP_TRY(dmalloc(&ctx->step_counter, 1));
+    PIC_HIP_TRY(dmalloc(&ctx->push_flag, 1));
     PIC_HIP_TRY(dmalloc(&ctx->step_partials, (size_t)7 * ctx->diag_blocks));
     PIC_HIP_TRY(hipMemcpyAsync(ctx->cosk, c.data(), nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     PIC_HIP_TRY(hipMemcpyAsync(ctx->sink, s.data(), nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -556,9 +558,19 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
         return pic_hip_poisson(ctx);
     };
     if (cfg.order == PIC_ORDER_PUSH_FIRST) {
-        push();
+        // The push that opens this step ran fused with the previous step's
+        // gather (or as the run's prologue); this step's gather is fused with
+        // the next push, which ctx->push_flag disables after the last step.
         if ((st = deposit()) != PIC_OK) return st;
         if ((st = solve()) != PIC_OK) return st;
+        launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
+                           ctx->Ey, ctx->dev, dt, 1, 1, ctx->stream, ctx->push_flag);
+        const bool parts = cfg.record_particles != 0;
+        launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3], ctx->s[4],
+                         ctx->s[5], parts ? (int64_t)ctx->n : 0, ctx->step_partials, ctx->diag_blocks,
+                         ctx->step_counter, record_from, cap, rec, ctx->stream);
+        PIC_HIP_TRY(hipGetLastError());
+        return PIC_OK;
     } else if (cfg.order == PIC_ORDER_DEPOSIT_SOLVE_PUSH) {
         if ((st = deposit()) != PIC_OK) return st;
         if ((st = solve()) != PIC_OK) return st;
@@ -590,7 +602,7 @@ struct GraphPair {
 pic_status capture_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, int record_from, int cap,
                         const StepRecords& rec, GraphPair& g, const StepHook& post_deposit) {
     PIC_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    pic_status st = enqueue_step(ctx, cfg, with_sort, true, record_from, cap, rec, post_deposit);
+    pic_status st = enqueue_step(ctx, cfg, with_sort, false, record_from, cap, rec, post_deposit);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (st != PIC_OK) {
@@ -639,17 +651,42 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
                     kinetic_energy ? dev_rec + 2 * cap : nullptr, charge_error ? dev_rec + 3 * cap : nullptr};
     launch_set_int(ctx->step_counter, 0, ctx->stream);
     const bool priv = cfg->backend == PIC_BACKEND_GPU_PRIV;
+    const bool fused = cfg->order == PIC_ORDER_PUSH_FIRST;
+    if (fused && nsteps > 0) {
+        launch_set_int(ctx->push_flag, 1, ctx->stream);
+        launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
+                           ctx->Ey, ctx->dev, (float)cfg->dt, 0, 1, ctx->stream);  // the first step's push
+    }
     if (cfg->use_graph && nsteps > 0) {
-        GraphPair g_sort, g_plain;
-        if ((st = capture_step(ctx, *cfg, priv, record_from, cap, rec, g_sort, post_deposit)) != PIC_OK) return st;
-        if (priv && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, g_plain, post_deposit)) != PIC_OK) return st;
+        // A sort permutes the particles into the other buffer set, so graphs
+        // exist for both: gs[p] sorts out of set p, gp[p] steps on set p.
+        // Capturing gs[0] then gs[1] leaves the host pointers where they began.
+        GraphPair gs[2], gp[2];
+        const bool plain = !priv || cfg->sort_interval > 1;
+        if (priv) {
+            if ((st = capture_step(ctx, *cfg, true, record_from, cap, rec, gs[0], post_deposit)) != PIC_OK) return st;
+            if (plain && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, gp[1], post_deposit)) != PIC_OK)
+                return st;
+            if ((st = capture_step(ctx, *cfg, true, record_from, cap, rec, gs[1], post_deposit)) != PIC_OK) return st;
+        }
+        if (plain && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, gp[0], post_deposit)) != PIC_OK)
+            return st;
+        int parity = 0;
         for (int step = 0; step < nsteps; ++step) {
-            const bool sort_now = priv && step % cfg->sort_interval == 0;
-            PIC_HIP_TRY(hipGraphLaunch(sort_now || !priv ? g_sort.exec : g_plain.exec, ctx->stream));
+            if (fused && step == nsteps - 1) launch_set_int(ctx->push_flag, 0, ctx->stream);
+            if (priv && step % cfg->sort_interval == 0) {
+                PIC_HIP_TRY(hipGraphLaunch(gs[parity].exec, ctx->stream));
+                parity ^= 1;
+            } else {
+                PIC_HIP_TRY(hipGraphLaunch(gp[parity].exec, ctx->stream));
+            }
         }
         PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (parity)
+            for (int i = 0; i < kStreams; ++i) std::swap(ctx->s[i], ctx->alt[i]);
     } else {
         for (int step = 0; step < nsteps; ++step) {
+            if (fused && step == nsteps - 1) launch_set_int(ctx->push_flag, 0, ctx->stream);
             const bool sort_now = priv && step % cfg->sort_interval == 0;
             if ((st = enqueue_step(ctx, *cfg, sort_now, false, record_from, cap, rec, post_deposit)) != PIC_OK) return st;
         }

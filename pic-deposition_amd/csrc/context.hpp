@@ -67,6 +67,7 @@ struct pic_hip_ctx {
     double* cosk = nullptr;      // cos/sin(2 pi (i + 1/2) dx / Lx), mode-1 diagnostic tables
     double* sink = nullptr;
     int* step_counter = nullptr;
+    int* push_flag = nullptr;    // device flag: the fused gather + push of a step pushes
     double* step_partials = nullptr;
     std::vector<hipEvent_t> prof_events;  // pairs (begin, end)
     int prof_max = 0, prof_used = 0;
@@ -110,6 +111,7 @@ struct pic_hip_ctx {
         (void)hipFree(cosk);
         (void)hipFree(sink);
         (void)hipFree(step_counter);
+        (void)hipFree(push_flag);
         (void)hipFree(step_partials);
         (void)hipHostFree(h_status);
         (void)hipHostFree(h_diag);

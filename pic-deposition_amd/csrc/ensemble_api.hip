@@ -293,10 +293,12 @@ struct Graph {
     }
 };
 
+// Captures one step.  A sort inside permutes into the other buffer set and
+// swaps the host pointers (graphs for both sets are captured in turn).
 pic_status capture(pic_hip_ensemble* e, float dt, bool with_sort, int record_from, int cap, const StepRecords& rec,
                    Graph& g) {
     PIC_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const pic_status st = enqueue_step(e, dt, with_sort, true, record_from, cap, rec);
+    const pic_status st = enqueue_step(e, dt, with_sort, false, record_from, cap, rec);
     hipGraph_t graph = nullptr;
     const hipError_t err = hipStreamEndCapture(e->stream, &graph);
     if (st != PIC_OK) {
@@ -391,8 +393,12 @@ pic_status pic_hip_ensemble_create(const pic_ensemble_member* members, int32_t c
         e->pchunk[(size_t)i + 1] = e->pchunk[(size_t)i] + chunks;
         e->gblk[(size_t)i + 1] = e->gblk[(size_t)i] + (m.G + 255) / 256;
         e->sblk[(size_t)i + 1] = e->sblk[(size_t)i] + ((int64_t)hx * u.domain.Ny + 255) / 256;
+        // Members share the persistent launch, so only the work per tile
+        // matters: at most 32768 particles per tile.
         const int* dt = kDefaultTiling[m.scheme];
-        m.g = make_tile_geom(m.d.Nx, m.d.Ny, dt[0], dt[1]);
+        int tx = dt[0], ty = dt[1];
+        default_tile_shape(m.scheme, m.d.Nx, m.d.Ny, m.n, 1, 32768, tx, ty);
+        m.g = make_tile_geom(m.d.Nx, m.d.Ny, tx, ty);
         m.ppl = dt[2];
     }
     if (slots + 64 > 0x7fffffc0ll) return fail(PIC_ERR_INVALID_ARGUMENT, "ensemble above 2^31 - 64 particle slots");
@@ -762,16 +768,25 @@ pic_status pic_hip_ensemble_run(pic_hip_ensemble* ens, const pic_ensemble_run_co
         launch_ens_gather_push(ens->particles(), ens->Ex, ens->Ey, ens->tables(), ens->nchunks, dt, 0,
                                ens->push_flag, ens->part_p, ens->stream);
     if (cfg->use_graph && nsteps > 0) {
-        Graph g_sort, g_plain;
-        if ((st = capture(ens, dt, true, record_from, cap, rec, g_sort)) != PIC_OK) return st;
-        if (cfg->sort_interval > 1 && nsteps > 1 &&
-            (st = capture(ens, dt, false, record_from, cap, rec, g_plain)) != PIC_OK)
-            return st;
+        // gs[p] sorts out of buffer set p, gp[p] steps on set p (see pic_hip_run).
+        Graph gs[2], gp[2];
+        const bool plain = cfg->sort_interval > 1 && nsteps > 1;
+        if ((st = capture(ens, dt, true, record_from, cap, rec, gs[0])) != PIC_OK) return st;
+        if (plain && (st = capture(ens, dt, false, record_from, cap, rec, gp[1])) != PIC_OK) return st;
+        if ((st = capture(ens, dt, true, record_from, cap, rec, gs[1])) != PIC_OK) return st;
+        if (plain && (st = capture(ens, dt, false, record_from, cap, rec, gp[0])) != PIC_OK) return st;
+        int parity = 0;
         for (int step = 0; step < nsteps; ++step) {
             if (step == nsteps - 1) launch_set_ints(ens->push_flag, 1, 0, ens->stream);
-            const bool sort_now = step % cfg->sort_interval == 0;
-            PIC_HIP_TRY(hipGraphLaunch(sort_now ? g_sort.exec : g_plain.exec, ens->stream));
+            if (step % cfg->sort_interval == 0) {
+                PIC_HIP_TRY(hipGraphLaunch(gs[parity].exec, ens->stream));
+                parity ^= 1;
+            } else {
+                PIC_HIP_TRY(hipGraphLaunch(gp[parity].exec, ens->stream));
+            }
         }
+        if (parity)
+            for (int i = 0; i < kStreams; ++i) std::swap(ens->s[i], ens->alt[i]);
         ens->sorted = true;
     } else {
         for (int step = 0; step < nsteps; ++step) {
@@ -799,16 +814,19 @@ pic_status pic_hip_ensemble_time(pic_hip_ensemble* ens, int what, int reps, doub
     if ((what == 0 || what == 2) && !ens->sorted)
         return fail(PIC_ERR_NOT_SORTED, "call pic_hip_ensemble_sort first");
     const float fdt = (float)dt;
-    Graph g;
+    Graph g[2];
     const StepRecords none{nullptr, nullptr, nullptr, nullptr};
+    int parity = 0;
     if (what >= 2) {
         launch_set_ints(ens->push_flag, 1, 1, ens->stream);
-        if ((st = capture(ens, fdt, what == 3, 0, 0, none, g)) != PIC_OK) return st;
+        if ((st = capture(ens, fdt, what == 3, 0, 0, none, g[0])) != PIC_OK) return st;
+        if (what == 3 && (st = capture(ens, fdt, true, 0, 0, none, g[1])) != PIC_OK) return st;
     }
     auto once = [&]() -> pic_status {
         if (what == 0) return enqueue_deposit(ens, fdt, 0, ens->gblocks);
         if (what == 1) return enqueue_sort(ens, false);
-        PIC_HIP_TRY(hipGraphLaunch(g.exec, ens->stream));
+        PIC_HIP_TRY(hipGraphLaunch(g[parity].exec, ens->stream));
+        if (what == 3) parity ^= 1;  // each sort graph permutes into the other buffer set
         return PIC_OK;
     };
     if ((st = once()) != PIC_OK) return st;  // warm-up
@@ -823,6 +841,8 @@ pic_status pic_hip_ensemble_time(pic_hip_ensemble* ens, int what, int reps, doub
     float ms = 0.0f;
     PIC_HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
     if (mean_ms) *mean_ms = ms / reps;
+    if (parity)
+        for (int i = 0; i < kStreams; ++i) std::swap(ens->s[i], ens->alt[i]);
     if (what >= 2) ens->sorted = true;
     return pic_hip_ensemble_sync(ens);
 }

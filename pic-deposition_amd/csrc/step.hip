@@ -10,15 +10,18 @@ namespace pichip {
 
 namespace {
 
-// Gather (v += q/m E dt) then push (x += v dt) of one particle per lane.
+// Gather (v += -(q/m) E dt) then push (x += v dt) of one particle per lane;
+// a non-null push_flag replaces do_push by its device value.
 __global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px, float* __restrict__ py,
                                                           float* __restrict__ pvx, float* __restrict__ pvy,
                                                           const float* __restrict__ pq, const float* __restrict__ pm,
                                                           int64_t n, const float* __restrict__ Ex,
                                                           const float* __restrict__ Ey, DevDomain d, float dt,
-                                                          int do_gather, int do_push) {
+                                                          int do_gather, int do_push,
+                                                          const int* __restrict__ push_flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (push_flag) do_push = *push_flag;
     float x = px[i], y = py[i];
     float vx = pvx[i], vy = pvy[i];
     if (do_gather) {
@@ -274,10 +277,10 @@ void launch_add_scalar(float* p, int64_t n, float v, hipStream_t s) {
 
 void launch_gather_push(float* x, float* y, float* vx, float* vy, const float* q, const float* m, int64_t n,
                         const float* Ex, const float* Ey, const DevDomain& d, float dt, int do_gather, int do_push,
-                        hipStream_t s) {
+                        hipStream_t s, const int* push_flag) {
     if (n > 0)
         hipLaunchKernelGGL(gather_push_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, vx, vy, q, m,
-                           n, Ex, Ey, d, dt, do_gather, do_push);
+                           n, Ex, Ey, d, dt, do_gather, do_push, push_flag);
 }
 
 void launch_diagnostics(const float* rho, const float* Ex, const float* Ey, int64_t G, const float* vx,
