@@ -133,11 +133,9 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(dmalloc(&c->sb.scan_tmp, c->sb.scan_tmp_cap));
     PIC_HIP_TRY(dmalloc(&c->sb.cell_off, (size_t)c->G + 1));
     PIC_HIP_TRY(dmalloc(&c->rho, (size_t)c->G));
-    PIC_HIP_TRY(dmalloc(&c->overflow, (size_t)c->G));
+    PIC_HIP_TRY(dmalloc(&c->acc64, (size_t)c->G));
     PIC_HIP_TRY(hipMemsetAsync(c->rho, 0, (size_t)c->G * sizeof(float), c->stream));
-    PIC_HIP_TRY(hipMemsetAsync(c->overflow, 0, (size_t)c->G * sizeof(float), c->stream));
-    PIC_HIP_TRY(dmalloc(&c->comp, (size_t)c->G));
-    PIC_HIP_TRY(hipMemsetAsync(c->comp, 0, (size_t)c->G * sizeof(float), c->stream));
+    PIC_HIP_TRY(hipMemsetAsync(c->acc64, 0, (size_t)c->G * sizeof(double), c->stream));
     PIC_HIP_TRY(dmalloc(&c->status, 1));
     PIC_HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
     PIC_HIP_TRY(dmalloc(&c->Ex, (size_t)c->G));
@@ -365,8 +363,7 @@ pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt)
     a.dt = (float)dt;
     a.rho = ctx->rho;
     a.partial = ctx->partial;
-    a.overflow = ctx->overflow;
-    a.comp = ctx->comp;
+    a.acc = ctx->acc64;
     a.status = ctx->status;
     a.stream = ctx->stream;
     if (ctx->profiling && ctx->prof_used < ctx->prof_max) {

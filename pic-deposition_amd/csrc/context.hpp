@@ -37,8 +37,7 @@ struct pic_hip_ctx {
     SortBuffers sb{};
     bool order_valid = false;
     float* rho = nullptr;
-    float* overflow = nullptr;
-    float* comp = nullptr;
+    double* acc64 = nullptr;  // FP64 global-atomic accumulation grid, zero between deposits
     int* status = nullptr;
     float* partial = nullptr;
     size_t partial_cap = 0;
@@ -90,8 +89,7 @@ struct pic_hip_ctx {
         (void)hipFree(sb.scan_tmp);
         (void)hipFree(sb.cell_off);
         (void)hipFree(rho);
-        (void)hipFree(overflow);
-        (void)hipFree(comp);
+        (void)hipFree(acc64);
         (void)hipFree(status);
         (void)hipFree(partial);
         (void)hipFree(phi);

@@ -448,8 +448,7 @@ pic_status pic_hip_deposit_subsample(pic_hip_ctx* ctx, int scheme, uint32_t stri
     a.dt = 0.0f;
     a.rho = ctx->rho;
     a.partial = nullptr;
-    a.overflow = ctx->overflow;
-    a.comp = ctx->comp;
+    a.acc = ctx->acc64;
     a.status = ctx->status;
     a.stream = ctx->stream;
     PIC_HIP_TRY(launch_deposit_subsample(a, stride, salt, (float)stride));

@@ -1,23 +1,23 @@
 // Charge-deposition kernels for gfx950.
 //
-//  * deposit_atomic_kernel  - "GPU_Atomics": one particle per lane, FP32 hardware
-//    atomics into rho.  Works on any particle order.  Counterpart of the
+//  * deposit_atomic_kernel  - "GPU_Atomics": one particle per lane, FP64
+//    hardware atomics into an accumulation grid that fold_acc_kernel rounds
+//    into rho (and clears).  Works on any particle order.  Counterpart of the
 //    reference depositKernelSoA (src/deposition/cuda/deposition_cuda.cu:205-210).
 //  * deposit_tile_kernel    - "GPU_Priv": needs cell-sorted particles and the
 //    cell offset table produced by the sort.  One wavefront owns one tile of
-//    T x R cells and a private LDS window of (T+3) x (R+3) nodes (halo -1..+2,
-//    enough for NGP/CIC/TSC/Esirkepov-density stencils).  Each lane streams K
+//    T x R cells and a private FP64 LDS window of (T+3+2D) x (R+3+2D) nodes
+//    (stencil halo -1..+2 plus a drift halo of D cells).  Each lane streams K
 //    consecutive particles with float4 loads, accumulates same-cell runs in
 //    registers, the wave then combines runs that straddle lanes with a DPP
-//    segmented scan, and only one lane per cell adds into LDS.  LDS adds use
-//    the returned old value to carry the exact rounding error (TwoSum) into a
-//    compensation window, so the per-tile partial sums are Kahan-compensated.
-//    Particles whose current cell is outside the tile (drift since the last
-//    sort) take an overflow path with global atomics.
-//  * deposit_merge_kernel   - writes rho node by node by adding the windows of
-//    the (at most 4 x 4) tiles that cover it in a fixed order, again with
-//    compensation.  Together with the per-wave windows this makes the sorted
-//    path's result independent of scheduling.
+//    segmented scan, and only one lane per cell adds into LDS (one
+//    non-returning FP64 LDS atomic per stencil value).  Particles whose
+//    current cell is outside the window's reach (drift since the last sort)
+//    take an overflow path with FP64 global atomics.
+//  * deposit_merge_kernel   - writes rho node by node by adding, in FP64 and
+//    in a fixed order, the windows of the tiles that cover it and the
+//    accumulation grid.  Together with the per-wave windows this makes the
+//    sorted path's result independent of scheduling.
 // The reference's own privatised kernel (deposition_cuda.cu:212-241) scans all
 // particles per tile; this design reads each particle once.
 
@@ -32,21 +32,21 @@ __global__ __launch_bounds__(256) void deposit_atomic_kernel(const float* __rest
                                                              const float* __restrict__ pvx,
                                                              const float* __restrict__ pvy,
                                                              const float* __restrict__ pq, int64_t n, DevDomain d,
-                                                             float dt, float* __restrict__ rho,
-                                                             float* __restrict__ comp, int* __restrict__ status) {
+                                                             float dt, double* __restrict__ gacc,
+                                                             int* __restrict__ status) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float vx = Stencil<S>::kNeedsVelocity ? pvx[i] : 0.0f;
     const float vy = Stencil<S>::kNeedsVelocity ? pvy[i] : 0.0f;
-    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * d.inv_vol, dt, d, rho, comp, status);
+    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * d.inv_vol, dt, d, gacc, status);
 }
 
-// rho += comp; comp = 0 (closes the compensated atomic sums).
-__global__ __launch_bounds__(256) void fold_comp_kernel(float* __restrict__ rho, float* __restrict__ comp, size_t n) {
+// rho = FP32(acc); acc = 0 (closes the FP64 atomic sums).
+__global__ __launch_bounds__(256) void fold_acc_kernel(float* __restrict__ rho, double* __restrict__ gacc, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        rho[i] += comp[i];
-        comp[i] = 0.0f;
+        rho[i] = (float)gacc[i];
+        gacc[i] = 0.0;
     }
 }
 
@@ -57,22 +57,22 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                                                            const float* __restrict__ pq,
                                                            const int* __restrict__ cell_off, DevDomain d, TileGeom g,
                                                            float dt, float* __restrict__ partial,
-                                                           float* __restrict__ overflow, float* __restrict__ comp,
-                                                           int* __restrict__ status) {
-    extern __shared__ float smem[];
+                                                           double* __restrict__ gacc, int* __restrict__ status) {
+    extern __shared__ double smem_d[];
+    float* smem = reinterpret_cast<float*>(smem_d);
     const int wv = threadIdx.x >> 6;
     const int tile = blockIdx.x * (blockDim.x >> 6) + wv;
     if (tile >= g.ntiles) return;  // wave-uniform
     deposit_one_tile<S, K, EX, PIPE>(tile, smem + (size_t)wv * g.lds_per_wave, px, py, pvx, pvy, pq, cell_off, d, g,
-                                     dt, partial, overflow, comp, status);
+                                     dt, partial, gacc, status);
 }
 
 __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
-                                                            float* __restrict__ overflow, float* __restrict__ comp,
-                                                            DevDomain d, TileGeom g, float* __restrict__ rho) {
+                                                            double* __restrict__ gacc, DevDomain d, TileGeom g,
+                                                            float* __restrict__ rho) {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
-    if (gx < d.Nx) rho[(size_t)gy * d.Nx + gx] = merge_node(gx, gy, partial, overflow, comp, d, g);
+    if (gx < d.Nx) rho[(size_t)gy * d.Nx + gx] = merge_node(gx, gy, partial, gacc, d, g);
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, size_t n, float v) {
@@ -86,10 +86,10 @@ void launch_atomic(const DepositArgs& a) {
     if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
     if (a.n > 0)
         hipLaunchKernelGGL(deposit_atomic_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, a.stream, a.x, a.y, a.vx,
-                           a.vy, a.q, a.n, a.dom, a.dt, a.rho, a.comp, a.status);
+                           a.vy, a.q, a.n, a.dom, a.dt, a.acc, a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
     const size_t G = (size_t)a.dom.Nx * a.dom.Ny;
-    hipLaunchKernelGGL(fold_comp_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.comp, G);
+    hipLaunchKernelGGL(fold_acc_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.acc, G);
 }
 
 template <int S, int K, int EX, int PIPE>
@@ -100,11 +100,10 @@ void launch_tile(const DepositArgs& a) {
     const size_t lds = (size_t)waves_per_block * g.lds_per_wave * sizeof(float);
     if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
     hipLaunchKernelGGL((deposit_tile_kernel<S, K, EX, PIPE>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream,
-                       a.x, a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.overflow, a.comp,
-                       a.status);
+                       a.x, a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.acc, a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
     hipLaunchKernelGGL(deposit_merge_kernel, dim3((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny), dim3(256), 0,
-                       a.stream, a.partial, a.overflow, a.comp, a.dom, g, a.rho);
+                       a.stream, a.partial, a.acc, a.dom, g, a.rho);
 }
 
 template <int S, int K>
@@ -142,30 +141,28 @@ __global__ __launch_bounds__(256) void deposit_subsample_kernel(const float* __r
                                                                 const float* __restrict__ pvy,
                                                                 const float* __restrict__ pq, int64_t n, DevDomain d,
                                                                 float dt, uint32_t threshold, uint64_t salt,
-                                                                float scale, float* __restrict__ rho,
-                                                                float* __restrict__ comp, int* __restrict__ status) {
+                                                                float scale, double* __restrict__ gacc,
+                                                                int* __restrict__ status) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !keep_particle(i, salt, threshold)) return;
     const float vx = Stencil<S>::kNeedsVelocity ? pvx[i] : 0.0f;
     const float vy = Stencil<S>::kNeedsVelocity ? pvy[i] : 0.0f;
-    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * scale * d.inv_vol, dt, d, rho, comp, status);
+    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * scale * d.inv_vol, dt, d, gacc, status);
 }
 
 template <int S>
 void launch_subsample(const DepositArgs& a, uint32_t threshold, uint64_t salt, float scale) {
     if (a.n > 0)
         hipLaunchKernelGGL(deposit_subsample_kernel<S>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, a.stream,
-                           a.x, a.y, a.vx, a.vy, a.q, a.n, a.dom, a.dt, threshold, salt, scale, a.rho, a.comp,
-                           a.status);
+                           a.x, a.y, a.vx, a.vy, a.q, a.n, a.dom, a.dt, threshold, salt, scale, a.acc, a.status);
     const size_t G = (size_t)a.dom.Nx * a.dom.Ny;
-    hipLaunchKernelGGL(fold_comp_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.comp, G);
+    hipLaunchKernelGGL(fold_acc_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.acc, G);
 }
 
 }  // namespace
 
 hipError_t launch_deposit_subsample(const DepositArgs& a, uint32_t stride, uint64_t salt, float scale) {
     const uint32_t threshold = (uint32_t)((1ull << 32) / stride);
-    fill_async(a.rho, (size_t)a.dom.Nx * a.dom.Ny, 0.0f, a.stream);
     switch (a.scheme) {
         case NGP: launch_subsample<NGP>(a, threshold, salt, scale); break;
         case CIC: launch_subsample<CIC>(a, threshold, salt, scale); break;
@@ -188,7 +185,7 @@ TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D) {
     g.WX = g.T + 3 + 2 * g.D;
     g.WY = g.R + 3 + 2 * g.D;
     g.wsize = g.WX * g.WY;
-    g.lds_per_wave = 2 * g.wsize + 4 * g.R + 1;
+    g.lds_per_wave = 2 * g.wsize + 4 * g.R + 2;  // FP64 window + row tables, even for 8-byte alignment
     const int rx = Nx % g.T, ry = Ny % g.R;
     g.fast = g.D == 0 && g.T >= 3 && g.R >= 3 && g.ntx >= 3 && g.nty >= 3 && (rx == 0 || rx >= 3) &&
              (ry == 0 || ry >= 3);
@@ -202,7 +199,6 @@ void fill_async(float* p, size_t n, float v, hipStream_t s) {
 
 hipError_t launch_deposit(const DepositArgs& a) {
     if (a.backend == 1) {
-        fill_async(a.rho, (size_t)a.dom.Nx * a.dom.Ny, 0.0f, a.stream);
         switch (a.scheme) {
             case NGP: launch_atomic<NGP>(a); break;
             case CIC: launch_atomic<CIC>(a); break;

@@ -58,17 +58,13 @@ __device__ __forceinline__ void wave_segmented_scan(float (&B)[V], float A) {
     lin_scan_step<kRowBcast31, 0xc>(B, A);
 }
 
-// Compensated add of a cell's V stencil values into the wave's LDS window:
-// all V returning adds are issued first, then the TwoSum errors go to the
-// compensation window.  (Adding 0 leaves both windows unchanged.)
+// Adds a cell's V stencil values into the wave's FP64 LDS window (one
+// non-returning LDS atomic per value; FP64 keeps the sum of thousands of
+// FP32 contributions exact to FP32 after the final rounding).
 template <int SX, int V>
-__device__ __forceinline__ void emit_cell(float* __restrict__ ws, float* __restrict__ wc, int wbase, int WX,
-                                          const float (&z)[V]) {
-    float old[V];
+__device__ __forceinline__ void emit_cell(double* __restrict__ win, int wbase, int WX, const float (&z)[V]) {
 #pragma unroll
-    for (int k = 0; k < V; ++k) old[k] = atomicAdd(ws + wbase + (k / SX) * WX + (k % SX), z[k]);
-#pragma unroll
-    for (int k = 0; k < V; ++k) atomicAdd(wc + wbase + (k / SX) * WX + (k % SX), two_sum_err(old[k], z[k]));
+    for (int k = 0; k < V; ++k) atomicAdd(win + wbase + (k / SX) * WX + (k % SX), (double)z[k]);
 }
 
 // General reference-Esirkepov walk (esirkepovWeights1DSparse,
@@ -129,17 +125,16 @@ __device__ __noinline__ int esk_walk_1d(float x0, float v, float dt, float inv_d
     return n;
 }
 
-// Compensated global add: grid + comp == exact running sum (TwoSum of each
-// atomic's old value and the addend).
-__device__ __forceinline__ void comp_atomic_add(float* __restrict__ grid, float* __restrict__ comp, size_t i, float v) {
-    const float old = unsafeAtomicAdd(grid + i, v);
-    unsafeAtomicAdd(comp + i, two_sum_err(old, v));
+// Global accumulation in FP64 (one non-returning hardware atomic per value;
+// sums stay exact to FP64 round-off whatever their order).
+__device__ __forceinline__ void acc_add(double* __restrict__ gacc, size_t i, float v) {
+    unsafeAtomicAdd(gacc + i, (double)v);
 }
 
 // Rare path of scatter_global: Esirkepov moves longer than one cell.
 __device__ __noinline__ void scatter_esk_long(float x, float y, float vx, float vy, float qv, float dt,
-                                              const DevDomain& d, float* __restrict__ grid,
-                                              float* __restrict__ comp, int* __restrict__ status) {
+                                              const DevDomain& d, double* __restrict__ gacc,
+                                              int* __restrict__ status) {
     int cxs[kEskMaxCells], cys[kEskMaxCells];
     float wxs[kEskMaxCells], wys[kEskMaxCells];
     const int nx = esk_walk_1d(x, vx, dt, d.inv_dx, d.dx, d.exact, cxs, wxs);
@@ -152,17 +147,17 @@ __device__ __noinline__ void scatter_esk_long(float x, float y, float vx, float 
         const int gx = wrap_index(cxs[i], d.Nx);
         for (int j = 0; j < ny; ++j) {
             const int gy = wrap_index(cys[j], d.Ny);
-            comp_atomic_add(grid, comp, (size_t)gy * d.Nx + gx, qv * wxs[i] * wys[j]);
+            acc_add(gacc, (size_t)gy * d.Nx + gx, qv * wxs[i] * wys[j]);
         }
     }
 }
 
-// Scatter one particle with compensated global FP32 atomics (used by
-// GPU_Atomics and by the tile kernel's overflow path).
+// Scatter one particle with FP64 global atomics (used by GPU_Atomics and by
+// the tile kernel's overflow path).
 template <int S>
 __device__ __forceinline__ void scatter_global(float x, float y, float vx, float vy, float qv, float dt,
-                                               const DevDomain& d, float* __restrict__ grid,
-                                               float* __restrict__ comp, int* __restrict__ status) {
+                                               const DevDomain& d, double* __restrict__ gacc,
+                                               int* __restrict__ status) {
     using St = Stencil<S>;
     float w[St::V];
     int cx, cy;
@@ -173,13 +168,13 @@ __device__ __forceinline__ void scatter_global(float x, float y, float vx, float
             if (w[k] != 0.0f) {
                 const int gx = wrap_index(cx + St::OX + k % St::SX, d.Nx);
                 const int gy = wrap_index(cy + St::OY + k / St::SX, d.Ny);
-                comp_atomic_add(grid, comp, (size_t)gy * d.Nx + gx, w[k]);
+                acc_add(gacc, (size_t)gy * d.Nx + gx, w[k]);
             }
         }
         return;
     }
     // Only the Esirkepov density can fail the fixed stencil (long moves).
-    scatter_esk_long(x, y, vx, vy, qv, dt, d, grid, comp, status);
+    scatter_esk_long(x, y, vx, vy, qv, dt, d, gacc, status);
 }
 
 // Q consecutive particle quads of one lane: physical start index of each quad
@@ -235,31 +230,29 @@ __device__ __forceinline__ void load_chunk(int vq0, int total_q, int th, const i
 }
 
 // A particle whose current cell is outside its tile (drift since the last
-// sort): its already computed stencil goes to the overflow grid with
-// compensated global atomics.  `ok` false (an Esirkepov move longer than a
+// sort): its already computed stencil goes to the FP64 accumulation grid
+// with global atomics.  `ok` false (an Esirkepov move longer than a
 // cell) takes the general walk.
 template <int S>
 __device__ __forceinline__ void overflow_scatter(bool ok, int cx, int cy, const float (&w)[Stencil<S>::V], float x,
                                                  float y, float vx, float vy, float qv, float dt, const DevDomain& d,
-                                                 float* __restrict__ overflow, float* __restrict__ comp,
-                                                 int* __restrict__ status) {
+                                                 double* __restrict__ gacc, int* __restrict__ status) {
     using St = Stencil<S>;
     if (ok) {
 #pragma unroll
         for (int k = 0; k < St::V; ++k) {
             const int gx = wrap_index(cx + St::OX + k % St::SX, d.Nx);
             const int gy = wrap_index(cy + St::OY + k / St::SX, d.Ny);
-            comp_atomic_add(overflow, comp, (size_t)gy * d.Nx + gx, w[k]);
+            acc_add(gacc, (size_t)gy * d.Nx + gx, w[k]);
         }
     } else {
-        scatter_esk_long(x, y, vx, vy, qv, dt, d, overflow, comp, status);
+        scatter_esk_long(x, y, vx, vy, qv, dt, d, gacc, status);
     }
 }
 
 // Per-wave state of the tiled deposit (one tile per wavefront).
 struct TileCtx {
-    float* win_s;
-    float* win_c;
+    double* win;  // FP64 LDS window
     int WX;
     int x0, y0, tw, th;  // the tile's own cells
     int ax0, ay0, aw, ah;  // cells deposited into the window: the tile widened by the drift halo
@@ -270,8 +263,7 @@ struct TileCtx {
 // compensated LDS add per (cell, stencil slot).
 template <int S, int Q, int EX>
 __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx& t, const DevDomain& d, float dt,
-                                              float* __restrict__ overflow, float* __restrict__ comp,
-                                              int* __restrict__ status) {
+                                              double* __restrict__ gacc, int* __restrict__ status) {
     using St = Stencil<S>;
     constexpr int V = St::V;
     float acc[V], H[V];
@@ -307,7 +299,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             const int lx = cx - t.ax0, ly = cy - t.ay0;
             const bool inside = ok && (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
             if (valid && !inside) {
-                overflow_scatter<S>(ok, cx, cy, w, xx, yy, vxx, vyy, qv, dt, d, overflow, comp, status);
+                overflow_scatter<S>(ok, cx, cy, w, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
             }
             const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
             const bool nseg = lk != ck;
@@ -318,7 +310,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                     multi = true;
                 } else {
                     const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
-                    emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, acc);
+                    emit_cell<St::SX, V>(t.win, wb, t.WX, acc);
                 }
             }
             const float keep = nseg ? 0.0f : 1.0f;
@@ -347,35 +339,33 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
 #pragma unroll
         for (int k = 0; k < V; ++k) H[k] = cont ? H[k] + Zp[k] : H[k];
         const int wb = ((hk >> 12) + 1 + St::OY) * t.WX + (hk & 4095) + 1 + St::OX;
-        emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, H);
+        emit_cell<St::SX, V>(t.win, wb, t.WX, H);
     }
     const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
     if (ck >= 0 && !cont_next) {
         const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
-        emit_cell<St::SX, V>(t.win_s, t.win_c, wb, t.WX, Z);
+        emit_cell<St::SX, V>(t.win, wb, t.WX, Z);
     }
 }
 
 // Deposits one tile with one wavefront: LDS window `lds` (g.lds_per_wave
-// floats), cell offsets and partial/overflow buffers already offset to the
-// tile's instance.
+// floats, 8-byte aligned), cell offsets and partial / accumulation buffers
+// already offset to the tile's instance.
 template <int S, int K, int EX, int PIPE>
 __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ lds, const float* __restrict__ px,
                                                  const float* __restrict__ py, const float* __restrict__ pvx,
                                                  const float* __restrict__ pvy, const float* __restrict__ pq,
                                                  const int* __restrict__ cell_off, const DevDomain& d,
                                                  const TileGeom& g, float dt, float* __restrict__ partial,
-                                                 float* __restrict__ overflow, float* __restrict__ comp,
-                                                 int* __restrict__ status) {
+                                                 double* __restrict__ gacc, int* __restrict__ status) {
     using St = Stencil<S>;
     constexpr int Q = K / 4;
     static_assert(K % 4 == 0, "K must be a multiple of 4 (float4 loads)");
     const int lane = threadIdx.x & 63;
 
     TileCtx t;
-    t.win_s = lds;
-    t.win_c = t.win_s + g.wsize;
-    int* tab = reinterpret_cast<int*>(t.win_c + g.wsize);
+    t.win = reinterpret_cast<double*>(lds);
+    int* tab = reinterpret_cast<int*>(t.win + g.wsize);
     int* t_qstart = tab;          // [R]   first aligned quad of row r
     int* t_rs = tab + g.R;        // [R]   first particle of row r
     int* t_re = tab + 2 * g.R;    // [R]   one past the last particle of row r
@@ -394,8 +384,7 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
     const int th = t.th;
 
     for (int i = lane; i < g.wsize; i += 64) {
-        t.win_s[i] = 0.0f;
-        t.win_c[i] = 0.0f;
+        t.win[i] = 0.0;
     }
     // Row table: lane r < th describes row y0 + r of the tile.
     int nq = 0;
@@ -434,13 +423,13 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
             if (b1 < total_q)
                 load_chunk<Q, St::kNeedsVelocity>(b1 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
                                                   pq, pvx, pvy, B);
-            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, status);
+            process_chunk<S, Q, EX>(A, t, d, dt, gacc, status);
             if (b1 >= total_q) break;
             const int b2 = b1 + step;
             if (b2 < total_q)
                 load_chunk<Q, St::kNeedsVelocity>(b2 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
                                                   pq, pvx, pvy, A);
-            process_chunk<S, Q, EX>(B, t, d, dt, overflow, comp, status);
+            process_chunk<S, Q, EX>(B, t, d, dt, gacc, status);
             base = b2;
         }
     } else {
@@ -448,13 +437,13 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
             Chunk<Q> A;
             load_chunk<Q, St::kNeedsVelocity>(base + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq,
                                               pvx, pvy, A);
-            process_chunk<S, Q, EX>(A, t, d, dt, overflow, comp, status);
+            process_chunk<S, Q, EX>(A, t, d, dt, gacc, status);
         }
     }
 
     wave_sync();
     float* out = partial + (size_t)tile * g.wsize;
-    for (int i = lane; i < g.wsize; i += 64) out[i] = t.win_s[i] + t.win_c[i];
+    for (int i = lane; i < g.wsize; i += 64) out[i] = (float)t.win[i];
     wave_sync();  // the window may be reused for the wave's next tile
 }
 
@@ -500,12 +489,11 @@ __device__ __forceinline__ int axis_pairs(int gn, int T, int nt, int n, int (&t)
     return 1;
 }
 
-// rho at node (gx, gy): windows covering it in fixed order (TwoSum
-// compensated), plus the overflow grid where it is non-zero (then cleared).
+// rho at node (gx, gy): the windows covering it, added in FP64 in a fixed
+// order, plus the FP64 accumulation grid where it is non-zero (then cleared).
 __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restrict__ partial,
-                                            float* __restrict__ overflow, float* __restrict__ comp,
-                                            const DevDomain& d, const TileGeom& g) {
-    float s = 0.0f, c = 0.0f;
+                                            double* __restrict__ gacc, const DevDomain& d, const TileGeom& g) {
+    double s = 0.0;
     if (g.fast) {
         int txs[2], wxs[2], tys[2], wys[2];
         const int ncx = axis_pairs(gx, g.T, g.ntx, d.Nx, txs, wxs);
@@ -514,8 +502,7 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
             for (int ix = 0; ix < ncx; ++ix) {
                 const float v =
                     partial[(size_t)(tys[iy] * g.ntx + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
-                c += two_sum_err(s, v);
-                s += v;
+                s += (double)v;
             }
         }
     } else {
@@ -543,24 +530,21 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
                     for (; gxi < x0 + tw + 2 + D; gxi += d.Nx) {
                         const int wx = gxi - (x0 - 1 - D);
                         const float v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
-                        c += two_sum_err(s, v);
-                        s += v;
+                        s += (double)v;
                     }
                 }
             }
         }
     }
-    float out = s + c;
-    // Overflow grid and its compensation are zero except where drifted
-    // particles were scattered; fold those nodes in and clear them.
+    // The accumulation grid is zero except where drifted particles were
+    // scattered; fold those nodes in and clear them.
     const size_t node = (size_t)gy * d.Nx + gx;
-    const float o = overflow[node], oc = comp[node];
-    if (o != 0.0f || oc != 0.0f) {
-        out += o + oc;
-        overflow[node] = 0.0f;
-        comp[node] = 0.0f;
+    const double o = gacc[node];
+    if (o != 0.0) {
+        s += o;
+        gacc[node] = 0.0;
     }
-    return out;
+    return (float)s;
 }
 
 }  // namespace

@@ -95,12 +95,12 @@ template <int S, int K, int EX>
 __global__ __launch_bounds__(256) void ens_deposit_kernel(EnsParticles p, const int* __restrict__ cell_off,
                                                           EnsTables t, const int2* __restrict__ items, int nitems,
                                                           int* __restrict__ counter, float dt,
-                                                          float* __restrict__ partial, float* __restrict__ overflow,
-                                                          float* __restrict__ comp, int* __restrict__ status,
+                                                          float* __restrict__ partial, double* __restrict__ gacc,
+                                                          int* __restrict__ status,
                                                           int lds_per_wave) {
-    extern __shared__ float smem[];
+    extern __shared__ double smem_d[];
     const int lane = threadIdx.x & 63;
-    float* lds = smem + (size_t)(threadIdx.x >> 6) * lds_per_wave;
+    float* lds = reinterpret_cast<float*>(smem_d) + (size_t)(threadIdx.x >> 6) * lds_per_wave;
     for (;;) {
         int it = 0;
         if (lane == 0) it = atomicAdd(counter, 1);
@@ -111,12 +111,12 @@ __global__ __launch_bounds__(256) void ens_deposit_kernel(EnsParticles p, const 
         const DevDomain d = m.d;
         const TileGeom g = m.g;
         deposit_one_tile<S, K, EX, 0>(w.y, lds, p.x, p.y, p.vx, p.vy, p.q, cell_off + m.cbase, d, g, dt,
-                                      partial + m.partial_base, overflow + m.gbase, comp + m.gbase, status);
+                                      partial + m.partial_base, gacc + m.gbase, status);
     }
 }
 
 __global__ __launch_bounds__(256) void ens_merge_kernel(const float* __restrict__ partial,
-                                                        float* __restrict__ overflow, float* __restrict__ comp,
+                                                        double* __restrict__ gacc,
                                                         EnsTables t, int64_t blk0, float* __restrict__ rho,
                                                         double* __restrict__ rho_part) {
     const int64_t b = blk0 + blockIdx.x;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void ens_merge_kernel(const float* __restrict_
         const DevDomain d = m.d;
         const TileGeom g = m.g;
         const int gx = (int)(node % d.Nx), gy = (int)(node / d.Nx);
-        const float r = merge_node(gx, gy, partial + m.partial_base, overflow + m.gbase, comp + m.gbase, d, g);
+        const float r = merge_node(gx, gy, partial + m.partial_base, gacc + m.gbase, d, g);
         rho[m.gbase + node] = r;
         v[0] = r;
     }
@@ -294,8 +294,8 @@ __global__ __launch_bounds__(256) void ens_diag_kernel(const double* __restrict_
 
 // ---- persistent deposit dispatch ---------------------------------------------
 
-using EnsKernel = void (*)(EnsParticles, const int*, EnsTables, const int2*, int, int*, float, float*, float*, float*,
-                           int*, int);
+using EnsKernel = void (*)(EnsParticles, const int*, EnsTables, const int2*, int, int*, float, float*, double*, int*,
+                           int);
 
 template <int S, int K>
 EnsKernel pick_ex(int exact) {
@@ -357,7 +357,7 @@ int ens_deposit_grid(const EnsWork& w, int ncu) {
 }
 
 hipError_t launch_ens_deposit(const EnsWork* groups, int ngroups, const EnsParticles& p, const int* cell_off,
-                              const EnsTables& t, float dt, float* partial, float* overflow, float* comp,
+                              const EnsTables& t, float dt, float* partial, double* gacc,
                               int* status, int* counters, int ncu, hipStream_t s) {
     (void)ncu;
     launch_set_ints(counters, ngroups, 0, s);
@@ -367,15 +367,15 @@ hipError_t launch_ens_deposit(const EnsWork* groups, int ngroups, const EnsParti
         const EnsKernel k = pick_kernel(w.scheme, w.ppl, w.exact);
         const size_t shmem = (size_t)4 * w.lds_per_wave * sizeof(float);
         hipLaunchKernelGGL(k, dim3((unsigned)w.grid), dim3(256), shmem, s, p, cell_off, t, w.items, w.nitems,
-                           counters + gi, dt, partial, overflow, comp, status, w.lds_per_wave);
+                           counters + gi, dt, partial, gacc, status, w.lds_per_wave);
     }
     return hipGetLastError();
 }
 
-void launch_ens_merge(const float* partial, float* overflow, float* comp, const EnsTables& t, int64_t blk0,
+void launch_ens_merge(const float* partial, double* gacc, const EnsTables& t, int64_t blk0,
                       int64_t nblk, float* rho, double* rho_part, hipStream_t s) {
     if (nblk > 0)
-        hipLaunchKernelGGL(ens_merge_kernel, dim3((unsigned)nblk), dim3(256), 0, s, partial, overflow, comp, t, blk0,
+        hipLaunchKernelGGL(ens_merge_kernel, dim3((unsigned)nblk), dim3(256), 0, s, partial, gacc, t, blk0,
                            rho, rho_part);
 }
 

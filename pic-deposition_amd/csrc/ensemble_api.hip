@@ -49,7 +49,8 @@ struct pic_hip_ensemble {
     float* alt[kStreams] = {};
     SortBuffers sb{};
     bool sorted = false;
-    float *rho = nullptr, *overflow = nullptr, *comp = nullptr, *Ex = nullptr, *Ey = nullptr, *fft_in = nullptr;
+    float *rho = nullptr, *Ex = nullptr, *Ey = nullptr, *fft_in = nullptr;
+    double* acc64 = nullptr;  // FP64 global-atomic accumulation grid of all members, zero between deposits
     float2 *spec = nullptr, *spec_ex = nullptr, *spec_ey = nullptr;
     double *k2 = nullptr, *cosk = nullptr, *sink = nullptr;
     float2* sym = nullptr;
@@ -82,7 +83,7 @@ struct pic_hip_ensemble {
             (void)hipFree(alt[i]);
         }
         void* bufs[] = {sb.keys,   sb.keys_alt, sb.perm,     sb.perm_alt,    sb.hist,    sb.scan_tmp, sb.cell_off,
-                        rho,       overflow,    comp,        Ex,             Ey,         fft_in,      spec,
+                        rho,       acc64,       Ex,          Ey,             fft_in,     spec,
                         spec_ex,   spec_ey,     k2,          cosk,           sink,       sym,         partial,
                         status,    counters,    diag_counters, push_flag,    rho_part,   field_part,  part_p,
                         rho_sum,   d_mem,       d_pchunk,    d_gblk,         d_sblk,     d_items};
@@ -243,9 +244,9 @@ pic_status enqueue_sort(pic_hip_ensemble* e, bool copy_back) {
 // grid blocks [blk0, blk0 + nblk).
 pic_status enqueue_deposit(pic_hip_ensemble* e, float dt, int64_t blk0, int64_t nblk) {
     PIC_HIP_TRY(launch_ens_deposit(e->groups.data(), (int)e->groups.size(), e->particles(), e->sb.cell_off,
-                                   e->tables(), dt, e->partial, e->overflow, e->comp, e->status, e->counters, e->ncu,
+                                   e->tables(), dt, e->partial, e->acc64, e->status, e->counters, e->ncu,
                                    e->stream));
-    launch_ens_merge(e->partial, e->overflow, e->comp, e->tables(), blk0, nblk, e->rho, e->rho_part, e->stream);
+    launch_ens_merge(e->partial, e->acc64, e->tables(), blk0, nblk, e->rho, e->rho_part, e->stream);
     PIC_HIP_TRY(hipGetLastError());
     return PIC_OK;
 }
@@ -447,7 +448,9 @@ pic_status pic_hip_ensemble_create(const pic_ensemble_member* members, int32_t c
     e->sb.scan_tmp_cap = sort_scan_tmp_elems(e->sb.hist_cap);
     PIC_HIP_TRY(dmalloc(&e->sb.scan_tmp, e->sb.scan_tmp_cap));
     PIC_HIP_TRY(dmalloc(&e->sb.cell_off, (size_t)keys + 1));
-    float** grids[] = {&e->rho, &e->overflow, &e->comp, &e->Ex, &e->Ey, &e->fft_in};
+    PIC_HIP_TRY(dmalloc(&e->acc64, (size_t)G));
+    PIC_HIP_TRY(hipMemsetAsync(e->acc64, 0, (size_t)G * sizeof(double), e->stream));
+    float** grids[] = {&e->rho, &e->Ex, &e->Ey, &e->fft_in};
     for (float** g : grids) {
         PIC_HIP_TRY(dmalloc(g, (size_t)G));
         PIC_HIP_TRY(hipMemsetAsync(*g, 0, (size_t)G * sizeof(float), e->stream));

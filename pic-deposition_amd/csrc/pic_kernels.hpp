@@ -43,8 +43,7 @@ struct DepositArgs {
     float dt;
     float* rho;       // [Ny][Nx] FP32 out
     float* partial;   // ntiles * wsize
-    float* overflow;  // [Ny][Nx], kept zero between calls
-    float* comp;      // [Ny][Nx] compensation of the global atomic sums, kept zero between calls
+    double* acc;      // [Ny][Nx] FP64 global-atomic accumulation grid, kept zero between calls
     int* status;      // bit 0: Esirkepov move longer than the general walk supports
     hipStream_t stream;
     hipEvent_t ev_begin = nullptr;  // if set, recorded around the main scatter kernel
@@ -223,10 +222,10 @@ void launch_ens_init(const EnsParticles& p, const EnsTables& t, int64_t nchunks,
 void launch_ens_keys(const EnsParticles& p, const EnsTables& t, int64_t nchunks, int* keys, hipStream_t s);
 // counters: one int per group, reset by the launch itself.
 hipError_t launch_ens_deposit(const EnsWork* groups, int ngroups, const EnsParticles& p, const int* cell_off,
-                              const EnsTables& t, float dt, float* partial, float* overflow, float* comp,
+                              const EnsTables& t, float dt, float* partial, double* gacc,
                               int* status, int* counters, int ncu, hipStream_t s);
 // rho of grid blocks [blk0, blk0 + nblk) and their FP64 block sums rho_part[b].
-void launch_ens_merge(const float* partial, float* overflow, float* comp, const EnsTables& t, int64_t blk0,
+void launch_ens_merge(const float* partial, double* gacc, const EnsTables& t, int64_t blk0,
                       int64_t nblk, float* rho, double* rho_part, hipStream_t s);
 // out[m] = sum of part[b * stride + k] over member m's grid blocks (k fixed).
 void launch_ens_member_grid_sum(const double* part, int stride, int k, const EnsTables& t, double* out,

@@ -26,16 +26,35 @@ __global__ __launch_bounds__(256) void keys_kernel(const float* __restrict__ x, 
     if (i < n) keys[i] = cell_key_exact(x[i], y[i], d);
 }
 
+// Per-block digit histogram.  Keys re-sorted between steps arrive nearly in
+// order, so a wave mostly holds one or two digits: the first two distinct
+// digits of each wave are counted with one LDS add each (ballot + popcount),
+// the rest with one LDS atomic per lane.
 __global__ __launch_bounds__(256) void radix_hist_kernel(const int* __restrict__ keys, int64_t n, int shift,
                                                          int nblocks, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kDigits];
     h[threadIdx.x] = 0;
     __syncthreads();
+    const int lane = threadIdx.x & 63;
     const int64_t base = (int64_t)blockIdx.x * kBlockItems;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const int64_t i = base + (int64_t)j * kThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[((unsigned)keys[i] >> shift) & (kDigits - 1)], 1u);
+        const bool valid = i < n;
+        const unsigned dg = valid ? ((unsigned)keys[i] >> shift) & (kDigits - 1) : 0u;
+        bool pending = valid;
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const uint64_t act = __ballot(pending);
+            if (act == 0) break;  // wave-uniform
+            const int leader = __ffsll((unsigned long long)act) - 1;
+            const unsigned d0 = (unsigned)__shfl((int)dg, leader, 64);
+            const bool same = pending && dg == d0;
+            const uint64_t m = __ballot(same);
+            if (lane == leader) atomicAdd(&h[d0], (uint32_t)__popcll(m));
+            if (same) pending = false;
+        }
+        if (pending) atomicAdd(&h[dg], 1u);
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
