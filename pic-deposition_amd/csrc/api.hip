@@ -150,7 +150,7 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     // Defaults from the MI355X tiling sweep at 1024^2 / 64 ppc.
     for (int k = 0; k < 4; ++k) {
         int tx = kDefaultTiling[k][0], ty = kDefaultTiling[k][1];
-        default_tile_shape(k, domain->Nx, domain->Ny, 0, 4096, 0, tx, ty);
+        default_tile_shape(k, domain->Nx, domain->Ny, (int64_t)capacity, 4096, INT64_MAX, 512, tx, ty);
         const pic_status st = apply_tiling(c.get(), k, tx, ty, kDefaultTiling[k][2], 0);
         if (st != PIC_OK) return st;
     }

@@ -89,25 +89,31 @@ constexpr size_t kMaxDepositLds = 64 * 1024;
 constexpr int kDefaultTiling[4][3] = {{64, 4, 8}, {64, 4, 8}, {32, 4, 8}, {32, 4, 4}};
 
 // Default tile shape for a grid: the first of the scheme's shapes (largest
-// first, down to 8 x 2) that gives at least `min_tiles` tiles and, when the
-// particle count is known (n > 0), at most `max_per_tile` particles per tile.
-// Few large tiles leave the GPU idle behind a handful of long waves (a
-// 256^2 grid at 512 particles per cell deposits 3.5x faster in 8 x 2 tiles
-// than in 32 x 4); more, smaller tiles only add window halo to the merge.
-inline void default_tile_shape(int scheme, int Nx, int Ny, int64_t n, int min_tiles, int64_t max_per_tile, int& tx,
-                               int& ty) {
+// first, down to 8 x 2) that gives at least `min_tiles` tiles and at most
+// `max_per_tile` particles per tile, without going below `min_per_tile`
+// particles per tile (one wave chunk is 512 particles) when the particle
+// count n is known.  Few large tiles leave the GPU idle behind a handful of
+// long waves (a 256^2 grid at 512 particles per cell deposits 3.5x faster in
+// 8 x 2 tiles than in 32 x 4); tiles with fewer particles than a chunk leave
+// lanes idle and only add window halo to the merge.
+inline void default_tile_shape(int scheme, int Nx, int Ny, int64_t n, int min_tiles, int64_t max_per_tile,
+                               int64_t min_per_tile, int& tx, int& ty) {
     static const int wide[][2] = {{64, 4}, {32, 4}, {16, 4}, {8, 4}, {8, 2}};
     static const int narrow[][2] = {{32, 4}, {16, 4}, {8, 4}, {8, 2}};
     const bool w = scheme == 0 || scheme == 1;
     const int count = w ? 5 : 4;
-    const int64_t G = (int64_t)Nx * Ny;
+    const int64_t G = std::max<int64_t>((int64_t)Nx * Ny, 1);
+    bool have = false;
     for (int i = 0; i < count; ++i) {
-        tx = w ? wide[i][0] : narrow[i][0];
-        ty = w ? wide[i][1] : narrow[i][1];
-        const int64_t tiles = (int64_t)((Nx + tx - 1) / tx) * ((Ny + ty - 1) / ty);
-        const bool enough = tiles >= min_tiles;
-        const bool small = n <= 0 || (n * (int64_t)tx * ty) / std::max<int64_t>(G, 1) <= max_per_tile;
-        if (enough && small) return;
+        const int cx = w ? wide[i][0] : narrow[i][0];
+        const int cy = w ? wide[i][1] : narrow[i][1];
+        const int64_t tiles = (int64_t)((Nx + cx - 1) / cx) * ((Ny + cy - 1) / cy);
+        const int64_t per_tile = n * cx * cy / G;
+        if (n > 0 && have && per_tile < min_per_tile) return;  // keep the previous, larger shape
+        tx = cx;
+        ty = cy;
+        have = true;
+        if (tiles >= min_tiles && (n <= 0 || per_tile <= max_per_tile)) return;
     }
 }
 

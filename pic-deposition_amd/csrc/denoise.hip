@@ -407,20 +407,25 @@ __global__ __launch_bounds__(256) void denoise_stats_kernel(const float* __restr
     }
 }
 
-// Gate decision (one thread): result = {residual ratio, fallback flag, scale, Q}.
-__global__ void denoise_decide_kernel(const double* __restrict__ stats, int nbs, const double* __restrict__ moments,
-                                      int nbm, const double* __restrict__ rho_sum, int64_t n, double cv, double kappa,
-                                      double tol, double* __restrict__ result, int* __restrict__ fallback_count) {
+// Gate decision (one block): result = {residual ratio, fallback flag, scale, Q}.
+__global__ __launch_bounds__(256) void denoise_decide_kernel(const double* __restrict__ stats, int nbs,
+                                                             const double* __restrict__ moments, int nbm,
+                                                             const double* __restrict__ rho_sum, int64_t n, double cv,
+                                                             double kappa, double tol, double* __restrict__ result,
+                                                             int* __restrict__ fallback_count) {
+    __shared__ double sh[4];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nbs; b += blockDim.x) {
+        a[0] += stats[2 * b];
+        a[1] += stats[2 * b + 1];
+    }
+    for (int b = threadIdx.x; b < nbm; b += blockDim.x) {
+        a[2] += moments[2 * b];
+        a[3] += moments[2 * b + 1];
+    }
+    const double s1 = block_sum256(a[0], sh), s2 = block_sum256(a[1], sh);
+    const double q1 = block_sum256(a[2], sh), q2 = block_sum256(a[3], sh);
     if (threadIdx.x != 0) return;
-    double s1 = 0.0, s2 = 0.0, q1 = 0.0, q2 = 0.0;
-    for (int b = 0; b < nbs; ++b) {
-        s1 += stats[2 * b];
-        s2 += stats[2 * b + 1];
-    }
-    for (int b = 0; b < nbm; ++b) {
-        q1 += moments[2 * b];
-        q2 += moments[2 * b + 1];
-    }
     const double mean = *rho_sum / (double)n;
     const double noise = kappa * q2 / (cv * cv);  // expected squared shot-noise norm of the plain deposit
     const double ratio = noise > 0.0 ? s2 / noise : INFINITY;
@@ -561,7 +566,7 @@ void launch_denoise_finish(const float* delta_out, double inv_scale, float* rho,
     hipLaunchKernelGGL(denoise_stats_kernel, dim3(nbs), dim3(256), 0, s, delta_out, rho, G, rho_sum, inv_scale,
                        stats);
     hipLaunchKernelGGL(charge_moments_kernel, dim3(nbm), dim3(256), 0, s, q, n, moments);
-    hipLaunchKernelGGL(denoise_decide_kernel, dim3(1), dim3(64), 0, s, stats, nbs, moments, nbm, rho_sum, G, cv, kappa,
+    hipLaunchKernelGGL(denoise_decide_kernel, dim3(1), dim3(256), 0, s, stats, nbs, moments, nbm, rho_sum, G, cv, kappa,
                        tol, result, fallback_count);
     hipLaunchKernelGGL(denoise_apply_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, delta_out, rho, out64,
                        G, rho_sum, inv_scale, result);

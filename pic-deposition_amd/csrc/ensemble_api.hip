@@ -398,7 +398,7 @@ pic_status pic_hip_ensemble_create(const pic_ensemble_member* members, int32_t c
         // matters: at most 32768 particles per tile.
         const int* dt = kDefaultTiling[m.scheme];
         int tx = dt[0], ty = dt[1];
-        default_tile_shape(m.scheme, m.d.Nx, m.d.Ny, m.n, 1, 32768, tx, ty);
+        default_tile_shape(m.scheme, m.d.Nx, m.d.Ny, m.n, 1, 32768, 512, tx, ty);
         m.g = make_tile_geom(m.d.Nx, m.d.Ny, tx, ty);
         m.ppl = dt[2];
     }

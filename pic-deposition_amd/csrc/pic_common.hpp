@@ -66,6 +66,19 @@ __device__ __forceinline__ int cell_key_exact(float x, float y, const DevDomain&
     return iy * d.Nx + ix;
 }
 
+// Sum of one double per thread over a 256-thread block in a fixed order
+// (butterfly within each wave, then the four wave totals in order); every
+// thread receives the total.  `sh` holds 4 doubles of shared memory.
+__device__ __forceinline__ double block_sum256(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+    return t;
+}
+
 // Knuth TwoSum error term: (a + b) == fl(a + b) + two_sum_err(a, b) exactly.
 __device__ __forceinline__ float two_sum_err(float a, float b) {
     const float s = a + b;

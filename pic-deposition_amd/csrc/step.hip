@@ -2,6 +2,8 @@
 // Poisson solve around hipFFT, FP64 diagnostics, and on-device particle
 // initialisers.
 
+#include <algorithm>
+
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
 #include "particle_device.hpp"
@@ -79,18 +81,12 @@ __global__ __launch_bounds__(256) void diag_partial_kernel(const float* __restri
 
 __global__ __launch_bounds__(256) void diag_final_kernel(const double* __restrict__ partials, int nblocks,
                                                          double* __restrict__ out) {
-    __shared__ double red[256];
+    __shared__ double sh[4];
     for (int k = 0; k < 5; ++k) {
         double s = 0.0;
         for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(size_t)k * nblocks + b];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = red[0];
-        __syncthreads();
+        s = block_sum256(s, sh);
+        if (threadIdx.x == 0) out[k] = s;
     }
 }
 
@@ -213,19 +209,12 @@ __global__ __launch_bounds__(256) void step_diag_partial_kernel(const float* __r
 __global__ __launch_bounds__(256) void step_diag_final_kernel(const double* __restrict__ partials, int nblocks,
                                                               double cv, int64_t G, int* __restrict__ counter,
                                                               int record_from, int cap, StepRecords rec) {
-    __shared__ double red[256];
-    __shared__ double tot[kStepSums];
+    __shared__ double sh[4];
+    double tot[kStepSums];
     for (int k = 0; k < kStepSums; ++k) {
         double s = 0.0;
         for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(size_t)k * nblocks + b];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) tot[k] = red[0];
-        __syncthreads();
+        tot[k] = block_sum256(s, sh);
     }
     if (threadIdx.x == 0) {
         const int step = *counter;
@@ -283,9 +272,17 @@ void launch_gather_push(float* x, float* y, float* vx, float* vy, const float* q
                            n, Ex, Ey, d, dt, do_gather, do_push, push_flag);
 }
 
+// Partial-sum blocks for a reduction over `items` values: one per 2048 (the
+// grid-stride loops then give each thread about eight), at most `cap`.
+static int reduction_blocks(int64_t items, int cap) {
+    const int64_t nb = (items + 2047) / 2048;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nb, cap));
+}
+
 void launch_diagnostics(const float* rho, const float* Ex, const float* Ey, int64_t G, const float* vx,
                         const float* vy, const float* q, const float* m, int64_t n, double* partials, int nblocks,
                         double* out, hipStream_t s) {
+    nblocks = reduction_blocks(std::max(G, n), nblocks);
     hipLaunchKernelGGL(diag_partial_kernel, dim3(nblocks), dim3(256), 0, s, rho, Ex, Ey, G, vx, vy, q, m, n,
                        partials);
     hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, s, partials, nblocks, out);
@@ -303,6 +300,7 @@ void launch_step_diag(const float* rho, const float* Ex, const float* Ey, const 
                       double* partials, int nblocks, int* counter, int record_from, int cap, const StepRecords& rec,
                       hipStream_t s) {
     const int64_t G = (int64_t)d.Nx * d.Ny;
+    nblocks = reduction_blocks(std::max(G, n), nblocks);
     hipLaunchKernelGGL(step_diag_partial_kernel, dim3(nblocks), dim3(256), 0, s, rho, Ex, Ey, cosk, sink, d.Nx, G, vx,
                        vy, q, m, n, partials);
     hipLaunchKernelGGL(step_diag_final_kernel, dim3(1), dim3(256), 0, s, partials, nblocks, d.dx * d.dy, G, counter,
