@@ -214,7 +214,7 @@ def learned_study(pichip):
         n = N * N * ppc
         best = None
         with pichip.Context(dom, n) as c:
-            for si, halo in ((1, 0), (4, 2), (10, 2), (10, 4)):
+            for si, halo in ((1, 0), (4, 1), (10, 2), (10, 4)):
                 c.set_tile_halo(halo)
 
                 def go(steps):

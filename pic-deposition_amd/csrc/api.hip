@@ -57,8 +57,8 @@ pic_status ensure_fft(pic_hip_ctx* c) {
     PIC_HIP_TRY(dmalloc(&c->phi, (size_t)c->G));
     PIC_HIP_TRY(dmalloc(&c->fft_in, (size_t)c->G));
     PIC_HIP_TRY(dmalloc(&c->spec, (size_t)c->dom.Ny * hx));
-    PIC_HIP_TRY(dmalloc(&c->spec_ex, (size_t)c->dom.Ny * hx));
-    PIC_HIP_TRY(dmalloc(&c->spec_ey, (size_t)c->dom.Ny * hx));
+    PIC_HIP_TRY(dmalloc(&c->spec_ex, (size_t)2 * c->dom.Ny * hx));  // Ex then Ey spectrum
+    c->spec_ey = c->spec_ex + (size_t)c->dom.Ny * hx;
     {
         // k^2 per axis (poisson_fft.cpp:23-35) and the forward-difference
         // symbols (e^{i k d} - 1)/d, tabulated in FP64.
@@ -86,10 +86,37 @@ pic_status ensure_fft(pic_hip_ctx* c) {
         PIC_HIP_TRY(hipMemcpy(c->ptab.dx_sym, sx.data(), hx * sizeof(float2), hipMemcpyHostToDevice));
         PIC_HIP_TRY(hipMemcpy(c->ptab.dy_sym, sy.data(), ny * sizeof(float2), hipMemcpyHostToDevice));
     }
+    c->lds_fft = poisson_lds_supported(c->dom.Nx, c->dom.Ny);
+    if (c->lds_fft) {
+        auto table = [](int L) {
+            std::vector<float2> t((size_t)L);
+            for (int m = 0; m < L; ++m) {
+                const double a = 2.0 * M_PI * (double)m / (double)L;
+                t[(size_t)m] = make_float2((float)std::cos(a), (float)std::sin(a));
+            }
+            return t;
+        };
+        const std::vector<float2> tx = table(c->dom.Nx), ty = table(c->dom.Ny);
+        PIC_HIP_TRY(dmalloc(&c->tw_x, tx.size()));
+        PIC_HIP_TRY(dmalloc(&c->tw_y, ty.size()));
+        PIC_HIP_TRY(hipMemcpy(c->tw_x, tx.data(), tx.size() * sizeof(float2), hipMemcpyHostToDevice));
+        PIC_HIP_TRY(hipMemcpy(c->tw_y, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice));
+        c->fft_ready = true;
+        return PIC_OK;
+    }
     PIC_FFT_TRY(hipfftPlan2d(&c->plan_r2c, c->dom.Ny, c->dom.Nx, HIPFFT_R2C));
     PIC_FFT_TRY(hipfftPlan2d(&c->plan_c2r, c->dom.Ny, c->dom.Nx, HIPFFT_C2R));
+    {
+        // Ex and Ey in one batched inverse: spectra spec_ex, spec_ey -> grids Ex, Ey.
+        int dims[2] = {c->dom.Ny, c->dom.Nx};
+        int inembed[2] = {c->dom.Ny, hx};
+        int onembed[2] = {c->dom.Ny, c->dom.Nx};
+        PIC_FFT_TRY(hipfftPlanMany(&c->plan_c2r_e, 2, dims, inembed, 1, c->dom.Ny * hx, onembed, 1, (int)c->G,
+                                   HIPFFT_C2R, 2));
+    }
     PIC_FFT_TRY(hipfftSetStream(c->plan_r2c, c->stream));
     PIC_FFT_TRY(hipfftSetStream(c->plan_c2r, c->stream));
+    PIC_FFT_TRY(hipfftSetStream(c->plan_c2r_e, c->stream));
     c->fft_ready = true;
     return PIC_OK;
 }
@@ -138,8 +165,8 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(hipMemsetAsync(c->acc64, 0, (size_t)c->G * sizeof(double), c->stream));
     PIC_HIP_TRY(dmalloc(&c->status, 1));
     PIC_HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
-    PIC_HIP_TRY(dmalloc(&c->Ex, (size_t)c->G));
-    PIC_HIP_TRY(dmalloc(&c->Ey, (size_t)c->G));
+    PIC_HIP_TRY(dmalloc(&c->Ex, (size_t)2 * c->G));  // Ex then Ey
+    c->Ey = c->Ex + c->G;
     PIC_HIP_TRY(hipMemsetAsync(c->Ex, 0, (size_t)c->G * sizeof(float), c->stream));
     PIC_HIP_TRY(hipMemsetAsync(c->Ey, 0, (size_t)c->G * sizeof(float), c->stream));
     c->diag_blocks = 1024;
@@ -147,11 +174,15 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(dmalloc(&c->diag_out, 8));
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(int) * 4, hipHostMallocDefault));
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_diag), sizeof(double) * 8, hipHostMallocDefault));
-    // Defaults from the MI355X tiling sweep at 1024^2 / 64 ppc.
+    // Defaults from the MI355X tiling sweeps (1024^2 / 64 ppc, 256^2 / 8-512 ppc):
+    // at least 4096 tiles where the particles allow it; tiles of fewer than
+    // two 512-particle chunks take 4 particles per lane (256-particle chunks).
     for (int k = 0; k < 4; ++k) {
         int tx = kDefaultTiling[k][0], ty = kDefaultTiling[k][1];
-        default_tile_shape(k, domain->Nx, domain->Ny, (int64_t)capacity, 4096, INT64_MAX, 512, tx, ty);
-        const pic_status st = apply_tiling(c.get(), k, tx, ty, kDefaultTiling[k][2], 0);
+        default_tile_shape(k, domain->Nx, domain->Ny, (int64_t)capacity, 4096, INT64_MAX, 256, tx, ty);
+        const int64_t per_tile = (int64_t)capacity * tx * ty / std::max<int64_t>(c->G, 1);
+        const int ppl = per_tile < 1024 ? 4 : kDefaultTiling[k][2];
+        const pic_status st = apply_tiling(c.get(), k, tx, ty, ppl, 0);
         if (st != PIC_OK) return st;
     }
     PIC_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -437,21 +468,46 @@ pic_status pic_hip_add_background(pic_hip_ctx* ctx, double value) {
     return PIC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Spectral solve of ctx->rho on the context's stream (kernel launches and FFT
+// executions only).  The mean removed before the transform comes from the
+// device FP64 sum `sum` (see launch_poisson_prep).  Without `with_phi` the
+// potential stays as its spectrum in ctx->spec until phi is read.
+pic_status enqueue_solve(pic_hip_ctx* ctx, const double* sum, double sum_scale, double sum_add, bool with_phi) {
+    const double cv = ctx->dev.dx * ctx->dev.dy;
+    if (ctx->lds_fft) {
+        launch_poisson_lds(ctx->rho, ctx->G, 0.0, cv, sum, sum_scale, sum_add, ctx->spec, ctx->spec_ex, ctx->spec_ey,
+                           ctx->ptab, ctx->dev, ctx->tw_x, ctx->tw_y, ctx->stream);
+        launch_poisson_lds_rows(ctx->spec_ex, ctx->spec_ey, ctx->Ex, ctx->Ey, ctx->dev, ctx->tw_x, ctx->stream);
+        if (with_phi) launch_poisson_lds_rows(ctx->spec, nullptr, ctx->phi, nullptr, ctx->dev, ctx->tw_x, ctx->stream);
+        ctx->phi_pending = !with_phi;
+        PIC_HIP_TRY(hipGetLastError());
+        return PIC_OK;
+    }
+    launch_poisson_prep(ctx->rho, ctx->fft_in, ctx->G, 0.0, cv, sum, sum_scale, sum_add, ctx->stream);
+    PIC_FFT_TRY(hipfftExecR2C(ctx->plan_r2c, ctx->fft_in, reinterpret_cast<hipfftComplex*>(ctx->spec)));
+    launch_poisson_spectral(ctx->spec, ctx->spec_ex, ctx->spec_ey, ctx->ptab, ctx->dev, ctx->stream);
+    if (with_phi) PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec), ctx->phi));
+    PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r_e, reinterpret_cast<hipfftComplex*>(ctx->spec_ex), ctx->Ex));
+    ctx->phi_pending = !with_phi;
+    PIC_HIP_TRY(hipGetLastError());
+    return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 pic_status pic_hip_poisson(pic_hip_ctx* ctx) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     pic_status st = ensure_fft(ctx);
     if (st != PIC_OK) return st;
-    const double cv = ctx->dev.dx * ctx->dev.dy;
     launch_diagnostics(ctx->rho, nullptr, nullptr, ctx->G, nullptr, nullptr, nullptr, nullptr, 0, ctx->diag_partials,
                        ctx->diag_blocks, ctx->diag_out, ctx->stream);
-    launch_poisson_prep(ctx->rho, ctx->fft_in, ctx->G, 0.0, cv, ctx->diag_out, ctx->stream);
-    PIC_FFT_TRY(hipfftExecR2C(ctx->plan_r2c, ctx->fft_in, reinterpret_cast<hipfftComplex*>(ctx->spec)));
-    launch_poisson_spectral(ctx->spec, ctx->spec_ex, ctx->spec_ey, ctx->ptab, ctx->dev, ctx->stream);
-    PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec), ctx->phi));
-    PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec_ex), ctx->Ex));
-    PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec_ey), ctx->Ey));
-    PIC_HIP_TRY(hipGetLastError());
-    return PIC_OK;
+    return enqueue_solve(ctx, ctx->diag_out, 1.0, 0.0, true);
 }
 
 pic_status pic_hip_get_fields(pic_hip_ctx* ctx, float* phi, float* Ex, float* Ey) {
@@ -459,6 +515,13 @@ pic_status pic_hip_get_fields(pic_hip_ctx* ctx, float* phi, float* Ex, float* Ey
     const size_t bytes = (size_t)ctx->G * sizeof(float);
     if (phi) {
         if (!ctx->phi) return fail(PIC_ERR_INVALID_ARGUMENT, "phi not computed yet (call pic_hip_poisson)");
+        if (ctx->phi_pending) {  // the step loop's last solve left the spectrum
+            if (ctx->lds_fft)
+                launch_poisson_lds_rows(ctx->spec, nullptr, ctx->phi, nullptr, ctx->dev, ctx->tw_x, ctx->stream);
+            else
+                PIC_FFT_TRY(hipfftExecC2R(ctx->plan_c2r, reinterpret_cast<hipfftComplex*>(ctx->spec), ctx->phi));
+            ctx->phi_pending = false;
+        }
         PIC_HIP_TRY(hipMemcpyAsync(phi, ctx->phi, bytes, hipMemcpyDeviceToHost, ctx->stream));
     }
     if (Ex) PIC_HIP_TRY(hipMemcpyAsync(Ex, ctx->Ex, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -526,14 +589,17 @@ pic_status ensure_step_buffers(pic_hip_ctx* ctx) {
     PIC_HIP_TRY(dmalloc(&ctx->step_counter, 1));
     PIC_HIP_TRY(dmalloc(&ctx->push_flag, 1));
     PIC_HIP_TRY(dmalloc(&ctx->step_partials, (size_t)7 * ctx->diag_blocks));
+    PIC_HIP_TRY(dmalloc(&ctx->run_sums, 5));
     PIC_HIP_TRY(hipMemcpyAsync(ctx->cosk, c.data(), nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     PIC_HIP_TRY(hipMemcpyAsync(ctx->sink, s.data(), nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PIC_OK;
 }
 
-// Enqueues one step (see pic_hip_run).
-pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, bool copy_back,
+// Enqueues one step (see pic_hip_run); `diag` adds the step's diagnostics
+// record.  The Poisson solve takes the mean of rho from the particles' charge
+// sum in ctx->run_sums (set at the start of the run).
+pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, bool copy_back, bool diag,
                         int record_from, int cap, const StepRecords& rec, const StepHook& post_deposit) {
     pic_status st = PIC_OK;
     const float dt = (float)cfg.dt;
@@ -552,7 +618,8 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
     };
     auto solve = [&]() -> pic_status {
         if (cfg.background != 0.0) launch_add_scalar(ctx->rho, ctx->G, (float)cfg.background, ctx->stream);
-        return pic_hip_poisson(ctx);
+        const double cv = ctx->dev.dx * ctx->dev.dy;
+        return enqueue_solve(ctx, ctx->run_sums + 3, 1.0 / cv, (double)(float)cfg.background * (double)ctx->G, false);
     };
     if (cfg.order == PIC_ORDER_PUSH_FIRST) {
         // The push that opens this step ran fused with the previous step's
@@ -563,9 +630,10 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
         launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
                            ctx->Ey, ctx->dev, dt, 1, 1, ctx->stream, ctx->push_flag);
         const bool parts = cfg.record_particles != 0;
-        launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3], ctx->s[4],
-                         ctx->s[5], parts ? (int64_t)ctx->n : 0, ctx->step_partials, ctx->diag_blocks,
-                         ctx->step_counter, record_from, cap, rec, ctx->stream);
+        if (diag)
+            launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3],
+                             ctx->s[4], ctx->s[5], parts ? (int64_t)ctx->n : 0, ctx->step_partials, ctx->diag_blocks,
+                             ctx->step_counter, record_from, cap, rec, ctx->stream);
         PIC_HIP_TRY(hipGetLastError());
         return PIC_OK;
     } else if (cfg.order == PIC_ORDER_DEPOSIT_SOLVE_PUSH) {
@@ -580,9 +648,10 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
     launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
                        ctx->Ey, ctx->dev, dt, 1, 0, ctx->stream);
     const bool parts = cfg.record_particles != 0;
-    launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3], ctx->s[4],
-                     ctx->s[5], parts ? (int64_t)ctx->n : 0, ctx->step_partials, ctx->diag_blocks, ctx->step_counter,
-                     record_from, cap, rec, ctx->stream);
+    if (diag)
+        launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3], ctx->s[4],
+                         ctx->s[5], parts ? (int64_t)ctx->n : 0, ctx->step_partials, ctx->diag_blocks,
+                         ctx->step_counter, record_from, cap, rec, ctx->stream);
     PIC_HIP_TRY(hipGetLastError());
     return PIC_OK;
 }
@@ -596,10 +665,10 @@ struct GraphPair {
     }
 };
 
-pic_status capture_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, int record_from, int cap,
-                        const StepRecords& rec, GraphPair& g, const StepHook& post_deposit) {
+pic_status capture_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_sort, bool diag, int record_from,
+                        int cap, const StepRecords& rec, GraphPair& g, const StepHook& post_deposit) {
     PIC_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    pic_status st = enqueue_step(ctx, cfg, with_sort, false, record_from, cap, rec, post_deposit);
+    pic_status st = enqueue_step(ctx, cfg, with_sort, false, diag, record_from, cap, rec, post_deposit);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (st != PIC_OK) {
@@ -646,7 +715,15 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
     std::unique_ptr<double, hipError_t (*)(void*)> rec_guard(dev_rec, hipFree);
     StepRecords rec{field_energy ? dev_rec : nullptr, mode1_amplitude ? dev_rec + cap : nullptr,
                     kinetic_energy ? dev_rec + 2 * cap : nullptr, charge_error ? dev_rec + 3 * cap : nullptr};
-    launch_set_int(ctx->step_counter, 0, ctx->stream);
+    // Diagnostics run only on the steps that are recorded; the device step
+    // counter starts at the first of them.
+    const bool want_rec = cap > 0 && (field_energy || mode1_amplitude || kinetic_energy || charge_error);
+    const int diag_from = want_rec ? record_from : nsteps;
+    launch_set_int(ctx->step_counter, diag_from, ctx->stream);
+    // The particles' total charge, from which every solve of the run takes the
+    // mean of rho (the deposit conserves charge).
+    launch_diagnostics(nullptr, nullptr, nullptr, 0, ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n,
+                       ctx->diag_partials, ctx->diag_blocks, ctx->run_sums, ctx->stream);
     const bool priv = cfg->backend == PIC_BACKEND_GPU_PRIV;
     const bool fused = cfg->order == PIC_ORDER_PUSH_FIRST;
     if (fused && nsteps > 0) {
@@ -656,26 +733,36 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
     }
     if (cfg->use_graph && nsteps > 0) {
         // A sort permutes the particles into the other buffer set, so graphs
-        // exist for both: gs[p] sorts out of set p, gp[p] steps on set p.
-        // Capturing gs[0] then gs[1] leaves the host pointers where they began.
-        GraphPair gs[2], gp[2];
+        // exist for both: gs[v][p] sorts out of set p, gp[v][p] steps on set p,
+        // v = 1 with the diagnostics record.  Capturing gs[v][0] then gs[v][1]
+        // leaves the host pointers where they began.
+        GraphPair gs[2][2], gp[2][2];
         const bool plain = !priv || cfg->sort_interval > 1;
-        if (priv) {
-            if ((st = capture_step(ctx, *cfg, true, record_from, cap, rec, gs[0], post_deposit)) != PIC_OK) return st;
-            if (plain && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, gp[1], post_deposit)) != PIC_OK)
+        for (int v = 0; v < 2; ++v) {
+            if (v == 0 ? diag_from == 0 : diag_from >= nsteps) continue;  // variant unused
+            const bool d = v == 1;
+            if (priv) {
+                if ((st = capture_step(ctx, *cfg, true, d, record_from, cap, rec, gs[v][0], post_deposit)) != PIC_OK)
+                    return st;
+                if (plain &&
+                    (st = capture_step(ctx, *cfg, false, d, record_from, cap, rec, gp[v][1], post_deposit)) != PIC_OK)
+                    return st;
+                if ((st = capture_step(ctx, *cfg, true, d, record_from, cap, rec, gs[v][1], post_deposit)) != PIC_OK)
+                    return st;
+            }
+            if (plain &&
+                (st = capture_step(ctx, *cfg, false, d, record_from, cap, rec, gp[v][0], post_deposit)) != PIC_OK)
                 return st;
-            if ((st = capture_step(ctx, *cfg, true, record_from, cap, rec, gs[1], post_deposit)) != PIC_OK) return st;
         }
-        if (plain && (st = capture_step(ctx, *cfg, false, record_from, cap, rec, gp[0], post_deposit)) != PIC_OK)
-            return st;
         int parity = 0;
         for (int step = 0; step < nsteps; ++step) {
             if (fused && step == nsteps - 1) launch_set_int(ctx->push_flag, 0, ctx->stream);
+            const int v = step >= diag_from ? 1 : 0;
             if (priv && step % cfg->sort_interval == 0) {
-                PIC_HIP_TRY(hipGraphLaunch(gs[parity].exec, ctx->stream));
+                PIC_HIP_TRY(hipGraphLaunch(gs[v][parity].exec, ctx->stream));
                 parity ^= 1;
             } else {
-                PIC_HIP_TRY(hipGraphLaunch(gp[parity].exec, ctx->stream));
+                PIC_HIP_TRY(hipGraphLaunch(gp[v][parity].exec, ctx->stream));
             }
         }
         PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -685,7 +772,9 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
         for (int step = 0; step < nsteps; ++step) {
             if (fused && step == nsteps - 1) launch_set_int(ctx->push_flag, 0, ctx->stream);
             const bool sort_now = priv && step % cfg->sort_interval == 0;
-            if ((st = enqueue_step(ctx, *cfg, sort_now, false, record_from, cap, rec, post_deposit)) != PIC_OK) return st;
+            if ((st = enqueue_step(ctx, *cfg, sort_now, false, step >= diag_from, record_from, cap, rec,
+                                   post_deposit)) != PIC_OK)
+                return st;
         }
     }
     double* outs[4] = {field_energy, mode1_amplitude, kinetic_energy, charge_error};

@@ -91,7 +91,7 @@ constexpr int kDefaultTiling[4][3] = {{64, 4, 8}, {64, 4, 8}, {32, 4, 8}, {32, 4
 // Default tile shape for a grid: the first of the scheme's shapes (largest
 // first, down to 8 x 2) that gives at least `min_tiles` tiles and at most
 // `max_per_tile` particles per tile, without going below `min_per_tile`
-// particles per tile (one wave chunk is 512 particles) when the particle
+// particles per tile (a wave chunk is 256 or 512 particles) when the particle
 // count n is known.  Few large tiles leave the GPU idle behind a handful of
 // long waves (a 256^2 grid at 512 particles per cell deposits 3.5x faster in
 // 8 x 2 tiles than in 32 x 4); tiles with fewer particles than a chunk leave

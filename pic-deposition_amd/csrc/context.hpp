@@ -52,10 +52,19 @@ struct pic_hip_ctx {
     float* Ey = nullptr;
     float* fft_in = nullptr;
     float2* spec = nullptr;     // phi spectrum [Ny][Nx/2+1]
-    float2* spec_ex = nullptr;  // Ex, Ey spectra
+    float2* spec_ex = nullptr;  // Ex, Ey spectra (one allocation, Ey after Ex)
     float2* spec_ey = nullptr;
     PoissonTables ptab{};
-    hipfftHandle plan_r2c = 0, plan_c2r = 0;
+    hipfftHandle plan_r2c = 0, plan_c2r = 0, plan_c2r_e = 0;  // plan_c2r_e: Ex and Ey batched
+    // Power-of-two grids up to 1024 per axis solve with the LDS transforms
+    // (twiddle tables tw_x, tw_y); the hipFFT plans serve the other grids.
+    bool lds_fft = false;
+    float2* tw_x = nullptr;
+    float2* tw_y = nullptr;
+    // The step loop leaves the potential's spectrum in `spec` and transforms it
+    // only when phi is read.
+    bool phi_pending = false;
+    double* run_sums = nullptr;  // per run: FP64 sums of the particle charges (diagnostics layout)
     bool fft_ready = false;
     double* diag_partials = nullptr;
     double* diag_out = nullptr;
@@ -77,6 +86,7 @@ struct pic_hip_ctx {
         for (hipEvent_t e : prof_events) (void)hipEventDestroy(e);
         if (plan_r2c) (void)hipfftDestroy(plan_r2c);
         if (plan_c2r) (void)hipfftDestroy(plan_c2r);
+        if (plan_c2r_e) (void)hipfftDestroy(plan_c2r_e);
         for (int i = 0; i < kStreams; ++i) {
             (void)hipFree(s[i]);
             (void)hipFree(alt[i]);
@@ -93,12 +103,10 @@ struct pic_hip_ctx {
         (void)hipFree(status);
         (void)hipFree(partial);
         (void)hipFree(phi);
-        (void)hipFree(Ex);
-        (void)hipFree(Ey);
+        (void)hipFree(Ex);  // Ey shares the allocation
         (void)hipFree(fft_in);
         (void)hipFree(spec);
-        (void)hipFree(spec_ex);
-        (void)hipFree(spec_ey);
+        (void)hipFree(spec_ex);  // spec_ey shares the allocation
         (void)hipFree(ptab.k2x);
         (void)hipFree(ptab.k2y);
         (void)hipFree(ptab.dx_sym);
@@ -111,6 +119,9 @@ struct pic_hip_ctx {
         (void)hipFree(step_counter);
         (void)hipFree(push_flag);
         (void)hipFree(step_partials);
+        (void)hipFree(run_sums);
+        (void)hipFree(tw_x);
+        (void)hipFree(tw_y);
         (void)hipHostFree(h_status);
         (void)hipHostFree(h_diag);
         if (stream) (void)hipStreamDestroy(stream);

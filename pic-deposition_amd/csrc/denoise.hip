@@ -61,11 +61,42 @@ __device__ __forceinline__ void block_sum_d(double (&v)[N]) {
 
 enum { kHidden = 0, kOutput = 1, kBackward = 2 };
 
+// Optional on-the-fly normalisation of a single-channel FP32 input:
+// x -> scale * (x / mean - 1), mean = *sum * sum_scale * inv_count.
+struct InNorm {
+    const double* sum;  // null: the input is used as stored
+    double sum_scale, inv_count;
+    double scale;
+};
+
+// Optional per-block FP64 sums over the output pixels of the last layer
+// (kOutput): u = 1 + out * inv_scale and (mean * u - rho)^2, mean as in InNorm,
+// into partial[2 * block + {0, 1}].
+struct OutStats {
+    double* partial;  // null: no sums
+    const float* rho;
+    const double* sum;
+    double sum_scale, inv_count;
+    double inv_scale;
+};
+
 template <int CIN, int NT, int MODE, int MT>
 __global__ __launch_bounds__(256) void conv3x3_kernel(const void* __restrict__ in_, const bf16x8* __restrict__ wpk,
                                                       const float* __restrict__ bias, void* __restrict__ out_,
-                                                      const __bf16* __restrict__ mask, CnnGeom g, int dil) {
+                                                      const __bf16* __restrict__ mask, CnnGeom g, int dil, InNorm nrm,
+                                                      OutStats ost) {
     const int lane = threadIdx.x & 63;
+    // Single-channel input transform x * mul + add (identity without nrm.sum).
+    float mul = 1.0f, add = 0.0f;
+    const bool norm_in = CIN == 1 && nrm.sum != nullptr;
+    if (norm_in) {
+        const double mean = *nrm.sum * nrm.sum_scale * nrm.inv_count;
+        mul = (float)(nrm.scale / mean);
+        add = (float)(-nrm.scale);
+    }
+    const bool stats = MODE == kOutput && ost.partial != nullptr;
+    double st_mean = 0.0, st_u = 0.0, st_r2 = 0.0;
+    if (stats) st_mean = *ost.sum * ost.sum_scale * ost.inv_count;
     const int grp = lane >> 4, col = lane & 15;
     const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * 4;
@@ -114,6 +145,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const void* __restrict__ i
                         const int yy = wrapn(y + (k / 3 - 1) * dil, g.H);
                         const int xx = wrapn(x + (k % 3 - 1) * dil, g.W);
                         v = in[pix(g, b, yy, xx)];
+                        if (norm_in) v = fmaf(v, mul, add);
                     }
                     a[j] = (__bf16)v;
                 }
@@ -134,13 +166,31 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const void* __restrict__ i
                     if (MODE == kHidden) {
                         static_cast<__bf16*>(out_)[p * 32 + c] = (__bf16)fmaxf(v + bias[c], 0.0f);
                     } else if (MODE == kOutput) {
-                        if (c == 0) static_cast<float*>(out_)[p] = v + bias[0];
+                        if (c == 0) {
+                            const float o = v + bias[0];
+                            static_cast<float*>(out_)[p] = o;
+                            if (stats) {
+                                const double u = 1.0 + (double)o * ost.inv_scale;
+                                const double r = st_mean * u - (double)ost.rho[p];
+                                st_u += u;
+                                st_r2 += r * r;
+                            }
+                        }
                     } else {
                         const float m = (float)mask[p * 32 + c];
                         static_cast<__bf16*>(out_)[p * 32 + c] = (__bf16)(m > 0.0f ? v : 0.0f);
                     }
                 }
             }
+        }
+    }
+    if (stats) {
+        __shared__ double sh[4];
+        const double su = block_sum256(st_u, sh);
+        const double sr = block_sum256(st_r2, sh);
+        if (threadIdx.x == 0) {
+            ost.partial[2 * blockIdx.x] = su;
+            ost.partial[2 * blockIdx.x + 1] = sr;
         }
     }
 }
@@ -386,33 +436,21 @@ __global__ __launch_bounds__(256) void charge_moments_kernel(const float* __rest
     }
 }
 
-// Per-block FP64 sums of 1 + delta_out and of (mean (1 + delta_out) - rho_plain)^2,
-// delta_out = net_out * inv_scale.
-__global__ __launch_bounds__(256) void denoise_stats_kernel(const float* __restrict__ net_out,
-                                                            const float* __restrict__ rho, int64_t n,
-                                                            const double* __restrict__ rho_sum, double inv_scale,
-                                                            double* __restrict__ partial) {
-    const double mean = *rho_sum / (double)n;
-    double v[2] = {0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double u = 1.0 + (double)net_out[i] * inv_scale;
-        const double r = mean * u - (double)rho[i];
-        v[0] += u;
-        v[1] += r * r;
-    }
-    block_sum_d<2>(v);
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = v[0];
-        partial[2 * blockIdx.x + 1] = v[1];
-    }
-}
-
-// Gate decision (one block): result = {residual ratio, fallback flag, scale, Q}.
-__global__ __launch_bounds__(256) void denoise_decide_kernel(const double* __restrict__ stats, int nbs,
-                                                             const double* __restrict__ moments, int nbm,
-                                                             const double* __restrict__ rho_sum, int64_t n, double cv,
-                                                             double kappa, double tol, double* __restrict__ result,
-                                                             int* __restrict__ fallback_count) {
+// Gate and renormalisation, one thread per node.  Every block first adds the
+// per-block sums of the output layer (u, squared residual) and of the charge
+// moments (q, q^2) in the same fixed order, so all blocks take the same
+// decision: the residual ratio ||rho_learned - rho_plain||^2 / (kappa sum q^2 /
+// cv^2) must not exceed tol, and the renormalisation scale makes sum(rho) * cv
+// equal sum(q).  Block 0 writes result = {ratio, fallback, scale, sum q} and
+// counts a fallback.
+__global__ __launch_bounds__(256) void denoise_apply_kernel(const float* __restrict__ net_out,
+                                                            float* __restrict__ rho, double* __restrict__ out64,
+                                                            int64_t n, const double* __restrict__ sum,
+                                                            double sum_scale, double inv_scale,
+                                                            const double* __restrict__ stats, int nbs,
+                                                            const double* __restrict__ moments, int nbm, double cv,
+                                                            double kappa, double tol, double* __restrict__ result,
+                                                            int* __restrict__ fallback_count) {
     __shared__ double sh[4];
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nbs; b += blockDim.x) {
@@ -425,40 +463,36 @@ __global__ __launch_bounds__(256) void denoise_decide_kernel(const double* __res
     }
     const double s1 = block_sum256(a[0], sh), s2 = block_sum256(a[1], sh);
     const double q1 = block_sum256(a[2], sh), q2 = block_sum256(a[3], sh);
-    if (threadIdx.x != 0) return;
-    const double mean = *rho_sum / (double)n;
+    const double mean = *sum * sum_scale / (double)n;
     const double noise = kappa * q2 / (cv * cv);  // expected squared shot-noise norm of the plain deposit
     const double ratio = noise > 0.0 ? s2 / noise : INFINITY;
     const double scale = (q1 / cv) / (mean * s1);  // makes sum(rho_out) * cv == sum(q)
     const bool ok = mean != 0.0 && isfinite(ratio) && isfinite(scale) && ratio <= tol;
-    result[0] = ratio;
-    result[1] = ok ? 0.0 : 1.0;
-    result[2] = scale;
-    result[3] = q1;
-    if (!ok && fallback_count) *fallback_count += 1;
-}
-
-__global__ __launch_bounds__(256) void denoise_apply_kernel(const float* __restrict__ net_out,
-                                                            float* __restrict__ rho, double* __restrict__ out64,
-                                                            int64_t n, const double* __restrict__ rho_sum,
-                                                            double inv_scale, const double* __restrict__ result) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        result[0] = ratio;
+        result[1] = ok ? 0.0 : 1.0;
+        result[2] = scale;
+        result[3] = q1;
+        if (!ok && fallback_count) *fallback_count += 1;
+    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (result[1] != 0.0) {  // fallback: the plain deposit stays
+    if (!ok) {  // fallback: the plain deposit stays
         out64[i] = (double)rho[i];
         return;
     }
-    const double mean = *rho_sum / (double)n;
-    const double v = mean * (1.0 + (double)net_out[i] * inv_scale) * result[2];
+    const double v = mean * (1.0 + (double)net_out[i] * inv_scale) * scale;
     out64[i] = v;
     rho[i] = (float)v;
 }
 
+// Launches one layer; returns the number of workgroups.
 template <int CIN, int NT, int MODE>
-void conv_launch(const void* in, const void* wpk, const float* bias, void* out, const void* mask, const CnnGeom& g,
-                 int dil, int ncu, hipStream_t s) {
+int conv_launch(const void* in, const void* wpk, const float* bias, void* out, const void* mask, const CnnGeom& g,
+                int dil, int ncu, hipStream_t s, const InNorm& nrm = InNorm{}, const OutStats& ost = OutStats{}) {
     const int64_t rowtiles16 = (int64_t)g.B * g.H * (g.W / 16);
-    // Four 16-pixel tiles per wave when there is enough work, else one.
+    // Four 16-pixel tiles per wave when there is enough work, else one (a
+    // single image runs faster with more, shorter waves).
     const bool wide = (g.W % 64) == 0 && rowtiles16 >= (int64_t)ncu * 64;
     const int mt = wide ? 4 : 1;
     const int64_t tiles = rowtiles16 / mt;
@@ -467,9 +501,19 @@ void conv_launch(const void* in, const void* wpk, const float* bias, void* out, 
     const auto* w = static_cast<const bf16x8*>(wpk);
     const auto* m = static_cast<const __bf16*>(mask);
     if (wide)
-        hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 4>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil);
+        hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 4>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
+                           ost);
     else
-        hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 1>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil);
+        hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 1>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
+                           ost);
+    return (int)grid.x;
+}
+
+int conv_blocks(const CnnGeom& g, int ncu) {
+    const int64_t rowtiles16 = (int64_t)g.B * g.H * (g.W / 16);
+    const bool wide = (g.W % 64) == 0 && rowtiles16 >= (int64_t)ncu * 64;
+    const int64_t tiles = rowtiles16 / (wide ? 4 : 1);
+    return (int)std::max<int64_t>(std::min<int64_t>((tiles + 3) / 4, (int64_t)ncu * 8), 1);
 }
 
 }  // namespace
@@ -557,21 +601,23 @@ void launch_moments(const float* v, int64_t n, double* partial, int nb, hipStrea
     hipLaunchKernelGGL(charge_moments_kernel, dim3(nb), dim3(256), 0, s, v, n, partial);
 }
 
-void launch_denoise_finish(const float* delta_out, double inv_scale, float* rho, double* out64, int64_t G,
-                           const double* rho_sum, const float* q, int64_t n, double cv, double kappa, double tol,
-                           double* scratch, double* result, int* fallback_count, hipStream_t s) {
-    const int nbs = 128, nbm = 128;
-    double* stats = scratch;
-    double* moments = scratch + 2 * nbs;
-    hipLaunchKernelGGL(denoise_stats_kernel, dim3(nbs), dim3(256), 0, s, delta_out, rho, G, rho_sum, inv_scale,
-                       stats);
-    hipLaunchKernelGGL(charge_moments_kernel, dim3(nbm), dim3(256), 0, s, q, n, moments);
-    hipLaunchKernelGGL(denoise_decide_kernel, dim3(1), dim3(256), 0, s, stats, nbs, moments, nbm, rho_sum, G, cv, kappa,
-                       tol, result, fallback_count);
-    hipLaunchKernelGGL(denoise_apply_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, delta_out, rho, out64,
-                       G, rho_sum, inv_scale, result);
+void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
+                    hipStream_t s) {
+    const __bf16* pk = reinterpret_cast<const __bf16*>(c.packed);
+    const double inv_g = 1.0 / (double)a.G;
+    const InNorm nrm{a.sum, a.sum_scale, inv_g, a.in_scale};
+    const OutStats ost{a.stats, a.rho, a.sum, a.sum_scale, inv_g, a.inv_out_scale};
+    conv_launch<1, 2, kHidden>(a.rho, pk + 8 * 64 * 0, c.w + cnn::kB1, c.a1, nullptr, g, dil[0], ncu, s, nrm);
+    conv_launch<32, 2, kHidden>(c.a1, pk + 8 * cnn::kPackF2, c.w + cnn::kB2, c.a2, nullptr, g, dil[1], ncu, s);
+    conv_launch<32, 2, kHidden>(c.a2, pk + 8 * cnn::kPackF3, c.w + cnn::kB3, c.a3, nullptr, g, dil[2], ncu, s);
+    const int nbs =
+        conv_launch<32, 1, kOutput>(c.a3, pk + 8 * cnn::kPackF4, c.w + cnn::kB4, c.out, nullptr, g, dil[3], ncu, s,
+                                    InNorm{}, ost);
+    hipLaunchKernelGGL(denoise_apply_kernel, dim3((unsigned)((a.G + 255) / 256)), dim3(256), 0, s, c.out, a.rho,
+                       a.out64, a.G, a.sum, a.sum_scale, a.inv_out_scale, a.stats, nbs, a.moments, a.nbm, a.cv,
+                       a.kappa, a.tol, a.result, a.fallback_count);
 }
 
-size_t denoise_scratch_doubles() { return 2 * 128 + 2 * 128; }
+size_t denoise_stats_doubles(int ncu) { return (size_t)2 * 8 * std::max(ncu, 1); }
 
 }  // namespace pichip

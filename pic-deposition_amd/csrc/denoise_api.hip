@@ -113,15 +113,46 @@ void enqueue_normalise(pic_hip_denoiser* n, pic_hip_ctx* ctx, float* dst, double
 
 double kappa_of(int scheme);
 
-// Normalise, network, renormalise and gate on the context's stream (kernel
-// launches only).
-void enqueue_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double tolerance, int* fallback_count) {
-    enqueue_normalise(net, ctx, net->c.x0, net->in_scale);
-    launch_cnn_forward(net->c, geom(net, 1), net->dil, net->ncu, ctx->stream);
+constexpr int kMomentBlocks = 128;
+
+// Charge moments of the context's particles into net->scratch[0, 2 kMomentBlocks).
+void enqueue_moments(pic_hip_denoiser* net, pic_hip_ctx* ctx) {
+    launch_moments(ctx->s[4], (int64_t)ctx->n, net->scratch, kMomentBlocks, ctx->stream);
+}
+
+// Network, renormalisation and gate on the context's stream (kernel launches
+// only).  A single call takes the charge moments and the sum of rho itself;
+// inside the step loop (`in_run`) the moments were taken at the start of the
+// run and the mean of rho comes from the particles' charge sum, which the
+// deposit conserves.
+void enqueue_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double tolerance, int* fallback_count,
+                     bool in_run) {
     const double cv = ctx->dev.dx * ctx->dev.dy;
-    launch_denoise_finish(net->c.out, 1.0 / net->out_scale, ctx->rho, net->out64, ctx->G, net->diag_out, ctx->s[4],
-                          (int64_t)ctx->n, cv, kappa_of(scheme), tolerance, net->scratch, net->result, fallback_count,
-                          ctx->stream);
+    DenoiseArgs a{};
+    a.rho = ctx->rho;
+    a.out64 = net->out64;
+    a.G = ctx->G;
+    if (in_run) {
+        a.sum = ctx->run_sums + 3;
+        a.sum_scale = 1.0 / cv;
+    } else {
+        enqueue_moments(net, ctx);
+        launch_diagnostics(ctx->rho, nullptr, nullptr, ctx->G, nullptr, nullptr, nullptr, nullptr, 0, net->diag_part,
+                           1024, net->diag_out, ctx->stream);
+        a.sum = net->diag_out;
+        a.sum_scale = 1.0;
+    }
+    a.in_scale = net->in_scale;
+    a.inv_out_scale = 1.0 / net->out_scale;
+    a.moments = net->scratch;
+    a.nbm = kMomentBlocks;
+    a.cv = cv;
+    a.kappa = kappa_of(scheme);
+    a.tol = tolerance;
+    a.stats = net->scratch + 2 * kMomentBlocks;
+    a.result = net->result;
+    a.fallback_count = fallback_count;
+    launch_denoise(net->c, geom(net, 1), net->dil, net->ncu, a, ctx->stream);
 }
 
 double kappa_of(int scheme) {
@@ -191,7 +222,8 @@ pic_status pic_hip_denoiser_create(int32_t Nx, int32_t Ny, int32_t max_batch, in
     PIC_HIP_TRY(dmalloc(&n->targets, P * std::max(max_pairs, 1)));
     PIC_HIP_TRY(dmalloc(&n->diag_part, (size_t)5 * 1024));
     PIC_HIP_TRY(dmalloc(&n->diag_out, 8));
-    PIC_HIP_TRY(dmalloc(&n->scratch, denoise_scratch_doubles()));
+    PIC_HIP_TRY(dmalloc(&n->scratch,
+                        std::max<size_t>(4 * kMomentBlocks, 2 * kMomentBlocks + denoise_stats_doubles(n->ncu))));
     PIC_HIP_TRY(dmalloc(&n->result, 4));
     PIC_HIP_TRY(dmalloc(&n->out64, P));
     PIC_HIP_TRY(dmalloc(&n->fallbacks, 1));
@@ -323,9 +355,9 @@ pic_status pic_hip_denoiser_train(pic_hip_denoiser* net, int32_t iterations, int
     }
     if (!net->scaled) {
         // RMS of the input and target sets (FP64 sums on the device).
-        const int nb = 128;
+        const int nb = kMomentBlocks;
         const size_t len = (size_t)pairs * net->plane();
-        double* part = net->scratch;  // 2 * 128 + 2 * 128 doubles
+        double* part = net->scratch;  // 4 * kMomentBlocks doubles
         launch_moments(net->inputs, (int64_t)len, part, nb, net->stream);
         launch_moments(net->targets, (int64_t)len, part + 2 * nb, nb, net->stream);
         std::vector<double> h(4 * nb);
@@ -378,7 +410,7 @@ pic_status pic_hip_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, 
     hipStream_t s = ctx->stream;
     PIC_HIP_TRY(hipEventRecord(net->ev_net, net->stream));
     PIC_HIP_TRY(hipStreamWaitEvent(s, net->ev_net, 0));
-    enqueue_denoise(ctx, net, scheme, tolerance, nullptr);
+    enqueue_denoise(ctx, net, scheme, tolerance, nullptr, false);
     PIC_HIP_TRY(hipGetLastError());
     PIC_HIP_TRY(hipEventRecord(net->ev_ctx, s));
     PIC_HIP_TRY(hipStreamWaitEvent(net->stream, net->ev_ctx, 0));
@@ -400,9 +432,10 @@ pic_status pic_hip_run_learned(pic_hip_ctx* ctx, pic_hip_denoiser* net, const pi
     PIC_HIP_TRY(hipEventRecord(net->ev_net, net->stream));
     PIC_HIP_TRY(hipStreamWaitEvent(ctx->stream, net->ev_net, 0));
     launch_set_int(net->fallbacks, 0, ctx->stream);
+    enqueue_moments(net, ctx);  // the particles' charges do not change during the run
     const int scheme = cfg->scheme;
     const StepHook hook = [ctx, net, scheme, tolerance]() -> pic_status {
-        enqueue_denoise(ctx, net, scheme, tolerance, net->fallbacks);
+        enqueue_denoise(ctx, net, scheme, tolerance, net->fallbacks, true);
         PIC_HIP_TRY(hipGetLastError());
         return PIC_OK;
     };

@@ -189,6 +189,12 @@ TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D) {
     const int rx = Nx % g.T, ry = Ny % g.R;
     g.fast = g.D == 0 && g.T >= 3 && g.R >= 3 && g.ntx >= 3 && g.nty >= 3 && (rx == 0 || rx >= 3) &&
              (ry == 0 || ry >= 3);
+    // At most (T + 2 + 2D) / T + 1 consecutive tiles cover a node along an
+    // axis; the range path needs them distinct and each window to hold one
+    // image of the node.
+    const bool ux = rx == 0 && g.WX <= Nx && (g.T + 2 + 2 * g.D) / g.T + 1 <= g.ntx;
+    const bool uy = ry == 0 && g.WY <= Ny && (g.R + 2 + 2 * g.D) / g.R + 1 <= g.nty;
+    g.uniform = ux && uy;
     return g;
 }
 

@@ -489,6 +489,24 @@ __device__ __forceinline__ int axis_pairs(int gn, int T, int nt, int n, int (&t)
     return 1;
 }
 
+// Equal tiles of T cells dividing the axis, windows [tT - 1 - D, tT + T + 2 + D):
+// the tiles whose windows hold node gn are the consecutive (unwrapped) tiles
+// floor((gn - 2 - D) / T) .. floor((gn + 1 + D) / T), the same tiles in the same
+// order as window_candidates finds them.  Writes (tile, window index) pairs.
+__device__ __forceinline__ int axis_range(int gn, int T, int nt, int D, int (&t)[4 + 2 * kMaxDriftHalo],
+                                          int (&w)[4 + 2 * kMaxDriftHalo]) {
+    const int a = gn - 2 - D;
+    const int tlo = a >= 0 ? a / T : -((-a + T - 1) / T);
+    const int thi = (gn + 1 + D) / T;
+    int cnt = 0;
+    for (int u = tlo; u <= thi && cnt < 4 + 2 * kMaxDriftHalo; ++u) {
+        t[cnt] = ((u % nt) + nt) % nt;
+        w[cnt] = gn - (u * T - 1 - D);
+        ++cnt;
+    }
+    return cnt;
+}
+
 // rho at node (gx, gy): the windows covering it, added in FP64 in a fixed
 // order, plus the FP64 accumulation grid where it is non-zero (then cleared).
 __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restrict__ partial,
@@ -504,6 +522,16 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
                     partial[(size_t)(tys[iy] * g.ntx + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
                 s += (double)v;
             }
+        }
+    } else if (g.uniform) {
+        int txs[4 + 2 * kMaxDriftHalo], wxs[4 + 2 * kMaxDriftHalo];
+        int tys[4 + 2 * kMaxDriftHalo], wys[4 + 2 * kMaxDriftHalo];
+        const int ncx = axis_range(gx, g.T, g.ntx, g.D, txs, wxs);
+        const int ncy = axis_range(gy, g.R, g.nty, g.D, tys, wys);
+        for (int iy = 0; iy < ncy; ++iy) {
+            const size_t row = (size_t)tys[iy] * g.ntx;
+            for (int ix = 0; ix < ncx; ++ix)
+                s += (double)partial[(row + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
         }
     } else {
         int txs[4 + 2 * kMaxDriftHalo], tys[4 + 2 * kMaxDriftHalo];

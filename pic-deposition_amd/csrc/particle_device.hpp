@@ -31,7 +31,17 @@ __device__ __forceinline__ void gather_cic(float x, float y, const float* __rest
 // added, so |v dt| >> L does not cost FP32 position bits; in exact arithmetic
 // this equals fmod(x + v dt, L).
 __device__ __forceinline__ float push_coord(float x, float v, float dt, float L) {
-    x += fmodf(v * dt, L);
+    const float s = v * dt;
+    if (fabsf(s) < L) {  // fmodf(s, L) == s
+        float t = x + s;
+        // fmodf(t, L) is t on (-L, L) and t - L (exact) on [L, 2L)
+        if (t >= 0.0f && t < L) return t;
+        if (t >= L && t < L + L) return t - L;
+        if (t < 0.0f && t > -L) return t + L;
+        t = fmodf(t, L);
+        return t < 0.0f ? t + L : t;
+    }
+    x += fmodf(s, L);
     x = fmodf(x, L);
     if (x < 0.0f) x += L;
     return x;

@@ -26,6 +26,7 @@ struct TileGeom {
     int WX, WY, wsize;
     int lds_per_wave;  // floats: 2 windows + row tables
     int fast;          // merge fast path: no halo, tiles >= 3 cells on each axis, >= 3 tiles per axis
+    int uniform;       // merge range path: equal tiles dividing each axis, one window image per node
 };
 
 TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D = 0);
@@ -120,9 +121,12 @@ void launch_init(float* x, float* y, float* vx, float* vy, float* q, float* m, i
                  const InitParams& p, hipStream_t s);
 
 // Spectral Poisson solve helpers (hipFFT plans live in api.hip).
-// in[i] = (rho[i] + bg) * cv - mean, mean from the FP64 charge sum.
+// in[i] = (rho[i] + bg) * cv - mean, where mean * G / cv is the FP64 sum of the
+// stored rho plus bg * G, taken as *rho_sum * sum_scale + sum_add (a sum of rho
+// has scale 1; the particles' charge sum, which the deposit conserves, has
+// scale 1 / cv).
 void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, double cv, const double* rho_sum,
-                         hipStream_t s);
+                         double sum_scale, double sum_add, hipStream_t s);
 struct PoissonTables {
     double* k2x;     // [Nx/2+1] kx^2
     double* k2y;     // [Ny]     ky^2
@@ -135,6 +139,19 @@ struct PoissonTables {
 // the cancellation of differencing an FP32 phi).
 void launch_poisson_spectral(float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
                              const DevDomain& d, hipStream_t s);
+
+// The same solve in three launches with the 1D transforms in LDS (csrc/poisson.hip),
+// for power-of-two grids from 4 to 1024 cells per axis.  launch_poisson_lds takes
+// rho (mean removed as in launch_poisson_prep) to the row half spectra
+// [Ny][Nx/2+1] of phi (in spec), Ex and Ey; launch_poisson_lds_rows turns row
+// half spectra X (and Y, may be null) into the real grids ox (and oy).
+// tw_x, tw_y: e^{2 pi i m / Nx}, m < Nx, and e^{2 pi i m / Ny}, m < Ny.
+bool poisson_lds_supported(int Nx, int Ny);
+void launch_poisson_lds(const float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
+                        double sum_add, float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
+                        const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s);
+void launch_poisson_lds_rows(const float2* X, const float2* Y, float* ox, float* oy, const DevDomain& d,
+                             const float2* tw_x, hipStream_t s);
 // Ex = -(phi(i+1,j) - phi(i,j))/dx, Ey = -(phi(i,j+1) - phi(i,j))/dy.
 void launch_efield(const float* phi, float* Ex, float* Ey, const DevDomain& d, hipStream_t s);
 void launch_add_scalar(float* p, int64_t n, float v, hipStream_t s);
@@ -296,14 +313,28 @@ void launch_normalize(const float* rho, int64_t n, const double* rho_sum, double
                       hipStream_t s);
 // Per-block FP64 sums of v and v^2 into partial[2 * block + {0, 1}].
 void launch_moments(const float* v, int64_t n, double* partial, int nb, hipStream_t s);
-// Renormalisation and gate: result = {ratio, fallback, scale, sum q}; when the
-// gate passes rho (FP32) and out64 hold the renormalised learned density,
-// otherwise rho is left as deposited, out64 is its FP64 copy and
-// *fallback_count (if not null) is incremented.
-// The network output is delta / inv_scale.
-void launch_denoise_finish(const float* delta_out, double inv_scale, float* rho, double* out64, int64_t G,
-                           const double* rho_sum, const float* q, int64_t n, double cv, double kappa, double tol,
-                           double* scratch, double* result, int* fallback_count, hipStream_t s);
-size_t denoise_scratch_doubles();
+// Learned density of one [H][W] grid: rho normalised on the fly (mean of rho =
+// *sum * sum_scale / G), the network, then the gate and renormalisation:
+// result = {ratio, fallback, scale, sum q}.  When the gate passes, rho (FP32)
+// and out64 hold the renormalised learned density; otherwise rho is left as
+// deposited, out64 is its FP64 copy and *fallback_count (if not null) is
+// incremented.  `moments` holds nbm per-block charge moments (launch_moments).
+struct DenoiseArgs {
+    float* rho;
+    double* out64;
+    int64_t G;
+    const double* sum;
+    double sum_scale;
+    double in_scale, inv_out_scale;  // network input scale, 1 / output scale
+    const double* moments;
+    int nbm;
+    double cv, kappa, tol;
+    double* stats;  // denoise_stats_doubles(ncu)
+    double* result;
+    int* fallback_count;
+};
+void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
+                    hipStream_t s);
+size_t denoise_stats_doubles(int ncu);
 
 }  // namespace pichip

@@ -1,0 +1,223 @@
+// Spectral Poisson solve in three launches for power-of-two grids up to 1024
+// cells per axis, with every 1D transform done by one wavefront in LDS.
+//
+// The solve is the one of poisson_fft.cpp:39-66 (rho * cv with the mean
+// removed, phi_hat = rho_hat / (k^2 Nx Ny), k = 0 zeroed) followed by the
+// spectral forward differences for E (the same operator as the reference's
+// real-space difference of phi in exact arithmetic):
+//
+//  * poisson_rows_kernel: the mean-removed rows 2p and 2p+1 as the real and
+//    imaginary part of one complex transform of length Nx; the two half
+//    spectra are separated with the conjugate symmetry of real input.
+//  * poisson_cols_kernel: one half-spectrum column: forward transform along y,
+//    the Green's function and the difference symbols, then inverse transforms
+//    along y of phi, Ex and Ey, written back as row half spectra.
+//  * poisson_rows_inv_kernel: one row: the half spectra of two real outputs X
+//    and Y combined into the full spectrum of X + iY, one inverse transform,
+//    real part to X and imaginary part to Y.  The imaginary parts of the DC and
+//    Nyquist bins are dropped, as a complex-to-real transform does.
+//
+// Transforms are unnormalised (forward e^{-i}, inverse e^{+i}); the 1/(Nx Ny)
+// of the solve sits in the Green's function.  The 1D transform is the
+// Stockham autosort form: radix-4 passes and a final radix-2 pass when the
+// length is not a power of four, twiddles from one table e^{2 pi i m / L}
+// tabulated in FP64 and rounded to FP32.
+
+#include "pic_common.hpp"
+#include "pic_kernels.hpp"
+
+namespace pichip {
+
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Unnormalised DFT of a[0, L) by the calling wavefront (the only wave of its
+// workgroup), SIGN = -1 forward, +1 inverse.  b is scratch of the same length
+// and tw[m] = e^{2 pi i m / L}.  Returns the buffer that holds the result.
+template <int SIGN>
+__device__ float2* wave_fft(float2* a, float2* b, int L, const float2* tw) {
+    const int lane = threadIdx.x;
+    for (int Ls = 1; Ls < L;) {
+        const int R = (L / Ls >= 4) ? 4 : 2;
+        const int NR = L / R;
+        const int tstep = L / (Ls * R);
+        for (int j = lane; j < NR; j += 64) {
+            const int k = j & (Ls - 1);
+            const int o = (j - k) * R + k;
+            if (R == 4) {
+                float2 x0 = a[j], x1 = a[j + NR], x2 = a[j + 2 * NR], x3 = a[j + 3 * NR];
+                if (k != 0) {
+                    float2 w1 = tw[k * tstep], w2 = tw[2 * k * tstep], w3 = tw[3 * k * tstep];
+                    if (SIGN < 0) {
+                        w1.y = -w1.y;
+                        w2.y = -w2.y;
+                        w3.y = -w3.y;
+                    }
+                    x1 = cmul(x1, w1);
+                    x2 = cmul(x2, w2);
+                    x3 = cmul(x3, w3);
+                }
+                const float2 s02 = make_float2(x0.x + x2.x, x0.y + x2.y);
+                const float2 d02 = make_float2(x0.x - x2.x, x0.y - x2.y);
+                const float2 s13 = make_float2(x1.x + x3.x, x1.y + x3.y);
+                const float2 d13 = make_float2(x1.x - x3.x, x1.y - x3.y);
+                // (SIGN i) * d13
+                const float2 r13 = SIGN > 0 ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+                b[o] = make_float2(s02.x + s13.x, s02.y + s13.y);
+                b[o + Ls] = make_float2(d02.x + r13.x, d02.y + r13.y);
+                b[o + 2 * Ls] = make_float2(s02.x - s13.x, s02.y - s13.y);
+                b[o + 3 * Ls] = make_float2(d02.x - r13.x, d02.y - r13.y);
+            } else {
+                float2 x0 = a[j], x1 = a[j + NR];
+                if (k != 0) {
+                    float2 w1 = tw[k * tstep];
+                    if (SIGN < 0) w1.y = -w1.y;
+                    x1 = cmul(x1, w1);
+                }
+                b[o] = make_float2(x0.x + x1.x, x0.y + x1.y);
+                b[o + Ls] = make_float2(x0.x - x1.x, x0.y - x1.y);
+            }
+        }
+        __syncthreads();
+        float2* t = a;
+        a = b;
+        b = t;
+        Ls *= R;
+    }
+    return a;
+}
+
+__global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restrict__ rho, float2* __restrict__ spec,
+                                                          int W, int64_t G, double bg, double cv,
+                                                          const double* __restrict__ rho_sum, double sum_scale,
+                                                          double sum_add, const float2* __restrict__ tw) {
+    extern __shared__ float2 sm[];
+    float2* a = sm;
+    float2* b = sm + W;
+    float2* t = sm + 2 * W;
+    const int lane = threadIdx.x;
+    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+    const double mean = (*rho_sum * sum_scale + sum_add + bg * (double)G) * cv / (double)G;
+    for (int n = lane; n < W; n += 64) {
+        const float u = (float)(((double)rho[(size_t)r0 * W + n] + bg) * cv - mean);
+        const float v = (float)(((double)rho[(size_t)r1 * W + n] + bg) * cv - mean);
+        a[n] = make_float2(u, v);
+        t[n] = tw[n];
+    }
+    __syncthreads();
+    const float2* Z = wave_fft<-1>(a, b, W, t);
+    const int hx = W / 2 + 1;
+    for (int k = lane; k < hx; k += 64) {
+        const float2 p = Z[k], m = Z[(W - k) & (W - 1)];
+        // A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / (2i)
+        spec[(size_t)r0 * hx + k] = make_float2(0.5f * (p.x + m.x), 0.5f * (p.y - m.y));
+        spec[(size_t)r1 * hx + k] = make_float2(0.5f * (p.y + m.y), -0.5f * (p.x - m.x));
+    }
+}
+
+__global__ __launch_bounds__(64) void poisson_cols_kernel(float2* __restrict__ spec, float2* __restrict__ sex,
+                                                          float2* __restrict__ sey, PoissonTables tb, DevDomain d,
+                                                          const float2* __restrict__ tw) {
+    extern __shared__ float2 sm[];
+    const int H = d.Ny;
+    float2* a = sm;
+    float2* b = sm + H;
+    float2* ex = sm + 2 * H;
+    float2* ey = sm + 3 * H;
+    float2* t = sm + 4 * H;
+    const int lane = threadIdx.x;
+    const int hx = d.Nx / 2 + 1;
+    const int kx = blockIdx.x;
+    for (int y = lane; y < H; y += 64) {
+        a[y] = spec[(size_t)y * hx + kx];
+        t[y] = tw[y];
+    }
+    __syncthreads();
+    float2* F = wave_fft<-1>(a, b, H, t);
+    float2* S = F == a ? b : a;  // free buffer
+    const float2 sx = tb.dx_sym[kx];
+    for (int ky = lane; ky < H; ky += 64) {
+        float2 p = make_float2(0.0f, 0.0f);
+        if (kx != 0 || ky != 0) {
+            const float f = (float)(1.0 / ((tb.k2x[kx] + tb.k2y[ky]) * (double)d.Nx * (double)d.Ny));
+            const float2 v = F[ky];
+            p = make_float2(v.x * f, v.y * f);
+        }
+        F[ky] = p;
+        const float2 sy = tb.dy_sym[ky];
+        ex[ky] = make_float2(-(sx.x * p.x - sx.y * p.y), -(sx.x * p.y + sx.y * p.x));
+        ey[ky] = make_float2(-(sy.x * p.x - sy.y * p.y), -(sy.x * p.y + sy.y * p.x));
+    }
+    __syncthreads();
+    const float2* P = wave_fft<1>(F, S, H, t);
+    for (int y = lane; y < H; y += 64) spec[(size_t)y * hx + kx] = P[y];
+    float2* scratch = P == F ? S : F;
+    __syncthreads();
+    const float2* X = wave_fft<1>(ex, scratch, H, t);
+    for (int y = lane; y < H; y += 64) sex[(size_t)y * hx + kx] = X[y];
+    float2* scratch2 = X == ex ? scratch : ex;
+    __syncthreads();
+    const float2* Y = wave_fft<1>(ey, scratch2, H, t);
+    for (int y = lane; y < H; y += 64) sey[(size_t)y * hx + kx] = Y[y];
+}
+
+__global__ __launch_bounds__(64) void poisson_rows_inv_kernel(const float2* __restrict__ X,
+                                                              const float2* __restrict__ Y, float* __restrict__ ox,
+                                                              float* __restrict__ oy, int W,
+                                                              const float2* __restrict__ tw) {
+    extern __shared__ float2 sm[];
+    float2* a = sm;
+    float2* b = sm + W;
+    float2* t = sm + 2 * W;
+    const int lane = threadIdx.x;
+    const int hx = W / 2 + 1;
+    const size_t row = blockIdx.x;
+    const float2* xr = X + row * hx;
+    const float2* yr = Y ? Y + row * hx : nullptr;
+    for (int k = lane; k < hx; k += 64) {
+        const float2 u = xr[k];
+        const float2 v = yr ? yr[k] : make_float2(0.0f, 0.0f);
+        if (k == 0 || k == W / 2) {
+            a[k] = make_float2(u.x, v.x);  // DC and Nyquist: real parts only
+        } else {
+            a[k] = make_float2(u.x - v.y, u.y + v.x);           // X + iY
+            a[W - k] = make_float2(u.x + v.y, -u.y + v.x);      // conj X + i conj Y
+        }
+    }
+    for (int n = lane; n < W; n += 64) t[n] = tw[n];
+    __syncthreads();
+    const float2* z = wave_fft<1>(a, b, W, t);
+    for (int n = lane; n < W; n += 64) {
+        const float2 v = z[n];
+        ox[row * W + n] = v.x;
+        if (oy) oy[row * W + n] = v.y;
+    }
+}
+
+}  // namespace
+
+bool poisson_lds_supported(int Nx, int Ny) {
+    auto pow2 = [](int v) { return v >= 4 && v <= 1024 && (v & (v - 1)) == 0; };
+    return pow2(Nx) && pow2(Ny);
+}
+
+void launch_poisson_lds(const float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
+                        double sum_add, float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
+                        const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s) {
+    const int W = d.Nx, H = d.Ny;
+    hipLaunchKernelGGL(poisson_rows_kernel, dim3((unsigned)(H / 2)), dim3(64), (size_t)3 * W * sizeof(float2), s, rho,
+                       spec, W, G, bg, cv, rho_sum, sum_scale, sum_add, tw_x);
+    hipLaunchKernelGGL(poisson_cols_kernel, dim3((unsigned)(W / 2 + 1)), dim3(64), (size_t)5 * H * sizeof(float2), s,
+                       spec, spec_ex, spec_ey, t, d, tw_y);
+}
+
+void launch_poisson_lds_rows(const float2* X, const float2* Y, float* ox, float* oy, const DevDomain& d,
+                             const float2* tw_x, hipStream_t s) {
+    hipLaunchKernelGGL(poisson_rows_inv_kernel, dim3((unsigned)d.Ny), dim3(64), (size_t)3 * d.Nx * sizeof(float2), s,
+                       X, Y, ox, oy, d.Nx, tw_x);
+}
+
+}  // namespace pichip

@@ -112,10 +112,11 @@ __global__ __launch_bounds__(256) void init_kernel(float* __restrict__ px, float
 // enters the single-precision FFT.
 __global__ __launch_bounds__(256) void poisson_prep_kernel(const float* __restrict__ rho, float* __restrict__ in,
                                                            int64_t G, double bg, double cv,
-                                                           const double* __restrict__ rho_sum) {
+                                                           const double* __restrict__ rho_sum, double sum_scale,
+                                                           double sum_add) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G) return;
-    const double mean = (*rho_sum + bg * (double)G) * cv / (double)G;
+    const double mean = (*rho_sum * sum_scale + sum_add + bg * (double)G) * cv / (double)G;
     in[i] = (float)(((double)rho[i] + bg) * cv - mean);
 }
 
@@ -242,9 +243,9 @@ __global__ __launch_bounds__(256) void copy_f32_kernel(const float* __restrict__
 }  // namespace
 
 void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, double cv, const double* rho_sum,
-                         hipStream_t s) {
+                         double sum_scale, double sum_add, hipStream_t s) {
     hipLaunchKernelGGL(poisson_prep_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, rho, in, G, bg, cv,
-                       rho_sum);
+                       rho_sum, sum_scale, sum_add);
 }
 
 void launch_poisson_spectral(float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,

@@ -83,8 +83,11 @@ def test_gather_constant_and_linear_fields(hip):
     np.testing.assert_allclose(o["vx"], x.astype(np.float64) * nx, rtol=2e-6, atol=2e-6)
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 1.0, 1.0), (128, 32, 2.0, 1.0), (512, 512, 1.0, 1.0), (100, 60, 1.0, 1.0)])
+@pytest.mark.parametrize("shape", [(64, 64, 1.0, 1.0), (128, 32, 2.0, 1.0), (512, 512, 1.0, 1.0), (1024, 1024, 1.0, 1.0),
+                                   (8, 4, 1.0, 0.5), (100, 60, 1.0, 1.0), (2048, 16, 1.0, 1.0)])
 def test_poisson_vs_oracle(hip, orc, shape):
+    """Power-of-two grids up to 1024 per axis take the LDS transforms, the
+    others hipFFT; both against the FP64 oracle."""
     nx, ny, lx, ly = shape
     say("poisson", shape)
     dom = hip.make_domain(lx, ly, nx, ny)

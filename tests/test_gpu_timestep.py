@@ -112,6 +112,42 @@ def test_graph_replay_matches_eager(hip, orc, scheme):
         np.testing.assert_allclose(np.sort(g1_state[k]), np.sort(eager_state[k]), rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_late_records_and_fields_after_run(hip, use_graph):
+    """Diagnostics run only on recorded steps: records taken from step 12 on
+    equal the tail of a run recorded from step 0.  The loop's solves leave the
+    potential as a spectrum; reading phi after the run transforms it and gives
+    the potential of a fresh solve of the final rho (the loop takes the mean of
+    rho from the particle charges, a fresh solve from the grid sum)."""
+    say("late records", use_graph)
+    G, n = 64, 64 * 64 * 16
+    dom = hip.make_domain(1.0, 1.0, G, G)
+    rng = np.random.default_rng(12)
+    p = {"x": rng.uniform(0, 1, n), "y": rng.uniform(0, 1, n), "vx": rng.normal(0, 0.05, n),
+         "vy": rng.normal(0, 0.05, n), "q": np.full(n, -1.0 / n), "m": np.ones(n)}
+    p["x"] = (p["x"] + 0.05 * np.sin(2 * np.pi * p["x"])) % 1.0
+    p = f32_round(p)
+    kw = dict(scheme=2, backend=2, dt=0.05, background=1.0, sort_interval=1, record_particles=True,
+              use_graph=use_graph)
+    with hip.Context(dom, n) as c:
+        upload(c, p)
+        full = c.run(30, record_from=0, **kw)
+        upload(c, p)
+        late = c.run(30, record_from=12, **kw)
+        phi_run, ex_run, ey_run = c.fields()
+        rho = c.rho()
+        c.poisson()
+        phi_new, ex_new, ey_new = c.fields()
+        np.testing.assert_array_equal(c.rho(), rho)
+    for k in full:
+        assert late[k].shape == (18,)
+        np.testing.assert_array_equal(late[k], full[k][12:], err_msg=k)
+    for a, b in ((phi_run, phi_new), (ex_run, ex_new), (ey_run, ey_new)):
+        scale = np.abs(b).max()
+        assert scale > 0
+        assert np.abs(a - b).max() <= 1e-5 * scale
+
+
 def test_energy_and_charge_records_match_oracle_loop(hip, orc):
     """Reduced config-3 shape (64x64, 100 ppc, two beams +-0.2, thermal 0.01,
     TSC, sort every 10 steps): per-step field energy, kinetic energy and
