@@ -259,8 +259,8 @@ struct TileCtx {
 };
 
 // Processes one chunk: K consecutive particles per lane accumulated as runs of
-// equal cell in registers, then one segmented wave scan, then one
-// compensated LDS add per (cell, stencil slot).
+// equal cell in registers, then one segmented wave scan, then one FP64 LDS
+// add per (cell, stencil slot).
 template <int S, int Q, int EX>
 __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx& t, const DevDomain& d, float dt,
                                               double* __restrict__ gacc, int* __restrict__ status) {
@@ -290,16 +290,31 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                 vxx = e == 0 ? cur.VX[j].x : e == 1 ? cur.VX[j].y : e == 2 ? cur.VX[j].z : cur.VX[j].w;
                 vyy = e == 0 ? cur.VY[j].x : e == 1 ? cur.VY[j].y : e == 2 ? cur.VY[j].z : cur.VY[j].w;
             }
-            const float qv = valid ? qq * d.inv_vol : 0.0f;
-            float w[V];
+            // Home cell first: the window test decides whether the weights go
+            // to the window, so the charge of any other particle is zeroed
+            // before the (linear in q) weights are formed.  Unwrapped cell:
+            // positions outside [0, L) fall outside every tile and take the
+            // (wrapping) overflow path.
             int cx, cy;
-            const bool ok = St::template weights<EX>(xx, yy, vxx, vyy, qv, dt, d, cx, cy, w);
-            // Unwrapped cell: positions outside [0, L) fall outside every tile
-            // and take the (wrapping) overflow path.
+            float fx, fy;
+            home_cell<EX>(xx, yy, d, cx, cy, fx, fy);
             const int lx = cx - t.ax0, ly = cy - t.ay0;
-            const bool inside = ok && (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
+            const bool in_win = (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
+            const float qv = valid ? qq * d.inv_vol : 0.0f;
+            // Slots past the tile's particles hold unloaded registers and
+            // quads may reach into the array's pad: a particle that is not
+            // deposited here gets zero charge and finite fractions, so its
+            // weights are exact zeros.
+            const bool take0 = valid && in_win;
+            float w[V];
+            const bool ok = St::template fill<EX>(take0 ? fx : 0.0f, take0 ? fy : 0.0f, vxx, vyy, take0 ? qv : 0.0f,
+                                                  dt, d, w);
+            const bool inside = ok && in_win;
             if (valid && !inside) {
-                overflow_scatter<S>(ok, cx, cy, w, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
+                // overflow (rare): weights with the particle's charge
+                float wo[V];
+                const bool ok_o = St::template fill<EX>(fx, fy, vxx, vyy, qv, dt, d, wo);
+                overflow_scatter<S>(ok_o, cx, cy, wo, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
             }
             const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
             const bool nseg = lk != ck;
@@ -314,7 +329,9 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                 }
             }
             const float keep = nseg ? 0.0f : 1.0f;
-            const bool take = valid && inside;
+            // w is zero unless the particle is valid and in the window; only an
+            // Esirkepov move longer than a cell (ok false) must be masked.
+            const bool take = St::kAlwaysOk ? true : (valid && inside);
 #pragma unroll
             for (int k = 0; k < V; ++k) acc[k] = fmaf(acc[k], keep, take ? w[k] : 0.0f);
             if (nseg) {

@@ -92,6 +92,15 @@ __device__ __forceinline__ float two_sum_err(float a, float b) {
 // SX x SY node stencil whose lower-left corner is (cx + OX, cy + OY).  Slot k
 // is node (cx + OX + k % SX, cy + OY + k / SX).  Weights include q / cellVolume.
 // ---------------------------------------------------------------------------
+// Home cell and in-cell fractions of a particle: the cell every stencil is
+// anchored to (weights = fill(fractions), so the window test can run first).
+template <int EX = -1>
+__device__ __forceinline__ void home_cell(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                          float& fy) {
+    cell_coord<EX>(x, d.inv_dx, d.dx, d.exact, cx, fx);
+    cell_coord<EX>(y, d.inv_dy, d.dy, d.exact, cy, fy);
+}
+
 template <int S>
 struct Stencil;
 
@@ -101,18 +110,23 @@ template <>
 struct Stencil<NGP> {
     static constexpr int SX = 2, SY = 2, V = 4, OX = 0, OY = 0;
     static constexpr bool kNeedsVelocity = false;
+    static constexpr bool kAlwaysOk = true;  // fill never rejects a particle
     template <int EX = -1>
-    __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
-                                                   int& cx, int& cy, float (&w)[V]) {
-        float fx, fy;
-        cell_coord<EX>(x, d.inv_dx, d.dx, d.exact, cx, fx);
-        cell_coord<EX>(y, d.inv_dy, d.dy, d.exact, cy, fy);
+    __device__ __forceinline__ static bool fill(float fx, float fy, float, float, float qv, float, const DevDomain&,
+                                                float (&w)[V]) {
         const bool rx = fx >= 0.5f, ry = fy >= 0.5f;
         w[0] = (!rx && !ry) ? qv : 0.0f;
         w[1] = (rx && !ry) ? qv : 0.0f;
         w[2] = (!rx && ry) ? qv : 0.0f;
         w[3] = (rx && ry) ? qv : 0.0f;
         return true;
+    }
+    template <int EX = -1>
+    __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
+                                                   const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
+        float fx, fy;
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
+        return fill<EX>(fx, fy, vx, vy, qv, dt, d, w);
     }
 };
 
@@ -121,12 +135,10 @@ template <>
 struct Stencil<CIC> {
     static constexpr int SX = 2, SY = 2, V = 4, OX = 0, OY = 0;
     static constexpr bool kNeedsVelocity = false;
+    static constexpr bool kAlwaysOk = true;  // fill never rejects a particle
     template <int EX = -1>
-    __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
-                                                   int& cx, int& cy, float (&w)[V]) {
-        float wx, wy;
-        cell_coord<EX>(x, d.inv_dx, d.dx, d.exact, cx, wx);
-        cell_coord<EX>(y, d.inv_dy, d.dy, d.exact, cy, wy);
+    __device__ __forceinline__ static bool fill(float wx, float wy, float, float, float qv, float, const DevDomain&,
+                                                float (&w)[V]) {
         // q(1-wx) and q wx first, then the y factors: 6 operations, each
         // weight within 2 ulp of q*(1-wx)*(1-wy) etc.
         const float a1 = qv * wx, a0 = qv - a1;
@@ -135,6 +147,13 @@ struct Stencil<CIC> {
         w[0] = a0 - w[2];
         w[1] = a1 - w[3];
         return true;
+    }
+    template <int EX = -1>
+    __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
+                                                   const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
+        float fx, fy;
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
+        return fill<EX>(fx, fy, vx, vy, qv, dt, d, w);
     }
 };
 
@@ -163,12 +182,17 @@ template <>
 struct Stencil<TSC> {
     static constexpr int SX = 4, SY = 4, V = 16, OX = -1, OY = -1;
     static constexpr bool kNeedsVelocity = false;
+    static constexpr bool kAlwaysOk = true;  // fill never rejects a particle
     template <int EX = -1>
-    __device__ __forceinline__ static bool weights(float x, float y, float, float, float qv, float, const DevDomain& d,
-                                                   int& cx, int& cy, float (&w)[V]) {
+    __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
+                                                   const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
         float fx, fy;
-        cell_coord<EX>(x, d.inv_dx, d.dx, d.exact, cx, fx);
-        cell_coord<EX>(y, d.inv_dy, d.dy, d.exact, cy, fy);
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
+        return fill<EX>(fx, fy, vx, vy, qv, dt, d, w);
+    }
+    template <int EX = -1>
+    __device__ __forceinline__ static bool fill(float fx, float fy, float, float, float qv, float, const DevDomain&,
+                                                float (&w)[V]) {
         float wx[4], wy[4];
         tsc_1d(fx, wx);
         tsc_1d(fy, wy);
@@ -191,10 +215,8 @@ struct Stencil<TSC> {
 // cell the support is within offsets -1..1 of the home cell.  Returns false
 // when a move crosses more than one face (handled by the general walk).
 template <int EX = -1>
-__device__ __forceinline__ bool esk_1d(float x0, float v, float dt, float inv_d, double dd, int exact, int& c0,
-                                       float (&w)[3]) {
-    float fr0;
-    cell_coord<EX>(x0, inv_d, dd, exact, c0, fr0);
+__device__ __forceinline__ bool esk_1d_from(float fr0, float v, float dt, float inv_d, double dd, int exact,
+                                            float (&w)[3]) {
     const float disp = v * dt;
     w[0] = w[1] = w[2] = 0.0f;
     // The reference tests |x1 - x0| < 1e-15 in FP64; the displacement itself
@@ -236,12 +258,20 @@ template <>
 struct Stencil<ESK> {
     static constexpr int SX = 3, SY = 3, V = 9, OX = -1, OY = -1;
     static constexpr bool kNeedsVelocity = true;
+    static constexpr bool kAlwaysOk = false;  // moves across more than one face take the walk
     template <int EX = -1>
     __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
                                                    const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
+        float fx, fy;
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
+        return fill<EX>(fx, fy, vx, vy, qv, dt, d, w);
+    }
+    template <int EX = -1>
+    __device__ __forceinline__ static bool fill(float fx, float fy, float vx, float vy, float qv, float dt,
+                                                const DevDomain& d, float (&w)[V]) {
         float wx[3], wy[3];
-        const bool okx = esk_1d<EX>(x, vx, dt, d.inv_dx, d.dx, d.exact, cx, wx);
-        const bool oky = esk_1d<EX>(y, vy, dt, d.inv_dy, d.dy, d.exact, cy, wy);
+        const bool okx = esk_1d_from<EX>(fx, vx, dt, d.inv_dx, d.dx, d.exact, wx);
+        const bool oky = esk_1d_from<EX>(fy, vy, dt, d.inv_dy, d.dy, d.exact, wy);
 #pragma unroll
         for (int a = 0; a < 3; ++a) wx[a] *= qv;
 #pragma unroll
