@@ -168,7 +168,8 @@ pic_status pic_hip_set_rho(pic_hip_ctx* ctx, const float* rho);
 /* rho += value on every cell (neutralising background 1/V). */
 pic_status pic_hip_add_background(pic_hip_ctx* ctx, double value);
 /* phi, Ex, Ey from rho: spectral solve with k = 0 removed (FP64 mean), FP32
- * FFTs, E = -forward difference of phi. */
+ * transforms (in LDS for power-of-two grids up to 1024 cells per axis, hipFFT
+ * otherwise), E = -forward difference of phi. */
 pic_status pic_hip_poisson(pic_hip_ctx* ctx);
 pic_status pic_hip_get_fields(pic_hip_ctx* ctx, float* phi, float* Ex, float* Ey);
 pic_status pic_hip_set_fields(pic_hip_ctx* ctx, const float* Ex, const float* Ey);
@@ -207,9 +208,13 @@ pic_status pic_hip_continuity_residual(pic_hip_ctx* ctx, double dt, double* max_
  * (computeDiagnostics, diagnostics.cpp:42-51).  Output arrays hold
  * nsteps - record_from values; NULL skips.  With GPU_PRIV the particles are
  * re-sorted by cell before the deposit of every sort_interval-th step (first
- * step included); between sorts the deposit stays exact through its overflow
- * path.  use_graph captures one step (with and without sort) as hipGraphs and
- * replays them. */
+ * step included); between sorts the deposit stays exact through the drift
+ * halo of its tile windows and its overflow path.  use_graph captures one step
+ * (with and without sort, with and without the diagnostics record) as
+ * hipGraphs and replays them.  Diagnostics run only on recorded steps.  The
+ * run's Poisson solves take the mean of rho from the particles' total charge
+ * (which the deposit conserves) and leave phi as a spectrum: pic_hip_get_fields
+ * transforms it when phi is read. */
 enum { PIC_ORDER_PUSH_FIRST = 0, PIC_ORDER_DEPOSIT_SOLVE_PUSH = 1, PIC_ORDER_DEPOSIT_PUSH_SOLVE = 2 };
 
 typedef struct {
