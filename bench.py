@@ -246,6 +246,36 @@ def learned_study(pichip):
     return out
 
 
+def profile_step(pichip, mode, steps):
+    """The C5 step loop alone (256^2, TSC, reference normalisation q = -1/N
+    with a thermal drift of ~0.2 cells per step, sort every 10 steps, drift
+    halo 4): a warm-up run, then `steps` graph-replayed steps."""
+    N = 256
+    ppc = 512 if mode == "plain512" else 8
+    n = N * N * ppc
+    dom = pichip.make_domain(1.0, 1.0, N, N)
+    net = pichip.Denoiser(N, N, max_batch=1, max_pairs=0, seed=1) if mode == "learned8" else None
+    with pichip.Context(dom, n) as c:
+        c.set_tile_halo(4)
+        c.init_particles(n, kind=1, seed=5, amp=0.02, mode=1, vth=0.2 / N / 0.1, q=-1.0 / n, m=1.0)
+        c.sync()
+
+        def go(k):
+            t = time.perf_counter()
+            if net:
+                net.run(c, k, 2, 2, dt=0.1, sort_interval=10, record_from=k, tolerance=1e9)
+            else:
+                c.run(k, 2, 2, 0.1, 0.0, pichip.ORDER_PUSH_FIRST, sort_interval=10, record_from=k, use_graph=True)
+            return time.perf_counter() - t
+
+        go(20)
+        wall = go(steps)
+    if net:
+        net.close()
+    return {"profile_step": mode, "grid": [N, N], "ppc": ppc, "steps": steps,
+            "ms_per_step_incl_capture": wall * 1000.0 / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +289,11 @@ def main():
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned-density study")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--profile-step", choices=("plain8", "learned8", "plain512"), default="",
+                    help="only run the 256^2 TSC step loop (sort every 10, drift halo 4) for --steps steps, for "
+                         "kernel profiles (the learned network is untrained: kernel times do not depend on weights)")
+    ap.add_argument("--profile-ensemble", action="store_true",
+                    help="only run the 64-member ensemble deposit and step timings, for kernel profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design "
@@ -266,6 +301,13 @@ def main():
         sys.exit(2)
 
     import pichip
+
+    if args.profile_step:
+        print(json.dumps(profile_step(pichip, args.profile_step, args.steps)), flush=True)
+        return
+    if args.profile_ensemble:
+        print(json.dumps(ensemble_bench(pichip)), flush=True)
+        return
 
     # The library's stream is the only one doing work; ctx.sync() (a
     # hipStreamSynchronize plus a check of the device status word) brackets the
