@@ -322,8 +322,10 @@ pic_status pic_hip_ensemble_get_tiling(pic_hip_ensemble* ens, int32_t member, in
 pic_status pic_hip_ensemble_set_tile_halo(pic_hip_ensemble* ens, int cells);
 /* Autotunes the members by class (grid, scheme, particles per cell rounded
  * to a power of two): each class's candidates are timed with only that
- * class's work items in the persistent kernels.  Needs a sort.  cache_path
- * as pic_hip_autotune. */
+ * class's work items in the persistent kernels.  The whole ensemble's deposit
+ * is then timed with the tuned and with the previous tilings, and the
+ * previous ones stay if they are faster.  Needs a sort.  cache_path as
+ * pic_hip_autotune. */
 pic_status pic_hip_ensemble_autotune(pic_hip_ensemble* ens, const char* cache_path, int reps);
 
 /* Step loop of every member (PIC_ORDER_PUSH_FIRST: push, [sort], deposit,

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pic_hip.h"
@@ -481,7 +482,12 @@ pic_status enqueue_solve(pic_hip_ctx* ctx, const double* sum, double sum_scale, 
     if (ctx->lds_fft) {
         launch_poisson_lds(ctx->rho, ctx->G, 0.0, cv, sum, sum_scale, sum_add, ctx->spec, ctx->spec_ex, ctx->spec_ey,
                            ctx->ptab, ctx->dev, ctx->tw_x, ctx->tw_y, ctx->stream);
-        launch_poisson_lds_rows(ctx->spec_ex, ctx->spec_ey, ctx->Ex, ctx->Ey, ctx->dev, ctx->tw_x, ctx->stream);
+        if (ctx->dom.Ny > 1) {
+            launch_poisson_lds_rows(ctx->spec_ex, ctx->spec_ey, ctx->Ex, ctx->Ey, ctx->dev, ctx->tw_x, ctx->stream);
+        } else {  // one row: Ey is exactly zero (phi does not vary along y)
+            launch_poisson_lds_rows(ctx->spec_ex, nullptr, ctx->Ex, nullptr, ctx->dev, ctx->tw_x, ctx->stream);
+            fill_async(ctx->Ey, (size_t)ctx->G, 0.0f, ctx->stream);
+        }
         if (with_phi) launch_poisson_lds_rows(ctx->spec, nullptr, ctx->phi, nullptr, ctx->dev, ctx->tw_x, ctx->stream);
         ctx->phi_pending = !with_phi;
         PIC_HIP_TRY(hipGetLastError());
@@ -961,9 +967,17 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
     if (n > 0 && (!x || !y || !q)) return fail(PIC_ERR_INVALID_ARGUMENT, "x, y and q are required");
     if (scheme == PIC_SCHEME_ESIRKEPOV && n > 0 && (!vx || !vy))
         return fail(PIC_ERR_INVALID_ARGUMENT, "the Esirkepov scheme needs vx and vy");
-    // A per-thread context, grown on demand (the reference keeps a static
-    // grow-only device pool, deposition_cuda.cu:243-273).
+    // A per-thread context and pinned staging buffer, grown on demand (the
+    // reference keeps a static grow-only device pool, deposition_cuda.cu:243-273).
+    struct Staging {
+        float* p = nullptr;
+        size_t cap = 0;
+        ~Staging() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
     thread_local std::unique_ptr<pic_hip_ctx, pic_status (*)(pic_hip_ctx*)> cache(nullptr, pic_hip_destroy);
+    thread_local Staging stage;
     if (!cache || cache->dom.Nx != domain->Nx || cache->dom.Ny != domain->Ny || cache->dom.Lx != domain->Lx ||
         cache->dom.Ly != domain->Ly || cache->cap < n + 64) {
         cache.reset();
@@ -973,17 +987,52 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
         cache.reset(c);
     }
     pic_hip_ctx* c = cache.get();
-    std::vector<float> buf[5];
     const double* src[5] = {x, y, vx, vy, q};
-    for (int i = 0; i < 5; ++i) {
-        if (!src[i]) continue;
-        buf[i].resize(n);
-        for (size_t k = 0; k < n; ++k) buf[i][k] = (float)src[i][k];
+    const int dst_stream[5] = {0, 1, 2, 3, 4};
+    if (stage.cap < 5 * n) {
+        if (stage.p) (void)hipHostFree(stage.p);
+        stage.p = nullptr;
+        stage.cap = 0;
+        PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&stage.p), std::max<size_t>(5 * n, 1) * sizeof(float),
+                                  hipHostMallocDefault));
+        stage.cap = 5 * n;
     }
-    pic_status st = pic_hip_upload(c, n ? buf[0].data() : nullptr, n ? buf[1].data() : nullptr,
-                                   buf[2].empty() ? nullptr : buf[2].data(), buf[3].empty() ? nullptr : buf[3].data(),
-                                   n ? buf[4].data() : nullptr, nullptr, n);
-    if (st != PIC_OK) return st;
+    // FP64 -> FP32 into pinned memory, split over host threads for large sets,
+    // then DMA copies on the context's stream.  (The workers see the staging
+    // buffer through `sp`: `stage` names a different object in each thread.)
+    float* const sp = stage.p;
+    auto convert = [sp, &src, n](size_t b, size_t e) {
+        for (int a = 0; a < 5; ++a) {
+            if (!src[a]) continue;
+            float* d = sp + (size_t)a * n;
+            for (size_t k = b; k < e; ++k) d[k] = (float)src[a][k];
+        }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = n >= ((size_t)1 << 18) ? hw : 1;
+    if (nt > 1) {
+        std::vector<std::thread> pool;
+        const size_t chunk = (n + nt - 1) / nt;
+        for (unsigned t = 1; t < nt; ++t) {
+            const size_t b = std::min(n, t * chunk), e = std::min(n, b + chunk);
+            pool.emplace_back(convert, b, e);
+        }
+        convert(0, std::min(n, chunk));
+        for (auto& th : pool) th.join();
+    } else {
+        convert(0, n);
+    }
+    for (int a = 0; a < 5 && n > 0; ++a) {
+        float* d = c->s[dst_stream[a]];
+        if (src[a])
+            PIC_HIP_TRY(hipMemcpyAsync(d, sp + (size_t)a * n, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        else
+            fill_async(d, n, 0.0f, c->stream);
+    }
+    if (n > 0) fill_async(c->s[5], n, 1.0f, c->stream);  // mass
+    c->n = n;
+    c->order_valid = false;
+    pic_status st = PIC_OK;
     if (sorted || backend == PIC_BACKEND_GPU_PRIV) {
         st = pic_hip_sort(c);
         if (st != PIC_OK) return st;

@@ -681,6 +681,32 @@ pic_status pic_hip_ensemble_autotune(pic_hip_ensemble* ens, const char* cache_pa
     EventPair ev;
     PIC_HIP_TRY(hipEventCreate(&ev.a));
     PIC_HIP_TRY(hipEventCreate(&ev.b));
+    // Median device time of the deposit of members [m0, m1) (the whole ensemble
+    // by default) with the current tilings.
+    auto time_deposit = [&](int m0, int m1, double& med) -> pic_status {
+        pic_status st = rebuild_work(ens, m0, m1);
+        if (st != PIC_OK) return st;
+        const int64_t b0 = ens->gblk[(size_t)m0], nb = ens->gblk[(size_t)m1] - b0;
+        if ((st = enqueue_deposit(ens, 0.0f, b0, nb)) != PIC_OK) return st;  // warm-up
+        std::vector<double> t;
+        for (int r = 0; r < reps; ++r) {
+            PIC_HIP_TRY(hipEventRecord(ev.a, ens->stream));
+            if ((st = enqueue_deposit(ens, 0.0f, b0, nb)) != PIC_OK) return st;
+            PIC_HIP_TRY(hipEventRecord(ev.b, ens->stream));
+            PIC_HIP_TRY(hipEventSynchronize(ev.b));
+            float ms = 0.0f;
+            PIC_HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            t.push_back(ms);
+        }
+        med = median_of(t);
+        return PIC_OK;
+    };
+    // The tilings before tuning, kept if the tuned set is not faster as a whole
+    // (classes are timed alone, and run together in the persistent kernels).
+    std::vector<TilingChoice> before;
+    for (const EnsMember& m : ens->mem) before.push_back({m.g.T, m.g.R, m.ppl});
+    double before_ms = 0.0;
+    if (pic_status st0 = time_deposit(0, ens->M, before_ms); st0 != PIC_OK) return st0;
     // Classes: runs of members with equal grid, scheme and particles per cell bucket.
     int i0 = 0;
     while (i0 < ens->M) {
@@ -711,21 +737,8 @@ pic_status pic_hip_ensemble_autotune(pic_hip_ensemble* ens, const char* cache_pa
         double best_ms = -1.0;
         for (const TilingChoice& c : tiling_candidates(a.d.Nx, a.d.Ny, a.scheme == PIC_SCHEME_ESIRKEPOV, ens->halo)) {
             if (apply(c) != PIC_OK) continue;
-            pic_status st = rebuild_work(ens, i0, i1);
-            if (st != PIC_OK) return st;
-            const int64_t b0 = ens->gblk[(size_t)i0], nb = ens->gblk[(size_t)i1] - b0;
-            if ((st = enqueue_deposit(ens, 0.0f, b0, nb)) != PIC_OK) return st;  // warm-up
-            std::vector<double> t;
-            for (int r = 0; r < reps; ++r) {
-                PIC_HIP_TRY(hipEventRecord(ev.a, ens->stream));
-                if ((st = enqueue_deposit(ens, 0.0f, b0, nb)) != PIC_OK) return st;
-                PIC_HIP_TRY(hipEventRecord(ev.b, ens->stream));
-                PIC_HIP_TRY(hipEventSynchronize(ev.b));
-                float ms = 0.0f;
-                PIC_HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-                t.push_back(ms);
-            }
-            const double med = median_of(t);
+            double med = 0.0;
+            if (pic_status st = time_deposit(i0, i1, med); st != PIC_OK) return st;
             if (best_ms < 0.0 || med < best_ms) {
                 best_ms = med;
                 best = c;
@@ -738,8 +751,16 @@ pic_status pic_hip_ensemble_autotune(pic_hip_ensemble* ens, const char* cache_pa
             return fail(PIC_ERR_INVALID_ARGUMENT, std::string("cannot write tuning cache ") + cache_path);
         i0 = i1;
     }
-    pic_status st = rebuild_work(ens);
+    double after_ms = 0.0;
+    pic_status st = time_deposit(0, ens->M, after_ms);
     if (st != PIC_OK) return st;
+    if (after_ms > before_ms) {
+        for (int i = 0; i < ens->M; ++i)
+            if ((st = set_member_tiling(ens, i, before[(size_t)i].tx, before[(size_t)i].ty, before[(size_t)i].ppl)) !=
+                PIC_OK)
+                return st;
+    }
+    if ((st = rebuild_work(ens)) != PIC_OK) return st;
     return pic_hip_ensemble_sync(ens);
 }
 

@@ -141,7 +141,7 @@ void launch_poisson_spectral(float2* spec, float2* spec_ex, float2* spec_ey, con
                              const DevDomain& d, hipStream_t s);
 
 // The same solve in three launches with the 1D transforms in LDS (csrc/poisson.hip),
-// for power-of-two grids from 4 to 1024 cells per axis.  launch_poisson_lds takes
+// for power-of-two grids of 4 to 1024 cells along x and 1 to 1024 along y.  launch_poisson_lds takes
 // rho (mean removed as in launch_poisson_prep) to the row half spectra
 // [Ny][Nx/2+1] of phi (in spec), Ex and Ey; launch_poisson_lds_rows turns row
 // half spectra X (and Y, may be null) into the real grids ox (and oy).

@@ -1,5 +1,6 @@
 // Spectral Poisson solve in three launches for power-of-two grids up to 1024
-// cells per axis, with every 1D transform done by one wavefront in LDS.
+// cells per axis (Nx >= 4, Ny >= 1), with every 1D transform done by one
+// wavefront in LDS.
 //
 // The solve is the one of poisson_fft.cpp:39-66 (rho * cv with the mean
 // removed, phi_hat = rho_hat / (k^2 Nx Ny), k = 0 zeroed) followed by the
@@ -8,7 +9,8 @@
 //
 //  * poisson_rows_kernel: the mean-removed rows 2p and 2p+1 as the real and
 //    imaginary part of one complex transform of length Nx; the two half
-//    spectra are separated with the conjugate symmetry of real input.
+//    spectra are separated with the conjugate symmetry of real input (a grid
+//    of one row, the quasi-1D geometry, transforms it alone).
 //  * poisson_cols_kernel: one half-spectrum column: forward transform along y,
 //    the Green's function and the difference symbols, then inverse transforms
 //    along y of phi, Ex and Ey, written back as row half spectra.
@@ -91,7 +93,7 @@ __device__ float2* wave_fft(float2* a, float2* b, int L, const float2* tw) {
 }
 
 __global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restrict__ rho, float2* __restrict__ spec,
-                                                          int W, int64_t G, double bg, double cv,
+                                                          int W, int H, int64_t G, double bg, double cv,
                                                           const double* __restrict__ rho_sum, double sum_scale,
                                                           double sum_add, const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
@@ -100,10 +102,11 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restric
     float2* t = sm + 2 * W;
     const int lane = threadIdx.x;
     const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+    const bool pair = r1 < H;  // a grid of one row transforms it alone
     const double mean = (*rho_sum * sum_scale + sum_add + bg * (double)G) * cv / (double)G;
     for (int n = lane; n < W; n += 64) {
         const float u = (float)(((double)rho[(size_t)r0 * W + n] + bg) * cv - mean);
-        const float v = (float)(((double)rho[(size_t)r1 * W + n] + bg) * cv - mean);
+        const float v = pair ? (float)(((double)rho[(size_t)r1 * W + n] + bg) * cv - mean) : 0.0f;
         a[n] = make_float2(u, v);
         t[n] = tw[n];
     }
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restric
         const float2 p = Z[k], m = Z[(W - k) & (W - 1)];
         // A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / (2i)
         spec[(size_t)r0 * hx + k] = make_float2(0.5f * (p.x + m.x), 0.5f * (p.y - m.y));
-        spec[(size_t)r1 * hx + k] = make_float2(0.5f * (p.y + m.y), -0.5f * (p.x - m.x));
+        if (pair) spec[(size_t)r1 * hx + k] = make_float2(0.5f * (p.y + m.y), -0.5f * (p.x - m.x));
     }
 }
 
@@ -200,16 +203,16 @@ __global__ __launch_bounds__(64) void poisson_rows_inv_kernel(const float2* __re
 }  // namespace
 
 bool poisson_lds_supported(int Nx, int Ny) {
-    auto pow2 = [](int v) { return v >= 4 && v <= 1024 && (v & (v - 1)) == 0; };
-    return pow2(Nx) && pow2(Ny);
+    auto pow2 = [](int v, int lo) { return v >= lo && v <= 1024 && (v & (v - 1)) == 0; };
+    return pow2(Nx, 4) && pow2(Ny, 1);
 }
 
 void launch_poisson_lds(const float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
                         double sum_add, float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
                         const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s) {
     const int W = d.Nx, H = d.Ny;
-    hipLaunchKernelGGL(poisson_rows_kernel, dim3((unsigned)(H / 2)), dim3(64), (size_t)3 * W * sizeof(float2), s, rho,
-                       spec, W, G, bg, cv, rho_sum, sum_scale, sum_add, tw_x);
+    hipLaunchKernelGGL(poisson_rows_kernel, dim3((unsigned)((H + 1) / 2)), dim3(64), (size_t)3 * W * sizeof(float2),
+                       s, rho, spec, W, H, G, bg, cv, rho_sum, sum_scale, sum_add, tw_x);
     hipLaunchKernelGGL(poisson_cols_kernel, dim3((unsigned)(W / 2 + 1)), dim3(64), (size_t)5 * H * sizeof(float2), s,
                        spec, spec_ex, spec_ey, t, d, tw_y);
 }

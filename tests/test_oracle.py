@@ -322,6 +322,22 @@ def test_golden_two_stream_growth(orc, scheme, name, order):
     assert gamma == pytest.approx(g["growth_rate_measured"][name], rel=5e-9)
 
 
+@pytest.mark.parametrize("scheme,name,order", [(0, "NGP", 0), (1, "CIC", 0), (2, "TSC", 0), (3, "Esirkepov", 2)])
+def test_golden_two_stream_quasi1d_growth(orc, scheme, name, order):
+    """The quasi-1D geometry (256 x 1 grid, Lx = 2 pi, all particles at
+    y = Ly / 2): the FP64 loop reproduces two_stream_quasi1d.csv."""
+    g = _gold("two_stream_quasi1d")
+    dom = orc.domain(g["Lx"], g["Ly"], g["Nx"], g["Ny"])
+    n = g["num_particles"]
+    v = orc.resonant_beam_velocity(dom, n, 1)
+    p = orc.init("two_stream", n, dom, seed=g["seed"], beam_velocity=v, perturbation=g["perturbation"], quasi1d=True)
+    assert np.all(p["y"] == 0.5 * g["Ly"])
+    rs = g["steps"] // 4
+    _, rec = orc.run_loop(p, dom, scheme, order, g["dt"], 1.0, g["steps"], rs)
+    gamma = orc.growth_rate(rec["mode1"], g["dt"], rs) * orc.plasma_frequency(dom, n)
+    assert gamma == pytest.approx(g["growth_rate_measured"][name], rel=5e-9)
+
+
 def test_golden_langmuir_frequency_and_noise(orc):
     g = _gold("langmuir_validation")
     dom = orc.domain(1.0, 1.0, g["grid"], g["grid"])
