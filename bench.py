@@ -366,6 +366,12 @@ def main():
                      "kernel": "deposit_tile_kernel<CIC>", "kernel_ms": kern_ms,
                      "algorithmic_bytes": deposit_bytes(1)},
     }
+    if not args.quick:
+        # measured streaming-read ceiling of this device (16-byte loads, 1 GiB),
+        # reported beside the published peak
+        stream = ctx.read_probe(1 << 30, 10)
+        out["roofline"]["stream_read_gbs"] = stream
+        out["roofline"]["frac_of_stream_read"] = achieved / stream
     # the CPU baseline deposits these same cell-sorted particles
     base_particles = None if args.no_cpu_baseline else ctx.download()
     if not args.quick:

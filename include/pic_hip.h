@@ -253,6 +253,11 @@ pic_status pic_hip_profile_end(pic_hip_ctx* ctx, double* total_ms, int* launches
  * a plain streaming kernel (copy_f32_kernel): a known-size reference for
  * calibrating hardware byte counters.  Synchronous. */
 pic_status pic_hip_copy_probe(pic_hip_ctx* ctx, size_t bytes);
+/* Streaming-read bandwidth of this device: `reps` timed reads (HIP events,
+ * median) of a `bytes` scratch buffer with 16-byte loads; *gbs in GB/s.  The
+ * measured ceiling that HBM-bound kernels are compared with beside the
+ * published peak.  Synchronous. */
+pic_status pic_hip_read_probe(pic_hip_ctx* ctx, size_t bytes, int reps, double* gbs);
 
 /* ---- tiling autotuner (build-defined; no reference counterpart) ----------
  * Times every LDS tiling candidate (tile shapes from 16x8 to 128x4 cells,

@@ -957,6 +957,42 @@ pic_status pic_hip_copy_probe(pic_hip_ctx* ctx, size_t bytes) {
     return PIC_OK;
 }
 
+pic_status pic_hip_read_probe(pic_hip_ctx* ctx, size_t bytes, int reps, double* gbs) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!gbs || reps < 1 || bytes < 16) return fail(PIC_ERR_INVALID_ARGUMENT, "need gbs, reps >= 1, bytes >= 16");
+    const int64_t n4 = (int64_t)(bytes / 16);
+    int dev = 0, ncu = 1;
+    PIC_HIP_TRY(hipGetDevice(&dev));
+    PIC_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int blocks = std::max(1, ncu) * 8;
+    float *a = nullptr, *part = nullptr;
+    PIC_HIP_TRY(dmalloc(&a, (size_t)n4 * 4));
+    std::unique_ptr<float, hipError_t (*)(void*)> ga(a, hipFree);
+    PIC_HIP_TRY(dmalloc(&part, (size_t)blocks));
+    std::unique_ptr<float, hipError_t (*)(void*)> gp(part, hipFree);
+    fill_async(a, (size_t)n4 * 4, 1.0f, ctx->stream);
+    launch_read_probe(a, n4, part, blocks, ctx->stream);  // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PIC_HIP_TRY(hipEventCreate(&e0));
+    PIC_HIP_TRY(hipEventCreate(&e1));
+    std::vector<double> t;
+    hipError_t err = hipSuccess;
+    for (int r = 0; r < reps && err == hipSuccess; ++r) {
+        (void)hipEventRecord(e0, ctx->stream);
+        launch_read_probe(a, n4, part, blocks, ctx->stream);
+        (void)hipEventRecord(e1, ctx->stream);
+        err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        t.push_back(ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    PIC_HIP_TRY(err);
+    *gbs = (double)n4 * 16.0 / (median_of(t) * 1e-3) / 1e9;
+    return PIC_OK;
+}
+
 // ---- drop-in seam -----------------------------------------------------------
 
 pic_status pic_hip_deposit_host(const double* x, const double* y, const double* vx, const double* vy,

@@ -185,6 +185,8 @@ void launch_esirkepov_current(const float* x, const float* y, const float* vx, c
 void launch_continuity_residual(const CurrentGrids& g, const DevDomain& d, double dt, double* partial, int nblocks,
                                 double* out, hipStream_t s);
 void launch_copy_f32(const float* src, float* dst, int64_t n, hipStream_t s);
+// Streaming read of n4 float4 values, one partial sum per block.
+void launch_read_probe(const float* src, int64_t n4, float* partial, int blocks, hipStream_t s);
 
 // ---- ensemble.hip -----------------------------------------------------------
 // Particle slots per chunk; each member's slot range is chunk aligned, so a

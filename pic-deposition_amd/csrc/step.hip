@@ -240,7 +240,32 @@ __global__ __launch_bounds__(256) void copy_f32_kernel(const float* __restrict__
     if (i < n) dst[i] = src[i];
 }
 
+// Streaming read of n float4 values: every thread adds its grid-stride share,
+// one partial per block (so the loads cannot be dropped).
+__global__ __launch_bounds__(256) void read_probe_kernel(const float4* __restrict__ src, int64_t n,
+                                                         float* __restrict__ partial) {
+    float s = 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 v = src[i];
+        s += (v.x + v.y) + (v.z + v.w);
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
 }  // namespace
+
+void launch_read_probe(const float* src, int64_t n4, float* partial, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(read_probe_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(src), n4, partial);
+}
 
 void launch_poisson_prep(const float* rho, float* in, int64_t G, double bg, double cv, const double* rho_sum,
                          double sum_scale, double sum_add, hipStream_t s) {

@@ -169,6 +169,7 @@ _SIGS = {
     "pic_hip_growth_rate": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int32]),
     "pic_hip_plasma_frequency": (ctypes.c_double, [ctypes.POINTER(Domain), ctypes.c_int64]),
     "pic_hip_copy_probe": (ctypes.c_int, [_P, ctypes.c_size_t]),
+    "pic_hip_read_probe": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "pic_hip_esirkepov_current": (ctypes.c_int, [_P, ctypes.c_double]),
     "pic_hip_get_current": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p]),
     "pic_hip_continuity_residual": (ctypes.c_int, [_P, ctypes.c_double, _f64p, _f64p]),
@@ -411,6 +412,12 @@ class Context:
 
     def copy_probe(self, nbytes: int) -> None:
         _check(_lib.pic_hip_copy_probe(self._h, int(nbytes)))
+
+    def read_probe(self, nbytes: int, reps: int = 10) -> float:
+        """pic_hip_read_probe: measured streaming-read bandwidth in GB/s."""
+        g = ctypes.c_double()
+        _check(_lib.pic_hip_read_probe(self._h, int(nbytes), int(reps), ctypes.byref(g)))
+        return float(g.value)
 
     def profile_begin(self, max_launches: int) -> None:
         _check(_lib.pic_hip_profile_begin(self._h, int(max_launches)))
