@@ -21,6 +21,10 @@
 //  * wgrad_kernel: weight gradient as a GEMM over pixels, M = input channels
 //    (or taps), N = output channels, K = 32 pixels of one row per MFMA; every
 //    wave writes its own partial and a fixed-order FP64 reduction finishes it.
+//  * inference (launch_denoise): the first layer normalises rho on load, the
+//    last writes per-block FP64 sums for the gate, and denoise_apply_kernel
+//    re-adds those sums in a fixed order in every block, decides the gate and
+//    renormalises: five launches from deposited rho to learned rho.
 
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"

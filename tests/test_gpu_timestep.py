@@ -146,16 +146,18 @@ def test_graph_replay_matches_eager(hip, orc, scheme):
         np.testing.assert_allclose(np.sort(g1_state[k]), np.sort(eager_state[k]), rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("shape", [(64, 64), (48, 40)])  # LDS transforms, hipFFT
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_late_records_and_fields_after_run(hip, use_graph):
+def test_late_records_and_fields_after_run(hip, use_graph, shape):
     """Diagnostics run only on recorded steps: records taken from step 12 on
     equal the tail of a run recorded from step 0.  The loop's solves leave the
     potential as a spectrum; reading phi after the run transforms it and gives
     the potential of a fresh solve of the final rho (the loop takes the mean of
     rho from the particle charges, a fresh solve from the grid sum)."""
-    say("late records", use_graph)
-    G, n = 64, 64 * 64 * 16
-    dom = hip.make_domain(1.0, 1.0, G, G)
+    say("late records", use_graph, shape)
+    nx, ny = shape
+    n = nx * ny * 16
+    dom = hip.make_domain(1.0, 1.0, nx, ny)
     rng = np.random.default_rng(12)
     p = {"x": rng.uniform(0, 1, n), "y": rng.uniform(0, 1, n), "vx": rng.normal(0, 0.05, n),
          "vy": rng.normal(0, 0.05, n), "q": np.full(n, -1.0 / n), "m": np.ones(n)}
