@@ -246,6 +246,58 @@ def learned_study(pichip):
     return out
 
 
+def timestep_c3(pichip, steps=100):
+    """C3: 512^2, 100 ppc (26,214,400 particles), two beams +-0.2 with a
+    0.01 thermal spread, q = -1/N, background +1/V, TSC, PUSH_FIRST, the
+    reference normalisation dt = 0.1 / omega_p (the beams cross the box ~100
+    times per step, so every step starts from an unordered set).  Graph-replayed
+    steps for each (backend, sort interval, halo) setting; the best is reported
+    against the per-step byte model (40 B gather+push + 12 B deposit + 40/10 B
+    sort per particle)."""
+    N = 512
+    n = N * N * 100
+    dom = pichip.make_domain(1.0, 1.0, N, N)
+    dt = 0.1 / pichip.plasma_frequency(dom, n)
+    res = {}
+    with pichip.Context(dom, n) as c:
+        for name, backend, si, halo in (("priv_sort1", 2, 1, 0), ("priv_sort10_halo2", 2, 10, 2),
+                                        ("atomics", 1, 1, 0)):
+            c.set_tile_halo(halo)
+
+            def go(k):
+                c.init_particles(n, kind=2, seed=42, amp=0.2, vth=0.01, q=-1.0 / n, m=1.0)
+                c.sync()
+                t = time.perf_counter()
+                c.run(k, 2, backend, dt, 1.0, pichip.ORDER_PUSH_FIRST, sort_interval=si, record_from=k,
+                      use_graph=True)
+                return time.perf_counter() - t
+
+            go(10)
+            res[name] = (go(steps + 10) - go(10)) * 1000.0 / steps
+        # The same beams with a resolved step: dt = 0.2 dx / 0.2, so no particle
+        # moves more than ~0.2 cells per step and the sort order stays useful.
+        dt_r = 0.2 * (1.0 / N) / 0.2
+        c.set_tile_halo(2)
+
+        def go_r(k):
+            c.init_particles(n, kind=2, seed=42, amp=0.2, vth=0.01, q=-1.0 / n, m=1.0)
+            c.sync()
+            t = time.perf_counter()
+            c.run(k, 2, 2, dt_r, 1.0, pichip.ORDER_PUSH_FIRST, sort_interval=10, record_from=k, use_graph=True)
+            return time.perf_counter() - t
+
+        go_r(10)
+        resolved = (go_r(steps + 10) - go_r(10)) * 1000.0 / steps
+    best = min(res, key=res.get)
+    model = 56 * n  # bytes per step of the byte model
+    return {"grid": [N, N], "particles": n, "dt": dt, "steps_timed": steps, "ms_per_step": res,
+            "best": best, "best_ms_per_step": res[best],
+            "model_bytes_per_step": model, "model_gbs": model / (res[best] / 1000.0) / 1e9,
+            "model_hbm_frac": model / (res[best] / 1000.0) / 1e9 / HBM_PEAK_GBS,
+            "resolved_dt": dt_r, "resolved_ms_per_step_sort10_halo2": resolved,
+            "resolved_model_hbm_frac": model / (resolved / 1000.0) / 1e9 / HBM_PEAK_GBS}
+
+
 def profile_step(pichip, mode, steps):
     """The C5 step loop alone (256^2, TSC, reference normalisation q = -1/N
     with a thermal drift of ~0.2 cells per step, sort every 10 steps, drift
@@ -287,6 +339,7 @@ def main():
     ap.add_argument("--tiling", type=str, default="", help="tile_x,tile_y,particles_per_lane,pipelined for CIC")
     ap.add_argument("--skip-ensemble", action="store_true", help="skip the 64-member ensemble measurement")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned-density study")
+    ap.add_argument("--skip-timestep", action="store_true", help="skip the C3 timestep measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--profile-step", choices=("plain8", "learned8", "plain512"), default="",
@@ -294,6 +347,8 @@ def main():
                          "kernel profiles (the learned network is untrained: kernel times do not depend on weights)")
     ap.add_argument("--profile-ensemble", action="store_true",
                     help="only run the 64-member ensemble deposit and step timings, for kernel profiles")
+    ap.add_argument("--profile-timestep", action="store_true",
+                    help="only run the C3 timestep measurement, for kernel profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design "
@@ -307,6 +362,9 @@ def main():
         return
     if args.profile_ensemble:
         print(json.dumps(ensemble_bench(pichip)), flush=True)
+        return
+    if args.profile_timestep:
+        print(json.dumps(timestep_c3(pichip, args.steps)), flush=True)
         return
 
     # The library's stream is the only one doing work; ctx.sync() (a
@@ -393,6 +451,8 @@ def main():
         others["cell sort"] = {"particles_per_s": N / (sort_ms / 1000.0), "ms": sort_ms}
         out["other"] = others
     ctx.close()
+    if not args.quick and not args.skip_timestep:
+        out["timestep_c3"] = timestep_c3(pichip)
     if not args.quick and not args.skip_ensemble:
         out["ensemble"] = ensemble_bench(pichip)
     if not args.quick and not args.skip_learned:

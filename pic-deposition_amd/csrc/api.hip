@@ -267,6 +267,12 @@ pic_status pic_hip_upload(pic_hip_ctx* ctx, const float* x, const float* y, cons
             fill_async(ctx->s[i], n, i == 5 ? 1.0f : 0.0f, ctx->stream);
         }
     }
+    for (int i = 0; i < kStreams; ++i) {
+        bool u = i >= 4;  // only charges and masses are candidates
+        if (u && src[i])
+            for (size_t k = 1; k < n && u; ++k) u = src[i][k] == src[i][0];
+        ctx->uniform[i] = u;
+    }
     ctx->n = n;
     ctx->order_valid = false;
     PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -303,6 +309,7 @@ pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* p, si
     launch_init(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)n, ctx->dev, ip,
                 ctx->stream);
     PIC_HIP_TRY(hipGetLastError());
+    for (int i = 0; i < kStreams; ++i) ctx->uniform[i] = i >= 4;  // one charge and one mass for all
     ctx->n = n;
     ctx->order_valid = false;
     return PIC_OK;
@@ -314,21 +321,27 @@ namespace {
 
 // Sort by cell and permute the six streams.  copy_back keeps the stream
 // pointers fixed (needed inside captured graphs, which bake pointers in).
+// Swaps the particle buffers of every stream a sort permutes.
+void swap_permuted(pic_hip_ctx* ctx) {
+    for (int i = 0; i < kStreams; ++i)
+        if (!ctx->uniform[i]) std::swap(ctx->s[i], ctx->alt[i]);
+}
+
 pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
     PIC_HIP_TRY(launch_sort_keys_perm(ctx->s[0], ctx->s[1], (int64_t)ctx->n, ctx->dev, ctx->sb, ctx->stream));
     StreamSet fwd{}, back{};
-    fwd.count = back.count = kStreams;
     for (int i = 0; i < kStreams; ++i) {
-        fwd.src[i] = ctx->s[i];
-        fwd.dst[i] = ctx->alt[i];
-        back.src[i] = ctx->alt[i];
-        back.dst[i] = ctx->s[i];
+        if (ctx->uniform[i]) continue;  // every particle holds the same value
+        fwd.src[fwd.count] = ctx->s[i];
+        fwd.dst[fwd.count++] = ctx->alt[i];
+        back.src[back.count] = ctx->alt[i];
+        back.dst[back.count++] = ctx->s[i];
     }
     launch_permute_streams(fwd, ctx->sb.perm, (int64_t)ctx->n, ctx->stream);
     if (copy_back)
         launch_permute_streams(back, nullptr, (int64_t)ctx->n, ctx->stream);
     else
-        for (int i = 0; i < kStreams; ++i) std::swap(ctx->s[i], ctx->alt[i]);
+        swap_permuted(ctx);
     PIC_HIP_TRY(hipGetLastError());
     ctx->order_valid = true;
     return PIC_OK;
@@ -772,8 +785,7 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
             }
         }
         PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (parity)
-            for (int i = 0; i < kStreams; ++i) std::swap(ctx->s[i], ctx->alt[i]);
+        if (parity) swap_permuted(ctx);
     } else {
         for (int step = 0; step < nsteps; ++step) {
             if (fused && step == nsteps - 1) launch_set_int(ctx->push_flag, 0, ctx->stream);
@@ -1037,26 +1049,33 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
     // then DMA copies on the context's stream.  (The workers see the staging
     // buffer through `sp`: `stage` names a different object in each thread.)
     float* const sp = stage.p;
-    auto convert = [sp, &src, n](size_t b, size_t e) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = n >= ((size_t)1 << 18) ? hw : 1;
+    std::vector<char> q_same(nt, 1);  // per worker: its charges all equal the first particle's
+    auto convert = [sp, &src, n, &q_same](unsigned t, size_t b, size_t e) {
         for (int a = 0; a < 5; ++a) {
             if (!src[a]) continue;
             float* d = sp + (size_t)a * n;
             for (size_t k = b; k < e; ++k) d[k] = (float)src[a][k];
         }
+        const float q0 = (float)src[4][0];
+        bool same = true;
+        for (size_t k = b; k < e && same; ++k) same = sp[4 * n + k] == q0;
+        q_same[t] = same ? 1 : 0;
     };
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const unsigned nt = n >= ((size_t)1 << 18) ? hw : 1;
-    if (nt > 1) {
-        std::vector<std::thread> pool;
-        const size_t chunk = (n + nt - 1) / nt;
-        for (unsigned t = 1; t < nt; ++t) {
-            const size_t b = std::min(n, t * chunk), e = std::min(n, b + chunk);
-            pool.emplace_back(convert, b, e);
+    if (n > 0) {
+        if (nt > 1) {
+            std::vector<std::thread> pool;
+            const size_t chunk = (n + nt - 1) / nt;
+            for (unsigned t = 1; t < nt; ++t) {
+                const size_t b = std::min(n, t * chunk), e = std::min(n, b + chunk);
+                pool.emplace_back(convert, t, b, e);
+            }
+            convert(0, 0, std::min(n, chunk));
+            for (auto& th : pool) th.join();
+        } else {
+            convert(0, 0, n);
         }
-        convert(0, std::min(n, chunk));
-        for (auto& th : pool) th.join();
-    } else {
-        convert(0, n);
     }
     for (int a = 0; a < 5 && n > 0; ++a) {
         float* d = c->s[dst_stream[a]];
@@ -1066,6 +1085,9 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
             fill_async(d, n, 0.0f, c->stream);
     }
     if (n > 0) fill_async(c->s[5], n, 1.0f, c->stream);  // mass
+    for (int i = 0; i < kStreams; ++i) c->uniform[i] = false;
+    c->uniform[5] = true;
+    c->uniform[4] = n > 0 && std::all_of(q_same.begin(), q_same.end(), [](char v) { return v != 0; });
     c->n = n;
     c->order_valid = false;
     pic_status st = PIC_OK;

@@ -34,6 +34,9 @@ struct pic_hip_ctx {
     hipStream_t stream = nullptr;
     float* s[kStreams] = {};
     float* alt[kStreams] = {};
+    // Stream k holds one value for every particle (charges and masses of a
+    // single species): a permutation leaves it unchanged, so sorts skip it.
+    bool uniform[kStreams] = {};
     SortBuffers sb{};
     bool order_valid = false;
     float* rho = nullptr;

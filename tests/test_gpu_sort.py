@@ -69,6 +69,51 @@ def test_sort_moves_all_streams_together(hip):
     assert np.all(np.diff(k2) >= 0)
 
 
+@pytest.mark.parametrize("uniform", ["q", "m", "qm"])
+def test_single_species_streams_across_sorts_and_runs(hip, uniform):
+    """Charges or masses that are one value for every particle are not
+    permuted by the sort (a permutation leaves them unchanged): the streams
+    stay consistent through sorts, graph runs with an odd number of sorts and
+    a later upload of per-particle values."""
+    say("uniform streams", uniform)
+    dom = hip.make_domain(1.0, 1.0, 32, 32)
+    n = 40_000
+    rng = np.random.default_rng(3)
+    cols = {k: rng.uniform(0.5, 1.5, n).astype(np.float32) for k in ("x", "y", "vx", "vy", "q", "m")}
+    cols["x"] -= 0.5
+    cols["y"] -= 0.5
+    for k in uniform:
+        cols[k] = np.full(n, 0.75, np.float32)
+    with hip.Context(dom, n) as c:
+        c.upload(cols["x"], cols["y"], cols["q"], cols["vx"], cols["vy"], cols["m"])
+        _, perm, _ = c.sort_perm()
+        c.upload(cols["x"], cols["y"], cols["q"], cols["vx"], cols["vy"], cols["m"])
+        c.sort()
+        out = c.download()
+        for k, v in cols.items():
+            assert np.array_equal(out[k], v[perm]), k
+        # 3 sorts in a graph run leave the particle buffers swapped; with dt = 0
+        # nothing moves, so every particle's (x, y, vx, vy, q, m) must survive
+        c.run(3, 1, 2, 0.0, 0.0, 0, sort_interval=1, use_graph=True)
+        after = c.download()
+        for k in uniform:
+            assert np.all(after[k] == 0.75), k
+        names = ("x", "y", "vx", "vy", "q", "m")
+        rows = lambda d: np.stack([d[k] for k in names], axis=1)[np.lexsort([d[k] for k in names[::-1]])]
+        assert np.array_equal(rows(after), rows(cols))
+        # per-particle values after uniform ones: sorted as usual again
+        cols2 = {k: rng.uniform(0.5, 1.5, n).astype(np.float32) for k in cols}
+        cols2["x"] -= 0.5
+        cols2["y"] -= 0.5
+        c.upload(cols2["x"], cols2["y"], cols2["q"], cols2["vx"], cols2["vy"], cols2["m"])
+        _, perm2, _ = c.sort_perm()
+        c.upload(cols2["x"], cols2["y"], cols2["q"], cols2["vx"], cols2["vy"], cols2["m"])
+        c.sort()
+        out2 = c.download()
+    for k, v in cols2.items():
+        assert np.array_equal(out2[k], v[perm2]), k
+
+
 @pytest.mark.parametrize("kind", ["equal", "descending", "empty", "one"])
 def test_degenerate_inputs(hip, orc, kind):
     say("degenerate", kind)
