@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void fold_acc_kernel(float* __restrict__ rho, 
     }
 }
 
-template <int S, int K, int EX, int PIPE>
+template <int S, int K, int EX, int PIPE, int LA>
 __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restrict__ px, const float* __restrict__ py,
                                                            const float* __restrict__ pvx,
                                                            const float* __restrict__ pvy,
@@ -63,8 +63,8 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
     const int wv = threadIdx.x >> 6;
     const int tile = blockIdx.x * (blockDim.x >> 6) + wv;
     if (tile >= g.ntiles) return;  // wave-uniform
-    deposit_one_tile<S, K, EX, PIPE>(tile, smem + (size_t)wv * g.lds_per_wave, px, py, pvx, pvy, pq, cell_off, d, g,
-                                     dt, partial, gacc, status);
+    deposit_one_tile<S, K, EX, PIPE, LA>(tile, smem + (size_t)wv * g.lds_per_wave, px, py, pvx, pvy, pq, cell_off, d,
+                                         g, dt, partial, gacc, status);
 }
 
 __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
@@ -92,28 +92,33 @@ void launch_atomic(const DepositArgs& a) {
     hipLaunchKernelGGL(fold_acc_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.acc, G);
 }
 
-template <int S, int K, int EX, int PIPE>
+template <int S, int K, int EX, int PIPE, int LA>
 void launch_tile(const DepositArgs& a) {
     const TileGeom& g = a.geom;
     const int waves_per_block = 4;
     const unsigned blocks = (unsigned)((g.ntiles + waves_per_block - 1) / waves_per_block);
     const size_t lds = (size_t)waves_per_block * g.lds_per_wave * sizeof(float);
     if (a.ev_begin) (void)hipEventRecord(a.ev_begin, a.stream);
-    hipLaunchKernelGGL((deposit_tile_kernel<S, K, EX, PIPE>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream,
+    hipLaunchKernelGGL((deposit_tile_kernel<S, K, EX, PIPE, LA>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream,
                        a.x, a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.acc, a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
     hipLaunchKernelGGL(deposit_merge_kernel, dim3((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny), dim3(256), 0,
                        a.stream, a.partial, a.acc, a.dom, g, a.rho);
 }
 
+// With a drift halo the particles have moved since the sort: the lookahead
+// variant (no register double-buffering) keeps their runs together.
 template <int S, int K>
 void launch_tile_kp(const DepositArgs& a) {
+    const bool drift = a.geom.D > 0;
     if (a.dom.exact) {
-        if (a.pipe) launch_tile<S, K, 1, 1>(a);
-        else launch_tile<S, K, 1, 0>(a);
+        if (drift) launch_tile<S, K, 1, 0, 1>(a);
+        else if (a.pipe) launch_tile<S, K, 1, 1, 0>(a);
+        else launch_tile<S, K, 1, 0, 0>(a);
     } else {
-        if (a.pipe) launch_tile<S, K, 0, 1>(a);
-        else launch_tile<S, K, 0, 0>(a);
+        if (drift) launch_tile<S, K, 0, 0, 1>(a);
+        else if (a.pipe) launch_tile<S, K, 0, 1, 0>(a);
+        else launch_tile<S, K, 0, 0, 0>(a);
     }
 }
 

@@ -258,14 +258,33 @@ struct TileCtx {
     int ax0, ay0, aw, ah;  // cells deposited into the window: the tile widened by the drift halo
 };
 
+// Window key ((ly << 12) | lx) of a particle's home cell, -1 when the particle
+// is not this lane's (outside its row range) or not in the window.
+template <int EX>
+__device__ __forceinline__ int window_key(float xx, float yy, bool valid, const TileCtx& t, const DevDomain& d) {
+    int cx, cy;
+    float fx, fy;
+    home_cell<EX>(xx, yy, d, cx, cy, fx, fy);
+    const int lx = cx - t.ax0, ly = cy - t.ay0;
+    const bool in_win = (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
+    return (valid && in_win) ? ((ly << 12) | lx) : -1;
+}
+
 // Processes one chunk: K consecutive particles per lane accumulated as runs of
 // equal cell in registers, then one segmented wave scan, then one FP64 LDS
 // add per (cell, stencil slot).
-template <int S, int Q, int EX>
+//
+// LA (lookahead, for particles that drifted since the sort): a particle whose
+// cell differs from the open run while the next particle returns to it is
+// added to the window on its own and the run stays open, so the scattered
+// strays of a drifted chunk do not cut its runs into pieces that every lane
+// then adds to the same few nodes.
+template <int S, int Q, int EX, int LA>
 __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx& t, const DevDomain& d, float dt,
                                               double* __restrict__ gacc, int* __restrict__ status) {
     using St = Stencil<S>;
     constexpr int V = St::V;
+    constexpr int K = 4 * Q;
     float acc[V], H[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
@@ -274,6 +293,20 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
     }
     int hk = -1, ck = -1;
     bool multi = false;
+    int nk[K];  // lookahead keys (LA only)
+    if (LA) {
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int p = cur.p0[j] + e;
+                const bool valid = p >= cur.lo[j] && p < cur.hi[j];
+                const float xx = e == 0 ? cur.X[j].x : e == 1 ? cur.X[j].y : e == 2 ? cur.X[j].z : cur.X[j].w;
+                const float yy = e == 0 ? cur.Y[j].x : e == 1 ? cur.Y[j].y : e == 2 ? cur.Y[j].z : cur.Y[j].w;
+                nk[4 * j + e] = window_key<EX>(xx, yy, valid, t, d);
+            }
+        }
+    }
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
 #pragma unroll
@@ -317,7 +350,19 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                 overflow_scatter<S>(ok_o, cx, cy, wo, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
             }
             const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
-            const bool nseg = lk != ck;
+            bool nseg = lk != ck;
+            // w is zero unless the particle is valid and in the window; only an
+            // Esirkepov move longer than a cell (ok false) must be masked.
+            const bool take = St::kAlwaysOk ? true : (valid && inside);
+            if (LA) {
+                const int idx = 4 * j + e;
+                const bool stray = nseg && ck >= 0 && idx + 1 < K && nk[idx + 1] == ck;
+                if (stray) {  // alone into the window; the run stays open
+                    const int wb = ((lk >> 12) + 1 + St::OY) * t.WX + (lk & 4095) + 1 + St::OX;
+                    emit_cell<St::SX, V>(t.win, wb, t.WX, w);
+                    continue;
+                }
+            }
             if (nseg && ck >= 0) {
                 if (!multi) {
 #pragma unroll
@@ -329,9 +374,6 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                 }
             }
             const float keep = nseg ? 0.0f : 1.0f;
-            // w is zero unless the particle is valid and in the window; only an
-            // Esirkepov move longer than a cell (ok false) must be masked.
-            const bool take = St::kAlwaysOk ? true : (valid && inside);
 #pragma unroll
             for (int k = 0; k < V; ++k) acc[k] = fmaf(acc[k], keep, take ? w[k] : 0.0f);
             if (nseg) {
@@ -368,7 +410,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
 // Deposits one tile with one wavefront: LDS window `lds` (g.lds_per_wave
 // floats, 8-byte aligned), cell offsets and partial / accumulation buffers
 // already offset to the tile's instance.
-template <int S, int K, int EX, int PIPE>
+template <int S, int K, int EX, int PIPE, int LA = 0>
 __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ lds, const float* __restrict__ px,
                                                  const float* __restrict__ py, const float* __restrict__ pvx,
                                                  const float* __restrict__ pvy, const float* __restrict__ pq,
@@ -440,13 +482,13 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
             if (b1 < total_q)
                 load_chunk<Q, St::kNeedsVelocity>(b1 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
                                                   pq, pvx, pvy, B);
-            process_chunk<S, Q, EX>(A, t, d, dt, gacc, status);
+            process_chunk<S, Q, EX, LA>(A, t, d, dt, gacc, status);
             if (b1 >= total_q) break;
             const int b2 = b1 + step;
             if (b2 < total_q)
                 load_chunk<Q, St::kNeedsVelocity>(b2 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py,
                                                   pq, pvx, pvy, A);
-            process_chunk<S, Q, EX>(B, t, d, dt, gacc, status);
+            process_chunk<S, Q, EX, LA>(B, t, d, dt, gacc, status);
             base = b2;
         }
     } else {
@@ -454,7 +496,7 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
             Chunk<Q> A;
             load_chunk<Q, St::kNeedsVelocity>(base + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq,
                                               pvx, pvy, A);
-            process_chunk<S, Q, EX>(A, t, d, dt, gacc, status);
+            process_chunk<S, Q, EX, LA>(A, t, d, dt, gacc, status);
         }
     }
 

@@ -91,7 +91,7 @@ __global__ void set_ints_kernel(int* __restrict__ p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-template <int S, int K, int EX>
+template <int S, int K, int EX, int LA>
 __global__ __launch_bounds__(256) void ens_deposit_kernel(EnsParticles p, const int* __restrict__ cell_off,
                                                           EnsTables t, const int2* __restrict__ items, int nitems,
                                                           int* __restrict__ counter, float dt,
@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void ens_deposit_kernel(EnsParticles p, const 
         const EnsMember& m = t.mem[w.x];
         const DevDomain d = m.d;
         const TileGeom g = m.g;
-        deposit_one_tile<S, K, EX, 0>(w.y, lds, p.x, p.y, p.vx, p.vy, p.q, cell_off + m.cbase, d, g, dt,
-                                      partial + m.partial_base, gacc + m.gbase, status);
+        deposit_one_tile<S, K, EX, 0, LA>(w.y, lds, p.x, p.y, p.vx, p.vy, p.q, cell_off + m.cbase, d, g, dt,
+                                          partial + m.partial_base, gacc + m.gbase, status);
     }
 }
 
@@ -297,26 +297,30 @@ __global__ __launch_bounds__(256) void ens_diag_kernel(const double* __restrict_
 using EnsKernel = void (*)(EnsParticles, const int*, EnsTables, const int2*, int, int*, float, float*, double*, int*,
                            int);
 
+// drift: the members' windows have a drift halo, so the lookahead variant of
+// the chunk processing runs (see process_chunk).
 template <int S, int K>
-EnsKernel pick_ex(int exact) {
-    return exact ? (EnsKernel)ens_deposit_kernel<S, K, 1> : (EnsKernel)ens_deposit_kernel<S, K, 0>;
+EnsKernel pick_ex(int exact, int drift) {
+    if (drift)
+        return exact ? (EnsKernel)ens_deposit_kernel<S, K, 1, 1> : (EnsKernel)ens_deposit_kernel<S, K, 0, 1>;
+    return exact ? (EnsKernel)ens_deposit_kernel<S, K, 1, 0> : (EnsKernel)ens_deposit_kernel<S, K, 0, 0>;
 }
 
 template <int S>
-EnsKernel pick_k(int ppl, int exact) {
-    return ppl == 4 ? pick_ex<S, 4>(exact) : pick_ex<S, 8>(exact);
+EnsKernel pick_k(int ppl, int exact, int drift) {
+    return ppl == 4 ? pick_ex<S, 4>(exact, drift) : pick_ex<S, 8>(exact, drift);
 }
 
-EnsKernel pick_kernel(int scheme, int ppl, int exact) {
+EnsKernel pick_kernel(int scheme, int ppl, int exact, int drift) {
     switch (scheme) {
         case NGP:
-            return pick_k<NGP>(ppl, exact);
+            return pick_k<NGP>(ppl, exact, drift);
         case CIC:
-            return pick_k<CIC>(ppl, exact);
+            return pick_k<CIC>(ppl, exact, drift);
         case TSC:
-            return pick_k<TSC>(ppl, exact);
+            return pick_k<TSC>(ppl, exact, drift);
         default:
-            return pick_ex<ESK, 4>(exact);  // Esirkepov runs 4 particles per lane
+            return pick_ex<ESK, 4>(exact, drift);  // Esirkepov runs 4 particles per lane
     }
 }
 
@@ -349,7 +353,7 @@ void launch_set_ints(int* p, int n, int v, hipStream_t s) {
 }
 
 int ens_deposit_grid(const EnsWork& w, int ncu) {
-    const EnsKernel k = pick_kernel(w.scheme, w.ppl, w.exact);
+    const EnsKernel k = pick_kernel(w.scheme, w.ppl, w.exact, w.drift);
     const size_t shmem = (size_t)4 * w.lds_per_wave * sizeof(float);
     const int64_t want = (int64_t)blocks_per_cu(k, shmem) * ncu;
     const int64_t need = ((int64_t)w.nitems + 3) / 4;  // four waves per workgroup
@@ -364,7 +368,7 @@ hipError_t launch_ens_deposit(const EnsWork* groups, int ngroups, const EnsParti
     for (int gi = 0; gi < ngroups; ++gi) {
         const EnsWork& w = groups[gi];
         if (w.nitems <= 0) continue;
-        const EnsKernel k = pick_kernel(w.scheme, w.ppl, w.exact);
+        const EnsKernel k = pick_kernel(w.scheme, w.ppl, w.exact, w.drift);
         const size_t shmem = (size_t)4 * w.lds_per_wave * sizeof(float);
         hipLaunchKernelGGL(k, dim3((unsigned)w.grid), dim3(256), shmem, s, p, cell_off, t, w.items, w.nitems,
                            counters + gi, dt, partial, gacc, status, w.lds_per_wave);

@@ -169,6 +169,7 @@ pic_status rebuild_work(pic_hip_ensemble* e, int m0 = 0, int m1 = -1) {
         w.scheme = kv.first.scheme;
         w.ppl = kv.first.ppl;
         w.exact = kv.first.exact;
+        w.drift = e->halo > 0 ? 1 : 0;
         w.lds_per_wave = 0;
         offs.push_back(items.size());
         for (int i : list) {

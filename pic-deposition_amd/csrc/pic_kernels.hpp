@@ -226,6 +226,7 @@ struct EnsParticles {
 // with this scheme, particles per lane and coordinate path.
 struct EnsWork {
     int scheme, ppl, exact;
+    int drift;  // the windows have a drift halo: lookahead chunk processing
     const int2* items;
     int nitems;
     int lds_per_wave;  // floats, the largest of the group's members
