@@ -151,7 +151,10 @@ pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* param
  * streams, and builds the cell offset table.  pic_hip_sort_perm computes the
  * same without moving the particles and copies out (synchronously) the sorted
  * keys, perm[i] = source index of the i-th sorted particle, and the Nx*Ny+1
- * cell offsets; NULL outputs are skipped. */
+ * cell offsets; NULL outputs are skipped.  The first pic_hip_sort or
+ * pic_hip_run of a context allocates 32 bytes per particle of capacity for
+ * record staging: a permutation far from the identity moves each particle's
+ * streams as one or two 16-byte records (decided on the device per sort). */
 pic_status pic_hip_sort(pic_hip_ctx* ctx);
 pic_status pic_hip_sort_perm(pic_hip_ctx* ctx, int32_t* keys_sorted, int32_t* perm, int32_t* cell_offsets);
 /* Keys of the current order (synchronous copy out). */

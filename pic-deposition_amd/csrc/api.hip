@@ -327,8 +327,18 @@ void swap_permuted(pic_hip_ctx* ctx) {
         if (!ctx->uniform[i]) std::swap(ctx->s[i], ctx->alt[i]);
 }
 
+// Buffers of the staged permutation (outside stream capture).
+pic_status ensure_sort_staging(pic_hip_ctx* ctx) {
+    if (ctx->sort_rec) return PIC_OK;
+    PIC_HIP_TRY(dmalloc(&ctx->far_count, 1));
+    PIC_HIP_TRY(dmalloc(&ctx->permute_mode, 1));
+    PIC_HIP_TRY(hipMemsetAsync(ctx->far_count, 0, sizeof(unsigned), ctx->stream));
+    PIC_HIP_TRY(hipMemsetAsync(ctx->permute_mode, 0, sizeof(int), ctx->stream));
+    PIC_HIP_TRY(dmalloc(&ctx->sort_rec, 2 * ctx->cap));
+    return PIC_OK;
+}
+
 pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
-    PIC_HIP_TRY(launch_sort_keys_perm(ctx->s[0], ctx->s[1], (int64_t)ctx->n, ctx->dev, ctx->sb, ctx->stream));
     StreamSet fwd{}, back{};
     for (int i = 0; i < kStreams; ++i) {
         if (ctx->uniform[i]) continue;  // every particle holds the same value
@@ -337,7 +347,21 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
         back.src[back.count] = ctx->alt[i];
         back.dst[back.count++] = ctx->s[i];
     }
-    launch_permute_streams(fwd, ctx->sb.perm, (int64_t)ctx->n, ctx->stream);
+    const int64_t n = (int64_t)ctx->n;
+    if (ctx->sort_rec) {
+        // The device decides per sort between a direct gather of each stream
+        // (an order close to the last sort's) and packed records (an order far
+        // from it: one or two lines per particle instead of one per stream).
+        launch_keys_far(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, 4 * ctx->dom.Nx, ctx->far_count,
+                        ctx->permute_mode, ctx->stream);
+        launch_pack_streams(fwd, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
+        PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));
+        launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream, ctx->permute_mode);
+        launch_gather_unpack(fwd, ctx->sb.perm, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
+    } else {
+        PIC_HIP_TRY(launch_sort_keys_perm(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb, ctx->stream));
+        launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream);
+    }
     if (copy_back)
         launch_permute_streams(back, nullptr, (int64_t)ctx->n, ctx->stream);
     else
@@ -353,6 +377,8 @@ extern "C" {
 
 pic_status pic_hip_sort(pic_hip_ctx* ctx) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    const pic_status st = ensure_sort_staging(ctx);
+    if (st != PIC_OK) return st;
     return sort_particles(ctx, false);
 }
 
@@ -728,6 +754,7 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
     pic_status st = ensure_step_buffers(ctx);
     if (st != PIC_OK) return st;
     if ((st = ensure_fft(ctx)) != PIC_OK) return st;
+    if ((st = ensure_sort_staging(ctx)) != PIC_OK) return st;
     const int cap = nsteps > record_from ? nsteps - record_from : 0;
     double* dev_rec = nullptr;
     PIC_HIP_TRY(dmalloc(&dev_rec, (size_t)4 * std::max(cap, 1)));
@@ -1092,7 +1119,7 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
     c->order_valid = false;
     pic_status st = PIC_OK;
     if (sorted || backend == PIC_BACKEND_GPU_PRIV) {
-        st = pic_hip_sort(c);
+        st = pic_hip_sort(c);  // the caller's order: usually far from sorted
         if (st != PIC_OK) return st;
     }
     st = pic_hip_deposit(c, scheme, backend, dt);

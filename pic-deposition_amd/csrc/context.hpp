@@ -37,6 +37,12 @@ struct pic_hip_ctx {
     // Stream k holds one value for every particle (charges and masses of a
     // single species): a permutation leaves it unchanged, so sorts skip it.
     bool uniform[kStreams] = {};
+    // Sort of a far-from-sorted order: the permuted streams go through packed
+    // records (sort_rec, 2 float4 per particle) chosen on the device per sort
+    // (far_count, permute_mode); allocated before the first sort that can use it.
+    float4* sort_rec = nullptr;
+    unsigned* far_count = nullptr;
+    int* permute_mode = nullptr;
     SortBuffers sb{};
     bool order_valid = false;
     float* rho = nullptr;
@@ -125,6 +131,9 @@ struct pic_hip_ctx {
         (void)hipFree(run_sums);
         (void)hipFree(tw_x);
         (void)hipFree(tw_y);
+        (void)hipFree(sort_rec);
+        (void)hipFree(far_count);
+        (void)hipFree(permute_mode);
         (void)hipHostFree(h_status);
         (void)hipHostFree(h_diag);
         if (stream) (void)hipStreamDestroy(stream);

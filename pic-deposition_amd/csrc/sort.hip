@@ -214,9 +214,84 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ s
     if (i < n) dst[i] = src[perm[i]];
 }
 
+// Keys of the current order, and the number of particles whose key moved by
+// more than `far` from the key at the same position after the last sort
+// (`keys` holds those on entry).  Many far moves mean the permutation will be
+// far from the identity.
+__global__ __launch_bounds__(256) void keys_far_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                       int64_t n, DevDomain d, int* __restrict__ keys, int far,
+                                                       unsigned* __restrict__ far_count) {
+    __shared__ unsigned wave_far[4];
+    unsigned cnt = 0;  // wave total, kept by every lane
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        bool f = false;
+        if (i < n) {
+            const int old = keys[i];
+            const int k = cell_key_exact(x[i], y[i], d);
+            keys[i] = k;
+            const long long dk = (long long)k - (long long)old;
+            f = dk > far || dk < -far;
+        }
+        cnt += (unsigned)__popcll(__ballot(f));
+    }
+    if ((threadIdx.x & 63) == 0) wave_far[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned c = wave_far[0] + wave_far[1] + wave_far[2] + wave_far[3];
+        if (c) atomicAdd(far_count, c);  // one atomic per block
+    }
+}
+
+// Permutation mode of this sort: 1 (staged gather through packed records)
+// when more than a quarter of the particles moved far, else 0; clears the
+// count for the next sort.
+__global__ void permute_mode_kernel(unsigned* __restrict__ far_count, int64_t n, int* __restrict__ mode) {
+    const unsigned c = *far_count;
+    *mode = (int64_t)c > n / 4 ? 1 : 0;
+    *far_count = 0;
+}
+
+// Mode 1: the permuted streams of particle i packed into one or two float4
+// records, so that the gather reads one or two lines per particle instead of
+// one per stream.
+__global__ __launch_bounds__(256) void pack_streams_kernel(StreamSet ss, int64_t n, const int* __restrict__ mode,
+                                                           float4* __restrict__ rec) {
+    if (*mode == 0) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < kMaxStreams; ++k)
+        if (k < ss.count) v[k] = ss.src[k][i];
+    const int w = ss.count > 4 ? 2 : 1;
+    rec[w * i] = make_float4(v[0], v[1], v[2], v[3]);
+    if (w == 2) rec[w * i + 1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+__global__ __launch_bounds__(256) void gather_unpack_kernel(StreamSet ss, const int* __restrict__ perm, int64_t n,
+                                                            const int* __restrict__ mode,
+                                                            const float4* __restrict__ rec) {
+    if (*mode == 0) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = perm[i];
+    const int w = ss.count > 4 ? 2 : 1;
+    const float4 a = rec[w * p];
+    const float4 b = w == 2 ? rec[w * p + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int k = 0; k < kMaxStreams; ++k)
+        if (k < ss.count) ss.dst[k][i] = v[k];
+}
+
 // dst_k[i] = src_k[perm[i]] for up to kMaxStreams streams: the permutation is
-// read once for all of them.  With perm == nullptr it is a plain copy.
-__global__ __launch_bounds__(256) void permute_streams_kernel(StreamSet ss, const int* __restrict__ perm, int64_t n) {
+// read once for all of them.  With perm == nullptr it is a plain copy.  With mode
+// set to 1 the staged gather does this sort and the kernel returns.
+__global__ __launch_bounds__(256) void permute_streams_kernel(StreamSet ss, const int* __restrict__ perm, int64_t n,
+                                                              const int* __restrict__ mode) {
+    if (mode && *mode != 0) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t p = perm ? (int64_t)perm[i] : i;
@@ -296,9 +371,30 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
     return launch_sort_precomputed(n, d.Nx * d.Ny, sb, s);
 }
 
-void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s) {
+void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s, const int* mode) {
     if (n > 0)
-        hipLaunchKernelGGL(permute_streams_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, perm, n);
+        hipLaunchKernelGGL(permute_streams_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, perm, n,
+                           mode);
+}
+
+void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far,
+                     unsigned* far_count, int* mode, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(keys_far_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                           s, x, y, n, d, keys, far, far_count);
+    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(1), 0, s, far_count, n, mode);
+}
+
+void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(pack_streams_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, n, mode, rec);
+}
+
+void launch_gather_unpack(const StreamSet& ss, const int* perm, int64_t n, const int* mode, const float4* rec,
+                          hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(gather_unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, perm, n, mode,
+                           rec);
 }
 
 void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n, hipStream_t s) {
