@@ -9,7 +9,8 @@
  *   - depositChargeCudaSoA / depositChargeCudaAoS
  *       src/deposition/cuda/deposition_cuda.hpp:11-12,
  *       src/deposition/cuda/deposition_cuda.cu:288-337
- *     -> pic_hip_deposit_host (synchronous drop-in: FP64 host in, FP64 rho out).
+ *     -> pic_hip_deposit_host, pic_hip_deposit_host_aos (synchronous drop-ins:
+ *        FP64 host in, FP64 rho out).
  *   - the device side of depositChargeSoA / depositChargeTimedSoA
  *       src/deposition/deposition.cpp:120-132,181-200,257-310
  *     -> pic_hip_create / pic_hip_upload / pic_hip_deposit / pic_hip_get_rho
@@ -110,6 +111,14 @@ const char* pic_hip_version(void);
 pic_status pic_hip_deposit_host(const double* x, const double* y, const double* vx, const double* vy,
                                 const double* q, size_t n, const pic_domain* domain, int scheme, int backend,
                                 int sorted, double dt, double* rho_out);
+/* Replaces depositChargeCudaAoS (deposition_cuda.cu:330-333, which copies the
+ * records into a new FP64 SoA with toSoA before the SoA path).  `records` holds
+ * n particles in the layout of ParticleAoS (src/pic/particles.hpp:13-20): six
+ * FP64 values x, y, vx, vy, q, m per 48-byte record.  Same semantics and
+ * results as pic_hip_deposit_host on the same particles; the host threads
+ * round straight from the records, without the intermediate SoA copy. */
+pic_status pic_hip_deposit_host_aos(const double* records, size_t n, const pic_domain* domain, int scheme,
+                                    int backend, int sorted, double dt, double* rho_out);
 
 /* ---- device-resident context --------------------------------------------
  * One HIP stream per context; calls enqueue work asynchronously on it and are
@@ -144,6 +153,16 @@ pic_status pic_hip_upload(pic_hip_ctx* ctx, const float* x, const float* y, cons
                           const float* q, const float* m, size_t n);
 /* Copies the current device order back; NULL pointers are skipped.  Synchronous. */
 pic_status pic_hip_download(pic_hip_ctx* ctx, float* x, float* y, float* vx, float* vy, float* q, float* m);
+/* AoS layout on the device side (ParticleAoS records, src/pic/particles.hpp:
+ * 13-20; the reference's device path accepts AoS only through toSoA,
+ * deposition_cuda.cu:330-333).  pic_hip_upload_aos copies n FP64 records
+ * (records 16-byte aligned) to the device as they are and converts them there
+ * into the FP32 streams (rounded to nearest: the same streams as
+ * pic_hip_upload of the values cast to float).  pic_hip_download_aos writes the
+ * current device order back as FP64 records (exact promotion).  Both are
+ * synchronous and stage through device memory in chunks of up to 2^22 records. */
+pic_status pic_hip_upload_aos(pic_hip_ctx* ctx, const double* records, size_t n);
+pic_status pic_hip_download_aos(pic_hip_ctx* ctx, double* records);
 pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* params, size_t n);
 
 /* Cell sort.  pic_hip_sort computes every particle's cell key

@@ -289,6 +289,61 @@ pic_status pic_hip_download(pic_hip_ctx* ctx, float* x, float* y, float* vx, flo
     return PIC_OK;
 }
 
+pic_status pic_hip_upload_aos(pic_hip_ctx* ctx, const double* records, size_t n) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (n > 0 && !records) return fail(PIC_ERR_INVALID_ARGUMENT, "null records");
+    if (n > 0 && (reinterpret_cast<uintptr_t>(records) & 15) != 0)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "records must be 16-byte aligned");
+    if (n + 64 > ctx->cap) return fail(PIC_ERR_INVALID_ARGUMENT, "particle count exceeds context capacity");
+    const size_t chunk = std::min<size_t>(n, kAosChunk);
+    double* dev = nullptr;
+    if (n > 0) PIC_HIP_TRY(dmalloc(&dev, chunk * kAosFields));
+    std::unique_ptr<double, hipError_t (*)(void*)> guard(dev, hipFree);
+    for (size_t b = 0; b < n; b += chunk) {
+        const size_t m = std::min(chunk, n - b);
+        PIC_HIP_TRY(hipMemcpyAsync(dev, records + b * kAosFields, m * kAosFields * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        float* const dst[6] = {ctx->s[0] + b, ctx->s[1] + b, ctx->s[2] + b, ctx->s[3] + b, ctx->s[4] + b,
+                               ctx->s[5] + b};
+        launch_aos_unpack(dev, (int64_t)m, dst, ctx->stream);
+        PIC_HIP_TRY(hipGetLastError());
+    }
+    for (int i = 0; i < kStreams; ++i) {
+        bool u = i >= 4;  // only charges and masses are candidates
+        if (u) {
+            const float v0 = n > 0 ? (float)records[i] : 0.0f;
+            for (size_t k = 1; k < n && u; ++k) u = (float)records[k * kAosFields + i] == v0;
+        }
+        ctx->uniform[i] = u;
+    }
+    ctx->n = n;
+    ctx->order_valid = false;
+    PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PIC_OK;
+}
+
+pic_status pic_hip_download_aos(pic_hip_ctx* ctx, double* records) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    const size_t n = ctx->n;
+    if (n == 0) return PIC_OK;
+    if (!records) return fail(PIC_ERR_INVALID_ARGUMENT, "null records");
+    const size_t chunk = std::min<size_t>(n, kAosChunk);
+    double* dev = nullptr;
+    PIC_HIP_TRY(dmalloc(&dev, chunk * kAosFields));
+    std::unique_ptr<double, hipError_t (*)(void*)> guard(dev, hipFree);
+    for (size_t b = 0; b < n; b += chunk) {
+        const size_t m = std::min(chunk, n - b);
+        const float* const src[6] = {ctx->s[0] + b, ctx->s[1] + b, ctx->s[2] + b, ctx->s[3] + b, ctx->s[4] + b,
+                                     ctx->s[5] + b};
+        launch_aos_pack(src, (int64_t)m, dev, ctx->stream);
+        PIC_HIP_TRY(hipGetLastError());
+        PIC_HIP_TRY(hipMemcpyAsync(records + b * kAosFields, dev, m * kAosFields * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    }
+    PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PIC_OK;
+}
+
 pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* p, size_t n) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     if (!p) return fail(PIC_ERR_INVALID_ARGUMENT, "null init params");
@@ -1034,14 +1089,12 @@ pic_status pic_hip_read_probe(pic_hip_ctx* ctx, size_t bytes, int reps, double* 
 
 // ---- drop-in seam -----------------------------------------------------------
 
-pic_status pic_hip_deposit_host(const double* x, const double* y, const double* vx, const double* vy,
-                                const double* q, size_t n, const pic_domain* domain, int scheme, int backend,
-                                int sorted, double dt, double* rho_out) {
+// The seam of both layouts: field a of particle k is src[a][k * stride] (SoA:
+// stride 1; AoS records: field pointers into the first record, stride 6).
+static pic_status seam_deposit(const double* const src_in[5], size_t stride, size_t n, const pic_domain* domain,
+                        int scheme, int backend, int sorted, double dt, double* rho_out) {
     if (!valid_domain(domain)) return fail(PIC_ERR_INVALID_ARGUMENT, "invalid domain");
     if (!rho_out) return fail(PIC_ERR_INVALID_ARGUMENT, "null rho_out");
-    if (n > 0 && (!x || !y || !q)) return fail(PIC_ERR_INVALID_ARGUMENT, "x, y and q are required");
-    if (scheme == PIC_SCHEME_ESIRKEPOV && n > 0 && (!vx || !vy))
-        return fail(PIC_ERR_INVALID_ARGUMENT, "the Esirkepov scheme needs vx and vy");
     // A per-thread context and pinned staging buffer, grown on demand (the
     // reference keeps a static grow-only device pool, deposition_cuda.cu:243-273).
     struct Staging {
@@ -1062,8 +1115,7 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
         cache.reset(c);
     }
     pic_hip_ctx* c = cache.get();
-    const double* src[5] = {x, y, vx, vy, q};
-    const int dst_stream[5] = {0, 1, 2, 3, 4};
+    const double* src[5] = {src_in[0], src_in[1], src_in[2], src_in[3], src_in[4]};
     if (stage.cap < 5 * n) {
         if (stage.p) (void)hipHostFree(stage.p);
         stage.p = nullptr;
@@ -1079,11 +1131,12 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     const unsigned nt = n >= ((size_t)1 << 18) ? hw : 1;
     std::vector<char> q_same(nt, 1);  // per worker: its charges all equal the first particle's
-    auto convert = [sp, &src, n, &q_same](unsigned t, size_t b, size_t e) {
+    auto convert = [sp, &src, n, stride, &q_same](unsigned t, size_t b, size_t e) {
         for (int a = 0; a < 5; ++a) {
             if (!src[a]) continue;
             float* d = sp + (size_t)a * n;
-            for (size_t k = b; k < e; ++k) d[k] = (float)src[a][k];
+            const double* f = src[a];
+            for (size_t k = b; k < e; ++k) d[k] = (float)f[k * stride];
         }
         const float q0 = (float)src[4][0];
         bool same = true;
@@ -1105,7 +1158,7 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
         }
     }
     for (int a = 0; a < 5 && n > 0; ++a) {
-        float* d = c->s[dst_stream[a]];
+        float* d = c->s[a];
         if (src[a])
             PIC_HIP_TRY(hipMemcpyAsync(d, sp + (size_t)a * n, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
         else
@@ -1125,6 +1178,25 @@ pic_status pic_hip_deposit_host(const double* x, const double* y, const double* 
     st = pic_hip_deposit(c, scheme, backend, dt);
     if (st != PIC_OK) return st;
     return pic_hip_get_rho_f64(c, rho_out);
+}
+
+pic_status pic_hip_deposit_host(const double* x, const double* y, const double* vx, const double* vy,
+                                const double* q, size_t n, const pic_domain* domain, int scheme, int backend,
+                                int sorted, double dt, double* rho_out) {
+    if (n > 0 && (!x || !y || !q)) return fail(PIC_ERR_INVALID_ARGUMENT, "x, y and q are required");
+    if (scheme == PIC_SCHEME_ESIRKEPOV && n > 0 && (!vx || !vy))
+        return fail(PIC_ERR_INVALID_ARGUMENT, "the Esirkepov scheme needs vx and vy");
+    const double* src[5] = {x, y, vx, vy, q};
+    return seam_deposit(src, 1, n, domain, scheme, backend, sorted, dt, rho_out);
+}
+
+pic_status pic_hip_deposit_host_aos(const double* records, size_t n, const pic_domain* domain, int scheme,
+                                    int backend, int sorted, double dt, double* rho_out) {
+    if (n > 0 && !records) return fail(PIC_ERR_INVALID_ARGUMENT, "null records");
+    const bool esk = scheme == PIC_SCHEME_ESIRKEPOV;
+    const double* src[5] = {records, records + 1, esk ? records + 2 : nullptr, esk ? records + 3 : nullptr,
+                            records + 4};
+    return seam_deposit(src, kAosFields, n, domain, scheme, backend, sorted, dt, rho_out);
 }
 
 }  // extern "C"

@@ -51,6 +51,8 @@ inline pic_status fail(pic_status s, const std::string& msg) {
     } while (0)
 
 constexpr int kStreams = 6;  // x, y, vx, vy, q, m
+constexpr size_t kAosFields = 6;            // FP64 values per ParticleAoS record
+constexpr size_t kAosChunk = size_t(1) << 22;  // records staged per device copy
 
 inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 

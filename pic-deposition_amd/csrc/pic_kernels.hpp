@@ -198,6 +198,10 @@ void launch_esirkepov_current(const float* x, const float* y, const float* vx, c
 void launch_continuity_residual(const CurrentGrids& g, const DevDomain& d, double dt, double* partial, int nblocks,
                                 double* out, hipStream_t s);
 void launch_copy_f32(const float* src, float* dst, int64_t n, hipStream_t s);
+// 48-byte FP64 particle records (x, y, vx, vy, q, m; rec 16-byte aligned)
+// <-> the six FP32 streams.
+void launch_aos_unpack(const double* rec, int64_t n, float* const dst[6], hipStream_t s);
+void launch_aos_pack(const float* const src[6], int64_t n, double* rec, hipStream_t s);
 // Streaming read of n4 float4 values, one partial sum per block.
 void launch_read_probe(const float* src, int64_t n4, float* partial, int blocks, hipStream_t s);
 

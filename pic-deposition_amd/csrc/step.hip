@@ -260,6 +260,37 @@ __global__ __launch_bounds__(256) void read_probe_kernel(const float4* __restric
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+// Particle records in the layout of ParticleAoS (src/pic/particles.hpp:13-20):
+// x, y, vx, vy, q, m as FP64, 48 bytes each.  Consecutive lanes read
+// consecutive records (three 16-byte loads) and write consecutive elements of
+// the six FP32 streams, rounded to nearest as a host (float) conversion is.
+__global__ __launch_bounds__(256) void aos_unpack_kernel(const double2* __restrict__ rec, int64_t n,
+                                                         float* __restrict__ x, float* __restrict__ y,
+                                                         float* __restrict__ vx, float* __restrict__ vy,
+                                                         float* __restrict__ q, float* __restrict__ m) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 a = rec[3 * i], b = rec[3 * i + 1], c = rec[3 * i + 2];
+    x[i] = (float)a.x;
+    y[i] = (float)a.y;
+    vx[i] = (float)b.x;
+    vy[i] = (float)b.y;
+    q[i] = (float)c.x;
+    m[i] = (float)c.y;
+}
+
+// The six FP32 streams back into FP64 records (exact promotion).
+__global__ __launch_bounds__(256) void aos_pack_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ vx, const float* __restrict__ vy,
+                                                       const float* __restrict__ q, const float* __restrict__ m,
+                                                       int64_t n, double2* __restrict__ rec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rec[3 * i] = make_double2((double)x[i], (double)y[i]);
+    rec[3 * i + 1] = make_double2((double)vx[i], (double)vy[i]);
+    rec[3 * i + 2] = make_double2((double)q[i], (double)m[i]);
+}
+
 }  // namespace
 
 void launch_read_probe(const float* src, int64_t n4, float* partial, int blocks, hipStream_t s) {
@@ -338,6 +369,18 @@ void launch_set_int(int* p, int v, hipStream_t s) { hipLaunchKernelGGL(set_int_k
 void launch_copy_f32(const float* src, float* dst, int64_t n, hipStream_t s) {
     if (n > 0)
         hipLaunchKernelGGL(copy_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+}
+
+void launch_aos_unpack(const double* rec, int64_t n, float* const dst[6], hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(aos_unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<const double2*>(rec), n, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5]);
+}
+
+void launch_aos_pack(const float* const src[6], int64_t n, double* rec, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(aos_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src[0], src[1],
+                           src[2], src[3], src[4], src[5], n, reinterpret_cast<double2*>(rec));
 }
 
 }  // namespace pichip

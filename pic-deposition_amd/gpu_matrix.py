@@ -24,7 +24,14 @@ With --resident, a second CSV in the same schema times the deposit alone on
 particles already resident in HBM (context API, HIP-synchronised wall time,
 GPU_Priv after one sort): the device-side figure behind each seam row.
 
-    python pic-deposition_amd/gpu_matrix.py --out benchmark_gpu_mi355x.csv [--resident resident.csv]
+gpuMatrix itself is SoA only; --layouts SoA,AoS adds the AoS rows of the CPU
+matrices (quickMatrix, benchmark_runner.cpp:15-56) on the device: the seam
+rows go through depositChargeAoS's device branch (pic_hip_deposit_host_aos,
+48-byte records), the resident rows upload the records with
+pic_hip_upload_aos (converted on the device).
+
+    python pic-deposition_amd/gpu_matrix.py --out benchmark_gpu_mi355x.csv [--resident resident.csv] \
+        [--layouts SoA,AoS]
 """
 import argparse
 import csv
@@ -69,9 +76,9 @@ def uniform_maxwellian(n, L=1.0, temperature=1.0, seed=42):
     return {"x": x, "y": y, "vx": vx, "vy": vy, "q": q, "m": np.ones(n)}
 
 
-def estimate_bytes(n, scheme, g):
-    """bytes read + written of estimateBytes for the SoA layout."""
-    return n * 40 + n * WRITES_PER_PARTICLE[scheme] * 8 + g * g * 8
+def estimate_bytes(n, scheme, g, layout="SoA"):
+    """bytes read + written of estimateBytes (40-byte SoA, 48-byte AoS record)."""
+    return n * (48 if layout == "AoS" else 40) + n * WRITES_PER_PARTICLE[scheme] * 8 + g * g * 8
 
 
 def charge_error(rho, p, dom):
@@ -79,37 +86,45 @@ def charge_error(rho, p, dom):
     return abs(float(rho.sum()) * cv - float(p["q"].sum())) / max(float(np.abs(p["q"]).sum()), 1e-12)
 
 
-def row(n, g, s, b, times_ms, err):
+def row(n, g, s, b, times_ms, err, layout="SoA"):
     mean = float(np.mean(times_ms))
     std = float(np.std(times_ms, ddof=1)) if len(times_ms) > 1 else 0.0
-    return {"num_particles": n, "grid": f"{g}x{g}", "scheme": SCHEME_NAMES[s], "layout": "SoA", "sorted": "off",
+    return {"num_particles": n, "grid": f"{g}x{g}", "scheme": SCHEME_NAMES[s], "layout": layout, "sorted": "off",
             "backend": BACKEND_NAMES[b], "threads": 1, "sort_ms": 0, "deposit_ms": mean, "deposit_std_ms": std,
             "deposition_ms": mean, "full_step_ms": 0, "throughput": n / (mean / 1000.0),
-            "bandwidth_gbs": estimate_bytes(n, s, g) / 1e9 / (mean / 1000.0), "charge_error": err,
+            "bandwidth_gbs": estimate_bytes(n, s, g, layout) / 1e9 / (mean / 1000.0), "charge_error": err,
             "energy_drift": 0, "spectral_noise": 0, "speedup": 1}
 
 
-def seam_case(n, g, s, b, repeats):
+def seam_case(n, g, s, b, repeats, layout="SoA"):
     dom = pichip.make_domain(1.0, 1.0, g, g)
     p = uniform_maxwellian(n)
     dt = 0.01  # Domain::dt, the reference's esirkepov_dt for the matrix
+    if layout == "AoS":
+        rec = pichip.to_aos(p)
+        call = lambda: pichip.deposit_charge_aos(rec, dom, scheme=s, backend=b, esirkepov_dt=dt)  # noqa: E731
+    else:
+        call = lambda: pichip.deposit_charge(p, dom, scheme=s, backend=b, esirkepov_dt=dt)  # noqa: E731
     rho = None
     for _ in range(3):
-        rho = pichip.deposit_charge(p, dom, scheme=s, backend=b, esirkepov_dt=dt)
+        rho = call()
     times = []
     for _ in range(repeats):
         t = time.perf_counter()
-        rho = pichip.deposit_charge(p, dom, scheme=s, backend=b, esirkepov_dt=dt)
+        rho = call()
         times.append((time.perf_counter() - t) * 1000.0)
-    return row(n, g, s, b, times, charge_error(rho, p, dom))
+    return row(n, g, s, b, times, charge_error(rho, p, dom), layout)
 
 
-def resident_case(n, g, s, b, repeats):
+def resident_case(n, g, s, b, repeats, layout="SoA"):
     dom = pichip.make_domain(1.0, 1.0, g, g)
     p = uniform_maxwellian(n)
     f32 = {k: v.astype(np.float32) for k, v in p.items()}
     with pichip.Context(dom, n) as c:
-        c.upload(f32["x"], f32["y"], f32["q"], f32["vx"], f32["vy"], f32["m"])
+        if layout == "AoS":
+            c.upload_aos(pichip.to_aos(p))
+        else:
+            c.upload(f32["x"], f32["y"], f32["q"], f32["vx"], f32["vy"], f32["m"])
         if b == 2:
             c.sort()
         for _ in range(3):
@@ -122,7 +137,7 @@ def resident_case(n, g, s, b, repeats):
             c.sync()
             times.append((time.perf_counter() - t) * 1000.0)
         rho = c.rho().astype(np.float64)
-    return row(n, g, s, b, times, charge_error(rho, p, dom))
+    return row(n, g, s, b, times, charge_error(rho, p, dom), layout)
 
 
 def main(argv=None):
@@ -133,20 +148,25 @@ def main(argv=None):
     ap.add_argument("--esirkepov", action="store_true", help="also Esirkepov, and GPU_Priv for every scheme")
     ap.add_argument("--counts", default="10000,100000,1000000")
     ap.add_argument("--grids", default="64,128,256")
+    ap.add_argument("--layouts", default="SoA", help="comma list of SoA, AoS")
     args = ap.parse_args(argv)
     cases = gpu_matrix(tuple(int(v) for v in args.counts.split(",")), tuple(int(v) for v in args.grids.split(",")),
                        esirkepov=args.esirkepov)
+    layouts = [v.strip() for v in args.layouts.split(",") if v.strip()]
+    if any(v not in ("SoA", "AoS") for v in layouts):
+        ap.error("--layouts takes SoA and/or AoS")
     outs = [(args.out, seam_case)] + ([(args.resident, resident_case)] if args.resident else [])
     for path, fn in outs:
         with open(path, "w", newline="") as f:
             w = csv.DictWriter(f, fieldnames=HEADER)
             w.writeheader()
             for n, g, s, b in cases:
-                r = fn(n, g, s, b, args.repeats)
-                w.writerow(r)
-                f.flush()
-                print(f"{os.path.basename(path)}: {n} {g}x{g} {SCHEME_NAMES[s]} {BACKEND_NAMES[b]} "
-                      f"{r['deposit_ms']:.4f} ms", flush=True)
+                for layout in layouts:
+                    r = fn(n, g, s, b, args.repeats, layout)
+                    w.writerow(r)
+                    f.flush()
+                    print(f"{os.path.basename(path)}: {n} {g}x{g} {SCHEME_NAMES[s]} {layout} {BACKEND_NAMES[b]} "
+                          f"{r['deposit_ms']:.4f} ms", flush=True)
     return 0
 
 

@@ -2,6 +2,7 @@
 
 #include "deposition_hip.hpp"
 
+#include <cstddef>
 #include <stdexcept>
 #include <string>
 
@@ -48,10 +49,22 @@ void depositChargeHipSoA(const ParticlesSoA& particles, FieldGrid& grid, const D
                                config.esirkepov_dt, grid.rho().data()));
 }
 
+// The records go to the library as they lie in memory: six FP64 fields in the
+// order x, y, vx, vy, q, m (src/pic/particles.hpp:13-20).
+static_assert(sizeof(ParticleAoS) == 6 * sizeof(double), "ParticleAoS must be six packed doubles");
+static_assert(offsetof(ParticleAoS, x) == 0 && offsetof(ParticleAoS, y) == 8 && offsetof(ParticleAoS, vx) == 16 &&
+                  offsetof(ParticleAoS, vy) == 24 && offsetof(ParticleAoS, q) == 32 && offsetof(ParticleAoS, m) == 40,
+              "ParticleAoS field order");
+
 void depositChargeHipAoS(const ParticlesAoS& particles, FieldGrid& grid, const DepositionConfig& config) {
-    // The reference's AoS device path converts too (deposition_cuda.cu:330-333).
-    const ParticlesSoA soa = toSoA(particles);
-    depositChargeHipSoA(soa, grid, config);
+    // The reference's AoS device path builds an FP64 SoA copy first
+    // (deposition_cuda.cu:330-333); here the records are rounded to FP32 straight
+    // from the caller's array.
+    const Domain& d = grid.domain();
+    const pic_domain pd{d.Lx, d.Ly, d.Nx, d.Ny};
+    const double* rec = particles.size() ? &particles.data()[0].x : nullptr;
+    check(pic_hip_deposit_host_aos(rec, particles.size(), &pd, schemeCode(config.scheme), backendCode(config.backend),
+                                   config.sorted ? 1 : 0, config.esirkepov_dt, grid.rho().data()));
 }
 
 }  // namespace pic

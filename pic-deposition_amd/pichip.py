@@ -136,6 +136,8 @@ _SIGS = {
     "pic_hip_deposit_host": (ctypes.c_int, [_f64p, _f64p, _f64p, _f64p, _f64p, ctypes.c_size_t,
                                             ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_double, _f64p]),
+    "pic_hip_deposit_host_aos": (ctypes.c_int, [_f64p, ctypes.c_size_t, ctypes.POINTER(Domain), ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_double, _f64p]),
     "pic_hip_create": (ctypes.c_int, [ctypes.POINTER(Domain), ctypes.c_size_t, ctypes.POINTER(_P)]),
     "pic_hip_destroy": (ctypes.c_int, [_P]),
     "pic_hip_stream": (_P, [_P]),
@@ -149,6 +151,8 @@ _SIGS = {
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pic_hip_upload": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_size_t]),
     "pic_hip_download": (ctypes.c_int, [_P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    "pic_hip_upload_aos": (ctypes.c_int, [_P, _f64p, ctypes.c_size_t]),
+    "pic_hip_download_aos": (ctypes.c_int, [_P, _f64p]),
     "pic_hip_init_particles": (ctypes.c_int, [_P, ctypes.POINTER(InitParams), ctypes.c_size_t]),
     "pic_hip_sort": (ctypes.c_int, [_P]),
     "pic_hip_sort_perm": (ctypes.c_int, [_P, _i32p, _i32p, _i32p]),
@@ -263,6 +267,38 @@ def deposit_host(x, y, q, domain: Domain, scheme=DepositionScheme.CIC, backend=D
     return rho.reshape(domain.Ny, domain.Nx)
 
 
+AOS_FIELDS = ("x", "y", "vx", "vy", "q", "m")  # ParticleAoS, src/pic/particles.hpp:13-20
+
+
+def to_aos(particles: dict) -> np.ndarray:
+    """FP64 records [n, 6] (x, y, vx, vy, q, m) from a dict of columns; missing
+    velocities are 0, a missing mass 1 (ParticleAoS defaults)."""
+    n = len(particles["x"])
+    rec = np.empty((n, 6), dtype=np.float64)
+    for j, k in enumerate(AOS_FIELDS):
+        rec[:, j] = particles[k] if particles.get(k) is not None else (1.0 if k == "m" else 0.0)
+    return rec
+
+
+def _records(records) -> np.ndarray:
+    rec = np.ascontiguousarray(records, dtype=np.float64)
+    if rec.ndim != 2 or rec.shape[1] != 6:
+        raise ValueError("particle records must have shape [n, 6] (x, y, vx, vy, q, m)")
+    if rec.ctypes.data % 16:
+        rec = np.require(np.array(rec, copy=True), requirements=["C", "A"])
+    return rec
+
+
+def deposit_host_aos(records, domain: Domain, scheme=DepositionScheme.CIC, backend=DepositionBackend.GPUAtomics,
+                     sorted_=False, dt=0.0) -> np.ndarray:
+    """pic_hip_deposit_host_aos: FP64 particle records [n, 6] in, FP64 rho[Ny, Nx] out."""
+    rec = _records(records)
+    rho = np.zeros(domain.Nx * domain.Ny, dtype=np.float64)
+    _check(_lib.pic_hip_deposit_host_aos(_ptr(rec, ctypes.c_double), rec.shape[0], ctypes.byref(domain), int(scheme),
+                                         int(backend), int(bool(sorted_)), float(dt), _ptr(rho, ctypes.c_double)))
+    return rho.reshape(domain.Ny, domain.Nx)
+
+
 @dataclass
 class Tiling:
     tile_x: int
@@ -339,6 +375,16 @@ class Context:
         out = [np.empty(n, dtype=np.float32) for _ in range(6)]
         _check(_lib.pic_hip_download(self._h, *[_ptr(a, ctypes.c_float) for a in out]))
         return dict(zip(("x", "y", "vx", "vy", "q", "m"), out))
+
+    def upload_aos(self, records) -> None:
+        """FP64 particle records [n, 6] (x, y, vx, vy, q, m), converted on the device."""
+        rec = _records(records)
+        _check(_lib.pic_hip_upload_aos(self._h, _ptr(rec, ctypes.c_double), rec.shape[0]))
+
+    def download_aos(self) -> np.ndarray:
+        rec = np.empty((self.count, 6), dtype=np.float64)
+        _check(_lib.pic_hip_download_aos(self._h, _ptr(rec, ctypes.c_double)))
+        return rec
 
     def init_particles(self, n: int, kind: int = 0, seed: int = 42, amp: float = 0.0, pert: float = 0.0,
                        vth: float = 0.0, vmax: float = 0.0, mode: int = 1, q: float = 1.0, m: float = 1.0) -> None:
@@ -642,6 +688,17 @@ def deposit_charge(particles: dict, domain: Domain, scheme="CIC", backend="GPU_P
         raise RuntimeError("CPU backend requested from the HIP library; the CPU schemes live in the host code")
     return deposit_host(particles["x"], particles["y"], particles["q"], domain, s, b, sorted_,
                         particles.get("vx"), particles.get("vy"), esirkepov_dt)
+
+
+def deposit_charge_aos(records, domain: Domain, scheme="CIC", backend="GPU_Priv", sorted_=False,
+                       esirkepov_dt=0.0) -> np.ndarray:
+    """GPU branch of depositChargeAoS (deposition.cpp:160-179): FP64 particle
+    records [n, 6] in, rho (FP64) out; the records are not changed."""
+    s = scheme_from_string(scheme) if isinstance(scheme, str) else DepositionScheme(scheme)
+    b = backend_from_string(backend) if isinstance(backend, str) else DepositionBackend(backend)
+    if b == DepositionBackend.CPU:
+        raise RuntimeError("CPU backend requested from the HIP library; the CPU schemes live in the host code")
+    return deposit_host_aos(records, domain, s, b, sorted_, esirkepov_dt)
 
 
 DENOISER_PARAMS = 19105

@@ -109,12 +109,14 @@ extern "C" int seam_run_tests() {
             pic::FieldGrid ref(d);
             pic::depositChargeSoA(p, ref, cpu);
             for (auto backend : backends) {
+                std::vector<double> soa_unsorted;
                 for (int sorted = 0; sorted < 2; ++sorted) {
                     pic::DepositionConfig cfg = cpu;
                     cfg.backend = backend;
                     cfg.sorted = sorted != 0;
                     pic::FieldGrid g(d);
                     pic::depositChargeSoA(p, g, cfg);
+                    if (!sorted) soa_unsorted = g.rho();
                     const double e = relL2(g.rho(), ref.rho());
                     expect(e <= 1e-6, pic::schemeName(scheme) + " SoA " + pic::backendName(backend) +
                                           (sorted ? " sorted" : " unsorted") + " vs CPU rel L2",
@@ -128,6 +130,8 @@ extern "C" int seam_run_tests() {
                 const double e = relL2(g.rho(), ref.rho());
                 expect(e <= 1e-6, pic::schemeName(scheme) + " AoS " + pic::backendName(backend) + " vs CPU rel L2",
                        e);
+                if (backend == pic::DepositionBackend::GPUPrivatized)  // deterministic: the same bits as SoA
+                    expect(g.rho() == soa_unsorted, pic::schemeName(scheme) + " AoS GPU_Priv == SoA bitwise", 0.0);
             }
         }
         // errors surface as std::runtime_error("HIP error: ...")

@@ -41,3 +41,25 @@ def test_matrix_rows(tmp_path, esirkepov):
             writes = {"NGP": 1, "CIC": 4, "TSC": 9, "Esirkepov": 4}[s]
             nbytes = n * 40 + n * writes * 8 + 64 * 64 * 8
             assert float(r["bandwidth_gbs"]) == pytest.approx(nbytes / 1e9 / (float(r["deposit_ms"]) / 1000), rel=1e-6)
+
+
+def test_matrix_aos_rows(tmp_path):
+    """--layouts SoA,AoS: every case twice, the AoS row with the 48-byte
+    record of estimateBytes (deposition.cpp:138)."""
+    import gpu_matrix
+    print("[case] gpu matrix AoS", flush=True)
+    seam, res = tmp_path / "seam.csv", tmp_path / "resident.csv"
+    args = ["--out", str(seam), "--resident", str(res), "--counts", "20000", "--grids", "64", "--repeats", "2",
+            "--layouts", "SoA,AoS"]
+    assert gpu_matrix.main(args) == 0
+    for path in (seam, res):
+        rows = list(csv.DictReader(path.read_text().splitlines()))
+        assert [(r["scheme"], r["layout"], r["backend"]) for r in rows] == [
+            (s, lay, b) for s in ("NGP", "CIC", "TSC") for b in ("GPU_Atomics", "GPU_Priv") for lay in ("SoA", "AoS")
+            if b == "GPU_Atomics" or s == "CIC"]
+        for r in rows:
+            assert float(r["charge_error"]) < 1e-6
+            rec = 48 if r["layout"] == "AoS" else 40
+            writes = {"NGP": 1, "CIC": 4, "TSC": 9}[r["scheme"]]
+            nbytes = 20000 * rec + 20000 * writes * 8 + 64 * 64 * 8
+            assert float(r["bandwidth_gbs"]) == pytest.approx(nbytes / 1e9 / (float(r["deposit_ms"]) / 1000), rel=1e-6)
