@@ -262,6 +262,46 @@ pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nst
 double pic_hip_dominant_frequency(const double* signal, int64_t n, double dt);
 double pic_hip_growth_rate(const double* signal, int64_t n, double dt, int32_t record_start_step);
 double pic_hip_plasma_frequency(const pic_domain* domain, int64_t num_particles);
+/* coldTwoStreamGrowthRate (validation.cpp:73-76): omega_p / (2 sqrt 2).
+ * resonantBeamVelocity (validation.cpp:78-85): omega_p / k, k = 2 pi mode / Lx. */
+double pic_hip_cold_two_stream_growth_rate(const pic_domain* domain, int64_t num_particles);
+double pic_hip_resonant_beam_velocity(const pic_domain* domain, int64_t num_particles, int32_t mode);
+
+/* The reference's particle initialisers (ParticlesSoA::initialize*,
+ * src/pic/particles_aos.cpp:67-180), drawn on the host with the C++ standard
+ * library's std::mt19937(seed), uniform_real_distribution over [0, L) and a
+ * fresh normal_distribution per velocity draw, in the reference's order; FP64
+ * out (the caller rounds to FP32 for pic_hip_upload).  The draws are the
+ * reference's wherever both are built with the same standard library (the
+ * uniform draws with any; libstdc++ here).
+ *   PIC_REF_MAXWELLIAN      x, y uniform; vx, vy ~ N(0, temperature); q = +1, -1
+ *                           alternating; m = 1 (:67-80)
+ *   PIC_REF_LANGMUIR        lattice x = Lx i / n, y = Ly / 2, vx = amplitude
+ *                           sin(2 pi mode x / Lx), vy = 0, q = -1/n (:82-95)
+ *   PIC_REF_WARM_LANGMUIR   the lattice plus N(0, temperature) draws on vx then
+ *                           vy per particle (:97-112)
+ *   PIC_REF_TWO_STREAM      x, y uniform; vx = +-beam_velocity (first half +)
+ *                           + perturbation sin(2 pi x / Lx); q = -1/n (:142-160)
+ *   PIC_REF_TWO_STREAM_1D   as TWO_STREAM with y = Ly / 2, x the only draw
+ *                           (:162-180) */
+enum {
+    PIC_REF_MAXWELLIAN = 0,
+    PIC_REF_LANGMUIR = 1,
+    PIC_REF_WARM_LANGMUIR = 2,
+    PIC_REF_TWO_STREAM = 3,
+    PIC_REF_TWO_STREAM_1D = 4
+};
+typedef struct {
+    int32_t kind;
+    uint32_t seed;
+    double temperature;
+    double amplitude;
+    int32_t mode;
+    double beam_velocity;
+    double perturbation;
+} pic_reference_init;
+pic_status pic_hip_reference_particles(const pic_domain* domain, const pic_reference_init* init, size_t n, double* x,
+                                       double* y, double* vx, double* vy, double* q, double* m);
 
 /* ---- kernel timing (build-defined; no reference counterpart) -------------
  * Between begin and end, every deposit records a pair of HIP events on the

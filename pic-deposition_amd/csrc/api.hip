@@ -10,6 +10,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -937,6 +938,78 @@ double pic_hip_plasma_frequency(const pic_domain* domain, int64_t num_particles)
     const double n0 = static_cast<double>(num_particles) / (domain->Lx * domain->Ly);
     const double q = 1.0 / static_cast<double>(num_particles);
     return std::sqrt(n0 * q * q);
+}
+
+double pic_hip_cold_two_stream_growth_rate(const pic_domain* domain, int64_t num_particles) {
+    return pic_hip_plasma_frequency(domain, num_particles) / (2.0 * std::sqrt(2.0));
+}
+
+double pic_hip_resonant_beam_velocity(const pic_domain* domain, int64_t num_particles, int32_t mode) {
+    if (!domain) return 0.0;
+    const double k = 2.0 * M_PI * static_cast<double>(mode) / domain->Lx;
+    return k < 1e-30 ? 0.0 : pic_hip_plasma_frequency(domain, num_particles) / k;
+}
+
+pic_status pic_hip_reference_particles(const pic_domain* domain, const pic_reference_init* init, size_t n, double* x,
+                                       double* y, double* vx, double* vy, double* q, double* m) {
+    if (!valid_domain(domain)) return fail(PIC_ERR_INVALID_ARGUMENT, "invalid domain");
+    if (!init) return fail(PIC_ERR_INVALID_ARGUMENT, "null init");
+    if (init->kind < PIC_REF_MAXWELLIAN || init->kind > PIC_REF_TWO_STREAM_1D)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "unknown reference initialiser");
+    if (n > 0 && (!x || !y || !vx || !vy || !q || !m)) return fail(PIC_ERR_INVALID_ARGUMENT, "null output array");
+    const double Lx = domain->Lx, Ly = domain->Ly;
+    std::mt19937 gen(init->seed);
+    std::uniform_real_distribution<double> along_x(0.0, Lx), along_y(0.0, Ly);
+    // A new distribution object per draw: no second polar-method value is
+    // carried from one draw to the next.
+    auto gauss = [&gen](double sd) { return std::normal_distribution<double>(0.0, sd)(gen); };
+    const double sd = std::sqrt(init->temperature);
+    const double per_particle = n ? -1.0 / static_cast<double>(n) : 0.0;
+    switch (init->kind) {
+        case PIC_REF_MAXWELLIAN:
+            for (size_t i = 0; i < n; ++i) {
+                x[i] = along_x(gen);
+                y[i] = along_y(gen);
+                vx[i] = gauss(sd);
+                vy[i] = gauss(sd);
+                m[i] = 1.0;
+                q[i] = i % 2 == 0 ? 1.0 : -1.0;
+            }
+            break;
+        case PIC_REF_LANGMUIR:
+        case PIC_REF_WARM_LANGMUIR: {
+            const double k = 2.0 * M_PI * static_cast<double>(init->mode) / Lx;
+            const bool warm = init->kind == PIC_REF_WARM_LANGMUIR;
+            for (size_t i = 0; i < n; ++i) {
+                x[i] = Lx * static_cast<double>(i) / static_cast<double>(n);
+                y[i] = 0.5 * Ly;
+                vx[i] = init->amplitude * std::sin(k * x[i]);
+                vy[i] = 0.0;
+                if (warm) {
+                    vx[i] += gauss(sd);
+                    vy[i] = gauss(sd);
+                }
+                m[i] = 1.0;
+                q[i] = per_particle;
+            }
+            break;
+        }
+        default: {  // the two beams
+            const double k = 2.0 * M_PI / Lx;
+            const bool one_row = init->kind == PIC_REF_TWO_STREAM_1D;
+            for (size_t i = 0; i < n; ++i) {
+                x[i] = along_x(gen);
+                y[i] = one_row ? 0.5 * Ly : along_y(gen);
+                const double beam = i < n / 2 ? init->beam_velocity : -init->beam_velocity;
+                vx[i] = beam + init->perturbation * std::sin(k * x[i]);
+                vy[i] = 0.0;
+                m[i] = 1.0;
+                q[i] = per_particle;
+            }
+            break;
+        }
+    }
+    return PIC_OK;
 }
 
 pic_status pic_hip_esirkepov_current(pic_hip_ctx* ctx, double dt) {

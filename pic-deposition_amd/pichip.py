@@ -108,6 +108,15 @@ class InitParams(ctypes.Structure):
                 ("mode", ctypes.c_int32), ("q", ctypes.c_double), ("m", ctypes.c_double)]
 
 
+class ReferenceInit(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("seed", ctypes.c_uint32), ("temperature", ctypes.c_double),
+                ("amplitude", ctypes.c_double), ("mode", ctypes.c_int32), ("beam_velocity", ctypes.c_double),
+                ("perturbation", ctypes.c_double)]
+
+
+REFERENCE_INIT_KINDS = {"maxwellian": 0, "langmuir": 1, "warm_langmuir": 2, "two_stream": 3, "two_stream_1d": 4}
+
+
 class EnsembleMember(ctypes.Structure):
     _fields_ = [("domain", Domain), ("scheme", ctypes.c_int32), ("num_particles", ctypes.c_int64),
                 ("init", InitParams)]
@@ -138,6 +147,10 @@ _SIGS = {
                                             ctypes.c_double, _f64p]),
     "pic_hip_deposit_host_aos": (ctypes.c_int, [_f64p, ctypes.c_size_t, ctypes.POINTER(Domain), ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, _f64p]),
+    "pic_hip_reference_particles": (ctypes.c_int, [ctypes.POINTER(Domain), ctypes.POINTER(ReferenceInit),
+                                                   ctypes.c_size_t, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
+    "pic_hip_cold_two_stream_growth_rate": (ctypes.c_double, [ctypes.POINTER(Domain), ctypes.c_int64]),
+    "pic_hip_resonant_beam_velocity": (ctypes.c_double, [ctypes.POINTER(Domain), ctypes.c_int64, ctypes.c_int32]),
     "pic_hip_create": (ctypes.c_int, [ctypes.POINTER(Domain), ctypes.c_size_t, ctypes.POINTER(_P)]),
     "pic_hip_destroy": (ctypes.c_int, [_P]),
     "pic_hip_stream": (_P, [_P]),
@@ -676,6 +689,27 @@ def growth_rate(signal, dt: float, record_start: int) -> float:
 
 def plasma_frequency(domain: Domain, n: int) -> float:
     return _lib.pic_hip_plasma_frequency(ctypes.byref(domain), int(n))
+
+
+def cold_two_stream_growth_rate(domain: Domain, n: int) -> float:
+    return _lib.pic_hip_cold_two_stream_growth_rate(ctypes.byref(domain), int(n))
+
+
+def resonant_beam_velocity(domain: Domain, n: int, mode: int = 1) -> float:
+    return _lib.pic_hip_resonant_beam_velocity(ctypes.byref(domain), int(n), int(mode))
+
+
+def reference_particles(kind: str, n: int, domain: Domain, seed: int = 42, temperature: float = 1.0,
+                        amplitude: float = 0.0, mode: int = 1, beam_velocity: float = 0.0,
+                        perturbation: float = 0.0) -> dict:
+    """The reference's initialisers (particles_aos.cpp:67-180), FP64 columns
+    x, y, vx, vy, q, m; kind in REFERENCE_INIT_KINDS."""
+    p = ReferenceInit(REFERENCE_INIT_KINDS[kind], int(seed), float(temperature), float(amplitude), int(mode),
+                      float(beam_velocity), float(perturbation))
+    cols = [np.empty(n, dtype=np.float64) for _ in range(6)]
+    _check(_lib.pic_hip_reference_particles(ctypes.byref(domain), ctypes.byref(p), int(n),
+                                            *[_ptr(a, ctypes.c_double) for a in cols]))
+    return dict(zip(("x", "y", "vx", "vy", "q", "m"), cols))
 
 
 def deposit_charge(particles: dict, domain: Domain, scheme="CIC", backend="GPU_Priv", sorted_=False,

@@ -353,3 +353,19 @@ def test_golden_langmuir_frequency_and_noise(orc):
         assert orc.spectral_noise_level(rho) == pytest.approx(g["spectral_noise"][name], rel=2e-9)
         err = orc.charge_error(orc.integrated_rho(rho, dom), orc.total_charge(m["q"]), orc.total_abs_charge(m["q"]))
         assert err < 1e-16
+
+
+@pytest.mark.parametrize("seed", [43, 44, 45, 46])
+def test_golden_multi_seed_two_stream_growth(orc, seed):
+    """validation_multi_seed.csv two-stream rows (runMultiSeedValidation,
+    physics_studies.cpp:479-569): the FP64 loop reproduces every scheme's
+    growth rate for the seeds beyond 42."""
+    g = _gold("multi_seed")["twostream_growth_rate_measured"][str(seed)]
+    dom = orc.domain(1.0, 1.0, 64, 64)
+    n = 5000
+    p = orc.init("two_stream", n, dom, seed=seed, beam_velocity=orc.resonant_beam_velocity(dom, n, 1),
+                 perturbation=0.01)
+    for scheme, name, order in ((0, "NGP", 0), (1, "CIC", 0), (2, "TSC", 0), (3, "Esirkepov", 2)):
+        _, rec = orc.run_loop(p, dom, scheme, order, 0.002, 1.0, 2000, 500)
+        gamma = orc.growth_rate(rec["mode1"], 0.002, 500) * orc.plasma_frequency(dom, n)
+        assert gamma == pytest.approx(g[name], rel=1e-9), name
