@@ -27,6 +27,14 @@ namespace pichip {
 
 namespace {
 
+// A wave's particles that share a stencil origin and sit in consecutive lanes
+// (cell-sorted input: a wave spans one to a few cells) are summed with a
+// segmented DPP scan and added by the run's last lane: one FP64 atomic per
+// node and run instead of one per particle, which on sorted input would all
+// hit the same few nodes and serialise.  Waves with many short runs
+// (unordered input) add particle by particle.
+constexpr int kAtomicMaxRuns = 16;
+
 template <int S>
 __global__ __launch_bounds__(256) void deposit_atomic_kernel(const float* __restrict__ px, const float* __restrict__ py,
                                                              const float* __restrict__ pvx,
@@ -34,11 +42,50 @@ __global__ __launch_bounds__(256) void deposit_atomic_kernel(const float* __rest
                                                              const float* __restrict__ pq, int64_t n, DevDomain d,
                                                              float dt, double* __restrict__ gacc,
                                                              int* __restrict__ status) {
+    using St = Stencil<S>;
+    constexpr int V = St::V;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float vx = Stencil<S>::kNeedsVelocity ? pvx[i] : 0.0f;
-    const float vy = Stencil<S>::kNeedsVelocity ? pvy[i] : 0.0f;
-    scatter_global<S>(px[i], py[i], vx, vy, pq[i] * d.inv_vol, dt, d, gacc, status);
+    const bool live = i < n;  // every lane stays for the wave-wide scan
+    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f, qv = 0.0f;
+    if (live) {
+        x = px[i];
+        y = py[i];
+        if (St::kNeedsVelocity) {
+            vx = pvx[i];
+            vy = pvy[i];
+        }
+        qv = pq[i] * d.inv_vol;
+    }
+    float w[V];
+    int cx = 0, cy = 0;
+    const bool ok = live && St::weights(x, y, vx, vy, qv, dt, d, cx, cy, w);
+    // Only the Esirkepov density can fail the fixed stencil (long moves).
+    if (live && !ok) scatter_esk_long(x, y, vx, vy, qv, dt, d, gacc, status);
+    const int key = ok ? wrap_index(cy, d.Ny) * d.Nx + wrap_index(cx, d.Nx) : -1;
+    const bool cont = key >= 0 && key == dpp_i<kWaveShr1, 0xf>(-1, key);
+    const int runs = __popcll(__ballot(key >= 0 && !cont));
+    if (runs > kAtomicMaxRuns) {  // wave-uniform
+        if (ok) {
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+                if (w[k] != 0.0f)
+                    acc_add(gacc, (size_t)wrap_index(cy + St::OY + k / St::SX, d.Ny) * d.Nx +
+                                      wrap_index(cx + St::OX + k % St::SX, d.Nx), w[k]);
+        }
+        return;
+    }
+    float z[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) z[k] = ok ? w[k] : 0.0f;
+    wave_segmented_scan<V>(z, cont ? 1.0f : 0.0f);
+    const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
+    if (key >= 0 && !cont_next) {
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+            if (z[k] != 0.0f)
+                acc_add(gacc, (size_t)wrap_index(cy + St::OY + k / St::SX, d.Ny) * d.Nx +
+                                  wrap_index(cx + St::OX + k % St::SX, d.Nx), z[k]);
+    }
 }
 
 // rho = FP32(acc); acc = 0 (closes the FP64 atomic sums).
