@@ -386,9 +386,8 @@ void swap_permuted(pic_hip_ctx* ctx) {
 // Buffers of the staged permutation (outside stream capture).
 pic_status ensure_sort_staging(pic_hip_ctx* ctx) {
     if (ctx->sort_rec) return PIC_OK;
-    PIC_HIP_TRY(dmalloc(&ctx->far_count, 1));
+    PIC_HIP_TRY(dmalloc(&ctx->far_part, kFarParts));
     PIC_HIP_TRY(dmalloc(&ctx->permute_mode, 1));
-    PIC_HIP_TRY(hipMemsetAsync(ctx->far_count, 0, sizeof(unsigned), ctx->stream));
     PIC_HIP_TRY(hipMemsetAsync(ctx->permute_mode, 0, sizeof(int), ctx->stream));
     PIC_HIP_TRY(dmalloc(&ctx->sort_rec, 2 * ctx->cap));
     return PIC_OK;
@@ -408,7 +407,7 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
         // The device decides per sort between a direct gather of each stream
         // (an order close to the last sort's) and packed records (an order far
         // from it: one or two lines per particle instead of one per stream).
-        launch_keys_far(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, 4 * ctx->dom.Nx, ctx->far_count,
+        launch_keys_far(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, 4 * ctx->dom.Nx, ctx->far_part,
                         ctx->permute_mode, ctx->stream);
         launch_pack_streams(fwd, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
         PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));

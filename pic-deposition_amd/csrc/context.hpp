@@ -39,9 +39,10 @@ struct pic_hip_ctx {
     bool uniform[kStreams] = {};
     // Sort of a far-from-sorted order: the permuted streams go through packed
     // records (sort_rec, 2 float4 per particle) chosen on the device per sort
-    // (far_count, permute_mode); allocated before the first sort that can use it.
+    // (far_part, per-block counts of far moves, and permute_mode); allocated before
+    // the first sort that can use it.
     float4* sort_rec = nullptr;
-    unsigned* far_count = nullptr;
+    unsigned* far_part = nullptr;  // kFarParts
     int* permute_mode = nullptr;
     SortBuffers sb{};
     bool order_valid = false;
@@ -132,7 +133,7 @@ struct pic_hip_ctx {
         (void)hipFree(tw_x);
         (void)hipFree(tw_y);
         (void)hipFree(sort_rec);
-        (void)hipFree(far_count);
+        (void)hipFree(far_part);
         (void)hipFree(permute_mode);
         (void)hipHostFree(h_status);
         (void)hipHostFree(h_diag);

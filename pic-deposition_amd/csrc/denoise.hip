@@ -495,29 +495,27 @@ template <int CIN, int NT, int MODE>
 int conv_launch(const void* in, const void* wpk, const float* bias, void* out, const void* mask, const CnnGeom& g,
                 int dil, int ncu, hipStream_t s, const InNorm& nrm = InNorm{}, const OutStats& ost = OutStats{}) {
     const int64_t rowtiles16 = (int64_t)g.B * g.H * (g.W / 16);
-    // Four 16-pixel tiles per wave when there is enough work, else one (a
-    // single image runs faster with more, shorter waves).
+    // Four 16-pixel tiles per wave when there is enough work.  Otherwise the
+    // 32 -> 32 layers take two (every wave loads the layer's 18 KB of packed
+    // weights, so two tiles halve that traffic: 8.8 -> 7.2 us at 256^2) and
+    // the single-channel layers one (more, shorter waves are faster there).
     const bool wide = (g.W % 64) == 0 && rowtiles16 >= (int64_t)ncu * 64;
-    const int mt = wide ? 4 : 1;
+    const int mt = wide ? 4 : (CIN == 32 && NT == 2 && (g.W % 32) == 0) ? 2 : 1;
     const int64_t tiles = rowtiles16 / mt;
     const int64_t blocks = std::min<int64_t>((tiles + 3) / 4, (int64_t)ncu * 8);
     const dim3 grid((unsigned)std::max<int64_t>(blocks, 1));
     const auto* w = static_cast<const bf16x8*>(wpk);
     const auto* m = static_cast<const __bf16*>(mask);
-    if (wide)
+    if (mt == 4)
         hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 4>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
+                           ost);
+    else if (mt == 2)
+        hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 2>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
                            ost);
     else
         hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 1>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
                            ost);
     return (int)grid.x;
-}
-
-int conv_blocks(const CnnGeom& g, int ncu) {
-    const int64_t rowtiles16 = (int64_t)g.B * g.H * (g.W / 16);
-    const bool wide = (g.W % 64) == 0 && rowtiles16 >= (int64_t)ncu * 64;
-    const int64_t tiles = rowtiles16 / (wide ? 4 : 1);
-    return (int)std::max<int64_t>(std::min<int64_t>((tiles + 3) / 4, (int64_t)ncu * 8), 1);
 }
 
 }  // namespace

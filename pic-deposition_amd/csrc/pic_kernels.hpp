@@ -99,8 +99,9 @@ void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hip
 // 1 when more than a quarter moved far (then the permutation goes through
 // launch_pack_streams before the radix passes and launch_gather_unpack after
 // them), else 0 (launch_permute_streams).
+constexpr int kFarParts = 1024;  // blocks (and per-block counts) of launch_keys_far
 void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far,
-                     unsigned* far_count, int* mode, hipStream_t s);
+                     unsigned* far_part, int* mode, hipStream_t s);
 // rec: n (up to 4 streams) or 2n (5 or 6 streams) float4 records.
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s);
 void launch_gather_unpack(const StreamSet& ss, const int* perm, int64_t n, const int* mode, const float4* rec,

@@ -1,6 +1,6 @@
 // Spectral Poisson solve in three launches for power-of-two grids up to 1024
-// cells per axis (Nx >= 4, Ny >= 1), with every 1D transform done by one
-// wavefront in LDS.
+// cells per axis (Nx >= 4, Ny >= 1), with every 1D transform done in LDS by
+// one workgroup or one wavefront.
 //
 // The solve is the one of poisson_fft.cpp:39-66 (rho * cv with the mean
 // removed, phi_hat = rho_hat / (k^2 Nx Ny), k = 0 zeroed) followed by the
@@ -11,9 +11,10 @@
 //    imaginary part of one complex transform of length Nx; the two half
 //    spectra are separated with the conjugate symmetry of real input (a grid
 //    of one row, the quasi-1D geometry, transforms it alone).
-//  * poisson_cols_kernel: one half-spectrum column: forward transform along y,
-//    the Green's function and the difference symbols, then inverse transforms
-//    along y of phi, Ex and Ey, written back as row half spectra.
+//  * poisson_cols_kernel: one half-spectrum column: forward transform along y
+//    and the Green's function and difference symbols by three wavefronts
+//    together, then the inverse transforms along y of phi, Ex and Ey, one
+//    wavefront each, written back as row half spectra.
 //  * poisson_rows_inv_kernel: one row: the half spectra of two real outputs X
 //    and Y combined into the full spectrum of X + iY, one inverse transform,
 //    real part to X and imaginary part to Y.  The imaginary parts of the DC and
@@ -36,17 +37,26 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Unnormalised DFT of a[0, L) by the calling wavefront (the only wave of its
-// workgroup), SIGN = -1 forward, +1 inverse.  b is scratch of the same length
-// and tw[m] = e^{2 pi i m / L}.  Returns the buffer that holds the result.
-template <int SIGN>
+// LDS ordering among the lanes of one wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Unnormalised DFT of a[0, L), SIGN = -1 forward, +1 inverse, by the whole
+// workgroup (BLOCK) or by the calling wavefront alone.  b is scratch of the
+// same length and tw[m] = e^{2 pi i m / L}.  Returns the buffer that holds the
+// result.
+template <int SIGN, bool BLOCK = true>
 __device__ float2* wave_fft(float2* a, float2* b, int L, const float2* tw) {
-    const int lane = threadIdx.x;
+    const int lane = BLOCK ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+    const int nthr = BLOCK ? (int)blockDim.x : 64;
     for (int Ls = 1; Ls < L;) {
         const int R = (L / Ls >= 4) ? 4 : 2;
         const int NR = L / R;
         const int tstep = L / (Ls * R);
-        for (int j = lane; j < NR; j += 64) {
+        for (int j = lane; j < NR; j += nthr) {
             const int k = j & (Ls - 1);
             const int o = (j - k) * R + k;
             if (R == 4) {
@@ -83,7 +93,10 @@ __device__ float2* wave_fft(float2* a, float2* b, int L, const float2* tw) {
                 b[o + Ls] = make_float2(x0.x - x1.x, x0.y - x1.y);
             }
         }
-        __syncthreads();
+        if (BLOCK)
+            __syncthreads();
+        else
+            wave_lds_sync();
         float2* t = a;
         a = b;
         b = t;
@@ -111,7 +124,7 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restric
         t[n] = tw[n];
     }
     __syncthreads();
-    const float2* Z = wave_fft<-1>(a, b, W, t);
+    const float2* Z = wave_fft<-1, false>(a, b, W, t);
     const int hx = W / 2 + 1;
     for (int k = lane; k < hx; k += 64) {
         const float2 p = Z[k], m = Z[(W - k) & (W - 1)];
@@ -121,20 +134,27 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restric
     }
 }
 
-__global__ __launch_bounds__(64) void poisson_cols_kernel(float2* __restrict__ spec, float2* __restrict__ sex,
-                                                          float2* __restrict__ sey, PoissonTables tb, DevDomain d,
-                                                          const float2* __restrict__ tw) {
+// Three wavefronts per column: the forward transform and the Green's function
+// by all of them, then one inverse transform each (phi, Ex, Ey) side by side.
+constexpr int kColsThreads = 192;
+
+__global__ __launch_bounds__(kColsThreads) void poisson_cols_kernel(float2* __restrict__ spec,
+                                                                    float2* __restrict__ sex,
+                                                                    float2* __restrict__ sey, PoissonTables tb,
+                                                                    DevDomain d, const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
     const int H = d.Ny;
     float2* a = sm;
     float2* b = sm + H;
     float2* ex = sm + 2 * H;
     float2* ey = sm + 3 * H;
-    float2* t = sm + 4 * H;
-    const int lane = threadIdx.x;
+    float2* s1 = sm + 4 * H;
+    float2* s2 = sm + 5 * H;
+    float2* t = sm + 6 * H;
+    const int tid = threadIdx.x;
     const int hx = d.Nx / 2 + 1;
     const int kx = blockIdx.x;
-    for (int y = lane; y < H; y += 64) {
+    for (int y = tid; y < H; y += kColsThreads) {
         a[y] = spec[(size_t)y * hx + kx];
         t[y] = tw[y];
     }
@@ -142,7 +162,7 @@ __global__ __launch_bounds__(64) void poisson_cols_kernel(float2* __restrict__ s
     float2* F = wave_fft<-1>(a, b, H, t);
     float2* S = F == a ? b : a;  // free buffer
     const float2 sx = tb.dx_sym[kx];
-    for (int ky = lane; ky < H; ky += 64) {
+    for (int ky = tid; ky < H; ky += kColsThreads) {
         float2 p = make_float2(0.0f, 0.0f);
         if (kx != 0 || ky != 0) {
             const float f = (float)(1.0 / ((tb.k2x[kx] + tb.k2y[ky]) * (double)d.Nx * (double)d.Ny));
@@ -155,16 +175,17 @@ __global__ __launch_bounds__(64) void poisson_cols_kernel(float2* __restrict__ s
         ey[ky] = make_float2(-(sy.x * p.x - sy.y * p.y), -(sy.x * p.y + sy.y * p.x));
     }
     __syncthreads();
-    const float2* P = wave_fft<1>(F, S, H, t);
-    for (int y = lane; y < H; y += 64) spec[(size_t)y * hx + kx] = P[y];
-    float2* scratch = P == F ? S : F;
-    __syncthreads();
-    const float2* X = wave_fft<1>(ex, scratch, H, t);
-    for (int y = lane; y < H; y += 64) sex[(size_t)y * hx + kx] = X[y];
-    float2* scratch2 = X == ex ? scratch : ex;
-    __syncthreads();
-    const float2* Y = wave_fft<1>(ey, scratch2, H, t);
-    for (int y = lane; y < H; y += 64) sey[(size_t)y * hx + kx] = Y[y];
+    const int wv = tid >> 6, lane = tid & 63;  // wave-uniform branches below
+    if (wv == 0) {
+        const float2* P = wave_fft<1, false>(F, S, H, t);
+        for (int y = lane; y < H; y += 64) spec[(size_t)y * hx + kx] = P[y];
+    } else if (wv == 1) {
+        const float2* X = wave_fft<1, false>(ex, s1, H, t);
+        for (int y = lane; y < H; y += 64) sex[(size_t)y * hx + kx] = X[y];
+    } else {
+        const float2* Y = wave_fft<1, false>(ey, s2, H, t);
+        for (int y = lane; y < H; y += 64) sey[(size_t)y * hx + kx] = Y[y];
+    }
 }
 
 __global__ __launch_bounds__(64) void poisson_rows_inv_kernel(const float2* __restrict__ X,
@@ -192,7 +213,7 @@ __global__ __launch_bounds__(64) void poisson_rows_inv_kernel(const float2* __re
     }
     for (int n = lane; n < W; n += 64) t[n] = tw[n];
     __syncthreads();
-    const float2* z = wave_fft<1>(a, b, W, t);
+    const float2* z = wave_fft<1, false>(a, b, W, t);
     for (int n = lane; n < W; n += 64) {
         const float2 v = z[n];
         ox[row * W + n] = v.x;
@@ -213,8 +234,8 @@ void launch_poisson_lds(const float* rho, int64_t G, double bg, double cv, const
     const int W = d.Nx, H = d.Ny;
     hipLaunchKernelGGL(poisson_rows_kernel, dim3((unsigned)((H + 1) / 2)), dim3(64), (size_t)3 * W * sizeof(float2),
                        s, rho, spec, W, H, G, bg, cv, rho_sum, sum_scale, sum_add, tw_x);
-    hipLaunchKernelGGL(poisson_cols_kernel, dim3((unsigned)(W / 2 + 1)), dim3(64), (size_t)5 * H * sizeof(float2), s,
-                       spec, spec_ex, spec_ey, t, d, tw_y);
+    hipLaunchKernelGGL(poisson_cols_kernel, dim3((unsigned)(W / 2 + 1)), dim3(kColsThreads),
+                       (size_t)7 * H * sizeof(float2), s, spec, spec_ex, spec_ey, t, d, tw_y);
 }
 
 void launch_poisson_lds_rows(const float2* X, const float2* Y, float* ox, float* oy, const DevDomain& d,

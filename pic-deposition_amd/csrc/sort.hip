@@ -214,13 +214,13 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ s
     if (i < n) dst[i] = src[perm[i]];
 }
 
-// Keys of the current order, and the number of particles whose key moved by
-// more than `far` from the key at the same position after the last sort
-// (`keys` holds those on entry).  Many far moves mean the permutation will be
-// far from the identity.
+// Keys of the current order, and per block the number of particles whose key
+// moved by more than `far` from the key at the same position after the last
+// sort (`keys` holds those on entry).  Many far moves mean the permutation
+// will be far from the identity.
 __global__ __launch_bounds__(256) void keys_far_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                        int64_t n, DevDomain d, int* __restrict__ keys, int far,
-                                                       unsigned* __restrict__ far_count) {
+                                                       unsigned* __restrict__ far_part) {
     __shared__ unsigned wave_far[4];
     unsigned cnt = 0;  // wave total, kept by every lane
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -238,19 +238,24 @@ __global__ __launch_bounds__(256) void keys_far_kernel(const float* __restrict__
     }
     if ((threadIdx.x & 63) == 0) wave_far[threadIdx.x >> 6] = cnt;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned c = wave_far[0] + wave_far[1] + wave_far[2] + wave_far[3];
-        if (c) atomicAdd(far_count, c);  // one atomic per block
-    }
+    if (threadIdx.x == 0) far_part[blockIdx.x] = wave_far[0] + wave_far[1] + wave_far[2] + wave_far[3];
 }
 
 // Permutation mode of this sort: 1 (staged gather through packed records)
-// when more than a quarter of the particles moved far, else 0; clears the
-// count for the next sort.
-__global__ void permute_mode_kernel(unsigned* __restrict__ far_count, int64_t n, int* __restrict__ mode) {
-    const unsigned c = *far_count;
-    *mode = (int64_t)c > n / 4 ? 1 : 0;
-    *far_count = 0;
+// when more than a quarter of the particles moved far, else 0.  One block
+// sums the `nparts` per-block counts.
+__global__ __launch_bounds__(256) void permute_mode_kernel(const unsigned* __restrict__ far_part, int nparts,
+                                                           int64_t n, int* __restrict__ mode) {
+    __shared__ unsigned long long sh[256];
+    unsigned long long c = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) c += far_part[i];
+    sh[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *mode = (int64_t)sh[0] > n / 4 ? 1 : 0;
 }
 
 // Mode 1: the permuted streams of particle i packed into one or two float4
@@ -378,11 +383,11 @@ void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hip
 }
 
 void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far,
-                     unsigned* far_count, int* mode, hipStream_t s) {
+                     unsigned* far_part, int* mode, hipStream_t s) {
+    const int blocks = (int)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), kFarParts);
     if (n > 0)
-        hipLaunchKernelGGL(keys_far_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0,
-                           s, x, y, n, d, keys, far, far_count);
-    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(1), 0, s, far_count, n, mode);
+        hipLaunchKernelGGL(keys_far_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, y, n, d, keys, far, far_part);
+    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, far_part, n > 0 ? blocks : 0, n, mode);
 }
 
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s) {
