@@ -168,17 +168,19 @@ def test_drifted_particles_take_overflow_path(hip, orc, scheme):
     (3, (16, 16, 8, 1), (48, 80)),
     (2, (64, 8, 4, 0), (64, 64)),
 ])
+@pytest.mark.parametrize("ppc", [24, 6])
 @pytest.mark.parametrize("scheme", SCHEMES)
-def test_drift_halo_windows(hip, orc, scheme, halo, tiling, shape):
+def test_drift_halo_windows(hip, orc, scheme, halo, tiling, shape, ppc):
     """Tile windows widened by a drift halo take particles that moved up to
     `halo` cells since the sort (the rest go to the overflow path): exact
     against the oracle, and a second deposit agrees to the last bits (the
-    overflow path's compensated atomics land in scheduling order)."""
-    say("halo", scheme, halo, tiling, shape)
+    overflow path's compensated atomics land in scheduling order).  24 per
+    cell runs the lookahead chunk loop, 6 per cell the plain one."""
+    say("halo", scheme, halo, tiling, shape, ppc)
     Nx, Ny = shape
     dom = hip.make_domain(1.0, 1.0, Nx, Ny)
     dt = 0.01
-    x, y, q, vx, vy = uniform_particles(Nx * Ny * 24, seed=21, vmax=(halo + 1.5) / max(Nx, Ny) / dt)
+    x, y, q, vx, vy = uniform_particles(Nx * Ny * ppc, seed=21, vmax=(halo + 1.5) / max(Nx, Ny) / dt)
     with hip.Context(dom, len(x)) as c:
         c.upload(x, y, q, vx, vy)
         c.sort()
