@@ -163,19 +163,19 @@ struct Stencil<CIC> {
 // B-spline values are the same polynomials re-expanded around f, which keeps
 // every subtraction exact in FP32.  Slots cover node offsets -1..2 per axis.
 __device__ __forceinline__ void tsc_1d(float f, float (&w)[4]) {
-    if (f >= 0.5f) {
-        const float a = 1.5f - f, b = f - 1.0f, c = f - 0.5f;
-        w[0] = 0.0f;
-        w[1] = 0.5f * a * a;
-        w[2] = 0.75f - b * b;
-        w[3] = 0.5f * c * c;
-    } else {
-        const float a = 0.5f - f, c = f + 0.5f;
-        w[0] = 0.5f * a * a;
-        w[1] = 0.75f - f * f;
-        w[2] = 0.5f * c * c;
-        w[3] = 0.0f;
-    }
+    // f >= 0.5: a = 1.5 - f, b = f - 1, c = f - 0.5 into slots 1..3;
+    // f <  0.5: a = 0.5 - f, b = f,     c = f + 0.5 into slots 0..2.
+    // One set of operations with per-case constants (the same FP32 operations
+    // as either case alone), then the slot placement.
+    const bool hi = f >= 0.5f;
+    const float a = (hi ? 1.5f : 0.5f) - f;
+    const float b = f - (hi ? 1.0f : 0.0f);
+    const float c = f - (hi ? 0.5f : -0.5f);
+    const float w0 = 0.5f * a * a, w1 = 0.75f - b * b, w2 = 0.5f * c * c;
+    w[0] = hi ? 0.0f : w0;
+    w[1] = hi ? w0 : w1;
+    w[2] = hi ? w1 : w2;
+    w[3] = hi ? w2 : 0.0f;
 }
 
 template <>
@@ -218,40 +218,27 @@ template <int EX = -1>
 __device__ __forceinline__ bool esk_1d_from(float fr0, float v, float dt, float inv_d, double dd, int exact,
                                             float (&w)[3]) {
     const float disp = v * dt;
-    w[0] = w[1] = w[2] = 0.0f;
     // The reference tests |x1 - x0| < 1e-15 in FP64; the displacement itself
-    // is the same quantity without the cancellation.
-    if (fabsf(disp) < 1e-15f) {
-        w[1] = 1.0f - fr0;
-        w[2] = fr0;
-        return true;
-    }
+    // is the same quantity without the cancellation.  A static particle gets
+    // the CIC pair.
+    const bool still = fabsf(disp) < 1e-15f;
+    // A moving particle: wa = path inside the home cell up to the face it
+    // heads for; a move past that face adds wb in the neighbour cell, and the
+    // pair is renormalised.  Both directions and both outcomes are evaluated
+    // with the same FP32 operations as the case alone and then selected.
     const float dc = cells<EX>(disp, inv_d, dd, exact);
-    if (dc > 0.0f) {
-        const float wa = 1.0f - fr0;  // path inside the home cell up to its upper face
-        if (dc <= wa) {
-            w[1] = 1.0f;
-            return true;
-        }
-        const float wb = dc - wa;
-        if (wb > 1.0f) return false;
-        const float inv = 1.0f / (wa + wb);
-        w[1] = wa * inv;
-        w[2] = wb * inv;
-        return true;
-    }
-    const float adc = -dc;
-    const float wa = fr0;  // path inside the home cell down to its lower face
-    if (adc <= wa) {
-        w[1] = 1.0f;
-        return true;
-    }
+    const bool up = dc > 0.0f;
+    const float adc = up ? dc : -dc;
+    const float wa = up ? 1.0f - fr0 : fr0;
+    const bool cross = !(adc <= wa);
     const float wb = adc - wa;
-    if (wb > 1.0f) return false;
     const float inv = 1.0f / (wa + wb);
-    w[1] = wa * inv;
-    w[0] = wb * inv;
-    return true;
+    const float wh = cross ? wa * inv : 1.0f;
+    const float wn = cross ? wb * inv : 0.0f;
+    w[0] = still || up ? 0.0f : wn;
+    w[1] = still ? 1.0f - fr0 : wh;
+    w[2] = still ? fr0 : (up ? wn : 0.0f);
+    return still || !(cross && wb > 1.0f);
 }
 
 template <>
