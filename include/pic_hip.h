@@ -175,6 +175,14 @@ pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* param
  * record staging: a permutation far from the identity moves each particle's
  * streams as one or two 16-byte records (decided on the device per sort). */
 pic_status pic_hip_sort(pic_hip_ctx* ctx);
+/* Sort by nearest node instead of by cell (key floor(x/dx) + [fraction >= 1/2],
+ * wrapped; build-defined, no reference counterpart).  On this order the
+ * GPU_PRIV NGP and TSC deposits run node-centred stencils of 1 and 9 slots
+ * instead of 4 and 16 (same weights, same results to FP32 rounding); CIC and
+ * Esirkepov deposits stay exact on it through their windows and overflow path
+ * but prefer pic_hip_sort's order.  pic_hip_run sorts NGP and TSC runs this way;
+ * pic_hip_deposit_host does for GPU_PRIV NGP and TSC. */
+pic_status pic_hip_sort_nodes(pic_hip_ctx* ctx);
 pic_status pic_hip_sort_perm(pic_hip_ctx* ctx, int32_t* keys_sorted, int32_t* perm, int32_t* cell_offsets);
 /* Keys of the current order (synchronous copy out). */
 pic_status pic_hip_get_keys(pic_hip_ctx* ctx, int32_t* keys);

@@ -393,7 +393,12 @@ pic_status ensure_sort_staging(pic_hip_ctx* ctx) {
     return PIC_OK;
 }
 
-pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
+// Stencils whose tile kernel prefers the node-keyed order (a 9-slot TSC and
+// a 1-slot NGP instead of 16 and 4 slots keyed by cell).
+bool prefers_node_order(int scheme) { return scheme == NGP || scheme == TSC; }
+
+// Sorts by cell key (particleCellIndex) or, with `node`, by nearest node.
+pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back, bool node) {
     StreamSet fwd{}, back{};
     for (int i = 0; i < kStreams; ++i) {
         if (ctx->uniform[i]) continue;  // every particle holds the same value
@@ -407,14 +412,15 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
         // The device decides per sort between a direct gather of each stream
         // (an order close to the last sort's) and packed records (an order far
         // from it: one or two lines per particle instead of one per stream).
-        launch_keys_far(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, 4 * ctx->dom.Nx, ctx->far_part,
+        launch_keys_far(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, 4 * ctx->dom.Nx, node, ctx->far_part,
                         ctx->permute_mode, ctx->stream);
         launch_pack_streams(fwd, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
         PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));
         launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream, ctx->permute_mode);
         launch_gather_unpack(fwd, ctx->sb.perm, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
     } else {
-        PIC_HIP_TRY(launch_sort_keys_perm(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb, ctx->stream));
+        launch_keys(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, ctx->stream, node);
+        PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));
         launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream);
     }
     if (copy_back)
@@ -423,6 +429,7 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back) {
         swap_permuted(ctx);
     PIC_HIP_TRY(hipGetLastError());
     ctx->order_valid = true;
+    ctx->order_node = node;
     return PIC_OK;
 }
 
@@ -434,7 +441,14 @@ pic_status pic_hip_sort(pic_hip_ctx* ctx) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     const pic_status st = ensure_sort_staging(ctx);
     if (st != PIC_OK) return st;
-    return sort_particles(ctx, false);
+    return sort_particles(ctx, false, false);
+}
+
+pic_status pic_hip_sort_nodes(pic_hip_ctx* ctx) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    const pic_status st = ensure_sort_staging(ctx);
+    if (st != PIC_OK) return st;
+    return sort_particles(ctx, false, true);
 }
 
 pic_status pic_hip_sort_perm(pic_hip_ctx* ctx, int32_t* keys_sorted, int32_t* perm, int32_t* cell_offsets) {
@@ -481,7 +495,11 @@ pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt)
     a.n = (int64_t)ctx->n;
     a.cell_off = ctx->sb.cell_off;
     a.dom = ctx->dev;
-    a.scheme = scheme;
+    // On the node-keyed order NGP and TSC run their node-centred stencils
+    // (same tiling and windows as the cell-keyed kernels).
+    a.scheme = backend == PIC_BACKEND_GPU_PRIV && ctx->order_node && prefers_node_order(scheme)
+                   ? (scheme == NGP ? NGP_NODE : TSC_NODE)
+                   : scheme;
     a.backend = backend;
     a.ppl = ctx->ppl[scheme];
     a.pipe = ctx->pipe[scheme];
@@ -709,7 +727,7 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
     };
     auto deposit = [&]() -> pic_status {
         if (with_sort) {
-            pic_status s2 = sort_particles(ctx, copy_back);
+            pic_status s2 = sort_particles(ctx, copy_back, prefers_node_order(cfg.scheme));
             if (s2 != PIC_OK) return s2;
         }
         pic_status s3 = pic_hip_deposit(ctx, cfg.scheme, cfg.backend, cfg.dt);
@@ -1244,7 +1262,10 @@ static pic_status seam_deposit(const double* const src_in[5], size_t stride, siz
     c->order_valid = false;
     pic_status st = PIC_OK;
     if (sorted || backend == PIC_BACKEND_GPU_PRIV) {
-        st = pic_hip_sort(c);  // the caller's order: usually far from sorted
+        // The caller's order: usually far from sorted.  The tiled NGP and TSC
+        // kernels take the node-keyed order.
+        if ((st = ensure_sort_staging(c)) != PIC_OK) return st;
+        st = sort_particles(c, false, backend == PIC_BACKEND_GPU_PRIV && prefers_node_order(scheme));
         if (st != PIC_OK) return st;
     }
     st = pic_hip_deposit(c, scheme, backend, dt);

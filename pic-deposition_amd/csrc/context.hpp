@@ -46,6 +46,7 @@ struct pic_hip_ctx {
     int* permute_mode = nullptr;
     SortBuffers sb{};
     bool order_valid = false;
+    bool order_node = false;  // the valid order is the node-keyed one (pic_hip_sort_nodes)
     float* rho = nullptr;
     double* acc64 = nullptr;  // FP64 global-atomic accumulation grid, zero between deposits
     int* status = nullptr;

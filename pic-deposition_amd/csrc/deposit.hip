@@ -274,6 +274,8 @@ hipError_t launch_deposit(const DepositArgs& a) {
             case NGP: launch_tile_k<NGP>(a); break;
             case CIC: launch_tile_k<CIC>(a); break;
             case TSC: launch_tile_k<TSC>(a); break;
+            case NGP_NODE: launch_tile_k<NGP_NODE>(a); break;
+            case TSC_NODE: launch_tile_k<TSC_NODE>(a); break;
             default: launch_tile_k<ESK>(a); break;
         }
     }

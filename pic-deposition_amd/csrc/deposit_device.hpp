@@ -260,11 +260,11 @@ struct TileCtx {
 
 // Window key ((ly << 12) | lx) of a particle's home cell, -1 when the particle
 // is not this lane's (outside its row range) or not in the window.
-template <int EX>
+template <int S, int EX>
 __device__ __forceinline__ int window_key(float xx, float yy, bool valid, const TileCtx& t, const DevDomain& d) {
     int cx, cy;
     float fx, fy;
-    home_cell<EX>(xx, yy, d, cx, cy, fx, fy);
+    Stencil<S>::template home<EX>(xx, yy, d, cx, cy, fx, fy);
     const int lx = cx - t.ax0, ly = cy - t.ay0;
     const bool in_win = (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
     return (valid && in_win) ? ((ly << 12) | lx) : -1;
@@ -303,7 +303,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                 const bool valid = p >= cur.lo[j] && p < cur.hi[j];
                 const float xx = e == 0 ? cur.X[j].x : e == 1 ? cur.X[j].y : e == 2 ? cur.X[j].z : cur.X[j].w;
                 const float yy = e == 0 ? cur.Y[j].x : e == 1 ? cur.Y[j].y : e == 2 ? cur.Y[j].z : cur.Y[j].w;
-                nk[4 * j + e] = window_key<EX>(xx, yy, valid, t, d);
+                nk[4 * j + e] = window_key<S, EX>(xx, yy, valid, t, d);
             }
         }
     }
@@ -330,7 +330,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             // (wrapping) overflow path.
             int cx, cy;
             float fx, fy;
-            home_cell<EX>(xx, yy, d, cx, cy, fx, fy);
+            St::template home<EX>(xx, yy, d, cx, cy, fx, fy);
             const int lx = cx - t.ax0, ly = cy - t.ay0;
             const bool in_win = (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
             const float qv = valid ? qq * d.inv_vol : 0.0f;

@@ -11,6 +11,10 @@
 namespace pichip {
 
 enum Scheme { NGP = 0, CIC = 1, TSC = 2, ESK = 3 };
+// Tile-kernel stencils for the node-keyed order (particles sorted by nearest
+// node instead of by cell): the same NGP / TSC weights on a stencil centred
+// on that node.  Internal kernel variants, not schemes of the C ABI.
+enum NodeScheme { NGP_NODE = 4, TSC_NODE = 5 };
 
 struct DevDomain {
     int Nx, Ny;
@@ -101,6 +105,33 @@ __device__ __forceinline__ void home_cell(float x, float y, const DevDomain& d, 
     cell_coord<EX>(y, d.inv_dy, d.dy, d.exact, cy, fy);
 }
 
+// Nearest node of a position (the node-keyed order's home): floor cell plus
+// one when the in-cell fraction is at least 1/2 (the NGP decision of
+// ngp.cpp:108-120), with the node past the last cell taken as node 0 of the
+// periodic grid.  fx, fy stay the floor-cell fractions.
+template <int EX = -1>
+__device__ __forceinline__ void node_cell(float x, float y, const DevDomain& d, int& nx, int& ny, float& fx,
+                                          float& fy) {
+    int cx, cy;
+    cell_coord<EX>(x, d.inv_dx, d.dx, d.exact, cx, fx);
+    cell_coord<EX>(y, d.inv_dy, d.dy, d.exact, cy, fy);
+    nx = cx + (fx >= 0.5f ? 1 : 0);
+    ny = cy + (fy >= 0.5f ? 1 : 0);
+    if (nx == d.Nx && cx == d.Nx - 1) nx = 0;
+    if (ny == d.Ny && cy == d.Ny - 1) ny = 0;
+}
+
+// Sort key of the node-keyed order: the wrapped nearest node, from the same
+// FP64 division as cell_key_exact.
+__device__ __forceinline__ int node_key_exact(float x, float y, const DevDomain& d) {
+    const double fx = (double)x / d.dx;
+    const double fy = (double)y / d.dy;
+    const double flx = floor(fx), fly = floor(fy);
+    const int ix = wrap_index((int)flx + ((float)(fx - flx) >= 0.5f ? 1 : 0), d.Nx);
+    const int iy = wrap_index((int)fly + ((float)(fy - fly) >= 0.5f ? 1 : 0), d.Ny);
+    return iy * d.Nx + ix;
+}
+
 template <int S>
 struct Stencil;
 
@@ -120,6 +151,12 @@ struct Stencil<NGP> {
         w[2] = (!rx && ry) ? qv : 0.0f;
         w[3] = (rx && ry) ? qv : 0.0f;
         return true;
+    }
+    // Home cell the window keys and the stencil offsets refer to.
+    template <int EX = -1>
+    __device__ __forceinline__ static void home(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                                float& fy) {
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
     }
     template <int EX = -1>
     __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
@@ -147,6 +184,12 @@ struct Stencil<CIC> {
         w[0] = a0 - w[2];
         w[1] = a1 - w[3];
         return true;
+    }
+    // Home cell the window keys and the stencil offsets refer to.
+    template <int EX = -1>
+    __device__ __forceinline__ static void home(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                                float& fy) {
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
     }
     template <int EX = -1>
     __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
@@ -183,6 +226,12 @@ struct Stencil<TSC> {
     static constexpr int SX = 4, SY = 4, V = 16, OX = -1, OY = -1;
     static constexpr bool kNeedsVelocity = false;
     static constexpr bool kAlwaysOk = true;  // fill never rejects a particle
+    // Home cell the window keys and the stencil offsets refer to.
+    template <int EX = -1>
+    __device__ __forceinline__ static void home(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                                float& fy) {
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
+    }
     template <int EX = -1>
     __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
                                                    const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
@@ -246,6 +295,12 @@ struct Stencil<ESK> {
     static constexpr int SX = 3, SY = 3, V = 9, OX = -1, OY = -1;
     static constexpr bool kNeedsVelocity = true;
     static constexpr bool kAlwaysOk = false;  // moves across more than one face take the walk
+    // Home cell the window keys and the stencil offsets refer to.
+    template <int EX = -1>
+    __device__ __forceinline__ static void home(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                                float& fy) {
+        home_cell<EX>(x, y, d, cx, cy, fx, fy);
+    }
     template <int EX = -1>
     __device__ __forceinline__ static bool weights(float x, float y, float vx, float vy, float qv, float dt,
                                                    const DevDomain& d, int& cx, int& cy, float (&w)[V]) {
@@ -266,6 +321,64 @@ struct Stencil<ESK> {
 #pragma unroll
             for (int a = 0; a < 3; ++a) w[b * 3 + a] = wx[a] * wy[b];
         return okx && oky;
+    }
+};
+
+// NGP on the node-keyed order: the home is the nearest node itself.
+template <>
+struct Stencil<NGP_NODE> {
+    static constexpr int SX = 1, SY = 1, V = 1, OX = 0, OY = 0;
+    static constexpr bool kNeedsVelocity = false;
+    static constexpr bool kAlwaysOk = true;
+    template <int EX = -1>
+    __device__ __forceinline__ static void home(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                                float& fy) {
+        node_cell<EX>(x, y, d, cx, cy, fx, fy);
+    }
+    template <int EX = -1>
+    __device__ __forceinline__ static bool fill(float, float, float, float, float qv, float, const DevDomain&,
+                                                float (&w)[V]) {
+        w[0] = qv;
+        return true;
+    }
+};
+
+// TSC on the node-keyed order: nodes n-1, n, n+1 around the nearest node n.
+// With f the floor-cell fraction these are the three non-zero weights of
+// tsc_1d (the same FP32 operations), without the fourth, empty slot.
+__device__ __forceinline__ void tsc_node_1d(float f, float (&w)[3]) {
+    const bool hi = f >= 0.5f;
+    const float a = (hi ? 1.5f : 0.5f) - f;
+    const float b = f - (hi ? 1.0f : 0.0f);
+    const float c = f - (hi ? 0.5f : -0.5f);
+    w[0] = 0.5f * a * a;
+    w[1] = 0.75f - b * b;
+    w[2] = 0.5f * c * c;
+}
+
+template <>
+struct Stencil<TSC_NODE> {
+    static constexpr int SX = 3, SY = 3, V = 9, OX = -1, OY = -1;
+    static constexpr bool kNeedsVelocity = false;
+    static constexpr bool kAlwaysOk = true;
+    template <int EX = -1>
+    __device__ __forceinline__ static void home(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                                float& fy) {
+        node_cell<EX>(x, y, d, cx, cy, fx, fy);
+    }
+    template <int EX = -1>
+    __device__ __forceinline__ static bool fill(float fx, float fy, float, float, float qv, float, const DevDomain&,
+                                                float (&w)[V]) {
+        float wx[3], wy[3];
+        tsc_node_1d(fx, wx);
+        tsc_node_1d(fy, wy);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) wx[a] *= qv;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) w[b * 3 + a] = wx[a] * wy[b];
+        return true;
     }
 };
 

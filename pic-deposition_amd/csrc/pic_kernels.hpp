@@ -100,13 +100,16 @@ void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hip
 // launch_pack_streams before the radix passes and launch_gather_unpack after
 // them), else 0 (launch_permute_streams).
 constexpr int kFarParts = 1024;  // blocks (and per-block counts) of launch_keys_far
-void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far,
+void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far, bool node,
                      unsigned* far_part, int* mode, hipStream_t s);
 // rec: n (up to 4 streams) or 2n (5 or 6 streams) float4 records.
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s);
 void launch_gather_unpack(const StreamSet& ss, const int* perm, int64_t n, const int* mode, const float4* rec,
                           hipStream_t s);
-void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s);
+// Cell keys (particleCellIndex), or with `node` the keys of the node-keyed
+// order (nearest node, node_key_exact).
+void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s,
+                 bool node = false);
 
 // ---- step.hip -------------------------------------------------------------
 // push_flag (device int, may be null) overrides do_push when set.

@@ -21,9 +21,9 @@ constexpr int kDigits = 256;
 constexpr int kWaves = kThreads / 64;
 
 __global__ __launch_bounds__(256) void keys_kernel(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
-                                                   DevDomain d, int* __restrict__ keys) {
+                                                   DevDomain d, int* __restrict__ keys, int node) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = cell_key_exact(x[i], y[i], d);
+    if (i < n) keys[i] = node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d);
 }
 
 // Per-block digit histogram.  Keys re-sorted between steps arrive nearly in
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ s
 // will be far from the identity.
 __global__ __launch_bounds__(256) void keys_far_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                        int64_t n, DevDomain d, int* __restrict__ keys, int far,
-                                                       unsigned* __restrict__ far_part) {
+                                                       int node, unsigned* __restrict__ far_part) {
     __shared__ unsigned wave_far[4];
     unsigned cnt = 0;  // wave total, kept by every lane
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void keys_far_kernel(const float* __restrict__
         bool f = false;
         if (i < n) {
             const int old = keys[i];
-            const int k = cell_key_exact(x[i], y[i], d);
+            const int k = node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d);
             keys[i] = k;
             const long long dk = (long long)k - (long long)old;
             f = dk > far || dk < -far;
@@ -324,9 +324,10 @@ size_t sort_hist_elems(int64_t n) {
 
 size_t sort_scan_tmp_elems(size_t m) { return (m + kBlockItems - 1) / kBlockItems + 1; }
 
-void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s) {
+void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s, bool node) {
     if (n > 0)
-        hipLaunchKernelGGL(keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n, d, keys);
+        hipLaunchKernelGGL(keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n, d, keys,
+                           node ? 1 : 0);
 }
 
 hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s) {
@@ -382,11 +383,12 @@ void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hip
                            mode);
 }
 
-void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far,
+void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far, bool node,
                      unsigned* far_part, int* mode, hipStream_t s) {
     const int blocks = (int)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), kFarParts);
     if (n > 0)
-        hipLaunchKernelGGL(keys_far_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, y, n, d, keys, far, far_part);
+        hipLaunchKernelGGL(keys_far_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, y, n, d, keys, far,
+                           node ? 1 : 0, far_part);
     hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, far_part, n > 0 ? blocks : 0, n, mode);
 }
 

@@ -168,6 +168,7 @@ _SIGS = {
     "pic_hip_download_aos": (ctypes.c_int, [_P, _f64p]),
     "pic_hip_init_particles": (ctypes.c_int, [_P, ctypes.POINTER(InitParams), ctypes.c_size_t]),
     "pic_hip_sort": (ctypes.c_int, [_P]),
+    "pic_hip_sort_nodes": (ctypes.c_int, [_P]),
     "pic_hip_sort_perm": (ctypes.c_int, [_P, _i32p, _i32p, _i32p]),
     "pic_hip_get_keys": (ctypes.c_int, [_P, _i32p]),
     "pic_hip_deposit": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
@@ -406,6 +407,10 @@ class Context:
 
     def sort(self) -> None:
         _check(_lib.pic_hip_sort(self._h))
+
+    def sort_nodes(self) -> None:
+        """pic_hip_sort_nodes: the node-keyed order (NGP / TSC tile kernels)."""
+        _check(_lib.pic_hip_sort_nodes(self._h))
 
     def sort_perm(self):
         n = self.count

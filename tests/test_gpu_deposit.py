@@ -304,3 +304,89 @@ def test_atomic_backend_sorted_single_cell(hip, orc):
         g = gpu_rho_sorted_atomics(hip, dom, x, y, q, z, z, scheme, 0.0)
         r = oracle_rho(orc, dom, x, y, q, z, z, scheme, 0.0)
         assert rel_l2(g, r) <= REL_L2, scheme
+
+
+def node_keys(x, y, nx, ny, Lx=1.0, Ly=1.0):
+    """Keys of the node-keyed order: floor cell + [FP32 fraction >= 1/2], wrapped."""
+    def axis(p, n, L):
+        f = p.astype(np.float64) / (L / n)
+        fl = np.floor(f)
+        up = (f - fl).astype(np.float32) >= np.float32(0.5)
+        return np.mod(fl.astype(np.int64) + up, n)
+    return axis(y, ny, Ly) * nx + axis(x, nx, Lx)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 1.0, 1.0), (100, 50, 1.0, 1.0), (33, 17, 3.7, 1.3), (256, 1, 6.283185307179586, 1.0)])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_node_order_deposit(hip, orc, scheme, shape):
+    """pic_hip_sort_nodes: particles ordered by nearest node; NGP and TSC run
+    their node-centred stencils, CIC and Esirkepov their cell-keyed ones on
+    the same order; all against the oracle."""
+    say("node order", scheme, shape)
+    nx, ny, Lx, Ly = shape
+    dom = hip.make_domain(Lx, Ly, nx, ny)
+    dt = 0.01
+    x, y, q, vx, vy = uniform_particles(nx * ny * 20, L=(Lx, Ly), seed=nx + scheme, vmax=0.4 * Lx / nx / dt)
+    with hip.Context(dom, len(x)) as c:
+        c.upload(x, y, q, vx, vy)
+        c.sort_nodes()
+        c.deposit(scheme, 2, dt)
+        g = c.rho().astype(np.float64)
+        p = c.download()
+        c.sort()
+        c.deposit(scheme, 2, dt)
+        g_cell = c.rho().astype(np.float64)
+    k = node_keys(p["x"], p["y"], nx, ny, Lx, Ly)
+    assert np.all(np.diff(k) >= 0)
+    r = oracle_rho(orc, dom, x, y, q, vx, vy, scheme, dt)
+    assert rel_l2(g, r) <= REL_L2
+    assert rel_l2(g, g_cell) <= REL_L2
+
+
+@pytest.mark.parametrize("scheme", [0, 2])
+def test_node_order_edge_positions(hip, orc, scheme):
+    """Cell midpoints (fraction exactly 1/2), the last half cell (nearest node
+    past the grid: node 0), x = 0 and positions just outside [0, L)."""
+    say("node order edges", scheme)
+    n = 32
+    dom = hip.make_domain(1.0, 1.0, n, n)
+    dx = 1.0 / n
+    xs = np.array([0.0, 0.5 * dx, (n - 0.5) * dx, 1.0 - 2.0 ** -24, 1.0 - 0.25 * dx, -0.25 * dx, 0.999 * dx,
+                   (n - 1) * dx, 3.5 * dx, -0.75 * dx, 1.0 + 0.25 * dx], dtype=np.float32)
+    x = np.repeat(xs, len(xs)).astype(np.float32)
+    y = np.tile(xs, len(xs)).astype(np.float32)
+    q = np.linspace(0.5, 1.5, len(x)).astype(np.float32)
+    z = np.zeros_like(x)
+    with hip.Context(dom, len(x)) as c:
+        c.upload(x, y, q, z, z)
+        c.sort_nodes()
+        c.deposit(scheme, 2, 0.0)
+        g = c.rho().astype(np.float64)
+    r = oracle_rho(orc, dom, x, y, q, z, z, scheme, 0.0)
+    assert rel_l2(g, r) <= REL_L2
+
+
+@pytest.mark.parametrize("ppc", [24, 6])
+@pytest.mark.parametrize("halo,tiling", [(2, (32, 4, 8, 0)), (3, (16, 16, 8, 1)), (4, (3, 3, 4, 0))])
+@pytest.mark.parametrize("scheme", [0, 2])
+def test_node_order_drift_halo(hip, orc, scheme, halo, tiling, ppc):
+    """Node-keyed order, particles pushed after the sort: windows with a drift
+    halo (lookahead at 24 per cell, plain chunks at 6) and the overflow path."""
+    say("node order halo", scheme, halo, tiling, ppc)
+    Nx, Ny = 48, 40
+    dom = hip.make_domain(1.0, 1.0, Nx, Ny)
+    dt = 0.01
+    x, y, q, vx, vy = uniform_particles(Nx * Ny * ppc, seed=7 + halo, vmax=(halo + 1.5) / Nx / dt)
+    with hip.Context(dom, len(x)) as c:
+        c.upload(x, y, q, vx, vy)
+        c.sort_nodes()
+        c.gather_push(dt, gather=False, push=True)
+        c.set_tiling(*tiling, scheme=scheme)
+        c.set_tile_halo(halo)
+        c.deposit(scheme, 2, dt)
+        g = c.rho().astype(np.float64)
+        c.deposit(scheme, 2, dt)
+        assert rel_l2(c.rho(), g) <= 1e-7
+        p = c.download()
+    r = oracle_rho(orc, dom, p["x"], p["y"], p["q"], p["vx"], p["vy"], scheme, dt)
+    assert rel_l2(g, r) <= REL_L2
