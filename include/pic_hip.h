@@ -380,7 +380,8 @@ pic_status pic_hip_ensemble_upload(pic_hip_ensemble* ens, int32_t member, const 
                                    const float* vx, const float* vy, const float* q, const float* m);
 pic_status pic_hip_ensemble_download(pic_hip_ensemble* ens, int32_t member, float* x, float* y, float* vx, float* vy,
                                      float* q, float* m);
-/* Cell sort of all members. */
+/* Sort of all members in one radix sort: CIC and Esirkepov members by cell,
+ * NGP and TSC members by nearest node (the order of pic_hip_sort_nodes). */
 pic_status pic_hip_ensemble_sort(pic_hip_ensemble* ens);
 /* Tiled deposit of every member with its own scheme (needs a sort). */
 pic_status pic_hip_ensemble_deposit(pic_hip_ensemble* ens, double dt);

@@ -71,18 +71,22 @@ __global__ __launch_bounds__(256) void ens_init_kernel(EnsParticles p, EnsTables
     }
 }
 
-// Member-offset cell keys: cbase + particleCellIndex; padding slots get the
-// member's last key cbase + G, so a stable sort keeps them at its end.
+// Member-offset keys: cbase + particleCellIndex, or for NGP and TSC members
+// cbase + nearest node (the node-keyed order their tile kernels take);
+// padding slots get the member's last key cbase + G, so a stable sort keeps
+// them at its end.
 __global__ __launch_bounds__(256) void ens_keys_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                        EnsTables t, int* __restrict__ keys) {
     const int64_t c = blockIdx.x;
     const int mi = find_member(t.pchunk, t.M, c);
     const EnsMember& m = t.mem[mi];
     const DevDomain d = m.d;
+    const bool node = m.scheme == NGP || m.scheme == TSC;
     for (int j = 0; j < kEnsChunk / 256; ++j) {
         const int64_t i = c * kEnsChunk + j * 256 + threadIdx.x;
         const int64_t local = i - m.pbase;
-        keys[i] = (int)(m.cbase + (local < m.n ? (int64_t)cell_key_exact(x[i], y[i], d) : m.G));
+        const int k = local < m.n ? (node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d)) : 0;
+        keys[i] = (int)(m.cbase + (local < m.n ? (int64_t)k : m.G));
     }
 }
 
@@ -313,12 +317,12 @@ EnsKernel pick_k(int ppl, int exact, int drift) {
 
 EnsKernel pick_kernel(int scheme, int ppl, int exact, int drift) {
     switch (scheme) {
-        case NGP:
-            return pick_k<NGP>(ppl, exact, drift);
+        case NGP:  // members sorted by nearest node
+            return pick_k<NGP_NODE>(ppl, exact, drift);
         case CIC:
             return pick_k<CIC>(ppl, exact, drift);
         case TSC:
-            return pick_k<TSC>(ppl, exact, drift);
+            return pick_k<TSC_NODE>(ppl, exact, drift);
         default:
             return pick_ex<ESK, 4>(exact, drift);  // Esirkepov runs 4 particles per lane
     }

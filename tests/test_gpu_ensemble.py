@@ -75,7 +75,10 @@ def test_deposit_matches_oracle_and_single_context(hip, orc):
             with hip.Context(mb.domain(), mb.num_particles) as c:
                 c.upload(p["x"], p["y"], p["q"], p["vx"], p["vy"], p["m"])
                 c.set_tiling(t.tile_x, t.tile_y, t.particles_per_lane, 0, scheme=mb.scheme)
-                c.sort()
+                if mb.scheme in (0, 2):  # the ensemble sorts NGP and TSC members by nearest node
+                    c.sort_nodes()
+                else:
+                    c.sort()
                 c.deposit(mb.scheme, 2, dt)
                 single = c.rho()
             assert np.array_equal(rho, single), i
