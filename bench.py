@@ -24,6 +24,7 @@ per step).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -209,12 +210,12 @@ def learned_study(pichip):
     cfit = float(np.exp(np.mean(np.log(ee) + 0.5 * np.log(pp))))
     out["plain_ppc_at_learned_error"] = (cfit / out["rms_delta_error"]["learned8"]) ** 2
 
-    def step_ms(ppc, learned):
+    def step_ms(ppc, learned, settings=((1, 0), (4, 1), (10, 2), (10, 4))):
         """Best wall time per step over (sort interval, drift halo) settings."""
         n = N * N * ppc
         best = None
         with pichip.Context(dom, n) as c:
-            for si, halo in ((1, 0), (4, 1), (10, 2), (10, 4)):
+            for si, halo in settings:
                 c.set_tile_halo(halo)
 
                 def go(steps):
@@ -242,6 +243,25 @@ def learned_study(pichip):
     out["step_ms_plain8"] = step_ms(8, False)
     out["step_ms_learned8"] = step_ms(8, True)
     out["speedup_learned8_vs_plain512"] = out["step_ms_plain512"] / out["step_ms_learned8"]
+    # The same comparison at equal density error: plain deposits with the
+    # particles per cell at which the fit puts learned8's error, their error
+    # and step time measured (the plain 512-ppc loop's best setting).
+    ppc_eq = int(math.ceil(out["plain_ppc_at_learned_error"]))
+    n_eq = N * N * ppc_eq
+    eq_errs = []
+    with pichip.Context(dom, n_eq) as c:
+        for seed, amp, mode in held:
+            truth = pichip.langmuir_delta_truth(N, N, amp, mode, 2)
+            c.init_particles(n_eq, kind=1, seed=seed, amp=amp, mode=mode, q=-1.0 / n_eq)
+            c.sort()
+            c.deposit(2, 2)
+            rho = c.rho().astype(np.float64)
+            eq_errs.append(float(np.sqrt(np.mean((rho / rho.mean() - 1 - truth) ** 2))))
+    s512 = out["step_setting_plain512"]
+    out["equal_error"] = {"plain_ppc": ppc_eq, "particles": n_eq, "plain_rms_delta_error": float(np.mean(eq_errs)),
+                          "learned8_rms_delta_error": out["rms_delta_error"]["learned8"],
+                          "plain_step_ms": step_ms(ppc_eq, False, ((s512["sort_interval"], s512["halo"]),))}
+    out["equal_error"]["speedup_learned8"] = out["equal_error"]["plain_step_ms"] / out["step_ms_learned8"]
     net.close()
     return out
 
@@ -473,7 +493,11 @@ def main():
             "learned8_rms_error": lr["rms_delta_error"]["learned8"],
             "plain512_rms_error": lr["rms_delta_error"]["plain512"],
             "learned8_step_ms": lr["step_ms_learned8"], "plain512_step_ms": lr["step_ms_plain512"],
-            "wall_time_ratio": lr["speedup_learned8_vs_plain512"]}
+            "wall_time_ratio": lr["speedup_learned8_vs_plain512"],
+            "equal_error_plain_ppc": lr["equal_error"]["plain_ppc"],
+            "equal_error_plain_rms_error": lr["equal_error"]["plain_rms_delta_error"],
+            "equal_error_plain_step_ms": lr["equal_error"]["plain_step_ms"],
+            "wall_time_ratio_at_equal_error": lr["equal_error"]["speedup_learned8"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(base_particles, dom_args)
     print(json.dumps(out), flush=True)

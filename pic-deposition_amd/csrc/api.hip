@@ -408,10 +408,12 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back, bool node) {
         back.dst[back.count++] = ctx->s[i];
     }
     const int64_t n = (int64_t)ctx->n;
-    if (ctx->sort_rec) {
+    if (ctx->sort_rec && n >= kStagedSortMin) {
         // The device decides per sort between a direct gather of each stream
         // (an order close to the last sort's) and packed records (an order far
         // from it: one or two lines per particle instead of one per stream).
+        // Below kStagedSortMin particles the random gather is short and the
+        // decision's own launches would cost more than they save.
         launch_keys_far(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, 4 * ctx->dom.Nx, node, ctx->far_part,
                         ctx->permute_mode, ctx->stream);
         launch_pack_streams(fwd, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);

@@ -53,6 +53,7 @@ inline pic_status fail(pic_status s, const std::string& msg) {
 constexpr int kStreams = 6;  // x, y, vx, vy, q, m
 constexpr size_t kAosFields = 6;            // FP64 values per ParticleAoS record
 constexpr size_t kAosChunk = size_t(1) << 22;  // records staged per device copy
+constexpr int64_t kStagedSortMin = int64_t(1) << 22;  // particles from which a sort may stage records
 
 inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
