@@ -190,60 +190,9 @@ __global__ __launch_bounds__(256) void scan_down_kernel(const uint32_t* __restri
     }
 }
 
-// Small inputs (the digit histograms of up to 2^20 particles): one workgroup of
-// 1024 threads scans the whole array in chunks of 16K with a running carry,
-// one launch instead of three.  In place (in == out) is allowed: a chunk is
-// read before the barrier and written after it.
-constexpr int kSmallThreads = 1024;
-constexpr int kSmallItems = 16;
-constexpr size_t kSmallScanMax = (size_t)1 << 16;
-
-__global__ __launch_bounds__(kSmallThreads) void scan_small_kernel(const uint32_t* in, size_t m, uint32_t* out) {
-    __shared__ uint32_t s_wave[kSmallThreads / 64];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint32_t carry = 0;
-    for (size_t off = 0; off < m; off += (size_t)kSmallThreads * kSmallItems) {
-        const size_t base = off + (size_t)t * kSmallItems;
-        uint32_t v[kSmallItems];
-        uint32_t sum = 0;
-#pragma unroll
-        for (int j = 0; j < kSmallItems; ++j) {
-            v[j] = base + j < m ? in[base + j] : 0u;
-            sum += v[j];
-        }
-        uint32_t incl = sum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += u;
-        }
-        if (lane == 63) s_wave[w] = incl;
-        __syncthreads();
-        uint32_t wpre = 0, tot = 0;
-#pragma unroll
-        for (int ww = 0; ww < kSmallThreads / 64; ++ww) {
-            const uint32_t c = s_wave[ww];
-            if (ww < w) wpre += c;
-            tot += c;
-        }
-        __syncthreads();  // s_wave is rewritten by the next chunk
-        uint32_t run = carry + wpre + incl - sum;
-#pragma unroll
-        for (int j = 0; j < kSmallItems; ++j) {
-            if (base + j < m) out[base + j] = run;
-            run += v[j];
-        }
-        carry += tot;
-    }
-}
-
 void exclusive_scan(const uint32_t* in, uint32_t* out, size_t m, uint32_t* tmp, hipStream_t s) {
     const size_t nb = (m + kBlockItems - 1) / kBlockItems;
     if (nb == 0) return;
-    if (m <= kSmallScanMax) {
-        hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kSmallThreads), 0, s, in, m, out);
-        return;
-    }
     hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp);
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kThreads), 0, s, tmp, nb);
     hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, out);
