@@ -166,6 +166,8 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     PIC_HIP_TRY(hipMemsetAsync(c->rho, 0, (size_t)c->G * sizeof(float), c->stream));
     PIC_HIP_TRY(hipMemsetAsync(c->acc64, 0, (size_t)c->G * sizeof(double), c->stream));
     PIC_HIP_TRY(dmalloc(&c->status, 1));
+    PIC_HIP_TRY(dmalloc(&c->acc_mark, 1));
+    PIC_HIP_TRY(hipMemsetAsync(c->acc_mark, 0, sizeof(int), c->stream));
     PIC_HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
     PIC_HIP_TRY(dmalloc(&c->Ex, (size_t)2 * c->G));  // Ex then Ey
     c->Ey = c->Ex + c->G;
@@ -506,6 +508,8 @@ pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt)
     a.ppl = ctx->ppl[scheme];
     a.pipe = ctx->pipe[scheme];
     a.geom = ctx->geom[scheme];
+    a.geom.acc_mark = ctx->acc_mark;
+    a.geom.epoch = ctx->deposit_epoch = ctx->deposit_epoch % 0x7ffffff0 + 1;  // never 0 (the initial marker)
     a.dt = (float)dt;
     a.rho = ctx->rho;
     a.partial = ctx->partial;

@@ -50,6 +50,8 @@ struct pic_hip_ctx {
     float* rho = nullptr;
     double* acc64 = nullptr;  // FP64 global-atomic accumulation grid, zero between deposits
     int* status = nullptr;
+    int* acc_mark = nullptr;  // overflow marker of the tiled deposit (TileGeom::acc_mark)
+    int deposit_epoch = 0;
     float* partial = nullptr;
     size_t partial_cap = 0;
     // LDS tiling per scheme (NGP, CIC, TSC, Esirkepov); the partial buffer is
@@ -136,6 +138,7 @@ struct pic_hip_ctx {
         (void)hipFree(sort_rec);
         (void)hipFree(far_part);
         (void)hipFree(permute_mode);
+        (void)hipFree(acc_mark);
         (void)hipHostFree(h_status);
         (void)hipHostFree(h_diag);
         if (stream) (void)hipStreamDestroy(stream);

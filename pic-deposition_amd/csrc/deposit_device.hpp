@@ -253,6 +253,8 @@ __device__ __forceinline__ void overflow_scatter(bool ok, int cx, int cy, const 
 // Per-wave state of the tiled deposit (one tile per wavefront).
 struct TileCtx {
     double* win;  // FP64 LDS window
+    int* acc_mark;  // overflow marker (TileGeom::acc_mark), nullptr for none
+    int epoch;
     int WX;
     int x0, y0, tw, th;  // the tile's own cells
     int ax0, ay0, aw, ah;  // cells deposited into the window: the tile widened by the drift halo
@@ -348,6 +350,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                 float wo[V];
                 const bool ok_o = St::template fill<EX>(fx, fy, vxx, vyy, qv, dt, d, wo);
                 overflow_scatter<S>(ok_o, cx, cy, wo, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
+                if (t.acc_mark) *t.acc_mark = t.epoch;  // the merge must fold the grid
             }
             const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
             bool nseg = lk != ck;
@@ -424,6 +427,8 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
 
     TileCtx t;
     t.win = reinterpret_cast<double*>(lds);
+    t.acc_mark = g.acc_mark;
+    t.epoch = g.epoch;
     int* tab = reinterpret_cast<int*>(t.win + g.wsize);
     int* t_qstart = tab;          // [R]   first aligned quad of row r
     int* t_rs = tab + g.R;        // [R]   first particle of row r
@@ -624,12 +629,15 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
         }
     }
     // The accumulation grid is zero except where drifted particles were
-    // scattered; fold those nodes in and clear them.
-    const size_t node = (size_t)gy * d.Nx + gx;
-    const double o = gacc[node];
-    if (o != 0.0) {
-        s += o;
-        gacc[node] = 0.0;
+    // scattered (and untouched when no tile marked this deposit); fold those
+    // nodes in and clear them.
+    if (g.acc_mark == nullptr || *g.acc_mark == g.epoch) {
+        const size_t node = (size_t)gy * d.Nx + gx;
+        const double o = gacc[node];
+        if (o != 0.0) {
+            s += o;
+            gacc[node] = 0.0;
+        }
     }
     return (float)s;
 }

@@ -27,6 +27,13 @@ struct TileGeom {
     int lds_per_wave;  // floats: 2 windows + row tables
     int fast;          // merge fast path: no halo, tiles >= 3 cells on each axis, >= 3 tiles per axis
     int uniform;       // merge range path: equal tiles dividing each axis, one window image per node
+    // Overflow marker of the single-context deposit: a tile that sends a
+    // particle to the FP64 accumulation grid stores `epoch` at *acc_mark, and
+    // the merge reads that grid only when *acc_mark == epoch.  A marker left
+    // by an earlier deposit with the same epoch (graph replays) costs a read of
+    // a zero grid, never a missed contribution.  nullptr: the grid is always read.
+    int* acc_mark = nullptr;
+    int epoch = 0;
 };
 
 TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D = 0);

@@ -390,3 +390,27 @@ def test_node_order_drift_halo(hip, orc, scheme, halo, tiling, ppc):
         p = c.download()
     r = oracle_rho(orc, dom, p["x"], p["y"], p["q"], p["vx"], p["vy"], scheme, dt)
     assert rel_l2(g, r) <= REL_L2
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_overflow_marker_sequence(hip, orc, scheme):
+    """Deposits alternating between drifted particles (overflow into the FP64
+    grid, which the merge folds) and freshly sorted ones (no overflow: the
+    merge skips the grid): each exact against the oracle."""
+    say("overflow sequence", scheme)
+    n = 48
+    dom = hip.make_domain(1.0, 1.0, n, n)
+    dt = 0.01
+    x, y, q, vx, vy = uniform_particles(n * n * 12, seed=99 + scheme, vmax=2.5 / n / dt)
+    with hip.Context(dom, len(x)) as c:
+        c.upload(x, y, q, vx, vy)
+        for k in range(5):
+            if k % 2 == 0:
+                c.sort()
+            else:
+                c.gather_push(dt, gather=False, push=True)  # drift of up to 2.5 cells: overflow
+            c.deposit(scheme, 2, dt)
+            g = c.rho().astype(np.float64)
+            p = c.download()
+            r = oracle_rho(orc, dom, p["x"], p["y"], p["q"], p["vx"], p["vy"], scheme, dt)
+            assert rel_l2(g, r) <= REL_L2, k
