@@ -30,15 +30,16 @@ __global__ __launch_bounds__(256) void keys_kernel(const float* __restrict__ x, 
 // order, so a wave mostly holds one or two digits: the first two distinct
 // digits of each wave are counted with one LDS add each (ballot + popcount),
 // the rest with one LDS atomic per lane.
+template <int ITEMS>
 __global__ __launch_bounds__(256) void radix_hist_kernel(const int* __restrict__ keys, int64_t n, int shift,
                                                          int nblocks, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kDigits];
     h[threadIdx.x] = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int64_t base = (int64_t)blockIdx.x * kBlockItems;
+    const int64_t base = (int64_t)blockIdx.x * (kThreads * ITEMS);
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         const int64_t i = base + (int64_t)j * kThreads + threadIdx.x;
         const bool valid = i < n;
         const unsigned dg = valid ? ((unsigned)keys[i] >> shift) & (kDigits - 1) : 0u;
@@ -60,9 +61,10 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const int* __restrict__
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: items of a block are processed in 16 slots of 256
+// Stable scatter: items of a block are processed in ITEMS slots of 256
 // consecutive items; within a slot, equal digits are ranked in thread order
 // with wave ballots, across waves and slots with LDS counters.
+template <int ITEMS>
 __global__ __launch_bounds__(256) void radix_scatter_kernel(const int* __restrict__ keys_in,
                                                             const int* __restrict__ vals_in, int64_t n, int shift,
                                                             int nblocks, const uint32_t* __restrict__ hist_scan,
@@ -76,8 +78,8 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(const int* __restric
     gbase[t] = hist_scan[(size_t)t * nblocks + blockIdx.x];
     run[t] = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * kBlockItems;
-    for (int j = 0; j < kItems; ++j) {
+    const int64_t base = (int64_t)blockIdx.x * (kThreads * ITEMS);
+    for (int j = 0; j < ITEMS; ++j) {
         const int64_t i = base + (int64_t)j * kThreads + t;
         const bool valid = i < n;
         int key = 0, val = 0;
@@ -317,9 +319,16 @@ int key_bits(int G) {
 
 }  // namespace
 
+// Radix blocks: 16 items per thread, or 4 for sets of up to kSmallSort
+// particles, whose passes would otherwise run on a few hundred workgroups
+// with 16 barrier-separated slots each.
+constexpr int64_t kSmallSort = int64_t(1) << 21;
+static int radix_items(int64_t n) { return n <= kSmallSort ? 4 : kItems; }
+
 size_t sort_hist_elems(int64_t n) {
-    const size_t nblocks = (size_t)((n + kBlockItems - 1) / kBlockItems);
-    return (nblocks ? nblocks : 1) * kDigits;
+    const size_t large = (size_t)((n + kBlockItems - 1) / kBlockItems);
+    const size_t small = (size_t)((std::min(n, kSmallSort) + kThreads * 4 - 1) / (kThreads * 4));
+    return std::max<size_t>(std::max(large, small), 1) * kDigits;
 }
 
 size_t sort_scan_tmp_elems(size_t m) { return (m + kBlockItems - 1) / kBlockItems + 1; }
@@ -332,7 +341,8 @@ void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, 
 
 hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s) {
     if (n > 0) {
-        const int nblocks = (int)((n + kBlockItems - 1) / kBlockItems);
+        const int items = radix_items(n);
+        const int nblocks = (int)((n + kThreads * items - 1) / (kThreads * items));
         const size_t m = (size_t)nblocks * kDigits;
         const int bits = key_bits(key_range);
         const int passes = bits == 0 ? 1 : (bits + 7) / 8;
@@ -342,11 +352,19 @@ hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hi
         int* vout = sb.perm;
         for (int p = 0; p < passes; ++p) {
             const int shift = 8 * p;
-            hipLaunchKernelGGL(radix_hist_kernel, dim3(nblocks), dim3(kThreads), 0, s, kin, n, shift, nblocks,
-                               sb.hist);
+            if (items == 4)
+                hipLaunchKernelGGL(radix_hist_kernel<4>, dim3(nblocks), dim3(kThreads), 0, s, kin, n, shift, nblocks,
+                                   sb.hist);
+            else
+                hipLaunchKernelGGL(radix_hist_kernel<kItems>, dim3(nblocks), dim3(kThreads), 0, s, kin, n, shift,
+                                   nblocks, sb.hist);
             exclusive_scan(sb.hist, sb.hist, m, sb.scan_tmp, s);
-            hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblocks), dim3(kThreads), 0, s, kin, vin, n, shift,
-                               nblocks, sb.hist, kout, vout);
+            if (items == 4)
+                hipLaunchKernelGGL(radix_scatter_kernel<4>, dim3(nblocks), dim3(kThreads), 0, s, kin, vin, n, shift,
+                                   nblocks, sb.hist, kout, vout);
+            else
+                hipLaunchKernelGGL(radix_scatter_kernel<kItems>, dim3(nblocks), dim3(kThreads), 0, s, kin, vin, n,
+                                   shift, nblocks, sb.hist, kout, vout);
             // next pass reads what this one wrote
             int* kt = kin;
             kin = kout;
