@@ -668,7 +668,8 @@ pic_status pic_hip_gather_push(pic_hip_ctx* ctx, double dt, int do_gather, int d
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     if (!std::isfinite(dt)) return fail(PIC_ERR_INVALID_ARGUMENT, "dt must be finite");
     launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
-                       ctx->Ey, ctx->dev, (float)dt, do_gather, do_push, ctx->stream);
+                       ctx->Ey, ctx->dev, (float)dt, do_gather, do_push, ctx->stream, nullptr, ctx->uniform[4],
+                       ctx->uniform[5]);
     PIC_HIP_TRY(hipGetLastError());
     return PIC_OK;
 }
@@ -752,7 +753,7 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
         if ((st = deposit()) != PIC_OK) return st;
         if ((st = solve()) != PIC_OK) return st;
         launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
-                           ctx->Ey, ctx->dev, dt, 1, 1, ctx->stream, ctx->push_flag);
+                           ctx->Ey, ctx->dev, dt, 1, 1, ctx->stream, ctx->push_flag, ctx->uniform[4], ctx->uniform[5]);
         const bool parts = cfg.record_particles != 0;
         if (diag)
             launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3],
@@ -770,7 +771,7 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
         if ((st = solve()) != PIC_OK) return st;
     }
     launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
-                       ctx->Ey, ctx->dev, dt, 1, 0, ctx->stream);
+                       ctx->Ey, ctx->dev, dt, 1, 0, ctx->stream, nullptr, ctx->uniform[4], ctx->uniform[5]);
     const bool parts = cfg.record_particles != 0;
     if (diag)
         launch_step_diag(ctx->rho, ctx->Ex, ctx->Ey, ctx->cosk, ctx->sink, ctx->dev, ctx->s[2], ctx->s[3], ctx->s[4],

@@ -122,7 +122,7 @@ void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, 
 // push_flag (device int, may be null) overrides do_push when set.
 void launch_gather_push(float* x, float* y, float* vx, float* vy, const float* q, const float* m, int64_t n,
                         const float* Ex, const float* Ey, const DevDomain& d, float dt, int do_gather, int do_push,
-                        hipStream_t s, const int* push_flag = nullptr);
+                        hipStream_t s, const int* push_flag = nullptr, bool q_one = false, bool m_one = false);
 
 // FP64 sums: out[0] = sum(rho), out[1] = sum(Ex^2+Ey^2), out[2] = sum(0.5 m v^2),
 // out[3] = sum(q), out[4] = sum(|q|).  `partials` holds 5 * nblocks doubles.

@@ -20,7 +20,7 @@ __global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px
                                                           int64_t n, const float* __restrict__ Ex,
                                                           const float* __restrict__ Ey, DevDomain d, float dt,
                                                           int do_gather, int do_push,
-                                                          const int* __restrict__ push_flag) {
+                                                          const int* __restrict__ push_flag, int q_one, int m_one) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (push_flag) do_push = *push_flag;
@@ -29,7 +29,9 @@ __global__ __launch_bounds__(256) void gather_push_kernel(float* __restrict__ px
     if (do_gather) {
         float ex, ey;
         gather_cic(x, y, Ex, Ey, d, ex, ey);
-        const float qm = -pq[i] / pm[i];
+        // A stream that holds one value for every particle (a single
+        // species' charge or mass) is read once, not per particle.
+        const float qm = -pq[q_one ? 0 : i] / pm[m_one ? 0 : i];
         vx += qm * ex * dt;
         vy += qm * ey * dt;
         pvx[i] = vx;
@@ -323,10 +325,10 @@ void launch_add_scalar(float* p, int64_t n, float v, hipStream_t s) {
 
 void launch_gather_push(float* x, float* y, float* vx, float* vy, const float* q, const float* m, int64_t n,
                         const float* Ex, const float* Ey, const DevDomain& d, float dt, int do_gather, int do_push,
-                        hipStream_t s, const int* push_flag) {
+                        hipStream_t s, const int* push_flag, bool q_one, bool m_one) {
     if (n > 0)
         hipLaunchKernelGGL(gather_push_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, vx, vy, q, m,
-                           n, Ex, Ey, d, dt, do_gather, do_push, push_flag);
+                           n, Ex, Ey, d, dt, do_gather, do_push, push_flag, q_one ? 1 : 0, m_one ? 1 : 0);
 }
 
 // Partial-sum blocks for a reduction over `items` values: one per 2048 (the
