@@ -556,19 +556,15 @@ __device__ __forceinline__ int axis_pairs(int gn, int T, int nt, int n, int (&t)
 // Equal tiles of T cells dividing the axis, windows [tT - 1 - D, tT + T + 2 + D):
 // the tiles whose windows hold node gn are the consecutive (unwrapped) tiles
 // floor((gn - 2 - D) / T) .. floor((gn + 1 + D) / T), the same tiles in the same
-// order as window_candidates finds them.  Writes (tile, window index) pairs.
-__device__ __forceinline__ int axis_range(int gn, int T, int nt, int D, int (&t)[4 + 2 * kMaxDriftHalo],
-                                          int (&w)[4 + 2 * kMaxDriftHalo]) {
+// order as window_candidates finds them.  Returns the first one's (wrapped)
+// tile index t0 and window index w0 and their count.
+__device__ __forceinline__ void axis_first(int gn, int T, int nt, int D, int& t0, int& w0, int& cnt) {
     const int a = gn - 2 - D;
     const int tlo = a >= 0 ? a / T : -((-a + T - 1) / T);
     const int thi = (gn + 1 + D) / T;
-    int cnt = 0;
-    for (int u = tlo; u <= thi && cnt < 4 + 2 * kMaxDriftHalo; ++u) {
-        t[cnt] = ((u % nt) + nt) % nt;
-        w[cnt] = gn - (u * T - 1 - D);
-        ++cnt;
-    }
-    return cnt;
+    cnt = min(thi - tlo + 1, 4 + 2 * kMaxDriftHalo);
+    t0 = ((tlo % nt) + nt) % nt;
+    w0 = gn - (tlo * T - 1 - D);
 }
 
 // rho at node (gx, gy): the windows covering it, added in FP64 in a fixed
@@ -576,26 +572,72 @@ __device__ __forceinline__ int axis_range(int gn, int T, int nt, int D, int (&t)
 __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restrict__ partial,
                                             double* __restrict__ gacc, const DevDomain& d, const TileGeom& g) {
     double s = 0.0;
+    // The marker is read before the windows so that its latency overlaps theirs.
+    const bool fold = g.acc_mark == nullptr || *g.acc_mark == g.epoch;
+    // All window loads of a node are issued before the first add (a load per
+    // add would wait out one L2 round trip per window); the adds keep the
+    // row-major order of the tiles.
     if (g.fast) {
         int txs[2], wxs[2], tys[2], wys[2];
         const int ncx = axis_pairs(gx, g.T, g.ntx, d.Nx, txs, wxs);
         const int ncy = axis_pairs(gy, g.R, g.nty, d.Ny, tys, wys);
-        for (int iy = 0; iy < ncy; ++iy) {
-            for (int ix = 0; ix < ncx; ++ix) {
-                const float v =
-                    partial[(size_t)(tys[iy] * g.ntx + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
-                s += (double)v;
-            }
+        // A missing second candidate repeats the first: every slot loads a
+        // valid window value and the adds mask by value (s starts at +0 and
+        // never becomes -0, so adding +0 leaves it unchanged).
+        if (ncx < 2) {
+            txs[1] = txs[0];
+            wxs[1] = wxs[0];
         }
+        if (ncy < 2) {
+            tys[1] = tys[0];
+            wys[1] = wys[0];
+        }
+        float v[2][2];
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix)
+                v[iy][ix] = partial[(size_t)(tys[iy] * g.ntx + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix) s += (iy < ncy && ix < ncx) ? (double)v[iy][ix] : 0.0;
     } else if (g.uniform) {
-        int txs[4 + 2 * kMaxDriftHalo], wxs[4 + 2 * kMaxDriftHalo];
-        int tys[4 + 2 * kMaxDriftHalo], wys[4 + 2 * kMaxDriftHalo];
-        const int ncx = axis_range(gx, g.T, g.ntx, g.D, txs, wxs);
-        const int ncy = axis_range(gy, g.R, g.nty, g.D, tys, wys);
-        for (int iy = 0; iy < ncy; ++iy) {
-            const size_t row = (size_t)tys[iy] * g.ntx;
-            for (int ix = 0; ix < ncx; ++ix)
-                s += (double)partial[(row + txs[ix]) * g.wsize + (size_t)wys[iy] * g.WX + wxs[ix]];
+        int tx0, wx0, ncx, ty0, wy0, ncy;
+        axis_first(gx, g.T, g.ntx, g.D, tx0, wx0, ncx);
+        axis_first(gy, g.R, g.nty, g.D, ty0, wy0, ncy);
+        const int total = ncx * ncy;
+        const size_t a0 = ((size_t)ty0 * g.ntx + tx0) * g.wsize + (size_t)wy0 * g.WX + wx0;
+        // Candidates in row-major order: along an axis the next candidate is
+        // the next tile (wrapped) and the window index drops by the tile size.
+        // The first kBatch are loaded together (slots past the end load the
+        // first candidate and add +0, as in the fast path); larger sets
+        // (tiles narrower than the drift halo) add the rest one by one.
+        constexpr int kBatch = 16;
+        int tx = tx0, wx = wx0, ty = ty0, wy = wy0, ix = 0;
+        auto advance = [&]() {
+            tx = tx + 1 == g.ntx ? 0 : tx + 1;
+            wx -= g.T;
+            if (++ix == ncx) {
+                ix = 0;
+                tx = tx0;
+                wx = wx0;
+                ty = ty + 1 == g.nty ? 0 : ty + 1;
+                wy -= g.R;
+            }
+        };
+        float v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const bool ok = k < total;
+            v[k] = partial[ok ? ((size_t)ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx : a0];
+            if (ok) advance();
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) s += k < total ? (double)v[k] : 0.0;
+        for (int k = kBatch; k < total; ++k) {
+            s += (double)partial[((size_t)ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
+            advance();
         }
     } else {
         int txs[4 + 2 * kMaxDriftHalo], tys[4 + 2 * kMaxDriftHalo];
@@ -631,7 +673,7 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
     // The accumulation grid is zero except where drifted particles were
     // scattered (and untouched when no tile marked this deposit); fold those
     // nodes in and clear them.
-    if (g.acc_mark == nullptr || *g.acc_mark == g.epoch) {
+    if (fold) {
         const size_t node = (size_t)gy * d.Nx + gx;
         const double o = gacc[node];
         if (o != 0.0) {
