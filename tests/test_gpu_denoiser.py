@@ -228,6 +228,36 @@ def test_learned_step_loop(hip, trained):
               f"learned8 {rel_l2(learned[key] / 8, ref):.3f}", flush=True)
 
 
+@pytest.mark.parametrize("background", [0.0, 1.0])
+def test_learned_run_matches_denoise(hip, trained, background):
+    """One learned step against pic_hip_denoise on the same plain deposit,
+    with and without a background (added to rho after the learned density is
+    written): the FP64 learned density equals the single call's to 1e-12
+    relative (the run takes the density mean from the particles' charge sum,
+    the single call from the grid sum), FP32 rho is its rounding plus the
+    background, and the total charge is exact."""
+    say("learned run vs denoise, background", background)
+    N = 64
+    n = N * N * 8
+    dom = hip.make_domain(1.0, 1.0, N, N)
+    with hip.Context(dom, n) as a, hip.Context(dom, n) as b:
+        for c in (a, b):
+            c.init_particles(n, kind=1, seed=41, amp=0.02, mode=2, q=hip.unit_plasma_charge(n, N, N), m=1.0)
+        _, fallbacks = trained.run(a, 1, 2, 2, dt=0.1, background=background, sort_interval=10, tolerance=4.0)
+        assert fallbacks == 0
+        learned = trained.output_f64().copy()
+        rho_a = a.rho()
+        # the same particles deposited the same way (the background only enters after the deposit)
+        b.run(1, 2, 2, 0.1, 0.0, hip.ORDER_PUSH_FIRST, sort_interval=10)
+        fb, ratio = trained.denoise(b, 2, tolerance=4.0)
+        assert not fb, ratio
+        np.testing.assert_allclose(learned, trained.output_f64(), rtol=1e-12, atol=0.0)
+        expect_rho = learned.astype(np.float32) + np.float32(background)
+        assert np.array_equal(rho_a, expect_rho)
+        q = a.download()["q"].astype(np.float64).sum()
+        assert abs(learned.sum() / (N * N) - q) <= 1e-13 * abs(q)
+
+
 def test_invalid_denoiser_arguments(hip):
     say("denoiser errors")
     with pytest.raises(hip.PicError, match="multiple of 32"):
