@@ -352,7 +352,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline CIC deposit only (for profiling runs)")
     ap.add_argument("--sweep", action="store_true", help="time every tiling candidate per scheme and exit")
@@ -456,7 +456,10 @@ def main():
         others = {}
 
         def timed(scheme, order):
-            ms_s, k_s = time_deposits(ctx, scheme, 2, max(5, args.steps // 2), 2)
+            # 20 warm-up deposits: the first ~20 ms of a new kernel's work run at
+            # lower clocks (TSC on the node order: 0.244 ms in the first 25
+            # deposits, 0.210 ms from the 75th on)
+            ms_s, k_s = time_deposits(ctx, scheme, 2, max(10, args.steps // 2), max(20, args.warmup))
             return {"particles_per_s": N / (ms_s / 1000.0), "ms": ms_s, "kernel_ms": k_s,
                     "hbm_frac": deposit_bytes(scheme) / (k_s / 1000.0) / 1e9 / HBM_PEAK_GBS, "order": order}
 
