@@ -12,6 +12,11 @@ Roofline: the tiled scatter kernel moves 12 B per particle (x, y, q in FP32)
 plus 4 B per cell of rho; achieved GB/s = those bytes / its mean duration,
 timed with HIP events on the context's stream.  Peak: 8 TB/s HBM3E.
 
+Warm-up: 200 untimed deposits (~35 ms) by default.  A process's first deposits
+run slower while the device's clocks ramp up (tile kernel 0.185 ms after 20
+deposits, 0.164 ms after 100, the same from then on); the steady state is the
+measured one.
+
 CPU baseline: the reference's own OpenMP deposit (oracle/_ref, compiled from
 the reference sources) on the same particles, timed like
 depositChargeTimedSoA, on this host's cores.
@@ -352,7 +357,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline CIC deposit only (for profiling runs)")
     ap.add_argument("--sweep", action="store_true", help="time every tiling candidate per scheme and exit")
@@ -456,9 +461,6 @@ def main():
         others = {}
 
         def timed(scheme, order):
-            # 20 warm-up deposits: the first ~20 ms of a new kernel's work run at
-            # lower clocks (TSC on the node order: 0.244 ms in the first 25
-            # deposits, 0.210 ms from the 75th on)
             ms_s, k_s = time_deposits(ctx, scheme, 2, max(10, args.steps // 2), max(20, args.warmup))
             return {"particles_per_s": N / (ms_s / 1000.0), "ms": ms_s, "kernel_ms": k_s,
                     "hbm_frac": deposit_bytes(scheme) / (k_s / 1000.0) / 1e9 / HBM_PEAK_GBS, "order": order}

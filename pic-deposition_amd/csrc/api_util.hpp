@@ -90,6 +90,9 @@ constexpr size_t kMaxDepositLds = 64 * 1024;
 // Default tiling per scheme (NGP, CIC, TSC, Esirkepov) from the MI355X sweep at
 // 1024^2 / 64 particles per cell: {tile_x, tile_y, particles per lane}.
 constexpr int kDefaultTiling[4][3] = {{64, 4, 8}, {64, 4, 8}, {32, 4, 8}, {32, 4, 4}};
+// Long-row tiling of a single grid with at least 4096 such tiles (the MI355X
+// sweep at 1024^2 / 64 ppc, interleaved repeats): {tile_x, tile_y, particles per lane}.
+constexpr int kRowTiling[4][3] = {{128, 2, 4}, {128, 2, 4}, {128, 1, 8}, {128, 1, 4}};
 
 // Default tile shape for a grid: the first of the scheme's shapes (largest
 // first, down to 8 x 2) that gives at least `min_tiles` tiles and at most
@@ -130,8 +133,8 @@ struct TilingChoice {
 // than the grid only adds halo), 4 or 8 particles per lane (4 only when
 // `ppl4_only`), windows within the workgroup LDS budget.
 inline std::vector<TilingChoice> tiling_candidates(int Nx, int Ny, bool ppl4_only, int halo) {
-    static const int shapes[][2] = {{8, 2},  {8, 4},  {16, 4},  {16, 8},  {32, 4},  {32, 8},
-                                    {64, 4}, {64, 8}, {128, 4}, {16, 16}, {32, 16}, {128, 2}};
+    static const int shapes[][2] = {{8, 2},  {8, 4},   {16, 4},  {16, 8},  {32, 4},  {32, 8},  {64, 4},
+                                    {64, 8}, {128, 4}, {16, 16}, {32, 16}, {128, 2}, {128, 1}};
     std::vector<TilingChoice> out;
     for (const auto& sh : shapes) {
         if (sh[0] > std::max(Nx, 16) || sh[1] > std::max(Ny, 2)) continue;

@@ -114,12 +114,13 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                                          g, dt, partial, gacc, status);
 }
 
+template <int B>
 __global__ __launch_bounds__(256) void deposit_merge_kernel(const float* __restrict__ partial,
                                                             double* __restrict__ gacc, DevDomain d, TileGeom g,
                                                             float* __restrict__ rho) {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
-    if (gx < d.Nx) rho[(size_t)gy * d.Nx + gx] = merge_node(gx, gy, partial, gacc, d, g);
+    if (gx < d.Nx) rho[(size_t)gy * d.Nx + gx] = merge_node<B>(gx, gy, partial, gacc, d, g);
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, size_t n, float v) {
@@ -149,8 +150,13 @@ void launch_tile(const DepositArgs& a) {
     hipLaunchKernelGGL((deposit_tile_kernel<S, K, EX, PIPE, LA>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream,
                        a.x, a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.acc, a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
-    hipLaunchKernelGGL(deposit_merge_kernel, dim3((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny), dim3(256), 0,
-                       a.stream, a.partial, a.acc, a.dom, g, a.rho);
+    const dim3 mgrid((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny);
+    if (g.mbatch <= 4)
+        hipLaunchKernelGGL(deposit_merge_kernel<4>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);
+    else if (g.mbatch <= 8)
+        hipLaunchKernelGGL(deposit_merge_kernel<8>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);
+    else
+        hipLaunchKernelGGL(deposit_merge_kernel<16>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);
 }
 
 // With a drift halo the particles have moved since the sort.  The lookahead
@@ -253,6 +259,8 @@ TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D) {
     const bool ux = rx == 0 && g.WX <= Nx && (g.T + 2 + 2 * g.D) / g.T + 1 <= g.ntx;
     const bool uy = ry == 0 && g.WY <= Ny && (g.R + 2 + 2 * g.D) / g.R + 1 <= g.nty;
     g.uniform = ux && uy;
+    const int most = ((g.T + 2 + 2 * g.D) / g.T + 1) * ((g.R + 2 + 2 * g.D) / g.R + 1);
+    g.mbatch = most <= 4 ? 4 : (most <= 8 ? 8 : 16);
     return g;
 }
 

@@ -569,6 +569,8 @@ __device__ __forceinline__ void axis_first(int gn, int T, int nt, int D, int& t0
 
 // rho at node (gx, gy): the windows covering it, added in FP64 in a fixed
 // order, plus the FP64 accumulation grid where it is non-zero (then cleared).
+// B: candidates loaded together on the range path (TileGeom::mbatch).
+template <int B = 16>
 __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restrict__ partial,
                                             double* __restrict__ gacc, const DevDomain& d, const TileGeom& g) {
     double s = 0.0;
@@ -610,10 +612,10 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
         const size_t a0 = ((size_t)ty0 * g.ntx + tx0) * g.wsize + (size_t)wy0 * g.WX + wx0;
         // Candidates in row-major order: along an axis the next candidate is
         // the next tile (wrapped) and the window index drops by the tile size.
-        // The first kBatch are loaded together (slots past the end load the
-        // first candidate and add +0, as in the fast path); larger sets
-        // (tiles narrower than the drift halo) add the rest one by one.
-        constexpr int kBatch = 16;
+        // The first B are loaded together (slots past the end load the first
+        // candidate and add +0, as in the fast path); larger sets add the
+        // rest one by one.
+        constexpr int kBatch = B;
         int tx = tx0, wx = wx0, ty = ty0, wy = wy0, ix = 0;
         auto advance = [&]() {
             tx = tx + 1 == g.ntx ? 0 : tx + 1;

@@ -27,6 +27,7 @@ struct TileGeom {
     int lds_per_wave;  // floats: 2 windows + row tables
     int fast;          // merge fast path: no halo, tiles >= 3 cells on each axis, >= 3 tiles per axis
     int uniform;       // merge range path: equal tiles dividing each axis, one window image per node
+    int mbatch = 16;   // range path: windows per node loaded together (4, 8 or 16; the most per node when <= 16)
     // Overflow marker of the single-context deposit: a tile that sends a
     // particle to the FP64 accumulation grid stores `epoch` at *acc_mark, and
     // the merge reads that grid only when *acc_mark == epoch.  A marker left

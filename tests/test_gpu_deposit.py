@@ -167,6 +167,8 @@ def test_drifted_particles_take_overflow_path(hip, orc, scheme):
     (4, (3, 3, 4, 0), (40, 24)),
     (3, (16, 16, 8, 1), (48, 80)),
     (2, (64, 8, 4, 0), (64, 64)),
+    (4, (3, 3, 4, 0), (36, 24)),  # range merge path, 25 windows per node: a batch of 16, then one by one
+    (1, (128, 2, 4, 0), (256, 64)),
 ])
 @pytest.mark.parametrize("ppc", [24, 6])
 @pytest.mark.parametrize("scheme", SCHEMES)
