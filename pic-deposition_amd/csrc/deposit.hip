@@ -259,6 +259,7 @@ TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D) {
     const bool ux = rx == 0 && g.WX <= Nx && (g.T + 2 + 2 * g.D) / g.T + 1 <= g.ntx;
     const bool uy = ry == 0 && g.WY <= Ny && (g.R + 2 + 2 * g.D) / g.R + 1 <= g.nty;
     g.uniform = ux && uy;
+    g.fast_rows = !g.fast && g.D == 0 && g.T >= 3 && g.R < 3 && g.ntx >= 3 && (rx == 0 || rx >= 3) && uy;
     const int most = ((g.T + 2 + 2 * g.D) / g.T + 1) * ((g.R + 2 + 2 * g.D) / g.R + 1);
     g.mbatch = most <= 4 ? 4 : (most <= 8 ? 8 : 16);
     return g;

@@ -604,6 +604,33 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
         for (int iy = 0; iy < 2; ++iy)
 #pragma unroll
             for (int ix = 0; ix < 2; ++ix) s += (iy < ncy && ix < ncx) ? (double)v[iy][ix] : 0.0;
+    } else if (g.fast_rows) {
+        // Tiles of one or two rows (no halo): along x as in the fast path,
+        // along y up to four consecutive tile rows, the same for every node
+        // of a grid row.
+        int txs[2], wxs[2];
+        const int ncx = axis_pairs(gx, g.T, g.ntx, d.Nx, txs, wxs);
+        if (ncx < 2) {
+            txs[1] = txs[0];
+            wxs[1] = wxs[0];
+        }
+        int ty0, wy0, ncy;
+        axis_first(gy, g.R, g.nty, 0, ty0, wy0, ncy);
+        float v[4][2];
+        int ty = ty0, wy = wy0;
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+            const int ry = iy < ncy ? ty : ty0, sy = iy < ncy ? wy : wy0;
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix)
+                v[iy][ix] = partial[((size_t)ry * g.ntx + txs[ix]) * g.wsize + (size_t)sy * g.WX + wxs[ix]];
+            ty = ty + 1 == g.nty ? 0 : ty + 1;
+            wy -= g.R;
+        }
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix) s += (iy < ncy && ix < ncx) ? (double)v[iy][ix] : 0.0;
     } else if (g.uniform) {
         int tx0, wx0, ncx, ty0, wy0, ncy;
         axis_first(gx, g.T, g.ntx, g.D, tx0, wx0, ncx);

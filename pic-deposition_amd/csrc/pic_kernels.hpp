@@ -26,6 +26,7 @@ struct TileGeom {
     int WX, WY, wsize;
     int lds_per_wave;  // floats: 2 windows + row tables
     int fast;          // merge fast path: no halo, tiles >= 3 cells on each axis, >= 3 tiles per axis
+    int fast_rows;     // merge row path: as fast along x; tiles of 1 or 2 rows dividing the y axis
     int uniform;       // merge range path: equal tiles dividing each axis, one window image per node
     int mbatch = 16;   // range path: windows per node loaded together (4, 8 or 16; the most per node when <= 16)
     // Overflow marker of the single-context deposit: a tile that sends a

@@ -169,6 +169,10 @@ def test_drifted_particles_take_overflow_path(hip, orc, scheme):
     (2, (64, 8, 4, 0), (64, 64)),
     (4, (3, 3, 4, 0), (36, 24)),  # range merge path, 25 windows per node: a batch of 16, then one by one
     (1, (128, 2, 4, 0), (256, 64)),
+    # without the halo (the second deposit) these take the merge's row path
+    (1, (16, 2, 4, 0), (64, 32)),
+    (2, (8, 1, 8, 0), (40, 24)),
+    (1, (12, 1, 4, 0), (45, 16)),  # ragged last tile column
 ])
 @pytest.mark.parametrize("ppc", [24, 6])
 @pytest.mark.parametrize("scheme", SCHEMES)
