@@ -53,6 +53,28 @@ pic_status apply_tiling(pic_hip_ctx* c, int scheme, int tx, int ty, int ppl, int
     return ensure_partial(c);
 }
 
+// Create-time tiling of scheme k for a drift halo: long-row tiles
+// (kRowTiling) on grids that hold at least 4096 of them, without a halo (a
+// halo around a one- or two-row tile is mostly halo); otherwise the shape of
+// default_tile_shape, with 4 particles per lane below 1024 particles per tile.
+void default_tiling(const pic_hip_ctx* c, int k, int64_t capacity, int halo, int& tx, int& ty, int& ppl) {
+    const int Nx = c->dom.Nx, Ny = c->dom.Ny;
+    const int rx = kRowTiling[k][0], ry = kRowTiling[k][1];
+    const int64_t row_tiles = (int64_t)((Nx + rx - 1) / rx) * ((Ny + ry - 1) / ry);
+    const int64_t row_per_tile = capacity * rx * ry / std::max<int64_t>(c->G, 1);
+    if (halo == 0 && rx <= Nx && row_tiles >= 4096 && row_per_tile >= 1024) {
+        tx = rx;
+        ty = ry;
+        ppl = kRowTiling[k][2];
+        return;
+    }
+    tx = kDefaultTiling[k][0];
+    ty = kDefaultTiling[k][1];
+    default_tile_shape(k, Nx, Ny, capacity, 4096, INT64_MAX, 256, tx, ty);
+    const int64_t per_tile = capacity * tx * ty / std::max<int64_t>(c->G, 1);
+    ppl = per_tile < 1024 ? 4 : kDefaultTiling[k][2];
+}
+
 pic_status ensure_fft(pic_hip_ctx* c) {
     if (c->fft_ready) return PIC_OK;
     const int hx = c->dom.Nx / 2 + 1;
@@ -185,23 +207,13 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     // a wave then streams one or two long cell ranges (1024^2 / 64 ppc: tile
     // kernel CIC 0.165 -> 0.159 ms, NGP 0.168 -> 0.153, TSC on the node order
     // 0.215 -> 0.203, Esirkepov 0.389 -> 0.375).
+    c->tile_capacity = (int64_t)capacity;
     for (int k = 0; k < 4; ++k) {
-        int tx = kDefaultTiling[k][0], ty = kDefaultTiling[k][1];
-        int ppl;
-        const int rx = kRowTiling[k][0], ry = kRowTiling[k][1];
-        const int64_t row_tiles = (int64_t)((domain->Nx + rx - 1) / rx) * ((domain->Ny + ry - 1) / ry);
-        const int64_t row_per_tile = (int64_t)capacity * rx * ry / std::max<int64_t>(c->G, 1);
-        if (rx <= domain->Nx && row_tiles >= 4096 && row_per_tile >= 1024) {
-            tx = rx;
-            ty = ry;
-            ppl = kRowTiling[k][2];
-        } else {
-            default_tile_shape(k, domain->Nx, domain->Ny, (int64_t)capacity, 4096, INT64_MAX, 256, tx, ty);
-            const int64_t per_tile = (int64_t)capacity * tx * ty / std::max<int64_t>(c->G, 1);
-            ppl = per_tile < 1024 ? 4 : kDefaultTiling[k][2];
-        }
+        int tx, ty, ppl;
+        default_tiling(c.get(), k, c->tile_capacity, 0, tx, ty, ppl);
         const pic_status st = apply_tiling(c.get(), k, tx, ty, ppl, 0);
         if (st != PIC_OK) return st;
+        c->tiling_default[k] = true;
     }
     PIC_HIP_TRY(hipStreamSynchronize(c->stream));
     *out = c.release();
@@ -238,7 +250,11 @@ pic_status pic_hip_sync(pic_hip_ctx* ctx) {
 pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int scheme, int tile_x, int tile_y, int particles_per_lane,
                               int pipelined) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
-    return apply_tiling(ctx, scheme, tile_x, tile_y, particles_per_lane, pipelined);
+    const pic_status st = apply_tiling(ctx, scheme, tile_x, tile_y, particles_per_lane, pipelined);
+    if (st == PIC_OK)
+        for (int k = 0; k < 4; ++k)
+            if (scheme < 0 || k == scheme) ctx->tiling_default[k] = false;
+    return st;
 }
 
 pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells) {
@@ -248,7 +264,10 @@ pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells) {
     const int old = ctx->halo;
     ctx->halo = cells;
     for (int k = 0; k < 4; ++k) {
-        const pic_status st = apply_tiling(ctx, k, ctx->geom[k].T, ctx->geom[k].R, ctx->ppl[k], ctx->pipe[k]);
+        // a scheme still on its create-time tiling takes the default for this halo
+        int tx = ctx->geom[k].T, ty = ctx->geom[k].R, ppl = ctx->ppl[k];
+        if (ctx->tiling_default[k]) default_tiling(ctx, k, ctx->tile_capacity, cells, tx, ty, ppl);
+        const pic_status st = apply_tiling(ctx, k, tx, ty, ppl, ctx->pipe[k]);
         if (st != PIC_OK) {
             ctx->halo = old;
             for (int j = 0; j < 4; ++j) (void)apply_tiling(ctx, j, ctx->geom[j].T, ctx->geom[j].R, ctx->ppl[j], ctx->pipe[j]);
@@ -1140,6 +1159,7 @@ pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path
     }
     pic_status st = apply_tiling(ctx, scheme, best.tx, best.ty, best.ppl, 0);
     if (st != PIC_OK) return st;
+    ctx->tiling_default[scheme] = false;
     if (tile_x) *tile_x = best.tx;
     if (tile_y) *tile_y = best.ty;
     if (particles_per_lane) *particles_per_lane = best.ppl;

@@ -59,6 +59,10 @@ struct pic_hip_ctx {
     TileGeom geom[4]{};
     int ppl[4] = {8, 8, 8, 4};
     int halo = 0;  // drift halo of every scheme's tile windows (cells)
+    // Scheme still on its create-time tiling (not set by pic_hip_set_tiling or
+    // pic_hip_autotune): pic_hip_set_tile_halo re-picks it for the new halo.
+    bool tiling_default[4] = {};
+    int64_t tile_capacity = 0;  // capacity the create-time tilings were chosen for
     int pipe[4] = {1, 1, 0, 0};
     float* phi = nullptr;
     float* Ex = nullptr;

@@ -420,3 +420,39 @@ def test_overflow_marker_sequence(hip, orc, scheme):
             p = c.download()
             r = oracle_rho(orc, dom, p["x"], p["y"], p["q"], p["vx"], p["vy"], scheme, dt)
             assert rel_l2(g, r) <= REL_L2, k
+
+
+def test_default_tiling_follows_the_drift_halo(hip, orc):
+    """A 1024^2 context with room for 1024 particles per long-row tile starts
+    on those tiles (CIC 128 x 2, TSC 128 x 1); a drift halo moves the schemes
+    still on their create-time tiling to the smaller default shapes and back,
+    while a tiling set by the caller stays; deposits on every tiling match the
+    oracle."""
+    say("default tiling vs halo")
+
+    def til(ctx, scheme):
+        t = ctx.tiling(scheme)
+        return (t.tile_x, t.tile_y, t.particles_per_lane)
+
+    N = 1024
+    n = 8 * N * N  # 1024 particles per 128 x 1 tile
+    dom = hip.make_domain(1.0, 1.0, N, N)
+    with hip.Context(dom, n) as c:
+        assert til(c, 1) == (128, 2, 4)
+        assert til(c, 2) == (128, 1, 8)
+        c.set_tile_halo(2)
+        assert til(c, 1) == (64, 4, 8)
+        assert til(c, 2)[:2] == (32, 4)
+        c.set_tile_halo(0)
+        assert til(c, 1) == (128, 2, 4)
+        c.set_tiling(32, 4, 8, 0, scheme=1)
+        c.set_tile_halo(1)
+        assert til(c, 1) == (32, 4, 8)
+        c.set_tile_halo(0)
+        x, y, q, vx, vy = uniform_particles(200000, seed=3, vmax=0.0)
+        c.upload(x, y, q, vx, vy)
+        c.sort()
+        for scheme in (1, 2):
+            c.deposit(scheme, 2, 0.01)
+            r = oracle_rho(orc, dom, x, y, q, vx, vy, scheme, 0.01)
+            assert rel_l2(c.rho().astype(np.float64), r) <= REL_L2
