@@ -141,7 +141,10 @@ pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int scheme, int tile_x, int tile
 /* Drift halo (0..4 cells, default 0) of every scheme's tile windows: a tile
  * deposits into LDS the particles sorted into it whose current cell is within
  * `cells` of the tile, so that particles that moved since the last sort stay
- * off the global-atomic overflow path.  Costs window LDS and merge reads. */
+ * off the global-atomic overflow path.  Costs window LDS and merge reads.
+ * Schemes still on their create-time tiling take the default shape for the
+ * new halo (long-row tiles on large grids only at halo 0); tilings set with
+ * pic_hip_set_tiling or pic_hip_autotune are kept. */
 pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells);
 int pic_hip_get_tile_halo(pic_hip_ctx* ctx);
 pic_status pic_hip_get_tiling(pic_hip_ctx* ctx, int scheme, int* tile_x, int* tile_y, int* particles_per_lane,
