@@ -5,21 +5,27 @@ Workload (the configuration the headline metric is quoted on): periodic
 particles, x drawn from n(x) ~ 1 + 0.01 cos(2 pi 4 x), y uniform, q = 1/64,
 cell-sorted and resident in HBM.  One step = one CIC deposit of all particles
 (tiled LDS kernel + deterministic merge).  Also measured: NGP / TSC /
-Esirkepov-density deposits, the global-atomic backend on sorted and shuffled
-orders, and the cell sort.
+Esirkepov-density deposits, the global-atomic backend on the cell-sorted order
+and on the initialiser's order (positions independent of the particle index,
+i.e. a random order with respect to the cells), the cell sort (from that
+random order, of an already sorted set, and after ten drift steps), and the
+per-phase timestep profile of Simulation::runProfiled at the reference's own
+cases.
 
 Roofline: the tiled scatter kernel moves 12 B per particle (x, y, q in FP32)
 plus 4 B per cell of rho; achieved GB/s = those bytes / its mean duration,
 timed with HIP events on the context's stream.  Peak: 8 TB/s HBM3E.
 
-Warm-up: 200 untimed deposits (~35 ms) by default.  A process's first deposits
-run slower while the device's clocks ramp up (tile kernel 0.185 ms after 20
-deposits, 0.164 ms after 100, the same from then on); the steady state is the
-measured one.
+Clock ramp: a process's first deposits run slower while the device's clocks
+ramp up from idle.  Before the `--warmup` deposits, a preconditioning phase
+deposits for `--precondition-s` seconds (default 0.5 s); its length is
+reported in the JSON line as its own field ("preconditioning"), and `warmup`
+echoes the warm-up deposits the command asked for.
 
 CPU baseline: the reference's own OpenMP deposit (oracle/_ref, compiled from
 the reference sources) on the same particles, timed like
-depositChargeTimedSoA, on this host's cores.
+depositChargeTimedSoA, with one thread per physical core of this host (lscpu,
+within the process's CPU affinity and OMP_NUM_THREADS), and serially.
 
 Also reported (not the headline): the 64-member Langmuir ensemble (persistent
 deposit kernels, autotuned tilings, graph-replayed steps) and the learned
@@ -65,6 +71,97 @@ def time_deposits(ctx, scheme, backend, steps, warmup):
     return (t1 - t0) * 1000.0 / steps, kern_ms / max(k, 1)
 
 
+def precondition(ctx, seconds):
+    """Untimed CIC deposits for `seconds` of wall time (the device's clocks
+    ramp up from idle over the first ~0.05 s of continuous work)."""
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(20):
+            ctx.deposit(1, 2, DT)
+        ctx.sync()
+        k += 20
+    return {"deposits": k, "seconds": time.perf_counter() - t0,
+            "why": "device clock ramp from idle; untimed, before the warm-up deposits"}
+
+
+def time_sorts(ctx, reps, prepare=None):
+    """Mean device time of `reps` sorts (HIP events around each sort on the
+    context's stream); `prepare` runs before each, untimed."""
+    tot = 0.0
+    for _ in range(reps):
+        if prepare:
+            prepare()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.sort()
+        ctx.sync()
+        tot += time.perf_counter() - t0
+    return tot * 1000.0 / reps
+
+
+def sort_and_shuffled(ctx):
+    """The cell sort from the initialiser's order (positions independent of
+    the index: a random order with respect to the cells), of an already sorted
+    set, and after ten steps of drift (~0.25 cells per step); and the CIC
+    global-atomic deposit on the random order."""
+    res = {}
+
+    def init():
+        ctx.init_particles(N, kind=1, seed=42, amp=0.01, mode=4, q=1.0 / PPC, vmax=0.5 / NX / DT)
+
+    init()
+    ms_a, k_a = time_deposits(ctx, 1, 1, 5, 1)
+    res["CIC GPU_Atomics (shuffled order)"] = {"particles_per_s": N / (ms_a / 1000.0), "ms": ms_a, "kernel_ms": k_a,
+                                               "order": "initialiser order (random with respect to the cells)"}
+    ms = time_sorts(ctx, 3, init)
+    res["cell sort (random order)"] = {"particles_per_s": N / (ms / 1000.0), "ms": ms}
+    ctx.sort()
+    ms = time_sorts(ctx, 3)
+    res["cell sort (sorted order)"] = {"particles_per_s": N / (ms / 1000.0), "ms": ms}
+
+    def drift():
+        ctx.sort()
+        for _ in range(10):
+            ctx.gather_push(DT * 0.5, gather=False, push=True)
+
+    ms = time_sorts(ctx, 3, drift)
+    res["cell sort (after 10 drift steps)"] = {"particles_per_s": N / (ms / 1000.0), "ms": ms,
+                                               "drift": "10 pushes of dt/2, |v| <= 0.5 dx/dt per axis"}
+    init()
+    ctx.sort()
+    return res
+
+
+def timestep_profiles(pichip):
+    """Simulation::runProfiled at the reference's cases
+    (BenchmarkRunner::runTimestepProfiles / runAmortizedTimestepProfiles,
+    benchmark_runner.cpp:224-279): uniform Maxwellian (T = 1, seed 42,
+    q = +-1), 128^2, Lx = 1, dt = 0.01, CIC; 1e4, 1e5, 1e6 particles sorting
+    every step (3 warm-up, 20 timed steps) and 1e5 sorting every 1, 10, 100
+    steps (3 warm-up, 100 timed); plus the 512^2 two-beam case (26,214,400
+    particles, TSC, sort every 10 steps, resolved dt).  Rows in the
+    timestep_profile.csv schema."""
+    rows = []
+    dom = pichip.make_domain(1.0, 1.0, 128, 128)
+    cases = [(10000, 1, 20), (100000, 1, 20), (1000000, 1, 20), (100000, 10, 100), (100000, 100, 100)]
+    for n, si, timed in cases:
+        p = pichip.reference_particles("maxwellian", n, dom, seed=42)
+        with pichip.Context(dom, n) as c:
+            c.upload(*(p[k].astype(np.float32) for k in ("x", "y", "q", "vx", "vy", "m")))
+            prof = c.run_profiled(3, timed, 1, 2, 0.01, 0.0, sort_interval=si)
+        rows.append({"num_particles": n, "grid": "128x128", "scheme": "CIC", **prof})
+    N3 = 512
+    n3 = N3 * N3 * 100
+    dom3 = pichip.make_domain(1.0, 1.0, N3, N3)
+    with pichip.Context(dom3, n3) as c:
+        c.init_particles(n3, kind=2, seed=42, amp=0.2, vth=0.01, q=-1.0 / n3, m=1.0)
+        c.set_tile_halo(2)
+        prof = c.run_profiled(3, 20, 2, 2, 0.2 * (1.0 / N3) / 0.2, 1.0, sort_interval=10)
+    rows.append({"num_particles": n3, "grid": "512x512", "scheme": "TSC", **prof})
+    return rows
+
+
 def measured_traffic():
     """HBM bytes per launch of the CIC tile kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB
@@ -79,38 +176,73 @@ def measured_traffic():
     return None
 
 
+def physical_cores():
+    """(physical cores within this process's CPU affinity per lscpu, logical
+    CPUs in the affinity)."""
+    import subprocess
+
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        allowed = set(cpus)
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(v) if v else 0 for v in line.split(",")[:3])
+            if cpu in allowed:
+                cores.add((sock, core))
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pass
+    return (len(cores) or len(cpus)), len(cpus)
+
+
 def cpu_baseline(p, dom_args, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, os.cpu_count() or 1))
+    phys, logical = physical_cores()
+    threads = phys
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        threads = min(threads, env)
+    threads = max(1, threads)
     od = oracle.domain(*dom_args)
     kind = "reference" if oracle.ref_available() else "port"
     x = p["x"].astype(np.float64)
     y = p["y"].astype(np.float64)
     q = p["q"].astype(np.float64)
     del p
+
+    def timed(nthreads, arrs, repeats):
+        if kind == "reference":
+            return oracle.ref_deposit_timed(1, *arrs, od, nthreads, repeats)
+        return oracle.deposit_omp_timed(1, *arrs, od, nthreads, repeats)
+
     # probe on a 1/16 sample to size the timed run to the budget
     m = len(x) // 16
-    if kind == "reference":
-        probe, _, _ = oracle.ref_deposit_timed(1, x[:m], y[:m], q[:m], od, threads, 1)
-    else:
-        probe, _, _ = oracle.deposit_omp_timed(1, x[:m], y[:m], q[:m], od, threads, 1)
+    probe, _, _ = timed(threads, (x[:m], y[:m], q[:m]), 1)
     per_full_s = probe * 16 / 1000.0
     repeats = int(max(2, min(10, budget_s / max(per_full_s, 1e-3) - 3)))
-    if kind == "reference":
-        mean_ms, std_ms, _ = oracle.ref_deposit_timed(1, x, y, q, od, threads, repeats)
-    else:
-        mean_ms, std_ms, _ = oracle.deposit_omp_timed(1, x, y, q, od, threads, repeats)
+    mean_ms, std_ms, _ = timed(threads, (x, y, q), repeats)
+    # serial: the reference's serial CIC (ngp.cpp) on a 1/8 sample of the
+    # same particles (the first eighth of the cell-sorted order: 128 grid
+    # rows), 3 warm-ups + 3 timed deposits
+    ms8 = len(x) // 8
+    s_mean, s_std, _ = timed(1, (x[:ms8], y[:ms8], q[:ms8]), 3)
     return {
         "value": len(x) / (mean_ms / 1000.0),
         "unit": "particles/s",
         "cores": threads,
+        "physical_cores": phys,
+        "logical_cpus": logical,
         "kind": kind,
         "sample": f"CIC deposit of the same {len(x):,} cell-sorted particles (FP32 promoted to FP64) on 1024^2, "
-                  f"OpenMP privatised path, 3 warm-up + {repeats} timed deposits, mean {mean_ms:.1f} ms "
-                  f"(std {std_ms:.1f})",
+                  f"OpenMP privatised path on {threads} threads, 3 warm-up + {repeats} timed deposits, mean "
+                  f"{mean_ms:.1f} ms (std {std_ms:.1f})",
+        "serial": {"value": ms8 / (s_mean / 1000.0), "unit": "particles/s", "cores": 1,
+                   "sample": f"serial CIC deposit (ngp.cpp) of the first {ms8:,} cell-sorted particles, 3 warm-up + "
+                             f"3 timed deposits, mean {s_mean:.1f} ms (std {s_std:.1f})"},
     }
 
 
@@ -357,7 +489,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precondition-s", type=float, default=0.5,
+                    help="seconds of untimed deposits before the warm-up (device clock ramp); reported in the output")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline CIC deposit only (for profiling runs)")
     ap.add_argument("--sweep", action="store_true", help="time every tiling candidate per scheme and exit")
@@ -365,6 +499,7 @@ def main():
     ap.add_argument("--skip-ensemble", action="store_true", help="skip the 64-member ensemble measurement")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned-density study")
     ap.add_argument("--skip-timestep", action="store_true", help="skip the C3 timestep measurement")
+    ap.add_argument("--skip-profile", action="store_true", help="skip the per-phase timestep profile rows")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--profile-step", choices=("plain8", "learned8", "plain512"), default="",
@@ -423,6 +558,7 @@ def main():
 
     if args.copy_probe:
         ctx.copy_probe(1 << 30)
+    pre = precondition(ctx, args.precondition_s)
     ms, kern_ms = time_deposits(ctx, 1, 2, args.steps, args.warmup)
     value = N / (ms / 1000.0)
     achieved = deposit_bytes(1) / (kern_ms / 1000.0) / 1e9
@@ -443,6 +579,7 @@ def main():
                                "cell-sorted FP32 SoA resident in HBM, tiled LDS backend (GPU_Priv)",
                    "particles": N, "grid": [NX, NY], "scheme": "CIC", "backend": "GPU_Priv",
                    "tiling": vars(ctx.tiling())},
+        "preconditioning": pre,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if not args.tiling else None,
@@ -475,20 +612,15 @@ def main():
         others["TSC on the cell order"] = timed(2, "cell")
         others["Esirkepov"] = timed(3, "cell")
         ms_a, k_a = time_deposits(ctx, 1, 1, 5, 1)
-        others["CIC GPU_Atomics (sorted order)"] = {"particles_per_s": N / (ms_a / 1000.0), "ms": ms_a}
-        # sort throughput (keys, 3 radix passes, permutation of 6 streams, offsets)
-        ctx.sort()
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(3):
-            ctx.sort()
-        ctx.sync()
-        sort_ms = (time.perf_counter() - t0) * 1000.0 / 3
-        others["cell sort"] = {"particles_per_s": N / (sort_ms / 1000.0), "ms": sort_ms}
+        others["CIC GPU_Atomics (sorted order)"] = {"particles_per_s": N / (ms_a / 1000.0), "ms": ms_a,
+                                                    "kernel_ms": k_a}
+        others.update(sort_and_shuffled(ctx))
         out["other"] = others
     ctx.close()
     if not args.quick and not args.skip_timestep:
         out["timestep_c3"] = timestep_c3(pichip)
+    if not args.quick and not args.skip_profile:
+        out["timestep_profile"] = timestep_profiles(pichip)
     if not args.quick and not args.skip_ensemble:
         out["ensemble"] = ensemble_bench(pichip)
     if not args.quick and not args.skip_learned:

@@ -264,6 +264,25 @@ typedef struct {
 pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nsteps, int32_t record_from,
                        double* field_energy, double* mode1_amplitude, double* kinetic_energy, double* charge_error);
 
+/* Per-phase timing of the timestep, replacing Simulation::runProfiled
+ * (src/pic/simulation.cpp:147-292, TimestepProfile simulation.hpp:40-48; rows
+ * written by benchmark_runner.cpp:224-306).  Runs warmup_steps + timed_steps
+ * eager steps of the reference's SoA order ([sort every sort_interval-th
+ * step] -> push -> deposit -> +background -> Poisson -> gather; Esirkepov
+ * pushes after the Poisson solve) and returns the mean milliseconds per timed
+ * step of each phase, measured with HIP events on the context's stream.
+ * GPU_Atomics sorts only with sort_interval >= 1 (SimulationConfig::sorted);
+ * NGP and TSC GPU_Priv steps sort by nearest node.  The deposit reads the
+ * pushed particles (the reference deposits a copy taken before the push,
+ * simulation.cpp:191,203-208).  use_graph and record_particles are ignored.
+ * Synchronous. */
+typedef struct {
+    double push_ms, sort_ms, deposit_ms, poisson_ms, gather_ms, total_ms;
+    int32_t sort_interval;
+} pic_timestep_profile;
+pic_status pic_hip_run_profiled(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t warmup_steps,
+                                int32_t timed_steps, pic_timestep_profile* out);
+
 /* Physics estimators on recorded series (host FP64; reference
  * src/pic/validation.cpp).  Dominant frequency: DFT peak bin of the
  * mean-removed series, k / (n dt) (:19-54).  Growth rate: least-squares slope

@@ -38,6 +38,16 @@ or_fourier_mode_amplitude    pure cos/sin field.
 or_esirkepov_current         discrete continuity identity (rho_new-rho_old)/dt
                              + div J = 0 to round-off; zero motion gives
                              J = 0 and rho_old = rho_new = CIC.
+or_integrated_rho,           bit-for-bit against the reference build
+or_field_energy,             (FieldGrid::integratedRho / fieldEnergy,
+or_total_charge,             ParticlesSoA::totalCharge / kineticEnergy:
+or_kinetic_energy            ref_* in ref_shim.cpp) on random grids and
+                             particles; closed forms: single Fourier mode
+                             field energy A^2 Lx Ly / 4, uniform-velocity
+                             kinetic energy N m v^2 / 2, uniform rho.
+or_total_abs_charge,         closed forms (diagnostics.cpp:18-29 is not
+or_charge_error              buildable here: it includes fftw3.h): alternating
+                             +-q sets, the 1e-12 denominator floor.
 or_cnn_forward               an independent numpy restatement with np.roll
                              (periodic correlation) to 1e-12; a centred delta
                              kernel is the identity and a shifted one a roll.
@@ -141,6 +151,10 @@ _REF_SIGS = {
     "ref_deposit_kernel_timed": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D, _L,
                                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                    ctypes.c_double, ctypes.c_int, _D, _D]),
+    "ref_integrated_rho": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "ref_field_energy": (ctypes.c_double, [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "ref_total_charge": (ctypes.c_double, [_D, _L]),
+    "ref_kinetic_energy": (ctypes.c_double, [_D, _D, _D, _L]),
     "ref_tsc_weights": (None, [ctypes.c_double, _D, _I]),
     "ref_esirkepov_weights_1d": (None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D]),
 }
@@ -397,6 +411,28 @@ def ref_init(kind, n, dom, p1=0.0, p2=0.0, mode=1, seed=42):
     ref().ref_init_soa(int(kind), int(n), dom.Lx, dom.Ly, dom.Nx, dom.Ny, float(p1), float(p2), int(mode), int(seed),
                        *[_pd(a) for a in arrs])
     return dict(zip(("x", "y", "vx", "vy", "q", "m"), arrs))
+
+
+def ref_integrated_rho(rho, dom):
+    """FieldGrid::integratedRho of the reference build."""
+    return ref().ref_integrated_rho(_pd(_d(np.ravel(rho))), dom.Lx, dom.Ly, dom.Nx, dom.Ny)
+
+
+def ref_field_energy(Ex, Ey, dom):
+    """FieldGrid::fieldEnergy of the reference build."""
+    return ref().ref_field_energy(_pd(_d(np.ravel(Ex))), _pd(_d(np.ravel(Ey))), dom.Lx, dom.Ly, dom.Nx, dom.Ny)
+
+
+def ref_total_charge(q):
+    """ParticlesSoA::totalCharge of the reference build."""
+    q = _d(q)
+    return ref().ref_total_charge(_pd(q), len(q))
+
+
+def ref_kinetic_energy(vx, vy, m):
+    """ParticlesSoA::kineticEnergy of the reference build."""
+    vx, vy, m = _d(vx), _d(vy), _d(m)
+    return ref().ref_kinetic_energy(_pd(vx), _pd(vy), _pd(m), len(vx))
 
 
 def ref_tsc_weights(xi):

@@ -295,11 +295,26 @@ void or_cell_keys(const double* x, const double* y, int64_t n, const or_domain* 
 // Permutation that sorts particles by cell key, cell_sort.cpp:22-46.
 // DEVIATION: stable (std::stable_sort) because the reference's std::sort tie
 // order is an introsort artefact.  perm[i] = source index of the i-th sorted particle.
+// Non-negative keys (cell and node indices) take a counting sort: per-key
+// counts, their exclusive prefix, then particles placed in index order, which
+// is the order std::stable_sort gives.  Other keys fall back to std::stable_sort.
 void or_sort_perm(const int32_t* keys, int64_t n, int32_t* perm) {
-    std::vector<int32_t> order(static_cast<size_t>(n));
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
-    std::copy(order.begin(), order.end(), perm);
+    int32_t lo = 0, hi = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        lo = std::min(lo, keys[i]);
+        hi = std::max(hi, keys[i]);
+    }
+    if (lo < 0) {
+        std::vector<int32_t> order(static_cast<size_t>(n));
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+        std::copy(order.begin(), order.end(), perm);
+        return;
+    }
+    std::vector<int64_t> next(static_cast<size_t>(hi) + 2, 0);
+    for (int64_t i = 0; i < n; ++i) next[static_cast<size_t>(keys[i]) + 1]++;
+    for (size_t k = 1; k < next.size(); ++k) next[k] += next[k - 1];
+    for (int64_t i = 0; i < n; ++i) perm[next[static_cast<size_t>(keys[i])]++] = static_cast<int32_t>(i);
 }
 
 // Exclusive-prefix cell offsets of a key array: off[c] = number of keys < c,

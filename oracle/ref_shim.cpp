@@ -196,6 +196,31 @@ double ref_deposit_kernel_timed(int scheme, int num_threads, const double* x, co
     return mean;
 }
 
+// FieldGrid::integratedRho and FieldGrid::fieldEnergy (field_grid.cpp:24-38)
+// on a grid holding the given rho, Ex, Ey.
+double ref_integrated_rho(const double* rho, double Lx, double Ly, int Nx, int Ny) {
+    pic::FieldGrid grid(makeDomain(Lx, Ly, Nx, Ny));
+    std::copy(rho, rho + static_cast<std::size_t>(Nx) * Ny, grid.rho().begin());
+    return grid.integratedRho();
+}
+
+double ref_field_energy(const double* Ex, const double* Ey, double Lx, double Ly, int Nx, int Ny) {
+    pic::FieldGrid grid(makeDomain(Lx, Ly, Nx, Ny));
+    std::copy(Ex, Ex + static_cast<std::size_t>(Nx) * Ny, grid.Ex().begin());
+    std::copy(Ey, Ey + static_cast<std::size_t>(Nx) * Ny, grid.Ey().begin());
+    return grid.fieldEnergy();
+}
+
+// ParticlesSoA::totalCharge and ParticlesSoA::kineticEnergy
+// (particles_aos.cpp:182-196).
+double ref_total_charge(const double* q, int64_t n) {
+    return makeSoA(q, q, nullptr, nullptr, q, nullptr, n).totalCharge();
+}
+
+double ref_kinetic_energy(const double* vx, const double* vy, const double* m, int64_t n) {
+    return makeSoA(vx, vy, vx, vy, vx, m, n).kineticEnergy();
+}
+
 // tscWeights (ngp.cpp:28-34) and esirkepovWeights1D (ngp.cpp:86-93).
 void ref_tsc_weights(double xi, double* w, int* i0) { pic::tscWeights(xi, w, *i0); }
 

@@ -261,17 +261,27 @@ pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     if (cells < 0 || cells > kMaxDriftHalo)
         return fail(PIC_ERR_INVALID_ARGUMENT, "tile halo must be 0.." + std::to_string(kMaxDriftHalo) + " cells");
+    // Every scheme's tiling as it stands, restored as a whole if any scheme's
+    // tiling does not fit the new halo.
     const int old = ctx->halo;
+    int saved[4][4];
+    for (int k = 0; k < 4; ++k) {
+        saved[k][0] = ctx->geom[k].T;
+        saved[k][1] = ctx->geom[k].R;
+        saved[k][2] = ctx->ppl[k];
+        saved[k][3] = ctx->pipe[k];
+    }
     ctx->halo = cells;
     for (int k = 0; k < 4; ++k) {
         // a scheme still on its create-time tiling takes the default for this halo
-        int tx = ctx->geom[k].T, ty = ctx->geom[k].R, ppl = ctx->ppl[k];
+        int tx = saved[k][0], ty = saved[k][1], ppl = saved[k][2];
         if (ctx->tiling_default[k]) default_tiling(ctx, k, ctx->tile_capacity, cells, tx, ty, ppl);
-        const pic_status st = apply_tiling(ctx, k, tx, ty, ppl, ctx->pipe[k]);
+        const pic_status st = apply_tiling(ctx, k, tx, ty, ppl, saved[k][3]);
         if (st != PIC_OK) {
+            const std::string msg = pic_hip_last_error();
             ctx->halo = old;
-            for (int j = 0; j < 4; ++j) (void)apply_tiling(ctx, j, ctx->geom[j].T, ctx->geom[j].R, ctx->ppl[j], ctx->pipe[j]);
-            return st;
+            for (int j = 0; j < 4; ++j) (void)apply_tiling(ctx, j, saved[j][0], saved[j][1], saved[j][2], saved[j][3]);
+            return fail(st, msg);
         }
     }
     return PIC_OK;
@@ -944,6 +954,97 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
 }
 
 extern "C" {
+
+// Simulation::runProfiled (src/pic/simulation.cpp:147-292), SoA branch, on the
+// device: per step [sort] -> push -> deposit -> background -> Poisson ->
+// gather (Esirkepov: [sort] -> deposit -> background -> Poisson -> push ->
+// gather), each phase bracketed by HIP events on the context's stream and
+// executed eagerly (no graph, no fused gather + push), so the phase times are
+// kernel times plus the launch gaps between them.  The reference deposits a
+// copy of the particles taken before the push (simulation.cpp:191,203-208);
+// here the deposit reads the pushed particles, as a timestep does.
+pic_status pic_hip_run_profiled(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t warmup_steps,
+                                int32_t timed_steps, pic_timestep_profile* out) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!cfg || !out) return fail(PIC_ERR_INVALID_ARGUMENT, "null step config or output");
+    if (warmup_steps < 0 || timed_steps < 1) return fail(PIC_ERR_INVALID_ARGUMENT, "need timed_steps >= 1");
+    if (cfg->scheme < 0 || cfg->scheme > 3) return fail(PIC_ERR_INVALID_ARGUMENT, "unknown scheme");
+    if (cfg->backend != PIC_BACKEND_GPU_ATOMICS && cfg->backend != PIC_BACKEND_GPU_PRIV)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "unknown backend");
+    if (cfg->backend == PIC_BACKEND_GPU_PRIV && cfg->sort_interval < 1)
+        return fail(PIC_ERR_INVALID_ARGUMENT, "GPU_Priv steps need sort_interval >= 1");
+    if (!std::isfinite(cfg->dt)) return fail(PIC_ERR_INVALID_ARGUMENT, "dt must be finite");
+    pic_status st = ensure_fft(ctx);
+    if (st != PIC_OK) return st;
+    if ((st = ensure_sort_staging(ctx)) != PIC_OK) return st;
+    // events e[0..7] per timed step mark the phase boundaries: sort | push
+    // (not Esirkepov) | deposit | background (untimed, as in the reference) |
+    // Poisson | push (Esirkepov) | gather
+    constexpr int kEv = 8;
+    std::vector<hipEvent_t> ev((size_t)kEv * timed_steps, nullptr);
+    struct EventGuard {
+        std::vector<hipEvent_t>& v;
+        ~EventGuard() {
+            for (hipEvent_t e : v)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } guard{ev};
+    for (auto& e : ev) PIC_HIP_TRY(hipEventCreate(&e));
+    const bool priv = cfg->backend == PIC_BACKEND_GPU_PRIV;
+    const bool esk = cfg->scheme == ESK;
+    // GPU_Atomics sorts only when asked to (sort_interval >= 1), like
+    // SimulationConfig::sorted; GPU_Priv always does
+    const int interval = std::max(1, cfg->sort_interval);
+    const bool sorted = priv || cfg->sort_interval >= 1;
+    const float dt = (float)cfg->dt;
+    auto push = [&]() {
+        launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
+                           ctx->Ey, ctx->dev, dt, 0, 1, ctx->stream);
+    };
+    for (int step = 0; step < warmup_steps + timed_steps; ++step) {
+        hipEvent_t* e = step >= warmup_steps ? &ev[(size_t)kEv * (step - warmup_steps)] : nullptr;
+        auto mark = [&](int k) -> hipError_t { return e ? hipEventRecord(e[k], ctx->stream) : hipSuccess; };
+        PIC_HIP_TRY(mark(0));
+        if (sorted && step % interval == 0)
+            if ((st = sort_particles(ctx, false, priv && prefers_node_order(cfg->scheme))) != PIC_OK) return st;
+        PIC_HIP_TRY(mark(1));
+        if (!esk) push();
+        PIC_HIP_TRY(mark(2));
+        if ((st = pic_hip_deposit(ctx, cfg->scheme, cfg->backend, cfg->dt)) != PIC_OK) return st;
+        PIC_HIP_TRY(mark(3));
+        if (cfg->background != 0.0) launch_add_scalar(ctx->rho, ctx->G, (float)cfg->background, ctx->stream);
+        PIC_HIP_TRY(mark(4));
+        if ((st = pic_hip_poisson(ctx)) != PIC_OK) return st;
+        PIC_HIP_TRY(mark(5));
+        if (esk) push();
+        PIC_HIP_TRY(mark(6));
+        launch_gather_push(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->s[5], (int64_t)ctx->n, ctx->Ex,
+                           ctx->Ey, ctx->dev, dt, 1, 0, ctx->stream, nullptr, ctx->uniform[4], ctx->uniform[5]);
+        PIC_HIP_TRY(mark(7));
+        PIC_HIP_TRY(hipGetLastError());
+    }
+    if ((st = pic_hip_sync(ctx)) != PIC_OK) return st;
+    // (phase, begin event, end event)
+    const int spans[5][3] = {{0, 0, 1}, {1, esk ? 5 : 1, esk ? 6 : 2}, {2, 2, 3}, {3, 4, 5}, {4, 6, 7}};
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int t = 0; t < timed_steps; ++t) {
+        const hipEvent_t* e = &ev[(size_t)kEv * t];
+        for (const auto& sp : spans) {
+            float ms = 0.0f;
+            PIC_HIP_TRY(hipEventElapsedTime(&ms, e[sp[1]], e[sp[2]]));
+            acc[sp[0]] += ms;
+        }
+    }
+    const double n = (double)timed_steps;
+    out->sort_ms = acc[0] / n;
+    out->push_ms = acc[1] / n;
+    out->deposit_ms = acc[2] / n;
+    out->poisson_ms = acc[3] / n;
+    out->gather_ms = acc[4] / n;
+    out->total_ms = out->push_ms + out->sort_ms + out->deposit_ms + out->poisson_ms + out->gather_ms;
+    out->sort_interval = interval;
+    return PIC_OK;
+}
 
 double pic_hip_dominant_frequency(const double* signal, int64_t n, double dt) {
     if (!signal || n < 8) return 0.0;

@@ -102,6 +102,16 @@ class Diagnostics(ctypes.Structure):
                                                 "total_charge", "total_abs_charge", "charge_error")]
 
 
+class TimestepProfile(ctypes.Structure):
+    """TimestepProfile (reference src/pic/simulation.hpp:40-48)."""
+    _fields_ = [("push_ms", ctypes.c_double), ("sort_ms", ctypes.c_double), ("deposit_ms", ctypes.c_double),
+                ("poisson_ms", ctypes.c_double), ("gather_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("sort_interval", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class InitParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("seed", ctypes.c_uint64), ("amp", ctypes.c_double),
                 ("pert", ctypes.c_double), ("vth", ctypes.c_double), ("vmax", ctypes.c_double),
@@ -183,6 +193,8 @@ _SIGS = {
     "pic_hip_diagnostics": (ctypes.c_int, [_P, ctypes.POINTER(Diagnostics)]),
     "pic_hip_run": (ctypes.c_int, [_P, ctypes.POINTER(StepConfig), ctypes.c_int32, ctypes.c_int32, _f64p, _f64p,
                                    _f64p, _f64p]),
+    "pic_hip_run_profiled": (ctypes.c_int, [_P, ctypes.POINTER(StepConfig), ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.POINTER(TimestepProfile)]),
     "pic_hip_dominant_frequency": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double]),
     "pic_hip_growth_rate": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int32]),
     "pic_hip_plasma_frequency": (ctypes.c_double, [ctypes.POINTER(Domain), ctypes.c_int64]),
@@ -506,6 +518,16 @@ class Context:
         if not record_particles:
             del out["kinetic"], out["charge_error"]
         return out
+
+    def run_profiled(self, warmup: int, timed: int, scheme=DepositionScheme.CIC,
+                     backend=DepositionBackend.GPUPrivatized, dt=0.01, background=0.0, sort_interval=1) -> dict:
+        """pic_hip_run_profiled (Simulation::runProfiled): mean ms per timed
+        step of push, sort, deposit, Poisson and gather, plus their total."""
+        cfg = StepConfig(int(scheme), int(backend), int(sort_interval), ORDER_PUSH_FIRST, float(dt), float(background),
+                         0, 0)
+        out = TimestepProfile()
+        _check(_lib.pic_hip_run_profiled(self._h, ctypes.byref(cfg), int(warmup), int(timed), ctypes.byref(out)))
+        return out.as_dict()
 
     def diagnostics(self) -> Diagnostics:
         d = Diagnostics()

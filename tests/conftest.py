@@ -25,3 +25,20 @@ def hip():
     import pichip
 
     return pichip
+
+
+_PARITY = []
+
+
+@pytest.fixture(scope="session")
+def parity_log():
+    """Appends (case, metric, value) lines that the end of the run prints
+    (pytest's terminal summary), so parity figures show in quiet runs too."""
+    return _PARITY.append
+
+
+def pytest_terminal_summary(terminalreporter, exitstatus, config):
+    if _PARITY:
+        terminalreporter.section("parity figures")
+        for line in _PARITY:
+            terminalreporter.write_line(line)

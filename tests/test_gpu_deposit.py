@@ -456,3 +456,31 @@ def test_default_tiling_follows_the_drift_halo(hip, orc):
             c.deposit(scheme, 2, 0.01)
             r = oracle_rho(orc, dom, x, y, q, vx, vy, scheme, 0.01)
             assert rel_l2(c.rho().astype(np.float64), r) <= REL_L2
+
+
+def test_failed_tile_halo_leaves_every_tiling_unchanged(hip):
+    """A caller tiling that fits the LDS budget without a halo but not with a
+    4-cell halo (CIC 64 x 16) makes set_tile_halo(4) fail; every scheme keeps
+    the tiling it had (NGP stays on its create-time long-row tile instead of
+    moving to the halo-4 default) and the halo stays 0."""
+    say("failed tile halo rollback")
+    N = 1024
+    dom = hip.make_domain(1.0, 1.0, N, N)
+
+    def til(ctx, scheme):
+        t = ctx.tiling(scheme)
+        return (t.tile_x, t.tile_y, t.particles_per_lane, t.pipelined)
+
+    with hip.Context(dom, 8 * N * N) as c:
+        c.set_tiling(64, 16, 8, 0, scheme=1)
+        before = [til(c, s) for s in range(4)]
+        with pytest.raises(hip.PicError, match="LDS"):
+            c.set_tile_halo(4)
+        assert c.tile_halo == 0
+        assert [til(c, s) for s in range(4)] == before
+        # create-time tilings still follow a halo that fits
+        c.set_tiling(32, 4, 8, 0, scheme=1)
+        c.set_tile_halo(2)
+        assert til(c, 0)[:2] != before[0][:2]
+        c.set_tile_halo(0)
+        assert til(c, 0) == before[0]

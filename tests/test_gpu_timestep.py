@@ -223,3 +223,51 @@ def test_full_size_config3_timestep_properties(hip):
     # rho includes the +1/V background, so integrated rho = sum q + 1 = 0 and the
     # reference's charge error is |0 - (-1)| / 1 = 1 at every step
     np.testing.assert_allclose(rec["charge_error"], 1.0, rtol=1e-5)
+
+
+@pytest.mark.parametrize("scheme,backend,interval", [(1, 2, 1), (1, 2, 10), (2, 2, 3), (1, 1, 0), (3, 2, 2)])
+def test_run_profiled_phases_and_state(hip, scheme, backend, interval):
+    """pic_hip_run_profiled (Simulation::runProfiled, simulation.cpp:147-292):
+    every phase time is non-negative, the total is their sum, the sort phase is
+    empty without sorting, and the particles it leaves equal those of the same
+    phase sequence issued through the single-call entry points."""
+    say("run_profiled", scheme, backend, interval)
+    N = 128
+    n = 20000
+    dom = hip.make_domain(1.0, 1.0, N, N)
+    p = hip.reference_particles("maxwellian", n, dom, seed=42)
+    f = {k: v.astype(np.float32) for k, v in p.items()}
+    warm, timed, dt = 2, 6, 0.01
+    with hip.Context(dom, n) as a, hip.Context(dom, n) as b:
+        for c in (a, b):
+            c.upload(f["x"], f["y"], f["q"], f["vx"], f["vy"], f["m"])
+        prof = a.run_profiled(warm, timed, scheme, backend, dt, background=0.5, sort_interval=interval)
+        print(prof, flush=True)
+        for k in ("push_ms", "sort_ms", "deposit_ms", "poisson_ms", "gather_ms"):
+            assert prof[k] >= 0.0, k
+        total = sum(prof[k] for k in ("push_ms", "sort_ms", "deposit_ms", "poisson_ms", "gather_ms"))
+        assert prof["total_ms"] == pytest.approx(total, rel=1e-12)
+        assert prof["sort_interval"] == max(1, interval)
+        if backend == 1 and interval == 0:
+            assert prof["sort_ms"] < 0.01  # two event records, nothing between them
+        else:
+            assert prof["sort_ms"] > 0.0
+        for step in range(warm + timed):
+            if (backend == 2 or interval >= 1) and step % max(1, interval) == 0:
+                if backend == 2 and scheme in (0, 2):
+                    b.sort_nodes()
+                else:
+                    b.sort()
+            if scheme != 3:
+                b.gather_push(dt, gather=False, push=True)
+            b.deposit(scheme, backend, dt)
+            b.add_background(0.5)
+            b.poisson()
+            if scheme == 3:
+                b.gather_push(dt, gather=False, push=True)
+            b.gather_push(dt, gather=True, push=False)
+        pa, pb = a.download(), b.download()
+        # particles that left their tile since the sort go through FP64
+        # atomics, which add in scheduling order: equal to FP32 rounding
+        for k in ("x", "y", "vx", "vy"):
+            np.testing.assert_allclose(pa[k], pb[k], rtol=1e-5, atol=1e-6, err_msg=k)

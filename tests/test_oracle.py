@@ -84,12 +84,99 @@ def test_sort_matches_reference_up_to_ties(orc):
     assert np.array_equal(perm, np.argsort(keys, kind="stable"))
 
 
+def test_sort_perm_is_the_stable_order(orc):
+    """The counting sort (non-negative keys) and the comparison fallback
+    (negative keys) both give numpy's stable argsort, ties in index order."""
+    rng = np.random.default_rng(4)
+    for keys in (rng.integers(0, 7, 5000), rng.integers(0, 1 << 20, 50000), np.zeros(100, np.int64),
+                 np.arange(1000)[::-1], rng.integers(-5, 5, 3000), np.array([], np.int64)):
+        keys = keys.astype(np.int32)
+        assert np.array_equal(orc.sort_perm(keys), np.argsort(keys, kind="stable"))
+
+
 def test_cell_offsets_closed_form(orc):
     rng = np.random.default_rng(3)
     keys = rng.integers(0, 50, 1000).astype(np.int32)
     off = orc.cell_offsets(keys, 64)
     expect = np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=64))])
     assert np.array_equal(off, expect)
+
+
+def c1_particles(orc):
+    """The 128^2 / 100 particles-per-cell case: 1,638,400 positions from the
+    reference's uniform initialiser (std::mt19937(42), x then y per particle,
+    particles_aos.cpp:67-80), q = 1/100, at rest."""
+    dom = orc.domain(128.0, 128.0, 128, 128)
+    p = orc.init("maxwellian", 128 * 128 * 100, dom, seed=42)
+    p["q"][:] = 1.0 / 100
+    p["vx"][:] = 0.0
+    p["vy"][:] = 0.0
+    return dom, p
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+def test_c1_deposit_bit_exact_vs_reference(orc, scheme):
+    """All four schemes on the full 128^2 x 100 ppc set: the oracle's serial
+    deposit equals the reference build's (depositChargeSoA, CPU backend) bit
+    for bit, and the initial positions are the reference initialiser's."""
+    _need_ref(orc)
+    dom, p = c1_particles(orc)
+    refp = orc.ref_init(0, len(p["x"]), dom, p1=1.0, seed=42)
+    assert np.array_equal(p["x"], refp["x"]) and np.array_equal(p["y"], refp["y"])
+    dt = 0.01 if scheme == 3 else 0.0
+    mine = orc.deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt)
+    ref = orc.ref_deposit(scheme, p["x"], p["y"], p["q"], dom, p["vx"], p["vy"], dt, threads=1)
+    assert np.array_equal(mine, ref)
+    np.testing.assert_allclose(orc.integrated_rho(mine, dom), 1638400 / 100, rtol=1e-12)
+
+
+@pytest.mark.parametrize("geom", [(1.0, 1.0, 32, 32), (128.0, 128.0, 128, 128), (3.7, 1.3, 24, 40)])
+def test_energy_sums_bit_exact_vs_reference(orc, geom):
+    """FieldGrid::integratedRho / fieldEnergy (field_grid.cpp:24-38) and
+    ParticlesSoA::totalCharge / kineticEnergy (particles_aos.cpp:182-196) of the
+    reference build against the oracle's sums, on random grids and particles."""
+    _need_ref(orc)
+    Lx, Ly, nx, ny = geom
+    dom = orc.domain(Lx, Ly, nx, ny)
+    rng = np.random.default_rng(nx * 7 + ny)
+    rho, ex, ey = (rng.normal(size=(ny, nx)) for _ in range(3))
+    assert orc.integrated_rho(rho, dom) == orc.ref_integrated_rho(rho, dom)
+    assert orc.field_energy(ex, ey, dom) == orc.ref_field_energy(ex, ey, dom)
+    n = 10007
+    vx, vy = rng.normal(size=n), rng.normal(size=n)
+    m = rng.uniform(0.5, 2.0, n)
+    q = rng.normal(size=n)
+    assert orc.kinetic_energy(vx, vy, m) == orc.ref_kinetic_energy(vx, vy, m)
+    assert orc.total_charge(q) == orc.ref_total_charge(q)
+
+
+def test_energy_sums_closed_forms(orc):
+    """Values fixed by the mathematics: a single Fourier mode's field energy,
+    a uniform beam's kinetic energy, a uniform density's integral, and the
+    charge error of diagnostics.cpp:18-29 (not buildable here) on hand cases.
+    A dropped 1/2, a missing cell volume or a swapped component fails one."""
+    Lx, Ly, nx, ny = 2.0, 0.5, 64, 16
+    dom = orc.domain(Lx, Ly, nx, ny)
+    xs = np.arange(nx) * (Lx / nx)
+    ys = np.arange(ny) * (Ly / ny)
+    A, B = 0.7, 1.9
+    ex = np.tile(A * np.cos(2 * np.pi * 3 * xs / Lx), (ny, 1))
+    ey = np.tile((B * np.sin(2 * np.pi * 2 * ys / Ly))[:, None], (1, nx))
+    # sum over the grid of cos^2 (sin^2) of a non-Nyquist mode is half the cells
+    np.testing.assert_allclose(orc.field_energy(ex, ey, dom), (A * A + B * B) * Lx * Ly / 4, rtol=1e-13)
+    np.testing.assert_allclose(orc.field_energy(ex, 0 * ey, dom), A * A * Lx * Ly / 4, rtol=1e-13)
+    np.testing.assert_allclose(orc.field_energy(0 * ex, ey, dom), B * B * Lx * Ly / 4, rtol=1e-13)
+    n = 1000
+    vx, vy, m = np.full(n, 0.3), np.full(n, -0.4), np.full(n, 2.0)
+    np.testing.assert_allclose(orc.kinetic_energy(vx, vy, m), n * 0.5 * 2.0 * 0.25, rtol=1e-13)
+    np.testing.assert_allclose(orc.kinetic_energy(vx, 0 * vy, m), n * 0.5 * 2.0 * 0.09, rtol=1e-13)
+    rho = np.full((ny, nx), 3.0)
+    np.testing.assert_allclose(orc.integrated_rho(rho, dom), 3.0 * Lx * Ly, rtol=1e-14)
+    q = np.tile([1.0, -1.0], 50)
+    assert orc.total_charge(q) == 0.0
+    assert orc.total_abs_charge(q) == 100.0
+    assert orc.charge_error(2.5, 2.0, 10.0) == 0.05
+    assert orc.charge_error(1.0, 3.0, 0.0) == 2.0 / 1e-12
 
 
 def test_gather_push_bit_exact_vs_reference(orc):
