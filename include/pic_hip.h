@@ -264,6 +264,27 @@ typedef struct {
 pic_status pic_hip_run(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nsteps, int32_t record_from,
                        double* field_energy, double* mode1_amplitude, double* kinetic_energy, double* charge_error);
 
+/* ---- allocation-free deposit ----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (work under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the global-atomic deposit of pic_hip_deposit with backend
+ * GPU_ATOMICS, on caller-owned device arrays, enqueued on `stream` (a
+ * hipStream_t; NULL is the null stream) of the current device, asynchronous.
+ * x, y, q (and vx, vy for the Esirkepov scheme; otherwise they may be NULL)
+ * hold n FP32 particles in any order; rho (Nx * Ny FP32, row-major) is
+ * overwritten.  The call allocates nothing: its FP64 accumulation grid is a
+ * file-scope device array of PIC_NOALLOC_MAX_CELLS cells (16 MiB plus one
+ * 256-byte status line per device, held until the process ends); larger grids
+ * are rejected with PIC_ERR_INVALID_ARGUMENT.  The array is cleared at the
+ * start of every call; calls that overlap on one device (several streams or
+ * host threads) race on it and must be serialised by the caller.  An
+ * Esirkepov move longer than 15 cells, which pic_hip_sync reports for a
+ * context, sets every cell of rho to NaN instead. */
+#define PIC_NOALLOC_MAX_CELLS 2097152
+pic_status pic_hip_deposit_noalloc(const pic_domain* domain, const float* x, const float* y, const float* vx,
+                                   const float* vy, const float* q, size_t n, int scheme, double dt, float* rho,
+                                   void* stream);
+
 /* Per-phase timing of the timestep, replacing Simulation::runProfiled
  * (src/pic/simulation.cpp:147-292, TimestepProfile simulation.hpp:40-48; rows
  * written by benchmark_runner.cpp:224-306).  Runs warmup_steps + timed_steps

@@ -193,6 +193,8 @@ _SIGS = {
     "pic_hip_diagnostics": (ctypes.c_int, [_P, ctypes.POINTER(Diagnostics)]),
     "pic_hip_run": (ctypes.c_int, [_P, ctypes.POINTER(StepConfig), ctypes.c_int32, ctypes.c_int32, _f64p, _f64p,
                                    _f64p, _f64p]),
+    "pic_hip_deposit_noalloc": (ctypes.c_int, [ctypes.POINTER(Domain), _P, _P, _P, _P, _P, ctypes.c_size_t,
+                                               ctypes.c_int, ctypes.c_double, _P, _P]),
     "pic_hip_run_profiled": (ctypes.c_int, [_P, ctypes.POINTER(StepConfig), ctypes.c_int32, ctypes.c_int32,
                                             ctypes.POINTER(TimestepProfile)]),
     "pic_hip_dominant_frequency": (ctypes.c_double, [_f64p, ctypes.c_int64, ctypes.c_double]),
@@ -270,6 +272,16 @@ def _f32(a):
 
 def version() -> str:
     return _lib.pic_hip_version().decode()
+
+
+NOALLOC_MAX_CELLS = 2097152
+
+
+def deposit_noalloc(domain: Domain, x, y, q, n: int, rho, scheme=DepositionScheme.CIC, vx=None, vy=None,
+                    dt: float = 0.0, stream=None) -> None:
+    """pic_hip_deposit_noalloc on device addresses (ints) of FP32 arrays."""
+    _check(_lib.pic_hip_deposit_noalloc(ctypes.byref(domain), x, y, vx, vy, q, int(n), int(scheme), float(dt), rho,
+                                        stream))
 
 
 def make_domain(Lx: float, Ly: float, Nx: int, Ny: int) -> Domain:
