@@ -485,6 +485,43 @@ def profile_step(pichip, mode, steps):
             "ms_per_step_incl_capture": wall * 1000.0 / steps}
 
 
+def profile_kernels(pichip, steps, copy_probe=False):
+    """The kernels whose counters are reported, `steps` launches each, at the
+    headline size (1024^2, 64 ppc): the CIC tile kernel (cell order), the TSC
+    tile kernel (node order), the Esirkepov tile kernel, the CIC global-atomic
+    kernel on the initialiser's (random) order, and the cell sort from that
+    random order, of the sorted set and after ten drift steps."""
+    dom = pichip.make_domain(1.0, 1.0, NX, NX)
+    c = pichip.Context(dom, N)
+
+    def init():
+        c.init_particles(N, kind=1, seed=42, amp=0.01, mode=4, q=1.0 / PPC, vmax=0.5 / NX / DT)
+
+    init()
+    c.sort()
+    if copy_probe:
+        c.copy_probe(1 << 30)
+    precondition(c, 0.3)
+    for scheme, sort in ((1, c.sort), (2, c.sort_nodes), (3, c.sort)):
+        sort()
+        for _ in range(steps):
+            c.deposit(scheme, 2, DT)
+    init()
+    for _ in range(steps):
+        c.deposit(1, 1, DT)
+    for _ in range(2):
+        init()
+        c.sort()
+    c.sort()
+    for _ in range(2):  # after ten drift steps of up to a quarter cell per axis
+        for _ in range(10):
+            c.gather_push(DT * 0.5, gather=False, push=True)
+        c.sort()
+    c.sync()
+    c.close()
+    return {"profile_kernels": steps, "particles": N, "grid": [NX, NX]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -507,6 +544,9 @@ def main():
                          "kernel profiles (the learned network is untrained: kernel times do not depend on weights)")
     ap.add_argument("--profile-ensemble", action="store_true",
                     help="only run the 64-member ensemble deposit and step timings, for kernel profiles")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only run the CIC / TSC / Esirkepov tile kernels, the atomic kernel on a random order and "
+                         "the sort, --steps launches each, for counter profiles")
     ap.add_argument("--profile-timestep", action="store_true",
                     help="only run the C3 timestep measurement, for kernel profiles")
     args = ap.parse_args()
@@ -525,6 +565,9 @@ def main():
         return
     if args.profile_timestep:
         print(json.dumps(timestep_c3(pichip, args.steps)), flush=True)
+        return
+    if args.profile_kernels:
+        print(json.dumps(profile_kernels(pichip, args.steps, args.copy_probe)), flush=True)
         return
 
     # The library's stream is the only one doing work; ctx.sync() (a

@@ -431,7 +431,7 @@ void swap_permuted(pic_hip_ctx* ctx) {
 // Buffers of the staged permutation (outside stream capture).
 pic_status ensure_sort_staging(pic_hip_ctx* ctx) {
     if (ctx->sort_rec) return PIC_OK;
-    PIC_HIP_TRY(dmalloc(&ctx->far_part, kFarParts));
+    PIC_HIP_TRY(dmalloc(&ctx->far_part, sort_part_elems((int64_t)ctx->cap)));
     PIC_HIP_TRY(dmalloc(&ctx->permute_mode, 1));
     PIC_HIP_TRY(hipMemsetAsync(ctx->permute_mode, 0, sizeof(int), ctx->stream));
     PIC_HIP_TRY(dmalloc(&ctx->sort_rec, 2 * ctx->cap));
@@ -455,16 +455,18 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back, bool node) {
     const int64_t n = (int64_t)ctx->n;
     if (ctx->sort_rec && n >= kStagedSortMin) {
         // The device decides per sort between a direct gather of each stream
-        // (an order close to the last sort's) and packed records (an order far
-        // from it: one or two lines per particle instead of one per stream).
-        // Below kStagedSortMin particles the random gather is short and the
-        // decision's own launches would cost more than they save.
-        launch_keys_far(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, 4 * ctx->dom.Nx, node, ctx->far_part,
-                        ctx->permute_mode, ctx->stream);
+        // (most particles kept their cell since the last sort, so the
+        // permutation moves long runs), packed records (more than an eighth
+        // changed cell: the runs are short, and one or two 16-byte records per
+        // particle read fewer lines than one word per stream), and a copy (the
+        // order is still sorted).  Below kStagedSortMin particles the random
+        // gather is short and the decision's own launches would cost more than
+        // they save.
+        launch_keys_hist(ctx->s[0], ctx->s[1], n, ctx->dev, (int)ctx->G, ctx->sb, 0, node,
+                         ctx->far_part, ctx->permute_mode, ctx->stream);
         launch_pack_streams(fwd, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
-        PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));
-        launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream, ctx->permute_mode);
-        launch_gather_unpack(fwd, ctx->sb.perm, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
+        PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream, ctx->permute_mode, true));
+        launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream, ctx->permute_mode, ctx->sort_rec);
     } else {
         launch_keys(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, ctx->stream, node);
         PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));

@@ -42,7 +42,7 @@ struct pic_hip_ctx {
     // (far_part, per-block counts of far moves, and permute_mode); allocated before
     // the first sort that can use it.
     float4* sort_rec = nullptr;
-    unsigned* far_part = nullptr;  // kFarParts
+    unsigned* far_part = nullptr;  // sort_part_elems(cap): per-block far / descent counts
     int* permute_mode = nullptr;
     SortBuffers sb{};
     bool order_valid = false;

@@ -75,19 +75,59 @@ __global__ __launch_bounds__(256) void ens_init_kernel(EnsParticles p, EnsTables
 // cbase + nearest node (the node-keyed order their tile kernels take);
 // padding slots get the member's last key cbase + G, so a stable sort keeps
 // them at its end.
-__global__ __launch_bounds__(256) void ens_keys_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                       EnsTables t, int* __restrict__ keys) {
-    const int64_t c = blockIdx.x;
-    const int mi = find_member(t.pchunk, t.M, c);
-    const EnsMember& m = t.mem[mi];
-    const DevDomain d = m.d;
+__device__ __forceinline__ int ens_key_at(const float* __restrict__ x, const float* __restrict__ y,
+                                          const EnsMember& m, int64_t i) {
+    const int64_t local = i - m.pbase;
+    if (local >= m.n) return (int)(m.cbase + m.G);
     const bool node = m.scheme == NGP || m.scheme == TSC;
-    for (int j = 0; j < kEnsChunk / 256; ++j) {
-        const int64_t i = c * kEnsChunk + j * 256 + threadIdx.x;
-        const int64_t local = i - m.pbase;
-        const int k = local < m.n ? (node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d)) : 0;
-        keys[i] = (int)(m.cbase + (local < m.n ? (int64_t)k : m.G));
+    const int k = node ? node_key_exact(x[i], y[i], m.d) : cell_key_exact(x[i], y[i], m.d);
+    return (int)(m.cbase + k);
+}
+
+// One block per chunk (every chunk belongs to one member); each wave takes a
+// contiguous quarter of it.  With `part`, also per chunk: part[2c] the slots
+// whose key differs from the key at the same position after the last sort
+// (`keys` holds those on entry), part[2c + 1] the slots whose key is below
+// the previous slot's (0 everywhere: the order is already sorted).
+__global__ __launch_bounds__(256) void ens_keys_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                       EnsTables t, int* __restrict__ keys,
+                                                       unsigned* __restrict__ part) {
+    constexpr int J = kEnsChunk / 256;
+    const int64_t c = blockIdx.x;
+    const EnsMember& m = t.mem[find_member(t.pchunk, t.M, c)];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t wb = c * kEnsChunk + (int64_t)w * (64 * J);
+    int kk[J];
+    unsigned nchg = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int64_t i = wb + j * 64 + lane;
+        kk[j] = ens_key_at(x, y, m, i);
+        if (part) nchg += (unsigned)__popcll(__ballot(kk[j] != keys[i]));
+        keys[i] = kk[j];
     }
+    if (!part) return;
+    int first_prev = -1;  // the slot before the wave's first (another chunk's for w == 0)
+    if (lane == 0 && wb > 0) {
+        const EnsMember& mp = w > 0 ? m : t.mem[find_member(t.pchunk, t.M, c - 1)];
+        first_prev = ens_key_at(x, y, mp, wb - 1);
+    }
+    unsigned ndesc = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int up = __shfl_up(kk[j], 1, 64);
+        const int last_prev = j > 0 ? __shfl(kk[j - 1], 63, 64) : __shfl(first_prev, 0, 64);
+        const int prev = lane > 0 ? up : last_prev;
+        ndesc += (unsigned)__popcll(__ballot(kk[j] < prev));
+    }
+    __shared__ unsigned cnt[2][4];
+    if (lane == 0) {
+        cnt[0][w] = nchg;
+        cnt[1][w] = ndesc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+        part[2 * c + threadIdx.x] = cnt[threadIdx.x][0] + cnt[threadIdx.x][1] + cnt[threadIdx.x][2] + cnt[threadIdx.x][3];
 }
 
 __global__ void set_ints_kernel(int* __restrict__ p, int n, int v) {
@@ -348,8 +388,10 @@ void launch_ens_init(const EnsParticles& p, const EnsTables& t, int64_t nchunks,
     if (nchunks > 0) hipLaunchKernelGGL(ens_init_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, p, t);
 }
 
-void launch_ens_keys(const EnsParticles& p, const EnsTables& t, int64_t nchunks, int* keys, hipStream_t s) {
-    if (nchunks > 0) hipLaunchKernelGGL(ens_keys_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, p.x, p.y, t, keys);
+void launch_ens_keys(const EnsParticles& p, const EnsTables& t, int64_t nchunks, int* keys, hipStream_t s,
+                     unsigned* part) {
+    if (nchunks > 0)
+        hipLaunchKernelGGL(ens_keys_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, p.x, p.y, t, keys, part);
 }
 
 void launch_set_ints(int* p, int n, int v, hipStream_t s) {

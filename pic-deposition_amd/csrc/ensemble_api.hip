@@ -57,6 +57,11 @@ struct pic_hip_ensemble {
     float* partial = nullptr;
     size_t partial_cap = 0;
     int* status = nullptr;
+    // sort: per-chunk key-change / descent counts, the device-chosen
+    // permutation mode, and the packed-record staging (two float4 per slot)
+    unsigned* sort_part = nullptr;
+    int* permute_mode = nullptr;
+    float4* sort_rec = nullptr;
     int* counters = nullptr;       // one per work group
     int* diag_counters = nullptr;  // one per member
     int* push_flag = nullptr;
@@ -86,7 +91,8 @@ struct pic_hip_ensemble {
                         rho,       acc64,       Ex,          Ey,             fft_in,     spec,
                         spec_ex,   spec_ey,     k2,          cosk,           sink,       sym,         partial,
                         status,    counters,    diag_counters, push_flag,    rho_part,   field_part,  part_p,
-                        rho_sum,   d_mem,       d_pchunk,    d_gblk,         d_sblk,     d_items};
+                        rho_sum,   d_mem,       d_pchunk,    d_gblk,         d_sblk,     d_items,
+                        sort_part, permute_mode, sort_rec};
         for (void* b : bufs) (void)hipFree(b);
         (void)hipHostFree(h_status);
         if (stream) (void)hipStreamDestroy(stream);
@@ -220,9 +226,10 @@ pic_status ensure_fft(pic_hip_ensemble* e) {
     return PIC_OK;
 }
 
+// The permutation is chosen on the device per sort (as for a context): a
+// copy when the order is still sorted, packed records when more than an
+// eighth of the slots changed key, else a gather per stream.
 pic_status enqueue_sort(pic_hip_ensemble* e, bool copy_back) {
-    launch_ens_keys(e->particles(), e->tables(), e->nchunks, e->sb.keys, e->stream);
-    PIC_HIP_TRY(launch_sort_precomputed(e->nslots, (int)e->keys_total, e->sb, e->stream));
     StreamSet fwd{}, back{};
     fwd.count = back.count = kStreams;
     for (int i = 0; i < kStreams; ++i) {
@@ -231,7 +238,11 @@ pic_status enqueue_sort(pic_hip_ensemble* e, bool copy_back) {
         back.src[i] = e->alt[i];
         back.dst[i] = e->s[i];
     }
-    launch_permute_streams(fwd, e->sb.perm, e->nslots, e->stream);
+    launch_ens_keys(e->particles(), e->tables(), e->nchunks, e->sb.keys, e->stream, e->sort_part);
+    launch_permute_mode(e->sort_part, (int)e->nchunks, e->nslots, e->permute_mode, e->stream);
+    launch_pack_streams(fwd, e->nslots, e->permute_mode, e->sort_rec, e->stream);
+    PIC_HIP_TRY(launch_sort_precomputed(e->nslots, (int)e->keys_total, e->sb, e->stream, e->permute_mode));
+    launch_permute_streams(fwd, e->sb.perm, e->nslots, e->stream, e->permute_mode, e->sort_rec);
     if (copy_back)
         launch_permute_streams(back, nullptr, e->nslots, e->stream);
     else
@@ -449,6 +460,9 @@ pic_status pic_hip_ensemble_create(const pic_ensemble_member* members, int32_t c
     e->sb.scan_tmp_cap = sort_scan_tmp_elems(e->sb.hist_cap);
     PIC_HIP_TRY(dmalloc(&e->sb.scan_tmp, e->sb.scan_tmp_cap));
     PIC_HIP_TRY(dmalloc(&e->sb.cell_off, (size_t)keys + 1));
+    PIC_HIP_TRY(dmalloc(&e->sort_part, 2 * (size_t)std::max<int64_t>(e->nchunks, 1)));
+    PIC_HIP_TRY(dmalloc(&e->permute_mode, 1));
+    PIC_HIP_TRY(dmalloc(&e->sort_rec, 2 * (size_t)cap));
     PIC_HIP_TRY(dmalloc(&e->acc64, (size_t)G));
     PIC_HIP_TRY(hipMemsetAsync(e->acc64, 0, (size_t)G * sizeof(double), e->stream));
     float** grids[] = {&e->rho, &e->Ex, &e->Ey, &e->fft_in};

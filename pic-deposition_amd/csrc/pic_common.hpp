@@ -60,11 +60,18 @@ __device__ __forceinline__ int wrap_index(int i, int n) {
     return r < 0 ? r + n : r;
 }
 
+// x / dx in FP64 as Domain::cellCoords divides (domain.hpp:34-43).  With a
+// power-of-two cell size the product with its (exact) reciprocal is the same
+// FP64 value (both are the exact quotient, which needs no rounding).
+__device__ __forceinline__ double cell_units_exact(float x, double dd, float inv, int exact) {
+    return exact ? (double)x * (double)inv : (double)x / dd;
+}
+
 // Cell key of a position exactly as particleCellIndex (src/sorting/cell_sort.cpp:8-13)
 // via Domain::cellCoords (domain.hpp:34-43): FP64 division, floor, wrap.
 __device__ __forceinline__ int cell_key_exact(float x, float y, const DevDomain& d) {
-    const double fx = (double)x / d.dx;
-    const double fy = (double)y / d.dy;
+    const double fx = cell_units_exact(x, d.dx, d.inv_dx, d.exact);
+    const double fy = cell_units_exact(y, d.dy, d.inv_dy, d.exact);
     const int ix = wrap_index((int)floor(fx), d.Nx);
     const int iy = wrap_index((int)floor(fy), d.Ny);
     return iy * d.Nx + ix;
@@ -124,8 +131,8 @@ __device__ __forceinline__ void node_cell(float x, float y, const DevDomain& d, 
 // Sort key of the node-keyed order: the wrapped nearest node, from the same
 // FP64 division as cell_key_exact.
 __device__ __forceinline__ int node_key_exact(float x, float y, const DevDomain& d) {
-    const double fx = (double)x / d.dx;
-    const double fy = (double)y / d.dy;
+    const double fx = cell_units_exact(x, d.dx, d.inv_dx, d.exact);
+    const double fy = cell_units_exact(y, d.dy, d.inv_dy, d.exact);
     const double flx = floor(fx), fly = floor(fy);
     const int ix = wrap_index((int)flx + ((float)(fx - flx) >= 0.5f ? 1 : 0), d.Nx);
     const int iy = wrap_index((int)fly + ((float)(fy - fly) >= 0.5f ? 1 : 0), d.Ny);

@@ -90,7 +90,11 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
                                  hipStream_t s);
 // Stable sort of precomputed keys in sb.keys (each in [0, key_range)):
 // same outputs as launch_sort_keys_perm, cell_off with key_range + 1 entries.
-hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s);
+// With `mode` (set by launch_keys_hist), mode 2 (keys already in order) skips
+// the radix passes and leaves sb.perm unset; first_hist_done: the first
+// pass's histogram table is already in sb.hist (launch_keys_hist).
+hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s,
+                                   const int* mode = nullptr, bool first_hist_done = false);
 // dst[i] = src[perm[i]] for one float stream.
 void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n, hipStream_t s);
 // dst_k[i] = src_k[perm[i]] for ss.count streams in one pass (perm == nullptr: copy).
@@ -100,21 +104,26 @@ struct StreamSet {
     float* dst[kMaxStreams];
     int count;
 };
-// dst[k][i] = src[k][perm[i]] (perm null: identity); skipped when *mode != 0.
+// dst[k][i] = src[k][perm[i]] (perm null or *mode == 2: identity); with
+// *mode == 1 from the records of launch_pack_streams (rec[W perm[i] + h],
+// W = 1 for up to four streams, else 2).
 void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s,
-                            const int* mode = nullptr);
-// Cell keys into `keys` (which holds the keys of the last sort's order), the
-// count of particles whose key moved by more than `far`, and from it *mode:
-// 1 when more than a quarter moved far (then the permutation goes through
-// launch_pack_streams before the radix passes and launch_gather_unpack after
-// them), else 0 (launch_permute_streams).
-constexpr int kFarParts = 1024;  // blocks (and per-block counts) of launch_keys_far
-void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far, bool node,
-                     unsigned* far_part, int* mode, hipStream_t s);
+                            const int* mode = nullptr, const float4* rec = nullptr);
+// Cell keys (node keys with `node`) of the current order into sb.keys (which
+// holds the keys of the last sort's order), the first radix pass's histogram
+// table, and from per-block counts (`part`, sort_part_elems(n) values) the
+// permutation mode *mode: 2 when the keys are already in order (the passes
+// are skipped and the streams copied), else 1 when more than an eighth of the
+// particles moved by more than `far` keys (the permutation goes through
+// launch_pack_streams before the radix passes and the records after them),
+// else 0 (launch_permute_streams gathers stream by stream).
+size_t sort_part_elems(int64_t n);
+// *mode from nparts pairs (changed, descents) of per-block counts.
+void launch_permute_mode(const unsigned* part, int nparts, int64_t n, int* mode, hipStream_t s);
+void launch_keys_hist(const float* x, const float* y, int64_t n, const DevDomain& d, int key_range, SortBuffers& sb,
+                      int far, bool node, unsigned* part, int* mode, hipStream_t s);
 // rec: n (up to 4 streams) or 2n (5 or 6 streams) float4 records.
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s);
-void launch_gather_unpack(const StreamSet& ss, const int* perm, int64_t n, const int* mode, const float4* rec,
-                          hipStream_t s);
 // Cell keys (particleCellIndex), or with `node` the keys of the node-keyed
 // order (nearest node, node_key_exact).
 void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s,
@@ -269,7 +278,10 @@ struct EnsWork {
 int ens_deposit_grid(const EnsWork& w, int ncu);
 
 void launch_ens_init(const EnsParticles& p, const EnsTables& t, int64_t nchunks, hipStream_t s);
-void launch_ens_keys(const EnsParticles& p, const EnsTables& t, int64_t nchunks, int* keys, hipStream_t s);
+// With `part` (2 * nchunks values), also the per-chunk counts that
+// launch_permute_mode turns into a permutation mode (see launch_keys_hist).
+void launch_ens_keys(const EnsParticles& p, const EnsTables& t, int64_t nchunks, int* keys, hipStream_t s,
+                     unsigned* part = nullptr);
 // counters: one int per group, reset by the launch itself.
 hipError_t launch_ens_deposit(const EnsWork* groups, int ngroups, const EnsParticles& p, const int* cell_off,
                               const EnsTables& t, float dt, float* partial, double* gacc,

@@ -1,11 +1,23 @@
-// Cell sort for gfx950: exact cell keys, stable LSD radix sort of (key, index)
-// pairs with 8-bit digits, permutation of the particle streams, and the
-// cell-offset table that the tiled deposit reads.
+// Cell sort for gfx950: exact cell keys, a stable LSD radix sort of (key,
+// index) pairs with digits of up to 10 bits (two passes for 2^20 cells), the
+// permutation of the particle streams, and the cell-offset table that the
+// tiled deposit reads.
 //
 // Keys follow particleCellIndex (reference src/sorting/cell_sort.cpp:8-13) in
 // FP64, so they are bit-identical to the reference's.  The reference orders
 // with std::sort (cell_sort.cpp:28-33), whose order among equal keys is an
 // introsort artefact; this sort is stable, i.e. equal to std::stable_sort.
+//
+// One radix pass = a histogram kernel (per block and digit), an exclusive
+// scan of that table (digit-major), and a scatter kernel.  A scatter block of
+// 4096 items ranks its items stably in LDS (each wave ranks a contiguous
+// quarter of the block in item order against a wave-private digit histogram;
+// a block-wide scan then combines the waves), stages them in
+// digit order in LDS and writes them out so that consecutive threads write
+// consecutive addresses of a digit's run.  Blocks map to item ranges
+// XCD-contiguously (blocks are dispatched round-robin over the 8 XCDs; block
+// b takes range (b mod 8) * q + b / 8), so the runs of neighbouring ranges,
+// which are adjacent in the output, are written through the same L2.
 
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
@@ -15,10 +27,14 @@ namespace pichip {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItems = 16;                       // items per thread per block
-constexpr int kBlockItems = kThreads * kItems;  // 4096
-constexpr int kDigits = 256;
+constexpr int kItems = 16;
+constexpr int kBlockItems = kThreads * kItems;  // 4096 (scan blocks)
 constexpr int kWaves = kThreads / 64;
+
+constexpr int kRT = 256;        // radix block threads
+constexpr int kRW = kRT / 64;   // waves per radix block
+constexpr int kMaxBins = 1024;  // digits of at most 10 bits
+constexpr int kXcds = 8;        // MI355X: workgroups are dispatched round-robin over 8 XCDs
 
 __global__ __launch_bounds__(256) void keys_kernel(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
                                                    DevDomain d, int* __restrict__ keys, int node) {
@@ -26,98 +42,308 @@ __global__ __launch_bounds__(256) void keys_kernel(const float* __restrict__ x, 
     if (i < n) keys[i] = node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d);
 }
 
-// Per-block digit histogram.  Keys re-sorted between steps arrive nearly in
-// order, so a wave mostly holds one or two digits: the first two distinct
-// digits of each wave are counted with one LDS add each (ballot + popcount),
-// the rest with one LDS atomic per lane.
-template <int ITEMS>
-__global__ __launch_bounds__(256) void radix_hist_kernel(const int* __restrict__ keys, int64_t n, int shift,
-                                                         int nblocks, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kDigits];
-    h[threadIdx.x] = 0;
-    __syncthreads();
+// Logical block (item range) of this workgroup; ranges >= nblocks are idle.
+__device__ __forceinline__ int logical_block(int q) { return (int)(blockIdx.x % kXcds) * q + (int)(blockIdx.x / kXcds); }
+
+// Adds the digits of one slot of 64 items to an LDS histogram.  Keys
+// re-sorted between steps arrive nearly in order, so a wave often holds one
+// or two digits: the first two distinct digits are counted with one LDS add
+// each (ballot + popcount), the rest with one LDS atomic per lane (a k-way
+// address conflict costs about k cycles, cheap for the few digits left).
+__device__ __forceinline__ void hist_add(uint32_t* __restrict__ h, bool valid, unsigned dg) {
     const int lane = threadIdx.x & 63;
-    const int64_t base = (int64_t)blockIdx.x * (kThreads * ITEMS);
+    uint64_t pending = __ballot(valid);
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const int64_t i = base + (int64_t)j * kThreads + threadIdx.x;
-        const bool valid = i < n;
-        const unsigned dg = valid ? ((unsigned)keys[i] >> shift) & (kDigits - 1) : 0u;
-        bool pending = valid;
-#pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            const uint64_t act = __ballot(pending);
-            if (act == 0) break;  // wave-uniform
-            const int leader = __ffsll((unsigned long long)act) - 1;
-            const unsigned d0 = (unsigned)__shfl((int)dg, leader, 64);
-            const bool same = pending && dg == d0;
-            const uint64_t m = __ballot(same);
-            if (lane == leader) atomicAdd(&h[d0], (uint32_t)__popcll(m));
-            if (same) pending = false;
-        }
-        if (pending) atomicAdd(&h[dg], 1u);
+    for (int it = 0; it < 2; ++it) {
+        if (pending == 0) break;  // wave-uniform
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const unsigned d0 = (unsigned)__builtin_amdgcn_readlane((int)dg, leader);
+        const uint64_t m = __ballot(dg == d0) & pending;
+        if (lane == leader) atomicAdd(&h[d0], (uint32_t)__popcll(m));
+        pending &= ~m;
     }
-    __syncthreads();
-    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    if ((pending >> lane) & 1ull) atomicAdd(&h[dg], 1u);
 }
 
-// Stable scatter: items of a block are processed in ITEMS slots of 256
-// consecutive items; within a slot, equal digits are ranked in thread order
-// with wave ballots, across waves and slots with LDS counters.
-template <int ITEMS>
-__global__ __launch_bounds__(256) void radix_scatter_kernel(const int* __restrict__ keys_in,
-                                                            const int* __restrict__ vals_in, int64_t n, int shift,
-                                                            int nblocks, const uint32_t* __restrict__ hist_scan,
-                                                            int* __restrict__ keys_out, int* __restrict__ vals_out) {
-    __shared__ uint32_t gbase[kDigits];
-    __shared__ uint32_t run[kDigits];
-    __shared__ uint32_t cnt[kWaves][kDigits];
-    const int t = threadIdx.x;
-    const int lane = t & 63;
-    const int w = t >> 6;
-    gbase[t] = hist_scan[(size_t)t * nblocks + blockIdx.x];
-    run[t] = 0;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * (kThreads * ITEMS);
-    for (int j = 0; j < ITEMS; ++j) {
-        const int64_t i = base + (int64_t)j * kThreads + t;
-        const bool valid = i < n;
-        int key = 0, val = 0;
-        if (valid) {
-            key = keys_in[i];
-            val = vals_in ? vals_in[i] : (int)i;
-        }
-        const unsigned dg = ((unsigned)key >> shift) & (kDigits - 1);
+// Lanes of this wave holding the same digit as this lane (a 64-bit lane
+// mask; empty for an invalid lane).  Up to twelve distinct digits are taken
+// one at a time (a read of the first pending lane's digit and one ballot
+// each: a nearly sorted slot holds a few), until a digit holds a single
+// lane; lanes still pending after that are matched with one ballot per digit
+// bit.
+template <int MAXBITS>
+__device__ __forceinline__ uint64_t digit_peers(bool valid, unsigned dg, int nbits) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(valid);
+    uint64_t m = 0;
+    for (int it = 0; it < 12 && pending != 0; ++it) {  // wave-uniform
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const unsigned d0 = (unsigned)__builtin_amdgcn_readlane((int)dg, leader);
+        const uint64_t same = __ballot(dg == d0) & pending;
+        if (dg == d0 && ((pending >> lane) & 1ull)) m = same;
+        pending &= ~same;
+        if (__popcll(same) < 2) break;  // a lone digit: an unordered slot
+    }
+    if (pending != 0) {  // wave-uniform
+        const bool mine = (pending >> lane) & 1ull;
+        uint32_t lo = (uint32_t)pending, hi = (uint32_t)(pending >> 32);
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) cnt[ww][t] = 0;
-        __syncthreads();
-        // Lanes of this wave holding the same digit (invalid lanes match nobody).
-        uint64_t match = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const uint64_t bal = __ballot(valid && ((dg >> b) & 1u));
-            match &= ((dg >> b) & 1u) ? bal : ~bal;
-        }
-        const int wrank = __popcll(match & lt_mask);
-        if (valid && wrank == 0) cnt[w][dg] = (uint32_t)__popcll(match);
-        __syncthreads();
-        {
-            uint32_t acc = run[t];
-#pragma unroll
-            for (int ww = 0; ww < kWaves; ++ww) {
-                const uint32_t c = cnt[ww][t];
-                cnt[ww][t] = acc;
-                acc += c;
+        for (int b = 0; b < MAXBITS; ++b) {
+            if (b < nbits) {
+                // sgn: all ones where this lane's bit b is set; the lanes
+                // agreeing with it on bit b are ~(ballot ^ sgn)
+                const uint32_t sgn = (uint32_t)((int)(dg << (31 - b)) >> 31);
+                const uint64_t bal = __ballot(sgn != 0u) & pending;
+                lo &= ~((uint32_t)bal ^ sgn);
+                hi &= ~((uint32_t)(bal >> 32) ^ sgn);
             }
-            run[t] = acc;
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t pos = gbase[dg] + cnt[w][dg] + (uint32_t)wrank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+        if (mine) m = ((uint64_t)hi << 32) | lo;
+    }
+    return m;
+}
+
+// Per-(digit, block) counts of a pass: table[d * nblocks + block].
+template <int I>
+__global__ __launch_bounds__(kRT) void rhist_kernel(const int* __restrict__ keys, int64_t n, int shift, int bins,
+                                                    int nblocks, int q, uint32_t* __restrict__ table,
+                                                    const int* __restrict__ mode) {
+    if (mode && *mode == 2) return;  // already in order: no pass runs
+    const int lb = logical_block(q);
+    if (lb >= nblocks) return;
+    __shared__ uint32_t h[kMaxBins];
+    for (int i = threadIdx.x; i < bins; i += kRT) h[i] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t wb = (int64_t)lb * (kRT * I) + (int64_t)w * (64 * I);
+    int k[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int64_t i = wb + j * 64 + lane;
+        k[j] = i < n ? keys[i] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < I; ++j) hist_add(h, k[j] >= 0, ((unsigned)k[j] >> shift) & (unsigned)(bins - 1));
+    __syncthreads();
+    for (int d = threadIdx.x; d < bins; d += kRT) table[(size_t)d * nblocks + lb] = h[d];
+}
+
+// Keys of the current order (into `keys`, which holds the keys of the last
+// sort's order on entry) and the first pass's histogram table, with per
+// block: part[2 lb] the particles whose key moved by more than `far` from the
+// key at the same position after the last sort, part[2 lb + 1] the positions
+// whose key is below the previous position's (0 everywhere: already sorted).
+template <int I>
+__global__ __launch_bounds__(kRT) void keys_rhist_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                         int64_t n, DevDomain d, int node, int* __restrict__ keys,
+                                                         int far, int shift, int bins, int nblocks, int q,
+                                                         uint32_t* __restrict__ table, unsigned* __restrict__ part) {
+    const int lb = logical_block(q);
+    if (lb >= nblocks) return;
+    __shared__ uint32_t h[kMaxBins];
+    __shared__ unsigned cnt[2][kRW];
+    for (int i = threadIdx.x; i < bins; i += kRT) h[i] = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t wb = (int64_t)lb * (kRT * I) + (int64_t)w * (64 * I);
+    int k[I];
+    unsigned nfar = 0, ndesc = 0;  // wave totals, kept by every lane
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int64_t i = wb + j * 64 + lane;
+        k[j] = -1;
+        bool f = false;
+        if (i < n) {
+            const int old = keys[i];
+            k[j] = node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d);
+            keys[i] = k[j];
+            const long long dk = (long long)k[j] - (long long)old;
+            f = dk > far || dk < -far;
         }
+        nfar += (unsigned)__popcll(__ballot(f));
+    }
+    // predecessor of the wave's first item (the previous wave's or block's last)
+    int first_prev = -1;
+    if (lane == 0 && wb > 0 && wb <= n)
+        first_prev = node ? node_key_exact(x[wb - 1], y[wb - 1], d) : cell_key_exact(x[wb - 1], y[wb - 1], d);
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int up = __shfl_up(k[j], 1, 64);
+        const int last_prev = j > 0 ? __shfl(k[j - 1], 63, 64) : __shfl(first_prev, 0, 64);
+        const int prev = lane > 0 ? up : last_prev;
+        ndesc += (unsigned)__popcll(__ballot(k[j] >= 0 && prev >= 0 && k[j] < prev));
+    }
+    __syncthreads();  // zeroed h
+#pragma unroll
+    for (int j = 0; j < I; ++j) hist_add(h, k[j] >= 0, ((unsigned)k[j] >> shift) & (unsigned)(bins - 1));
+    if (lane == 0) {
+        cnt[0][w] = nfar;
+        cnt[1][w] = ndesc;
+    }
+    __syncthreads();
+    for (int dd = threadIdx.x; dd < bins; dd += kRT) table[(size_t)dd * nblocks + lb] = h[dd];
+    if (threadIdx.x < 2) {
+        unsigned s = 0;
+        for (int ww = 0; ww < kRW; ++ww) s += cnt[threadIdx.x][ww];
+        part[2 * (size_t)lb + threadIdx.x] = s;
+    }
+}
+
+// Stable scatter of one pass.  Each wave ranks its contiguous eighth of the
+// block slot by slot: lanes holding the same digit are matched with one
+// ballot per digit bit, the group's first lane adds the group size to the
+// wave's LDS histogram (returning the count before it) and the others read
+// it from that lane, so ranks follow item order.  Then, per digit, the waves'
+// counts become exclusive offsets and the digit totals a block-wide exclusive
+// scan; every item goes to its digit-ordered slot of an LDS stage, from which
+// the block writes out run by run.  On an order already sorted (*mode == 2)
+// the pass only copies the keys through when it is the last of an odd number
+// of passes (so that the sorted keys end in the same buffer).
+template <int I>
+__global__ __launch_bounds__(kRT, 2) void rscatter_kernel(const int* __restrict__ keys_in,
+                                                       const int* __restrict__ vals_in, int64_t n, int shift,
+                                                       int nbits, int nblocks, int q,
+                                                       const uint32_t* __restrict__ scan, int* __restrict__ keys_out,
+                                                       int* __restrict__ vals_out, const int* __restrict__ mode,
+                                                       int identity_copy) {
+    constexpr int B = kRT * I;
+    const int lb = logical_block(q);
+    if (lb >= nblocks) return;
+    const int64_t b0 = (int64_t)lb * B;
+    const int nb = (int)min((int64_t)B, n - b0);
+    if (mode && *mode == 2) {
+        if (identity_copy)
+            for (int i = threadIdx.x; i < nb; i += kRT) keys_out[b0 + i] = keys_in[b0 + i];
+        return;
+    }
+    // The wave histograms are dead once every item holds its local position,
+    // so the stage reuses their memory (40 KiB per block: four blocks per CU).
+    union Lds {
+        int2 stage[B];
+        uint32_t wh[kRW * kMaxBins];
+    };
+    __shared__ Lds lds;
+    int2* stage = lds.stage;
+    uint32_t* wh = lds.wh;
+    __shared__ uint32_t gbase[kMaxBins];  // later: global position of the block's first item of each digit minus bpre
+    __shared__ uint32_t bpre[kMaxBins];
+    __shared__ uint32_t wtot[kRW];
+    const int bins = 1 << nbits;
+    const unsigned dmask = (unsigned)(bins - 1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < kRW * bins; i += kRT) wh[i] = 0;
+    for (int dd = threadIdx.x; dd < bins; dd += kRT) gbase[dd] = scan[(size_t)dd * nblocks + lb];
+    const int64_t wb = b0 + (int64_t)w * (64 * I);
+    int key[I], val[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int64_t i = wb + j * 64 + lane;
+        key[j] = -1;
+        val[j] = 0;
+        if (i < n) {
+            key[j] = keys_in[i];
+            val[j] = vals_in ? vals_in[i] : (int)i;
+        }
+    }
+    // Ranking in three phases so that the slots' latencies overlap: the
+    // lanes sharing a digit in every slot (no memory access), then the group
+    // leaders' histogram adds in slot order (LDS atomics of one wave execute
+    // in program order, which keeps the ranks in item order), then the
+    // broadcasts of the returned counts.
+    uint64_t mm[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) mm[j] = digit_peers<10>(key[j] >= 0, ((unsigned)key[j] >> shift) & dmask, nbits);
+    __syncthreads();  // zeroed wave histograms, gbase loaded
+    uint32_t* myh = wh + w * bins;
+    uint32_t old[I], below[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        // group members below this lane: its rank in the group
+        below[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mm[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mm[j], 0u));
+        old[j] = 0;
+        const unsigned dg = ((unsigned)key[j] >> shift) & dmask;
+        if (key[j] >= 0 && below[j] == 0) old[j] = atomicAdd(&myh[dg], (uint32_t)__popcll(mm[j]));
+    }
+    uint32_t rank[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int leader = key[j] >= 0 ? __ffsll((unsigned long long)mm[j]) - 1 : lane;
+        rank[j] = (uint32_t)__shfl((int)old[j], leader, 64) + below[j];
+    }
+    __syncthreads();
+    // wave counts -> exclusive offsets per digit; digit totals into bpre
+    for (int dd = threadIdx.x; dd < bins; dd += kRT) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int ww = 0; ww < kRW; ++ww) {
+            const uint32_t c = wh[ww * bins + dd];
+            wh[ww * bins + dd] = s;
+            s += c;
+        }
+        bpre[dd] = s;
+    }
+    __syncthreads();
+    // block-wide exclusive scan of the digit totals, kDpt consecutive digits
+    // per thread
+    {
+        constexpr int kDpt = kMaxBins / kRT;
+        const int d0 = kDpt * threadIdx.x;
+        uint32_t t[kDpt];
+        uint32_t v = 0;
+#pragma unroll
+        for (int e = 0; e < kDpt; ++e) {
+            t[e] = d0 + e < bins ? bpre[d0 + e] : 0u;
+            v += t[e];
+        }
+        uint32_t incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) wtot[w] = incl;
         __syncthreads();
+        uint32_t ex = incl - v;
+#pragma unroll
+        for (int ww = 0; ww < kRW; ++ww)
+            if (ww < w) ex += wtot[ww];
+#pragma unroll
+        for (int e = 0; e < kDpt; ++e) {
+            if (d0 + e < bins) {
+                bpre[d0 + e] = ex;
+                gbase[d0 + e] -= ex;
+            }
+            ex += t[e];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const unsigned dg = ((unsigned)key[j] >> shift) & dmask;
+        rank[j] += key[j] >= 0 ? bpre[dg] + myh[dg] : 0u;  // local position in the digit-ordered block
+    }
+    __syncthreads();  // the wave histograms are read: their memory becomes the stage
+#pragma unroll
+    for (int j = 0; j < I; ++j)
+        if (key[j] >= 0) stage[rank[j]] = make_int2(key[j], val[j]);
+    __syncthreads();
+    // write-out: item i of the digit-ordered stage goes to gbase[digit] + i
+    int2 kv[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int i = threadIdx.x + j * kRT;
+        kv[j] = i < nb ? stage[i] : make_int2(0, 0);
+    }
+    uint32_t off[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) off[j] = gbase[((unsigned)kv[j].x >> shift) & dmask];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int i = threadIdx.x + j * kRT;
+        if (i < nb) {
+            const uint32_t pos = off[j] + (uint32_t)i;
+            keys_out[pos] = kv[j].x;
+            vals_out[pos] = kv[j].y;
+        }
     }
 }
 
@@ -145,20 +371,44 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return wpre + incl - v;
 }
 
+// Each thread takes 16 consecutive values (four 16-byte loads; the table
+// length is padded to a multiple of 4 by the caller's allocation and the
+// values past m are read as 0).
+__device__ __forceinline__ void load16(const uint32_t* __restrict__ in, size_t base, size_t m, uint32_t (&v)[kItems]) {
+#pragma unroll
+    for (int q4 = 0; q4 < kItems / 4; ++q4) {
+        const size_t b = base + 4 * q4;
+        if (b + 4 <= m) {
+            const uint4 u = *reinterpret_cast<const uint4*>(in + b);
+            v[4 * q4] = u.x;
+            v[4 * q4 + 1] = u.y;
+            v[4 * q4 + 2] = u.z;
+            v[4 * q4 + 3] = u.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[4 * q4 + e] = b + e < m ? in[b + e] : 0u;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void scan_reduce_kernel(const uint32_t* __restrict__ in, size_t m,
-                                                          uint32_t* __restrict__ sums) {
+                                                          uint32_t* __restrict__ sums, const int* __restrict__ mode) {
+    if (mode && *mode == 2) return;
     __shared__ uint32_t s_wave[kWaves];
     const size_t base = (size_t)blockIdx.x * kBlockItems + (size_t)threadIdx.x * kItems;
+    uint32_t v[kItems];
+    load16(in, base, m, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j)
-        if (base + j < m) s += in[base + j];
+    for (int j = 0; j < kItems; ++j) s += v[j];
     uint32_t total;
     block_exclusive_scan(s, s_wave, total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(256) void scan_top_kernel(uint32_t* __restrict__ sums, size_t nb) {
+__global__ __launch_bounds__(256) void scan_top_kernel(uint32_t* __restrict__ sums, size_t nb,
+                                                       const int* __restrict__ mode) {
+    if (mode && *mode == 2) return;
     __shared__ uint32_t s_wave[kWaves];
     uint32_t carry = 0;
     for (size_t off = 0; off < nb; off += kThreads) {
@@ -173,41 +423,68 @@ __global__ __launch_bounds__(256) void scan_top_kernel(uint32_t* __restrict__ su
 
 __global__ __launch_bounds__(256) void scan_down_kernel(const uint32_t* __restrict__ in, size_t m,
                                                         const uint32_t* __restrict__ sums,
-                                                        uint32_t* __restrict__ out) {
+                                                        uint32_t* __restrict__ out, const int* __restrict__ mode) {
+    if (mode && *mode == 2) return;
     __shared__ uint32_t s_wave[kWaves];
     const size_t base = (size_t)blockIdx.x * kBlockItems + (size_t)threadIdx.x * kItems;
     uint32_t v[kItems];
+    load16(in, base, m, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        v[j] = (base + j < m) ? in[base + j] : 0u;
-        s += v[j];
-    }
+    for (int j = 0; j < kItems; ++j) s += v[j];
     uint32_t total;
     uint32_t run = block_exclusive_scan(s, s_wave, total) + sums[blockIdx.x];
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        if (base + j < m) out[base + j] = run;
-        run += v[j];
+    for (int q4 = 0; q4 < kItems / 4; ++q4) {
+        const size_t b = base + 4 * q4;
+        uint32_t o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            o[e] = run;
+            run += v[4 * q4 + e];
+        }
+        if (b + 4 <= m)
+            *reinterpret_cast<uint4*>(out + b) = make_uint4(o[0], o[1], o[2], o[3]);
+        else
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (b + e < m) out[b + e] = o[e];
     }
 }
 
-void exclusive_scan(const uint32_t* in, uint32_t* out, size_t m, uint32_t* tmp, hipStream_t s) {
+void exclusive_scan(const uint32_t* in, uint32_t* out, size_t m, uint32_t* tmp, hipStream_t s, const int* mode) {
     const size_t nb = (m + kBlockItems - 1) / kBlockItems;
     if (nb == 0) return;
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp);
-    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kThreads), 0, s, tmp, nb);
-    hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, out);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, mode);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kThreads), 0, s, tmp, nb, mode);
+    hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, out, mode);
 }
 
-// off[c] = first sorted index with key >= c, off[G] = n.
+// off[c] = first sorted index with key >= c, off[G] = n.  Four keys per thread.
 __global__ __launch_bounds__(256) void cell_offsets_kernel(const int* __restrict__ keys, int64_t n, int G,
                                                            int* __restrict__ off) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    const int lo = (i == 0) ? -1 : keys[i - 1];
-    const int hi = (i == n) ? G : keys[i];
-    for (int c = lo + 1; c <= hi; ++c) off[c] = (int)i;
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 > n) return;
+    int k[4];
+    if (i0 + 4 <= n) {
+        const int4 v = *reinterpret_cast<const int4*>(keys + i0);
+        k[0] = v.x;
+        k[1] = v.y;
+        k[2] = v.z;
+        k[3] = v.w;
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) k[e] = (i0 + e < n) ? keys[i0 + e] : G;
+    }
+    int lo = i0 == 0 ? -1 : keys[i0 - 1];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t i = i0 + e;
+        if (i > n) break;
+        const int hi = (i == n) ? G : k[e];
+        for (int c = lo + 1; c <= hi; ++c) off[c] = (int)i;
+        lo = hi;
+    }
 }
 
 __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ src, float* __restrict__ dst,
@@ -216,96 +493,79 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ s
     if (i < n) dst[i] = src[perm[i]];
 }
 
-// Keys of the current order, and per block the number of particles whose key
-// moved by more than `far` from the key at the same position after the last
-// sort (`keys` holds those on entry).  Many far moves mean the permutation
-// will be far from the identity.
-__global__ __launch_bounds__(256) void keys_far_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                       int64_t n, DevDomain d, int* __restrict__ keys, int far,
-                                                       int node, unsigned* __restrict__ far_part) {
-    __shared__ unsigned wave_far[4];
-    unsigned cnt = 0;  // wave total, kept by every lane
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
-        const int64_t i = i0 + threadIdx.x;
-        bool f = false;
-        if (i < n) {
-            const int old = keys[i];
-            const int k = node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d);
-            keys[i] = k;
-            const long long dk = (long long)k - (long long)old;
-            f = dk > far || dk < -far;
-        }
-        cnt += (unsigned)__popcll(__ballot(f));
+// Permutation mode of this sort from the per-block counts of the keys
+// kernel: 2 when no key is below its predecessor's (the order is already
+// sorted: the passes are skipped and the streams copied), else 1 (staged
+// gather through packed records) when more than an eighth of the particles
+// moved by more than `far` keys, else 0.  One block sums the counts.
+__global__ __launch_bounds__(256) void permute_mode_kernel(const unsigned* __restrict__ part, int nparts, int64_t n,
+                                                           int* __restrict__ mode) {
+    __shared__ unsigned long long sh[2][256];
+    unsigned long long f = 0, dsc = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) {
+        f += part[2 * (size_t)i];
+        dsc += part[2 * (size_t)i + 1];
     }
-    if ((threadIdx.x & 63) == 0) wave_far[threadIdx.x >> 6] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) far_part[blockIdx.x] = wave_far[0] + wave_far[1] + wave_far[2] + wave_far[3];
-}
-
-// Permutation mode of this sort: 1 (staged gather through packed records)
-// when more than a quarter of the particles moved far, else 0.  One block
-// sums the `nparts` per-block counts.
-__global__ __launch_bounds__(256) void permute_mode_kernel(const unsigned* __restrict__ far_part, int nparts,
-                                                           int64_t n, int* __restrict__ mode) {
-    __shared__ unsigned long long sh[256];
-    unsigned long long c = 0;
-    for (int i = threadIdx.x; i < nparts; i += 256) c += far_part[i];
-    sh[threadIdx.x] = c;
+    sh[0][threadIdx.x] = f;
+    sh[1][threadIdx.x] = dsc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        if ((int)threadIdx.x < o) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *mode = (int64_t)sh[0] > n / 4 ? 1 : 0;
+    if (threadIdx.x == 0) *mode = sh[1][0] == 0 ? 2 : ((int64_t)sh[0][0] > n / 8 ? 1 : 0);
 }
 
-// Mode 1: the permuted streams of particle i packed into one or two float4
-// records, so that the gather reads one or two lines per particle instead of
-// one per stream.
+// The stream kernels below take one item per thread: their reads are random
+// (the permutation) or line-sized, and the most waves in flight hide them best.
+unsigned stream_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
+
+// Mode 1: the permuted streams of particle i packed into W float4 records
+// (W = 2 for five or six streams), so that the gather reads one or two lines
+// per particle instead of one per stream.
+template <int W>
 __global__ __launch_bounds__(256) void pack_streams_kernel(StreamSet ss, int64_t n, const int* __restrict__ mode,
                                                            float4* __restrict__ rec) {
-    if (*mode == 0) return;
+    if (*mode != 1) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float v[4 * W];
 #pragma unroll
-    for (int k = 0; k < kMaxStreams; ++k)
-        if (k < ss.count) v[k] = ss.src[k][i];
-    const int w = ss.count > 4 ? 2 : 1;
-    rec[w * i] = make_float4(v[0], v[1], v[2], v[3]);
-    if (w == 2) rec[w * i + 1] = make_float4(v[4], v[5], v[6], v[7]);
+    for (int k = 0; k < 4 * W; ++k) v[k] = k < ss.count ? ss.src[k][i] : 0.0f;
+    rec[W * i] = make_float4(v[0], v[1], v[2], v[3]);
+    if (W == 2) rec[W * i + 1] = make_float4(v[4 % (4 * W)], v[5 % (4 * W)], v[6 % (4 * W)], v[7 % (4 * W)]);
 }
 
-__global__ __launch_bounds__(256) void gather_unpack_kernel(StreamSet ss, const int* __restrict__ perm, int64_t n,
-                                                            const int* __restrict__ mode,
-                                                            const float4* __restrict__ rec) {
-    if (*mode == 0) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t p = perm[i];
-    const int w = ss.count > 4 ? 2 : 1;
-    const float4 a = rec[w * p];
-    const float4 b = w == 2 ? rec[w * p + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-    for (int k = 0; k < kMaxStreams; ++k)
-        if (k < ss.count) ss.dst[k][i] = v[k];
-}
-
-// dst_k[i] = src_k[perm[i]] for up to kMaxStreams streams: the permutation is
-// read once for all of them.  With perm == nullptr it is a plain copy.  With mode
-// set to 1 the staged gather does this sort and the kernel returns.
+// dst_k[i] = src_k[perm[i]] for up to kMaxStreams streams, by mode: 0 a
+// gather of each stream (the permutation read once for all of them), 1 the
+// gather of the packed records, 2 (already sorted) or perm == nullptr a copy.
+template <int W>
 __global__ __launch_bounds__(256) void permute_streams_kernel(StreamSet ss, const int* __restrict__ perm, int64_t n,
-                                                              const int* __restrict__ mode) {
-    if (mode && *mode != 0) return;
+                                                              const int* __restrict__ mode,
+                                                              const float4* __restrict__ rec) {
+    const int md = mode ? *mode : 0;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t p = perm ? (int64_t)perm[i] : i;
-    float v[kMaxStreams];
+    float v[kMaxStreams > 4 * W ? kMaxStreams : 4 * W];
+    if (md == 1) {
+        const int64_t p = perm[i];
 #pragma unroll
-    for (int k = 0; k < kMaxStreams; ++k)
-        if (k < ss.count) v[k] = ss.src[k][p];
+        for (int h = 0; h < W; ++h) {
+            const float4 r = rec[W * p + h];
+            v[4 * h] = r.x;
+            v[4 * h + 1] = r.y;
+            v[4 * h + 2] = r.z;
+            v[4 * h + 3] = r.w;
+        }
+    } else {
+        const int64_t p = (perm == nullptr || md == 2) ? i : (int64_t)perm[i];
+#pragma unroll
+        for (int k = 0; k < kMaxStreams; ++k)
+            if (k < ss.count) v[k] = ss.src[k][p];
+    }
 #pragma unroll
     for (int k = 0; k < kMaxStreams; ++k)
         if (k < ss.count) ss.dst[k][i] = v[k];
@@ -317,21 +577,42 @@ int key_bits(int G) {
     return bits;
 }
 
+// Blocking of the radix passes: 16 items per thread (8192-item blocks), or 4
+// for sets of up to kSmallSort particles, whose passes would otherwise run on
+// a few hundred workgroups.
+constexpr int64_t kSmallSort = int64_t(1) << 21;
+
+struct RadixPlan {
+    int items;    // per thread
+    int nblocks;  // item ranges
+    int q;        // ranges per XCD
+    int passes;
+    int bits;     // per pass
+};
+
+RadixPlan radix_plan(int64_t n, int key_range) {
+    RadixPlan p;
+    p.items = n <= kSmallSort ? 4 : 16;
+    const int64_t bi = (int64_t)kRT * p.items;
+    p.nblocks = (int)std::max<int64_t>(1, (n + bi - 1) / bi);
+    p.q = (p.nblocks + kXcds - 1) / kXcds;
+    const int kb = key_bits(key_range);
+    p.passes = kb == 0 ? 1 : (kb + 9) / 10;
+    p.bits = kb == 0 ? 1 : (kb + p.passes - 1) / p.passes;
+    return p;
+}
+
 }  // namespace
 
-// Radix blocks: 16 items per thread, or 4 for sets of up to kSmallSort
-// particles, whose passes would otherwise run on a few hundred workgroups
-// with 16 barrier-separated slots each.
-constexpr int64_t kSmallSort = int64_t(1) << 21;
-static int radix_items(int64_t n) { return n <= kSmallSort ? 4 : kItems; }
-
 size_t sort_hist_elems(int64_t n) {
-    const size_t large = (size_t)((n + kBlockItems - 1) / kBlockItems);
-    const size_t small = (size_t)((std::min(n, kSmallSort) + kThreads * 4 - 1) / (kThreads * 4));
-    return std::max<size_t>(std::max(large, small), 1) * kDigits;
+    const size_t small = (size_t)((std::min(n, kSmallSort) + kRT * 4 - 1) / (kRT * 4));
+    const size_t large = (size_t)((n + kRT * 16 - 1) / (kRT * 16));
+    return std::max<size_t>(std::max(small, large), 1) * kMaxBins;
 }
 
 size_t sort_scan_tmp_elems(size_t m) { return (m + kBlockItems - 1) / kBlockItems + 1; }
+
+size_t sort_part_elems(int64_t n) { return 2 * (sort_hist_elems(n) / kMaxBins); }
 
 void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, hipStream_t s, bool node) {
     if (n > 0)
@@ -339,32 +620,35 @@ void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, 
                            node ? 1 : 0);
 }
 
-hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s) {
+hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s, const int* mode,
+                                   bool first_hist_done) {
     if (n > 0) {
-        const int items = radix_items(n);
-        const int nblocks = (int)((n + kThreads * items - 1) / (kThreads * items));
-        const size_t m = (size_t)nblocks * kDigits;
-        const int bits = key_bits(key_range);
-        const int passes = bits == 0 ? 1 : (bits + 7) / 8;
+        const RadixPlan p = radix_plan(n, key_range);
+        const int bins = 1 << p.bits;
+        const size_t m = (size_t)p.nblocks * bins;
+        const dim3 grid((unsigned)(p.q * kXcds));
         int* kin = sb.keys;
         int* kout = sb.keys_alt;
         int* vin = nullptr;  // identity permutation on the first pass
         int* vout = sb.perm;
-        for (int p = 0; p < passes; ++p) {
-            const int shift = 8 * p;
-            if (items == 4)
-                hipLaunchKernelGGL(radix_hist_kernel<4>, dim3(nblocks), dim3(kThreads), 0, s, kin, n, shift, nblocks,
-                                   sb.hist);
+        for (int pass = 0; pass < p.passes; ++pass) {
+            const int shift = p.bits * pass;
+            if (pass > 0 || !first_hist_done) {
+                if (p.items == 4)
+                    hipLaunchKernelGGL(rhist_kernel<4>, grid, dim3(kRT), 0, s, kin, n, shift, bins, p.nblocks, p.q,
+                                       sb.hist, mode);
+                else
+                    hipLaunchKernelGGL(rhist_kernel<16>, grid, dim3(kRT), 0, s, kin, n, shift, bins, p.nblocks, p.q,
+                                       sb.hist, mode);
+            }
+            exclusive_scan(sb.hist, sb.hist, m, sb.scan_tmp, s, mode);
+            const int copy = (pass == p.passes - 1 && (p.passes & 1)) ? 1 : 0;
+            if (p.items == 4)
+                hipLaunchKernelGGL(rscatter_kernel<4>, grid, dim3(kRT), 0, s, kin, vin, n, shift, p.bits, p.nblocks,
+                                   p.q, sb.hist, kout, vout, mode, copy);
             else
-                hipLaunchKernelGGL(radix_hist_kernel<kItems>, dim3(nblocks), dim3(kThreads), 0, s, kin, n, shift,
-                                   nblocks, sb.hist);
-            exclusive_scan(sb.hist, sb.hist, m, sb.scan_tmp, s);
-            if (items == 4)
-                hipLaunchKernelGGL(radix_scatter_kernel<4>, dim3(nblocks), dim3(kThreads), 0, s, kin, vin, n, shift,
-                                   nblocks, sb.hist, kout, vout);
-            else
-                hipLaunchKernelGGL(radix_scatter_kernel<kItems>, dim3(nblocks), dim3(kThreads), 0, s, kin, vin, n,
-                                   shift, nblocks, sb.hist, kout, vout);
+                hipLaunchKernelGGL(rscatter_kernel<16>, grid, dim3(kRT), 0, s, kin, vin, n, shift, p.bits,
+                                   p.nblocks, p.q, sb.hist, kout, vout, mode, copy);
             // next pass reads what this one wrote
             int* kt = kin;
             kin = kout;
@@ -384,7 +668,7 @@ hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hi
             sb.perm_alt = t;
         }
     }
-    hipLaunchKernelGGL(cell_offsets_kernel, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys, n,
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3((unsigned)((n / 4 + 1 + 255) / 256)), dim3(256), 0, s, sb.keys, n,
                        key_range, sb.cell_off);
     return hipGetLastError();
 }
@@ -395,31 +679,37 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
     return launch_sort_precomputed(n, d.Nx * d.Ny, sb, s);
 }
 
-void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s, const int* mode) {
+void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s, const int* mode,
+                            const float4* rec) {
     if (n > 0)
-        hipLaunchKernelGGL(permute_streams_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, perm, n,
-                           mode);
+        hipLaunchKernelGGL(ss.count > 4 ? permute_streams_kernel<2> : permute_streams_kernel<1>,
+                           dim3(stream_grid(n)), dim3(256), 0, s, ss, perm, n, mode, rec);
 }
 
-void launch_keys_far(const float* x, const float* y, int64_t n, const DevDomain& d, int* keys, int far, bool node,
-                     unsigned* far_part, int* mode, hipStream_t s) {
-    const int blocks = (int)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), kFarParts);
-    if (n > 0)
-        hipLaunchKernelGGL(keys_far_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, y, n, d, keys, far,
-                           node ? 1 : 0, far_part);
-    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, far_part, n > 0 ? blocks : 0, n, mode);
+void launch_keys_hist(const float* x, const float* y, int64_t n, const DevDomain& d, int key_range, SortBuffers& sb,
+                      int far, bool node, unsigned* part, int* mode, hipStream_t s) {
+    const RadixPlan p = radix_plan(n, key_range);
+    const int bins = 1 << p.bits;
+    const dim3 grid((unsigned)(p.q * kXcds));
+    if (n > 0) {
+        if (p.items == 4)
+            hipLaunchKernelGGL(keys_rhist_kernel<4>, grid, dim3(kRT), 0, s, x, y, n, d, node ? 1 : 0, sb.keys, far, 0,
+                               bins, p.nblocks, p.q, sb.hist, part);
+        else
+            hipLaunchKernelGGL(keys_rhist_kernel<16>, grid, dim3(kRT), 0, s, x, y, n, d, node ? 1 : 0, sb.keys, far,
+                               0, bins, p.nblocks, p.q, sb.hist, part);
+    }
+    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, part, n > 0 ? p.nblocks : 0, n, mode);
+}
+
+void launch_permute_mode(const unsigned* part, int nparts, int64_t n, int* mode, hipStream_t s) {
+    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, part, nparts, n, mode);
 }
 
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s) {
     if (n > 0)
-        hipLaunchKernelGGL(pack_streams_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, n, mode, rec);
-}
-
-void launch_gather_unpack(const StreamSet& ss, const int* perm, int64_t n, const int* mode, const float4* rec,
-                          hipStream_t s) {
-    if (n > 0)
-        hipLaunchKernelGGL(gather_unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ss, perm, n, mode,
-                           rec);
+        hipLaunchKernelGGL(ss.count > 4 ? pack_streams_kernel<2> : pack_streams_kernel<1>, dim3(stream_grid(n)),
+                           dim3(256), 0, s, ss, n, mode, rec);
 }
 
 void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n, hipStream_t s) {
