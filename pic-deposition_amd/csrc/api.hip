@@ -640,11 +640,12 @@ namespace {
 // executions only).  The mean removed before the transform comes from the
 // device FP64 sum `sum` (see launch_poisson_prep).  Without `with_phi` the
 // potential stays as its spectrum in ctx->spec until phi is read.
-pic_status enqueue_solve(pic_hip_ctx* ctx, const double* sum, double sum_scale, double sum_add, bool with_phi) {
+pic_status enqueue_solve(pic_hip_ctx* ctx, const double* sum, double sum_scale, double sum_add, bool with_phi,
+                         const LearnedApply& la = LearnedApply{}) {
     const double cv = ctx->dev.dx * ctx->dev.dy;
     if (ctx->lds_fft) {
         launch_poisson_lds(ctx->rho, ctx->G, 0.0, cv, sum, sum_scale, sum_add, ctx->spec, ctx->spec_ex, ctx->spec_ey,
-                           ctx->ptab, ctx->dev, ctx->tw_x, ctx->tw_y, ctx->stream);
+                           ctx->ptab, ctx->dev, ctx->tw_x, ctx->tw_y, ctx->stream, la);
         if (ctx->dom.Ny > 1) {
             launch_poisson_lds_rows(ctx->spec_ex, ctx->spec_ey, ctx->Ex, ctx->Ey, ctx->dev, ctx->tw_x, ctx->stream);
         } else {  // one row: Ey is exactly zero (phi does not vary along y)
@@ -787,9 +788,17 @@ pic_status enqueue_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
         return s3;
     };
     auto solve = [&]() -> pic_status {
-        if (cfg.background != 0.0) launch_add_scalar(ctx->rho, ctx->G, (float)cfg.background, ctx->stream);
+        // a learned density deferred by the post-deposit hook is written, and
+        // the background added to it, by the solve's first pass
+        LearnedApply la = ctx->pending_apply;
+        ctx->pending_apply = LearnedApply{};
+        if (la.net_out)
+            la.bg = (float)cfg.background;
+        else if (cfg.background != 0.0)
+            launch_add_scalar(ctx->rho, ctx->G, (float)cfg.background, ctx->stream);
         const double cv = ctx->dev.dx * ctx->dev.dy;
-        return enqueue_solve(ctx, ctx->run_sums + 3, 1.0 / cv, (double)(float)cfg.background * (double)ctx->G, false);
+        return enqueue_solve(ctx, ctx->run_sums + 3, 1.0 / cv, (double)(float)cfg.background * (double)ctx->G, false,
+                             la);
     };
     if (cfg.order == PIC_ORDER_PUSH_FIRST) {
         // The push that opens this step ran fused with the previous step's
@@ -880,6 +889,7 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
     if (st != PIC_OK) return st;
     if ((st = ensure_fft(ctx)) != PIC_OK) return st;
     if ((st = ensure_sort_staging(ctx)) != PIC_OK) return st;
+    ctx->pending_apply = LearnedApply{};  // nothing left over from a failed enqueue
     const int cap = nsteps > record_from ? nsteps - record_from : 0;
     double* dev_rec = nullptr;
     PIC_HIP_TRY(dmalloc(&dev_rec, (size_t)4 * std::max(cap, 1)));

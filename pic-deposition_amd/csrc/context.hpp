@@ -94,6 +94,10 @@ struct pic_hip_ctx {
     int* step_counter = nullptr;
     int* push_flag = nullptr;    // device flag: the fused gather + push of a step pushes
     double* step_partials = nullptr;
+    // A learned density's gate / renormalisation enqueued by a step's
+    // post-deposit hook and not yet launched: the step's Poisson solve takes
+    // it into its first pass (net_out null: none pending).
+    pichip::LearnedApply pending_apply{};
     std::vector<hipEvent_t> prof_events;  // pairs (begin, end)
     int prof_max = 0, prof_used = 0;
     bool profiling = false;

@@ -604,7 +604,7 @@ void launch_moments(const float* v, int64_t n, double* partial, int nb, hipStrea
 }
 
 void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
-                    hipStream_t s) {
+                    hipStream_t s, LearnedApply* defer) {
     const __bf16* pk = reinterpret_cast<const __bf16*>(c.packed);
     const double inv_g = 1.0 / (double)a.G;
     const InNorm nrm{a.sum, a.sum_scale, inv_g, a.in_scale};
@@ -615,6 +615,11 @@ void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], 
     const int nbs =
         conv_launch<32, 1, kOutput>(c.a3, pk + 8 * cnn::kPackF4, c.w + cnn::kB4, c.out, nullptr, g, dil[3], ncu, s,
                                     InNorm{}, ost);
+    if (defer) {
+        *defer = LearnedApply{c.out, a.out64, a.sum, a.sum_scale, a.inv_out_scale, a.stats, nbs, a.moments, a.nbm,
+                              a.cv, a.kappa, a.tol, a.result, a.fallback_count, 0.0f};
+        return;
+    }
     hipLaunchKernelGGL(denoise_apply_kernel, dim3((unsigned)((a.G + 255) / 256)), dim3(256), 0, s, c.out, a.rho,
                        a.out64, a.G, a.sum, a.sum_scale, a.inv_out_scale, a.stats, nbs, a.moments, a.nbm, a.cv,
                        a.kappa, a.tol, a.result, a.fallback_count);

@@ -152,7 +152,10 @@ void enqueue_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double
     a.stats = net->scratch + 2 * kMomentBlocks;
     a.result = net->result;
     a.fallback_count = fallback_count;
-    launch_denoise(net->c, geom(net, 1), net->dil, net->ncu, a, ctx->stream);
+    // Inside a step loop whose solve runs in LDS, the gate and the write of the
+    // learned density are left to the solve's first pass (ctx->pending_apply).
+    launch_denoise(net->c, geom(net, 1), net->dil, net->ncu, a, ctx->stream,
+                   in_run && ctx->lds_fft ? &ctx->pending_apply : nullptr);
 }
 
 double kappa_of(int scheme) {

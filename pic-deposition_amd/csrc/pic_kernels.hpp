@@ -182,9 +182,29 @@ void launch_poisson_spectral(float2* spec, float2* spec_ex, float2* spec_ey, con
 // half spectra X (and Y, may be null) into the real grids ox (and oy).
 // tw_x, tw_y: e^{2 pi i m / Nx}, m < Nx, and e^{2 pi i m / Ny}, m < Ny.
 bool poisson_lds_supported(int Nx, int Ny);
-void launch_poisson_lds(const float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
+// The gate, renormalisation and write of a learned density (the last step of
+// launch_denoise), deferred into the first pass of launch_poisson_lds when
+// net_out is set: rho (and out64) become the learned density, or stay the
+// plain deposit when the gate rejects it, then rho += bg, before the solve
+// reads them.  One launch fewer per step of a learned run.
+struct LearnedApply {
+    const float* net_out;  // null: nothing deferred
+    double* out64;
+    const double* sum;
+    double sum_scale, inv_scale;
+    const double* stats;
+    int nbs;
+    const double* moments;
+    int nbm;
+    double cv, kappa, tol;
+    double* result;
+    int* fallback_count;
+    float bg;
+};
+void launch_poisson_lds(float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
                         double sum_add, float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
-                        const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s);
+                        const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s,
+                        const LearnedApply& la = LearnedApply{});
 void launch_poisson_lds_rows(const float2* X, const float2* Y, float* ox, float* oy, const DevDomain& d,
                              const float2* tw_x, hipStream_t s);
 // Ex = -(phi(i+1,j) - phi(i,j))/dx, Ey = -(phi(i,j+1) - phi(i,j))/dy.
@@ -378,8 +398,10 @@ struct DenoiseArgs {
     double* result;
     int* fallback_count;
 };
+// With `defer`, the final gate / renormalisation launch is not made: its
+// arguments are written to *defer for launch_poisson_lds (bg left 0).
 void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
-                    hipStream_t s);
+                    hipStream_t s, LearnedApply* defer = nullptr);
 size_t denoise_stats_doubles(int ncu);
 
 }  // namespace pichip

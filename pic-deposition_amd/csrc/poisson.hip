@@ -26,6 +26,7 @@
 // length is not a power of four, twiddles from one table e^{2 pi i m / L}
 // tabulated in FP64 and rounded to FP32.
 
+#include "learned_gate.hpp"
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
 
@@ -105,10 +106,14 @@ __device__ float2* wave_fft(float2* a, float2* b, int L, const float2* tw) {
     return a;
 }
 
-__global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restrict__ rho, float2* __restrict__ spec,
+// With a deferred learned density (la.net_out set), each block first takes the
+// gate decision and then writes its rows of rho (and la.out64) as
+// denoise_apply_kernel would, plus la.bg, before transforming them.
+__global__ __launch_bounds__(64) void poisson_rows_kernel(float* __restrict__ rho, float2* __restrict__ spec,
                                                           int W, int H, int64_t G, double bg, double cv,
                                                           const double* __restrict__ rho_sum, double sum_scale,
-                                                          double sum_add, const float2* __restrict__ tw) {
+                                                          double sum_add, const float2* __restrict__ tw,
+                                                          LearnedApply la) {
     extern __shared__ float2 sm[];
     float2* a = sm;
     float2* b = sm + W;
@@ -117,9 +122,36 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(const float* __restric
     const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
     const bool pair = r1 < H;  // a grid of one row transforms it alone
     const double mean = (*rho_sum * sum_scale + sum_add + bg * (double)G) * cv / (double)G;
+    LearnedGate gt{};
+    if (la.net_out) {
+        gt = learned_gate_wave(la, G);
+        if (blockIdx.x == 0 && lane == 0) {
+            la.result[0] = gt.ratio;
+            la.result[1] = gt.ok ? 0.0 : 1.0;
+            la.result[2] = gt.scale;
+            la.result[3] = gt.q1;
+            if (!gt.ok && la.fallback_count) *la.fallback_count += 1;
+        }
+    }
+    // rho of node i, replaced first by the learned density when one is deferred
+    auto node = [&](size_t i) -> float {
+        float v = rho[i];
+        if (la.net_out) {
+            if (gt.ok) {
+                const double lv = gt.mean * (1.0 + (double)la.net_out[i] * la.inv_scale) * gt.scale;
+                la.out64[i] = lv;
+                v = (float)lv;
+            } else {
+                la.out64[i] = (double)v;  // fallback: the plain deposit stays
+            }
+            if (la.bg != 0.0f) v += la.bg;
+            rho[i] = v;
+        }
+        return v;
+    };
     for (int n = lane; n < W; n += 64) {
-        const float u = (float)(((double)rho[(size_t)r0 * W + n] + bg) * cv - mean);
-        const float v = pair ? (float)(((double)rho[(size_t)r1 * W + n] + bg) * cv - mean) : 0.0f;
+        const float u = (float)(((double)node((size_t)r0 * W + n) + bg) * cv - mean);
+        const float v = pair ? (float)(((double)node((size_t)r1 * W + n) + bg) * cv - mean) : 0.0f;
         a[n] = make_float2(u, v);
         t[n] = tw[n];
     }
@@ -228,12 +260,13 @@ bool poisson_lds_supported(int Nx, int Ny) {
     return pow2(Nx, 4) && pow2(Ny, 1);
 }
 
-void launch_poisson_lds(const float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
+void launch_poisson_lds(float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
                         double sum_add, float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
-                        const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s) {
+                        const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s,
+                        const LearnedApply& la) {
     const int W = d.Nx, H = d.Ny;
     hipLaunchKernelGGL(poisson_rows_kernel, dim3((unsigned)((H + 1) / 2)), dim3(64), (size_t)3 * W * sizeof(float2),
-                       s, rho, spec, W, H, G, bg, cv, rho_sum, sum_scale, sum_add, tw_x);
+                       s, rho, spec, W, H, G, bg, cv, rho_sum, sum_scale, sum_add, tw_x, la);
     hipLaunchKernelGGL(poisson_cols_kernel, dim3((unsigned)(W / 2 + 1)), dim3(kColsThreads),
                        (size_t)7 * H * sizeof(float2), s, spec, spec_ex, spec_ey, t, d, tw_y);
 }

@@ -147,16 +147,25 @@ __global__ __launch_bounds__(kRT) void keys_rhist_kernel(const float* __restrict
     const int64_t wb = (int64_t)lb * (kRT * I) + (int64_t)w * (64 * I);
     int k[I];
     unsigned nfar = 0, ndesc = 0;  // wave totals, kept by every lane
+    // all loads first (clamped to the last particle, so none is conditional)
+    float xs[I], ys[I];
+    int olds[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int64_t i = min(wb + j * 64 + lane, n - 1);
+        xs[j] = x[i];
+        ys[j] = y[i];
+        olds[j] = keys[i];
+    }
 #pragma unroll
     for (int j = 0; j < I; ++j) {
         const int64_t i = wb + j * 64 + lane;
         k[j] = -1;
         bool f = false;
         if (i < n) {
-            const int old = keys[i];
-            k[j] = node ? node_key_exact(x[i], y[i], d) : cell_key_exact(x[i], y[i], d);
+            k[j] = node ? node_key_exact(xs[j], ys[j], d) : cell_key_exact(xs[j], ys[j], d);
             keys[i] = k[j];
-            const long long dk = (long long)k[j] - (long long)old;
+            const long long dk = (long long)k[j] - (long long)olds[j];
             f = dk > far || dk < -far;
         }
         nfar += (unsigned)__popcll(__ballot(f));
