@@ -33,6 +33,15 @@ template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ int dpp_i(int old, int v) {
     return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
 }
+// Shifted value with zero where the source lane is outside the row or wave.
+// With every row enabled this is the bound_ctrl form, which needs no zeroed
+// destination first and folds into the arithmetic that consumes it (one
+// v_fmac_f32_dpp instead of a zeroing move, a DPP move and the FMA).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_z(float v) {
+    if constexpr (ROW_MASK == 0xf) return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+    else return __builtin_amdgcn_update_dpp(0.0f, v, CTRL, ROW_MASK, 0xf, false);
+}
 
 // One Hillis-Steele step of the linear recurrence B_i = b_i + a_i * B_{i-1}.
 template <int CTRL, int ROW_MASK, int V>
@@ -40,8 +49,8 @@ __device__ __forceinline__ void lin_scan_step(float (&B)[V], float& A) {
     const float As = dpp_f<CTRL, ROW_MASK>(1.0f, A);
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-        const float Bs = dpp_f<CTRL, ROW_MASK>(0.0f, B[k]);
-        B[k] = fmaf(A, Bs, B[k]);
+        const float Bs = dpp_z<CTRL, ROW_MASK>(B[k]);
+        B[k] = fmaf(Bs, A, B[k]);
     }
     A *= As;
 }
@@ -396,7 +405,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
     wave_segmented_scan<V>(Z, a);
     float Zp[V];
 #pragma unroll
-    for (int k = 0; k < V; ++k) Zp[k] = dpp_f<kWaveShr1, 0xf>(0.0f, Z[k]);
+    for (int k = 0; k < V; ++k) Zp[k] = dpp_z<kWaveShr1, 0xf>(Z[k]);
     if (multi) {
 #pragma unroll
         for (int k = 0; k < V; ++k) H[k] = cont ? H[k] + Zp[k] : H[k];
