@@ -166,7 +166,7 @@ def measured_traffic():
     """HBM bytes per launch of the CIC tile kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB
     copy kernel); None if the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "deposit_cic_1024x1024_64ppc_hbm_counters.md")
+    path = os.path.join(ROOT, "profiles", "deposit_sort_1024x1024_64ppc_hbm_counters.md")
     try:
         for line in open(path):
             if line.startswith("TRAFFIC_BYTES_PER_LAUNCH="):
