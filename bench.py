@@ -8,9 +8,9 @@ cell-sorted and resident in HBM.  One step = one CIC deposit of all particles
 Esirkepov-density deposits, the global-atomic backend on the cell-sorted order
 and on the initialiser's order (positions independent of the particle index,
 i.e. a random order with respect to the cells), the cell sort (from that
-random order, of an already sorted set, and after ten drift steps), and the
-per-phase timestep profile of Simulation::runProfiled at the reference's own
-cases.
+random order, of an already sorted set, and after ten drift steps), the
+Esirkepov current of the same particles, and the per-phase timestep profile
+of Simulation::runProfiled at the reference's own cases.
 
 Roofline: the tiled scatter kernel moves 12 B per particle (x, y, q in FP32)
 plus 4 B per cell of rho; achieved GB/s = those bytes / its mean duration,
@@ -98,6 +98,22 @@ def time_sorts(ctx, reps, prepare=None):
         ctx.sync()
         tot += time.perf_counter() - t0
     return tot * 1000.0 / reps
+
+
+def time_current(ctx, reps=10, warmup=3):
+    """Esirkepov current of the resident particles (cell-sorted order), wall
+    time per call bracketed by stream synchronisation; algorithmic bytes 20 B
+    per particle (x, y, vx, vy, q) + 16 B per cell (Jx, Jy, rho_old, rho_new)."""
+    for _ in range(warmup):
+        ctx.esirkepov_current(DT)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.esirkepov_current(DT)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1000.0 / reps
+    return {"particles_per_s": N / (ms / 1000.0), "ms": ms,
+            "hbm_frac": (20 * N + 16 * NX * NY) / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS, "order": "cell"}
 
 
 def sort_and_shuffled(ctx):
@@ -654,6 +670,7 @@ def main():
         ctx.sort()
         others["TSC on the cell order"] = timed(2, "cell")
         others["Esirkepov"] = timed(3, "cell")
+        others["Esirkepov current"] = time_current(ctx)
         ms_a, k_a = time_deposits(ctx, 1, 1, 5, 1)
         others["CIC GPU_Atomics (sorted order)"] = {"particles_per_s": N / (ms_a / 1000.0), "ms": ms_a,
                                                     "kernel_ms": k_a}
