@@ -1193,8 +1193,23 @@ pic_status pic_hip_esirkepov_current(pic_hip_ctx* ctx, double dt) {
                         &ctx->cur.Jx_c, &ctx->cur.Jy_c, &ctx->cur.rho_old_c, &ctx->cur.rho_new_c};
         for (int k = 0; k < 8; ++k) *f[k] = base + (size_t)k * ctx->G;
     }
-    launch_esirkepov_current(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], (int64_t)ctx->n, ctx->dev,
-                             (float)dt, ctx->cur, ctx->status, ctx->stream);
+    if (ctx->order_valid && !ctx->order_node && ctx->n > 0) {
+        // Cell-sorted particles: tiles of 64 x 2 cells (smaller grids: the grid).
+        const TileGeom tg = make_tile_geom(ctx->dom.Nx, ctx->dom.Ny, 64, 2, 0);
+        const size_t need = current_partial_doubles(tg);
+        if (need > ctx->cur_partial_cap) {
+            (void)hipFree(ctx->cur_partial);
+            ctx->cur_partial = nullptr;
+            ctx->cur_partial_cap = 0;
+            PIC_HIP_TRY(dmalloc(&ctx->cur_partial, need));
+            ctx->cur_partial_cap = need;
+        }
+        launch_esirkepov_current_tiled(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], ctx->sb.cell_off,
+                                       ctx->dev, tg, (float)dt, ctx->cur_partial, ctx->cur, ctx->status, ctx->stream);
+    } else {
+        launch_esirkepov_current(ctx->s[0], ctx->s[1], ctx->s[2], ctx->s[3], ctx->s[4], (int64_t)ctx->n, ctx->dev,
+                                 (float)dt, ctx->cur, ctx->status, ctx->stream);
+    }
     PIC_HIP_TRY(hipGetLastError());
     return PIC_OK;
 }

@@ -89,6 +89,8 @@ struct pic_hip_ctx {
     int* h_status = nullptr;     // pinned
     double* h_diag = nullptr;    // pinned
     CurrentGrids cur{};           // Esirkepov current buffers (allocated on first use)
+    double* cur_partial = nullptr;  // tile windows of the tiled current (4 planes)
+    size_t cur_partial_cap = 0;
     double* cosk = nullptr;      // cos/sin(2 pi (i + 1/2) dx / Lx), mode-1 diagnostic tables
     double* sink = nullptr;
     int* step_counter = nullptr;
@@ -135,6 +137,7 @@ struct pic_hip_ctx {
         (void)hipFree(diag_partials);
         (void)hipFree(diag_out);
         (void)hipFree(cur.Jx);  // one allocation holds all eight grids
+        (void)hipFree(cur_partial);
         (void)hipFree(cosk);
         (void)hipFree(sink);
         (void)hipFree(step_counter);

@@ -576,15 +576,13 @@ __device__ __forceinline__ void axis_first(int gn, int T, int nt, int D, int& t0
     w0 = gn - (tlo * T - 1 - D);
 }
 
-// rho at node (gx, gy): the windows covering it, added in FP64 in a fixed
-// order, plus the FP64 accumulation grid where it is non-zero (then cleared).
-// B: candidates loaded together on the range path (TileGeom::mbatch).
-template <int B = 16>
-__device__ __forceinline__ float merge_node(int gx, int gy, const float* __restrict__ partial,
-                                            double* __restrict__ gacc, const DevDomain& d, const TileGeom& g) {
+// Sum at node (gx, gy) of the windows covering it (FP32 or FP64 window
+// values P), added in FP64 in a fixed order.  B: candidates loaded together
+// on the range path (TileGeom::mbatch).
+template <int B = 16, typename P = float>
+__device__ __forceinline__ double merge_windows(int gx, int gy, const P* __restrict__ partial,
+                                                const DevDomain& d, const TileGeom& g) {
     double s = 0.0;
-    // The marker is read before the windows so that its latency overlaps theirs.
-    const bool fold = g.acc_mark == nullptr || *g.acc_mark == g.epoch;
     // All window loads of a node are issued before the first add (a load per
     // add would wait out one L2 round trip per window); the adds keep the
     // row-major order of the tiles.
@@ -603,7 +601,7 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
             tys[1] = tys[0];
             wys[1] = wys[0];
         }
-        float v[2][2];
+        P v[2][2];
 #pragma unroll
         for (int iy = 0; iy < 2; ++iy)
 #pragma unroll
@@ -625,7 +623,7 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
         }
         int ty0, wy0, ncy;
         axis_first(gy, g.R, g.nty, 0, ty0, wy0, ncy);
-        float v[4][2];
+        P v[4][2];
         int ty = ty0, wy = wy0;
 #pragma unroll
         for (int iy = 0; iy < 4; ++iy) {
@@ -664,7 +662,7 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
                 wy -= g.R;
             }
         };
-        float v[kBatch];
+        P v[kBatch];
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
             const bool ok = k < total;
@@ -701,13 +699,24 @@ __device__ __forceinline__ float merge_node(int gx, int gy, const float* __restr
                     while (gxi < x0 - 1 - D) gxi += d.Nx;
                     for (; gxi < x0 + tw + 2 + D; gxi += d.Nx) {
                         const int wx = gxi - (x0 - 1 - D);
-                        const float v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
+                        const P v = partial[(size_t)(ty * g.ntx + tx) * g.wsize + (size_t)wy * g.WX + wx];
                         s += (double)v;
                     }
                 }
             }
         }
     }
+    return s;
+}
+
+// rho at node (gx, gy): the windows covering it, added in FP64 in a fixed
+// order, plus the FP64 accumulation grid where it is non-zero (then cleared).
+template <int B = 16>
+__device__ __forceinline__ float merge_node(int gx, int gy, const float* __restrict__ partial,
+                                            double* __restrict__ gacc, const DevDomain& d, const TileGeom& g) {
+    // The marker is read before the windows so that its latency overlaps theirs.
+    const bool fold = g.acc_mark == nullptr || *g.acc_mark == g.epoch;
+    double s = merge_windows<B>(gx, gy, partial, d, g);
     // The accumulation grid is zero except where drifted particles were
     // scattered (and untouched when no tile marked this deposit); fold those
     // nodes in and clear them.

@@ -236,6 +236,14 @@ struct CurrentGrids {
 void launch_esirkepov_current(const float* x, const float* y, const float* vx, const float* vy, const float* q,
                               int64_t n, const DevDomain& d, float dt, const CurrentGrids& g, int* status,
                               hipStream_t s);
+// The same on cell-sorted particles with their cell offsets: one wavefront
+// per tile of tg (at most 128 cells, no drift halo), FP64 LDS windows written to
+// `partial` (current_partial_doubles(tg) doubles), then a merge.
+size_t current_partial_doubles(const TileGeom& tg);
+size_t current_tile_lds(const TileGeom& tg);
+void launch_esirkepov_current_tiled(const float* x, const float* y, const float* vx, const float* vy, const float* q,
+                                    const int* cell_off, const DevDomain& d, const TileGeom& tg, float dt,
+                                    double* partial, const CurrentGrids& g, int* status, hipStream_t s);
 // out[0] = max |(rho_new - rho_old)/dt + Dx Jx + Dy Jy|, out[1] = max |(rho_new - rho_old)/dt| (FP64).
 void launch_continuity_residual(const CurrentGrids& g, const DevDomain& d, double dt, double* partial, int nblocks,
                                 double* out, hipStream_t s);
