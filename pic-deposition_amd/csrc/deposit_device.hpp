@@ -267,7 +267,24 @@ struct TileCtx {
     int WX;
     int x0, y0, tw, th;  // the tile's own cells
     int ax0, ay0, aw, ah;  // cells deposited into the window: the tile widened by the drift halo
+    int mrow;              // moment stencils: home nodes per row of the moment window (the tile width)
 };
+
+// Adds a run of window key k into LDS: the V stencil values at the stencil's
+// nodes of the window, or, for a moment stencil, the V moments at the home
+// node's slot of the moment window.
+template <int S, int V>
+__device__ __forceinline__ void emit_run(const TileCtx& t, int k, const float (&z)[V]) {
+    using St = Stencil<S>;
+    if constexpr (S == TSC_MOM) {
+        const int base = ((k >> 12) * t.mrow + (k & 4095)) * V;
+#pragma unroll
+        for (int i = 0; i < V; ++i) atomicAdd(t.win + base + i, (double)z[i]);
+    } else {
+        const int wb = ((k >> 12) + 1 + St::OY) * t.WX + (k & 4095) + 1 + St::OX;
+        emit_cell<St::SX, V>(t.win, wb, t.WX, z);
+    }
+}
 
 // Window key ((ly << 12) | lx) of a particle's home cell, -1 when the particle
 // is not this lane's (outside its row range) or not in the window.
@@ -356,9 +373,10 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             const bool inside = ok && in_win;
             if (valid && !inside) {
                 // overflow (rare): weights with the particle's charge
+                constexpr int SO = S == TSC_MOM ? (int)TSC_NODE : S;  // node weights, not moments
                 float wo[V];
-                const bool ok_o = St::template fill<EX>(fx, fy, vxx, vyy, qv, dt, d, wo);
-                overflow_scatter<S>(ok_o, cx, cy, wo, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
+                const bool ok_o = Stencil<SO>::template fill<EX>(fx, fy, vxx, vyy, qv, dt, d, wo);
+                overflow_scatter<SO>(ok_o, cx, cy, wo, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
                 if (t.acc_mark) *t.acc_mark = t.epoch;  // the merge must fold the grid
             }
             const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
@@ -370,8 +388,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                 const int idx = 4 * j + e;
                 const bool stray = nseg && ck >= 0 && idx + 1 < K && nk[idx + 1] == ck;
                 if (stray) {  // alone into the window; the run stays open
-                    const int wb = ((lk >> 12) + 1 + St::OY) * t.WX + (lk & 4095) + 1 + St::OX;
-                    emit_cell<St::SX, V>(t.win, wb, t.WX, w);
+                    emit_run<S, V>(t, lk, w);
                     continue;
                 }
             }
@@ -381,8 +398,7 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
                     for (int k = 0; k < V; ++k) H[k] = acc[k];
                     multi = true;
                 } else {
-                    const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
-                    emit_cell<St::SX, V>(t.win, wb, t.WX, acc);
+                    emit_run<S, V>(t, ck, acc);
                 }
             }
             const float keep = nseg ? 0.0f : 1.0f;
@@ -409,13 +425,11 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
     if (multi) {
 #pragma unroll
         for (int k = 0; k < V; ++k) H[k] = cont ? H[k] + Zp[k] : H[k];
-        const int wb = ((hk >> 12) + 1 + St::OY) * t.WX + (hk & 4095) + 1 + St::OX;
-        emit_cell<St::SX, V>(t.win, wb, t.WX, H);
+        emit_run<S, V>(t, hk, H);
     }
     const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
     if (ck >= 0 && !cont_next) {
-        const int wb = ((ck >> 12) + 1 + St::OY) * t.WX + (ck & 4095) + 1 + St::OX;
-        emit_cell<St::SX, V>(t.win, wb, t.WX, Z);
+        emit_run<S, V>(t, ck, Z);
     }
 }
 
@@ -438,7 +452,11 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
     t.win = reinterpret_cast<double*>(lds);
     t.acc_mark = g.acc_mark;
     t.epoch = g.epoch;
-    int* tab = reinterpret_cast<int*>(t.win + g.wsize);
+    t.mrow = g.T;
+    // LDS window: node values, or for a moment stencil (no drift halo) the
+    // V moments of every home node of the tile.
+    const int wlen = S == TSC_MOM ? St::V * g.T * g.R : g.wsize;
+    int* tab = reinterpret_cast<int*>(t.win + wlen);
     int* t_qstart = tab;          // [R]   first aligned quad of row r
     int* t_rs = tab + g.R;        // [R]   first particle of row r
     int* t_re = tab + 2 * g.R;    // [R]   one past the last particle of row r
@@ -456,7 +474,7 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
     t.WX = g.WX;
     const int th = t.th;
 
-    for (int i = lane; i < g.wsize; i += 64) {
+    for (int i = lane; i < wlen; i += 64) {
         t.win[i] = 0.0;
     }
     // Row table: lane r < th describes row y0 + r of the tile.
@@ -516,7 +534,34 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
 
     wave_sync();
     float* out = partial + (size_t)tile * g.wsize;
-    for (int i = lane; i < g.wsize; i += 64) out[i] = (float)t.win[i];
+    if constexpr (S == TSC_MOM) {
+        // Window node (wx, wy) is tile node (wx - 1, wy - 1): the sum over the
+        // home nodes within one node of it, in a fixed order, of the weight
+        // of its offset mapped from the home node's moments.
+        for (int i = lane; i < g.wsize; i += 64) {
+            const int wy = i / g.WX, wx = i - wy * g.WX;
+            double sum = 0.0;
+            for (int b = 0; b < 3; ++b) {
+                const int hy = wy - b;
+                if (hy < 0 || hy >= t.th) continue;
+                for (int a = 0; a < 3; ++a) {
+                    const int hx = wx - a;
+                    if (hx < 0 || hx >= t.tw) continue;
+                    const double* m = t.win + (hy * t.mrow + hx) * St::V;
+                    double col[3];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        col[j] = tsc_moment_coeff(a, 0) * m[3 * j] + tsc_moment_coeff(a, 1) * m[3 * j + 1] +
+                                 tsc_moment_coeff(a, 2) * m[3 * j + 2];
+                    sum += tsc_moment_coeff(b, 0) * col[0] + tsc_moment_coeff(b, 1) * col[1] +
+                           tsc_moment_coeff(b, 2) * col[2];
+                }
+            }
+            out[i] = (float)sum;
+        }
+    } else {
+        for (int i = lane; i < g.wsize; i += 64) out[i] = (float)t.win[i];
+    }
     wave_sync();  // the window may be reused for the wave's next tile
 }
 

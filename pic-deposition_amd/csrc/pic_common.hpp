@@ -14,7 +14,7 @@ enum Scheme { NGP = 0, CIC = 1, TSC = 2, ESK = 3 };
 // Tile-kernel stencils for the node-keyed order (particles sorted by nearest
 // node instead of by cell): the same NGP / TSC weights on a stencil centred
 // on that node.  Internal kernel variants, not schemes of the C ABI.
-enum NodeScheme { NGP_NODE = 4, TSC_NODE = 5 };
+enum NodeScheme { NGP_NODE = 4, TSC_NODE = 5, TSC_MOM = 6 };
 
 struct DevDomain {
     int Nx, Ny;
@@ -390,5 +390,49 @@ struct Stencil<TSC_NODE> {
         return true;
     }
 };
+
+// TSC on the node-keyed order carried as charge moments.  Slot i + 3 j holds
+// q d_x^i d_y^j, d the particle's offset from its home node in cells (the
+// `b` of tsc_node_1d, in [-1/2, 1/2)).  The node weights are polynomials in d
+// (tsc_node_1d: node h-1 0.5 (1/2 - d)^2, node h 3/4 - d^2, node h+1
+// 0.5 (1/2 + d)^2), so the 3 x 3 node values of any sum of particles of one
+// home node follow linearly from its nine moment sums (tsc_moment_coeff).
+// The tile kernel adds moments per home node and maps them to node values
+// once per tile: fewer operations per particle than forming the weights.
+template <>
+struct Stencil<TSC_MOM> {
+    static constexpr int SX = 3, SY = 3, V = 9, OX = -1, OY = -1;
+    static constexpr bool kNeedsVelocity = false;
+    static constexpr bool kAlwaysOk = true;
+    template <int EX = -1>
+    __device__ __forceinline__ static void home(float x, float y, const DevDomain& d, int& cx, int& cy, float& fx,
+                                                float& fy) {
+        node_cell<EX>(x, y, d, cx, cy, fx, fy);
+    }
+    template <int EX = -1>
+    __device__ __forceinline__ static bool fill(float fx, float fy, float, float, float qv, float, const DevDomain&,
+                                                float (&w)[V]) {
+        const float dx = fx - (fx >= 0.5f ? 1.0f : 0.0f);
+        const float dy = fy - (fy >= 0.5f ? 1.0f : 0.0f);
+        const float m10 = qv * dx, m01 = qv * dy;
+        const float m20 = m10 * dx, m11 = m10 * dy, m21 = m20 * dy;
+        w[0] = qv;
+        w[1] = m10;
+        w[2] = m20;
+        w[3] = m01;
+        w[4] = m11;
+        w[5] = m21;
+        w[6] = m01 * dy;
+        w[7] = m11 * dy;
+        w[8] = m21 * dy;
+        return true;
+    }
+};
+
+// Coefficient of d^i in the TSC weight of node offset a - 1 (a = 0, 1, 2).
+__device__ __forceinline__ double tsc_moment_coeff(int a, int i) {
+    // node h-1: 1/8 - d/2 + d^2/2; node h: 3/4 - d^2; node h+1: 1/8 + d/2 + d^2/2
+    return i == 0 ? (a == 1 ? 0.75 : 0.125) : i == 1 ? (a == 0 ? -0.5 : a == 2 ? 0.5 : 0.0) : (a == 1 ? -1.0 : 0.5);
+}
 
 }  // namespace pichip
