@@ -140,9 +140,15 @@ void launch_atomic(const DepositArgs& a) {
     hipLaunchKernelGGL(fold_acc_kernel, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, a.stream, a.rho, a.acc, G);
 }
 
-// LDS floats per wave of the moment stencil: the V moments of every home
-// node of the tile (FP64) plus the row tables.
+}  // namespace
+
 int moment_lds_per_wave(const TileGeom& g) { return 2 * Stencil<TSC_MOM>::V * g.T * g.R + 4 * g.R + 2; }
+
+bool tsc_uses_moments(const TileGeom& g) {
+    return g.D == 0 && (size_t)4 * moment_lds_per_wave(g) * sizeof(float) <= 64 * 1024;
+}
+
+namespace {
 
 template <int S, int K, int EX, int PIPE, int LA>
 void launch_tile(const DepositArgs& a) {
@@ -303,8 +309,7 @@ hipError_t launch_deposit(const DepositArgs& a) {
             case TSC: launch_tile_k<TSC>(a); break;
             case NGP_NODE: launch_tile_k<NGP_NODE>(a); break;
             case TSC_NODE:
-                // without a drift halo the moment form, when its window fits
-                if (a.geom.D == 0 && (size_t)4 * moment_lds_per_wave(a.geom) * sizeof(float) <= 64 * 1024)
+                if (tsc_uses_moments(a.geom))
                     launch_tile_moments(a);
                 else
                     launch_tile_k<TSC_NODE>(a);

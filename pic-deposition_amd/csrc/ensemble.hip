@@ -363,6 +363,10 @@ EnsKernel pick_kernel(int scheme, int ppl, int exact, int drift) {
             return pick_k<CIC>(ppl, exact, drift);
         case TSC:
             return pick_k<TSC_NODE>(ppl, exact, drift);
+        case TSC_MOM:  // only without a drift halo
+            if (ppl == 4)
+                return exact ? (EnsKernel)ens_deposit_kernel<TSC_MOM, 4, 1, 0> : (EnsKernel)ens_deposit_kernel<TSC_MOM, 4, 0, 0>;
+            return exact ? (EnsKernel)ens_deposit_kernel<TSC_MOM, 8, 1, 0> : (EnsKernel)ens_deposit_kernel<TSC_MOM, 8, 0, 0>;
         default:
             return pick_ex<ESK, 4>(exact, drift);  // Esirkepov runs 4 particles per lane
     }

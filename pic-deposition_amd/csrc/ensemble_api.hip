@@ -149,8 +149,10 @@ pic_status rebuild_work(pic_hip_ensemble* e, int m0 = 0, int m1 = -1) {
     }
     PIC_HIP_TRY(hipMemcpyAsync(e->d_mem, e->mem.data(), e->mem.size() * sizeof(EnsMember), hipMemcpyHostToDevice,
                                e->stream));
-    // Groups by (scheme, particles per lane, coordinate path); members inside a
-    // group ordered by particles per tile, largest first.
+    // Groups by (stencil, particles per lane, coordinate path); members inside a
+    // group ordered by particles per tile, largest first.  TSC members whose
+    // windows take the moment form (as a single context's deposit would) form
+    // groups of their own.
     struct Key {
         int scheme, ppl, exact;
         bool operator<(const Key& o) const {
@@ -160,7 +162,8 @@ pic_status rebuild_work(pic_hip_ensemble* e, int m0 = 0, int m1 = -1) {
     std::map<Key, std::vector<int>> by;
     for (int i = m0; i < m1; ++i) {
         const EnsMember& m = e->mem[(size_t)i];
-        by[Key{m.scheme, m.ppl, m.d.exact}].push_back(i);
+        const int st = (m.scheme == PIC_SCHEME_TSC && tsc_uses_moments(m.g)) ? (int)TSC_MOM : m.scheme;
+        by[Key{st, m.ppl, m.d.exact}].push_back(i);
     }
     std::vector<int2> items;
     std::vector<size_t> offs;
@@ -180,7 +183,7 @@ pic_status rebuild_work(pic_hip_ensemble* e, int m0 = 0, int m1 = -1) {
         offs.push_back(items.size());
         for (int i : list) {
             const EnsMember& m = e->mem[(size_t)i];
-            w.lds_per_wave = std::max(w.lds_per_wave, m.g.lds_per_wave);
+            w.lds_per_wave = std::max(w.lds_per_wave, w.scheme == TSC_MOM ? moment_lds_per_wave(m.g) : m.g.lds_per_wave);
             for (int t = 0; t < m.g.ntiles; ++t) items.push_back(make_int2(i, t));
         }
         w.nitems = (int)(items.size() - offs.back());

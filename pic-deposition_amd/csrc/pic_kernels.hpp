@@ -40,6 +40,13 @@ struct TileGeom {
 
 TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D = 0);
 
+// LDS floats per wave of the TSC moment stencil: the nine FP64 moments of
+// every home node of the tile plus the row tables.
+int moment_lds_per_wave(const TileGeom& g);
+// TSC on the node-keyed order runs as charge moments: no drift halo and the
+// moment windows of four waves fit 64 KiB of LDS.
+bool tsc_uses_moments(const TileGeom& g);
+
 struct DepositArgs {
     const float *x, *y, *vx, *vy, *q;
     int64_t n;
@@ -292,7 +299,7 @@ struct EnsParticles {
 // One persistent deposit launch: the (member, tile) items of the members
 // with this scheme, particles per lane and coordinate path.
 struct EnsWork {
-    int scheme, ppl, exact;
+    int scheme, ppl, exact;  // scheme: a C-ABI scheme, or TSC_MOM for TSC members in the moment form
     int drift;  // the windows have a drift halo: lookahead chunk processing
     const int2* items;
     int nitems;
