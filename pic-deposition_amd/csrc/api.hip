@@ -26,7 +26,7 @@ using namespace pichip;
 namespace {
 
 pic_status ensure_partial(pic_hip_ctx* c) {
-    size_t need = 0;
+    size_t need = (size_t)c->tsc_cell_geom.ntiles * c->tsc_cell_geom.wsize;
     for (int k = 0; k < 4; ++k) need = std::max(need, (size_t)c->geom[k].ntiles * c->geom[k].wsize);
     if (need <= c->partial_cap) return PIC_OK;
     (void)hipFree(c->partial);
@@ -49,6 +49,30 @@ pic_status apply_tiling(pic_hip_ctx* c, int scheme, int tx, int ty, int ppl, int
         c->geom[k] = g;
         c->ppl[k] = ppl;
         c->pipe[k] = pipe ? 1 : 0;
+    }
+    return ensure_partial(c);
+}
+
+// Tiling of TSC on the cell-keyed order (pic_hip_ctx::tsc_cell_geom): the TSC
+// tiling, or while that is the create-time one without a drift halo, the
+// cell-keyed long rows where the grid holds 4096 of them with at least 1024
+// particles each.  Called after every change of the TSC tiling or the halo.
+pic_status sync_tsc_cell(pic_hip_ctx* c) {
+    c->tsc_cell_geom = c->geom[PIC_SCHEME_TSC];
+    c->tsc_cell_ppl = c->ppl[PIC_SCHEME_TSC];
+    c->tsc_cell_pipe = c->pipe[PIC_SCHEME_TSC];
+    if (c->tiling_default[PIC_SCHEME_TSC] && c->halo == 0) {
+        const int rx = kRowTilingTscCell[0], ry = kRowTilingTscCell[1];
+        const int Nx = c->dom.Nx, Ny = c->dom.Ny;
+        const int64_t tiles = (int64_t)((Nx + rx - 1) / rx) * ((Ny + ry - 1) / ry);
+        const int64_t per_tile = c->tile_capacity * rx * ry / std::max<int64_t>(c->G, 1);
+        const TileGeom g = make_tile_geom(Nx, Ny, rx, ry, 0);
+        if (rx <= Nx && tiles >= 4096 && per_tile >= 1024 &&
+            (size_t)4 * g.lds_per_wave * sizeof(float) <= kMaxDepositLds) {
+            c->tsc_cell_geom = g;
+            c->tsc_cell_ppl = kRowTilingTscCell[2];
+            c->tsc_cell_pipe = 0;
+        }
     }
     return ensure_partial(c);
 }
@@ -215,6 +239,10 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
         if (st != PIC_OK) return st;
         c->tiling_default[k] = true;
     }
+    {
+        const pic_status st = sync_tsc_cell(c.get());
+        if (st != PIC_OK) return st;
+    }
     PIC_HIP_TRY(hipStreamSynchronize(c->stream));
     *out = c.release();
     return PIC_OK;
@@ -251,10 +279,10 @@ pic_status pic_hip_set_tiling(pic_hip_ctx* ctx, int scheme, int tile_x, int tile
                               int pipelined) {
     if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
     const pic_status st = apply_tiling(ctx, scheme, tile_x, tile_y, particles_per_lane, pipelined);
-    if (st == PIC_OK)
-        for (int k = 0; k < 4; ++k)
-            if (scheme < 0 || k == scheme) ctx->tiling_default[k] = false;
-    return st;
+    if (st != PIC_OK) return st;
+    for (int k = 0; k < 4; ++k)
+        if (scheme < 0 || k == scheme) ctx->tiling_default[k] = false;
+    return sync_tsc_cell(ctx);
 }
 
 pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells) {
@@ -281,10 +309,11 @@ pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells) {
             const std::string msg = pic_hip_last_error();
             ctx->halo = old;
             for (int j = 0; j < 4; ++j) (void)apply_tiling(ctx, j, saved[j][0], saved[j][1], saved[j][2], saved[j][3]);
+            (void)sync_tsc_cell(ctx);
             return fail(st, msg);
         }
     }
-    return PIC_OK;
+    return sync_tsc_cell(ctx);
 }
 
 int pic_hip_get_tile_halo(pic_hip_ctx* ctx) { return ctx ? ctx->halo : 0; }
@@ -553,6 +582,11 @@ pic_status pic_hip_deposit(pic_hip_ctx* ctx, int scheme, int backend, double dt)
     a.ppl = ctx->ppl[scheme];
     a.pipe = ctx->pipe[scheme];
     a.geom = ctx->geom[scheme];
+    if (a.scheme == TSC && backend == PIC_BACKEND_GPU_PRIV) {  // TSC on the cell-keyed order
+        a.ppl = ctx->tsc_cell_ppl;
+        a.pipe = ctx->tsc_cell_pipe;
+        a.geom = ctx->tsc_cell_geom;
+    }
     a.geom.acc_mark = ctx->acc_mark;
     a.geom.epoch = ctx->deposit_epoch = ctx->deposit_epoch % 0x7ffffff0 + 1;  // never 0 (the initial marker)
     a.dt = (float)dt;
@@ -1252,7 +1286,13 @@ pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path
     double best_ms = 0.0;
     const auto cached = tuning_cache_read(cache_path);
     auto hit = cached.find(key);
-    if (hit != cached.end() && apply_tiling(ctx, scheme, hit->second.tx, hit->second.ty, hit->second.ppl, 0) == PIC_OK) {
+    // the tilings tried apply to both orders of TSC
+    ctx->tiling_default[scheme] = false;
+    auto apply = [&](int tx, int ty, int ppl) {
+        const pic_status st = apply_tiling(ctx, scheme, tx, ty, ppl, 0);
+        return st != PIC_OK ? st : sync_tsc_cell(ctx);
+    };
+    if (hit != cached.end() && apply(hit->second.tx, hit->second.ty, hit->second.ppl) == PIC_OK) {
         best = hit->second;
     } else {
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1262,7 +1302,7 @@ pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path
         std::unique_ptr<ihipEvent_t, hipError_t (*)(hipEvent_t)> g1(e1, hipEventDestroy);
         best_ms = -1.0;
         for (const TilingChoice& c : tiling_candidates(ctx->dom.Nx, ctx->dom.Ny, false, ctx->halo)) {
-            if (apply_tiling(ctx, scheme, c.tx, c.ty, c.ppl, 0) != PIC_OK) continue;
+            if (apply(c.tx, c.ty, c.ppl) != PIC_OK) continue;
             pic_status st = pic_hip_deposit(ctx, scheme, PIC_BACKEND_GPU_PRIV, 0.0);  // warm-up
             if (st != PIC_OK) return st;
             std::vector<double> t;
@@ -1285,9 +1325,8 @@ pic_status pic_hip_autotune(pic_hip_ctx* ctx, int scheme, const char* cache_path
         if (!tuning_cache_store(cache_path, key, best))
             return fail(PIC_ERR_INVALID_ARGUMENT, std::string("cannot write tuning cache ") + cache_path);
     }
-    pic_status st = apply_tiling(ctx, scheme, best.tx, best.ty, best.ppl, 0);
+    pic_status st = apply(best.tx, best.ty, best.ppl);
     if (st != PIC_OK) return st;
-    ctx->tiling_default[scheme] = false;
     if (tile_x) *tile_x = best.tx;
     if (tile_y) *tile_y = best.ty;
     if (particles_per_lane) *particles_per_lane = best.ppl;

@@ -64,6 +64,12 @@ struct pic_hip_ctx {
     bool tiling_default[4] = {};
     int64_t tile_capacity = 0;  // capacity the create-time tilings were chosen for
     int pipe[4] = {1, 1, 0, 0};
+    // Tiling of TSC on the cell-keyed order: the TSC tiling, except while TSC
+    // is on its create-time tiling without a drift halo, when the cell-keyed
+    // kernel takes its own long rows (the create-time long rows are chosen for
+    // the node-keyed order's moment form).
+    TileGeom tsc_cell_geom{};
+    int tsc_cell_ppl = 8, tsc_cell_pipe = 0;
     float* phi = nullptr;
     float* Ex = nullptr;
     float* Ey = nullptr;

@@ -23,6 +23,8 @@
 
 #include "deposit_device.hpp"
 
+#include <algorithm>
+
 namespace pichip {
 
 namespace {
@@ -142,10 +144,16 @@ void launch_atomic(const DepositArgs& a) {
 
 }  // namespace
 
-int moment_lds_per_wave(const TileGeom& g) { return 2 * Stencil<TSC_MOM>::V * g.T * g.R + 4 * g.R + 2; }
+int moment_lds_per_wave(const TileGeom& g, int ppl) {
+    const int during = 2 * Stencil<TSC_MOM>::V * g.T * g.R + 4 * g.R + 2 + mom_ring_size(ppl);
+    // The FP64 node window overlays the moments (and, once they are done
+    // with, the row tables and the ring) for at most 64 home nodes, and
+    // follows the ring otherwise.
+    return g.T * g.R <= 64 ? std::max(during, 2 * g.wsize) : during + 2 * g.wsize;
+}
 
-bool tsc_uses_moments(const TileGeom& g) {
-    return g.D == 0 && (size_t)4 * moment_lds_per_wave(g) * sizeof(float) <= 64 * 1024;
+bool tsc_uses_moments(const TileGeom& g, int ppl) {
+    return g.D == 0 && (size_t)4 * moment_lds_per_wave(g, ppl) * sizeof(float) <= 64 * 1024;
 }
 
 namespace {
@@ -153,7 +161,7 @@ namespace {
 template <int S, int K, int EX, int PIPE, int LA>
 void launch_tile(const DepositArgs& a) {
     TileGeom g = a.geom;
-    if (S == TSC_MOM) g.lds_per_wave = moment_lds_per_wave(g);
+    if (S == TSC_MOM) g.lds_per_wave = moment_lds_per_wave(g, K);
     const int waves_per_block = 4;
     const unsigned blocks = (unsigned)((g.ntiles + waves_per_block - 1) / waves_per_block);
     const size_t lds = (size_t)waves_per_block * g.lds_per_wave * sizeof(float);
@@ -201,14 +209,20 @@ void launch_tile_k(const DepositArgs& a) {
 // TSC on the node-keyed order without a drift halo: the moment form (no
 // lookahead or register double-buffering variants: the latter, 134 VGPRs,
 // was 8-10% slower).
-void launch_tile_moments(const DepositArgs& a) {
-    if (a.ppl == 4) {
-        if (a.dom.exact) launch_tile<TSC_MOM, 4, 1, 0, 0>(a);
-        else launch_tile<TSC_MOM, 4, 0, 0, 0>(a);
+template <int K>
+void launch_tile_moments_k(const DepositArgs& a) {
+    if (a.pipe) {
+        if (a.dom.exact) launch_tile<TSC_MOM, K, 1, 1, 0>(a);
+        else launch_tile<TSC_MOM, K, 0, 1, 0>(a);
     } else {
-        if (a.dom.exact) launch_tile<TSC_MOM, 8, 1, 0, 0>(a);
-        else launch_tile<TSC_MOM, 8, 0, 0, 0>(a);
+        if (a.dom.exact) launch_tile<TSC_MOM, K, 1, 0, 0>(a);
+        else launch_tile<TSC_MOM, K, 0, 0, 0>(a);
     }
+}
+
+void launch_tile_moments(const DepositArgs& a) {
+    if (a.ppl == 4) launch_tile_moments_k<4>(a);
+    else launch_tile_moments_k<8>(a);
 }
 
 // Bernoulli thinning by hash: particle i (in the current order) is kept when
@@ -309,7 +323,7 @@ hipError_t launch_deposit(const DepositArgs& a) {
             case TSC: launch_tile_k<TSC>(a); break;
             case NGP_NODE: launch_tile_k<NGP_NODE>(a); break;
             case TSC_NODE:
-                if (tsc_uses_moments(a.geom))
+                if (tsc_uses_moments(a.geom, a.ppl))
                     launch_tile_moments(a);
                 else
                     launch_tile_k<TSC_NODE>(a);

@@ -56,15 +56,17 @@ __device__ __forceinline__ void lin_scan_step(float (&B)[V], float& A) {
 }
 
 // Inclusive wave-wide scan of B_i = b_i + a_i * B_{i-1} with a_i in {0, 1}:
-// a segmented sum where a_i = 0 starts a new segment at lane i.
-template <int V>
+// a segmented sum where a_i = 0 starts a new segment at lane i.  With
+// STEPS < 6 the scan stays within groups of 2^STEPS lanes (16 at 4 steps, 32
+// at 5): a group's first lane starts a segment whatever its a_i.
+template <int V, int STEPS = 6>
 __device__ __forceinline__ void wave_segmented_scan(float (&B)[V], float A) {
     lin_scan_step<kRowShr + 1, 0xf>(B, A);
-    lin_scan_step<kRowShr + 2, 0xf>(B, A);
-    lin_scan_step<kRowShr + 4, 0xf>(B, A);
-    lin_scan_step<kRowShr + 8, 0xf>(B, A);
-    lin_scan_step<kRowBcast15, 0xa>(B, A);
-    lin_scan_step<kRowBcast31, 0xc>(B, A);
+    if constexpr (STEPS >= 2) lin_scan_step<kRowShr + 2, 0xf>(B, A);
+    if constexpr (STEPS >= 3) lin_scan_step<kRowShr + 4, 0xf>(B, A);
+    if constexpr (STEPS >= 4) lin_scan_step<kRowShr + 8, 0xf>(B, A);
+    if constexpr (STEPS >= 5) lin_scan_step<kRowBcast15, 0xa>(B, A);
+    if constexpr (STEPS >= 6) lin_scan_step<kRowBcast31, 0xc>(B, A);
 }
 
 // Adds a cell's V stencil values into the wave's FP64 LDS window (one
@@ -310,6 +312,7 @@ __device__ __forceinline__ int window_key(float xx, float yy, bool valid, const 
 template <int S, int Q, int EX, int LA>
 __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx& t, const DevDomain& d, float dt,
                                               double* __restrict__ gacc, int* __restrict__ status) {
+    static_assert(S != TSC_MOM, "the moment stencil has its own chunk processing (mom_chunk)");
     using St = Stencil<S>;
     constexpr int V = St::V;
     constexpr int K = 4 * Q;
@@ -373,10 +376,9 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             const bool inside = ok && in_win;
             if (valid && !inside) {
                 // overflow (rare): weights with the particle's charge
-                constexpr int SO = S == TSC_MOM ? (int)TSC_NODE : S;  // node weights, not moments
                 float wo[V];
-                const bool ok_o = Stencil<SO>::template fill<EX>(fx, fy, vxx, vyy, qv, dt, d, wo);
-                overflow_scatter<SO>(ok_o, cx, cy, wo, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
+                const bool ok_o = St::template fill<EX>(fx, fy, vxx, vyy, qv, dt, d, wo);
+                overflow_scatter<S>(ok_o, cx, cy, wo, xx, yy, vxx, vyy, qv, dt, d, gacc, status);
                 if (t.acc_mark) *t.acc_mark = t.epoch;  // the merge must fold the grid
             }
             const int lk = (valid && inside) ? ((ly << 12) | lx) : ck;
@@ -430,6 +432,202 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
     const int cont_next = dpp_i<kWaveShl1, 0xf>(0, cont ? 1 : 0);
     if (ck >= 0 && !cont_next) {
         emit_run<S, V>(t, ck, Z);
+    }
+}
+
+// ---- TSC moments on the node-keyed order (no drift halo) -------------------
+//
+// Each lane sums the moments of its K particles about one reference node, the
+// nearest node of its first particle: per particle the two offsets from that
+// node, the |offset| <= 1/2 test and the nine moment sums, with no per-particle
+// key or run bookkeeping.  On node-sorted input a lane's particles share that
+// node except where a node's run ends inside the lane; such particles (and any
+// that drifted since the sort, or whose reference node is not one of the
+// tile's) are queued as particle indices in an LDS ring and deposited 64 at a
+// time, one per lane, by the general per-particle path.  The lanes then
+// combine equal reference nodes with one segmented wave scan and one FP64 LDS
+// add per moment and run.  Window moments carry q without 1 / cell volume (the
+// tile's node values are scaled once when they are formed).
+// Lanes whose moment sums one segmented scan combines before the LDS adds
+// (2^4 = 16: one row of the wave).  Three steps time the same; two leave
+// enough adds to the same node in one instruction to be slower.
+constexpr int kMomScanSteps = 4;
+// Ring entries per wave for K particles per lane: at least the 63 + 64 K
+// entries that can be pending after a chunk.
+__host__ __device__ constexpr int mom_ring_size(int K) { return K <= 4 ? 512 : 1024; }
+
+// Offset in cells of coordinate x from node n.  With a power-of-two cell size
+// x * inv_d is exact in FP32 and so is the difference (the same value as the
+// floor-cell fraction minus 0 or 1); otherwise the product with the FP64
+// reciprocal.
+template <int EX>
+__device__ __forceinline__ float node_offset(float x, float inv_d, double rinv, float n) {
+    if constexpr (EX == 1) return fmaf(x, inv_d, -n);
+    else return (float)fma((double)x, rinv, -(double)n);
+}
+
+// Fast path over one chunk.  Appends the indices of the particles it leaves to
+// the ring (`tail` counts appended entries, wave-uniform).
+template <int Q, int EX, int RING>
+__device__ __forceinline__ void mom_chunk(Chunk<Q>& c, const TileCtx& t, const DevDomain& d, double rdx, double rdy,
+                                          int* __restrict__ ring, int& tail) {
+    static_assert(63 + 64 * 4 * Q <= RING && (RING & (RING - 1)) == 0, "ring too small for one chunk's entries");
+    const int lane = threadIdx.x & 63;
+    using St = Stencil<TSC_MOM>;
+    constexpr int V = St::V;
+    // Quad slots outside the row ranges (a row's first and last quads, lanes
+    // past the tile's particles) get zero charge and the coordinates of the
+    // lane's first particle in range, so they pass the node test with it and
+    // add exact zeros; a lane with no particle in range is not live.
+    bool partial = false;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) partial |= !(c.p0[j] >= c.lo[j] && c.p0[j] + 4 <= c.hi[j]);
+    bool live = true;
+    if (partial) {
+        float x0 = 0.0f, y0 = 0.0f;
+        bool found = false;
+#pragma unroll
+        for (int k = 4 * Q - 1; k >= 0; --k) {
+            const int j = k >> 2, e = k & 3;
+            const int p = c.p0[j] + e;
+            const bool valid = p >= c.lo[j] && p < c.hi[j];
+            const float4 X = c.X[j], Y = c.Y[j];
+            const float xx = e == 0 ? X.x : e == 1 ? X.y : e == 2 ? X.z : X.w;
+            const float yy = e == 0 ? Y.x : e == 1 ? Y.y : e == 2 ? Y.z : Y.w;
+            x0 = valid ? xx : x0;
+            y0 = valid ? yy : y0;
+            found |= valid;
+        }
+        live = found;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const bool v0 = c.p0[j] >= c.lo[j] && c.p0[j] < c.hi[j];
+            const bool v1 = c.p0[j] + 1 >= c.lo[j] && c.p0[j] + 1 < c.hi[j];
+            const bool v2 = c.p0[j] + 2 >= c.lo[j] && c.p0[j] + 2 < c.hi[j];
+            const bool v3 = c.p0[j] + 3 >= c.lo[j] && c.p0[j] + 3 < c.hi[j];
+            c.X[j] = make_float4(v0 ? c.X[j].x : x0, v1 ? c.X[j].y : x0, v2 ? c.X[j].z : x0, v3 ? c.X[j].w : x0);
+            c.Y[j] = make_float4(v0 ? c.Y[j].x : y0, v1 ? c.Y[j].y : y0, v2 ? c.Y[j].z : y0, v3 ? c.Y[j].w : y0);
+            c.QQ[j] = make_float4(v0 ? c.QQ[j].x : 0.0f, v1 ? c.QQ[j].y : 0.0f, v2 ? c.QQ[j].z : 0.0f,
+                                  v3 ? c.QQ[j].w : 0.0f);
+        }
+    }
+    // Reference node: the nearest node of the first slot (node_cell), kept
+    // unwrapped for the offsets; its window key, -1 outside the tile.
+    int cx, cy;
+    float fx, fy;
+    cell_coord<EX>(c.X[0].x, d.inv_dx, d.dx, d.exact, cx, fx);
+    cell_coord<EX>(c.Y[0].x, d.inv_dy, d.dy, d.exact, cy, fy);
+    const int nux = cx + (fx >= 0.5f ? 1 : 0), nuy = cy + (fy >= 0.5f ? 1 : 0);
+    const int nwx = (nux == d.Nx && cx == d.Nx - 1) ? 0 : nux;
+    const int nwy = (nuy == d.Ny && cy == d.Ny - 1) ? 0 : nuy;
+    const int lx = nwx - t.ax0, ly = nwy - t.ay0;
+    const bool inw = (unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah;
+    const int key = !live ? -2 : inw ? ((ly << 12) | lx) : -1;
+    const bool fast = key >= 0;
+    const float nfx = (float)nux, nfy = (float)nuy;
+    // A particle is taken when both offsets are within 1/2 of the reference
+    // node (lim = -1 takes none); the ring gets the untaken particles of the
+    // lanes with a reference node: those past one of its runs, or drifted.
+    const float lim = fast ? 0.5f : -1.0f;
+    const uint64_t fast_lanes = __builtin_amdgcn_ballot_w64(fast);
+
+    float s[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) s[k] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float xx = e == 0 ? c.X[j].x : e == 1 ? c.X[j].y : e == 2 ? c.X[j].z : c.X[j].w;
+            const float yy = e == 0 ? c.Y[j].x : e == 1 ? c.Y[j].y : e == 2 ? c.Y[j].z : c.Y[j].w;
+            const float qq = e == 0 ? c.QQ[j].x : e == 1 ? c.QQ[j].y : e == 2 ? c.QQ[j].z : c.QQ[j].w;
+            const float ox = node_offset<EX>(xx, d.inv_dx, rdx, nfx);
+            const float oy = node_offset<EX>(yy, d.inv_dy, rdy, nfy);
+            // |offset| = 1/2 (a tie between two nodes) gives the same TSC
+            // weights on either node, so it may stay with the reference node.
+            const bool untaken = fmaxf(fabsf(ox), fabsf(oy)) > lim;
+            const float qf = untaken ? 0.0f : qq;
+            const float m10 = qf * ox, m01 = qf * oy;
+            const float m20 = m10 * ox, m11 = m10 * oy, m21 = m20 * oy;
+            s[0] += qf;
+            s[1] += m10;
+            s[2] += m20;
+            s[3] += m01;
+            s[4] += m11;
+            s[5] += m21;
+            s[6] = fmaf(m01, oy, s[6]);
+            s[7] = fmaf(m11, oy, s[7]);
+            s[8] = fmaf(m21, oy, s[8]);
+            // Slots outside the row ranges are taken in a lane with a reference
+            // node (they repeat its first particle), so only particles are queued.
+            const uint64_t qm = __builtin_amdgcn_ballot_w64(untaken) & fast_lanes;
+            if (qm) {
+                if (untaken && fast) {
+                    const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
+                    ring[(tail + r) & (RING - 1)] = c.p0[j] + e;
+                }
+                tail += (int)__popcll(qm);
+            }
+        }
+    }
+    // A lane whose reference node is not the tile's (a particle that drifted
+    // or lies outside [0, L)) leaves every particle in range to the ring.
+    if (__builtin_amdgcn_ballot_w64(live && !fast)) {
+#pragma unroll
+        for (int k = 0; k < 4 * Q; ++k) {
+            const int j = k >> 2, e = k & 3;
+            const int p = c.p0[j] + e;
+            const bool queue = live && !fast && p >= c.lo[j] && p < c.hi[j];
+            const uint64_t qm = __builtin_amdgcn_ballot_w64(queue);
+            if (qm) {
+                if (queue) {
+                    const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
+                    ring[(tail + r) & (RING - 1)] = p;
+                }
+                tail += (int)__popcll(qm);
+            }
+        }
+    }
+    // Lanes of equal reference node (consecutive on sorted input) are summed
+    // by a segmented scan within each 16-lane row; the last lane of a run's
+    // piece in a row adds the moments to LDS (a few adds per node and chunk
+    // instead of two more scan steps over nine values).
+    const int kprev = dpp_i<kWaveShr1, 0xf>(-3, key);
+    const bool cont = fast && key == kprev;
+    wave_segmented_scan<V, kMomScanSteps>(s, cont ? 1.0f : 0.0f);
+    const int knext = dpp_i<kWaveShl1, 0xf>(-3, key);
+    constexpr int kLast = (1 << kMomScanSteps) - 1;
+    if (fast && (knext != key || (lane & kLast) == kLast)) emit_run<TSC_MOM, V>(t, key, s);
+}
+
+// Deposits ring entries [head, head + cnt), cnt <= 64, one per lane, with the
+// general path: nearest node, window test, moments into the window or node
+// weights to the FP64 accumulation grid.
+template <int EX, int RING>
+__device__ __forceinline__ void mom_ring_batch(const TileCtx& t, const int* __restrict__ ring, int head, int cnt,
+                                               const float* __restrict__ px, const float* __restrict__ py,
+                                               const float* __restrict__ pq, const DevDomain& d,
+                                               double* __restrict__ gacc, int* __restrict__ status) {
+    const int lane = threadIdx.x & 63;
+    if (lane >= cnt) return;
+    const int p = ring[(head + lane) & (RING - 1)];
+    const float xx = px[p], yy = py[p], qq = pq[p];
+    int cx, cy;
+    float fx, fy;
+    node_cell<EX>(xx, yy, d, cx, cy, fx, fy);
+    const int lx = cx - t.ax0, ly = cy - t.ay0;
+    if ((unsigned)lx < (unsigned)t.aw && (unsigned)ly < (unsigned)t.ah) {
+        float w[9];
+        Stencil<TSC_MOM>::template fill<EX>(fx, fy, 0.0f, 0.0f, qq, 0.0f, d, w);
+        emit_run<TSC_MOM, 9>(t, (ly << 12) | lx, w);
+    } else {
+        const float qv = qq * d.inv_vol;
+        float w[9];
+        Stencil<TSC_NODE>::template fill<EX>(fx, fy, 0.0f, 0.0f, qv, 0.0f, d, w);
+        overflow_scatter<TSC_NODE>(true, cx, cy, w, xx, yy, 0.0f, 0.0f, qv, 0.0f, d, gacc, status);
+        if (t.acc_mark) *t.acc_mark = t.epoch;  // the merge must fold the grid
     }
 }
 
@@ -488,10 +686,9 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
         t_rs[lane] = s;
         t_re[lane] = e;
     }
-    // Inclusive prefix of nq over the (at most 64) row lanes.
+    // Inclusive prefix of nq over the th (at most 64) row lanes.
     int incl = nq;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
+    for (int o = 1; o < th; o <<= 1) {
         const int u = __shfl_up(incl, o, 64);
         if (lane >= o) incl += u;
     }
@@ -501,7 +698,55 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
     const int total_q = t_qcum[th];
     const int step = 64 * Q;
 
-    if (PIPE) {
+    if constexpr (S == TSC_MOM) {
+        constexpr int RING = mom_ring_size(K);
+        int* ring = t_qcum + g.R + 2;  // 8-byte aligned: the node window follows it
+        int head = 0, tail = 0;
+        const double rdx = EX == 1 ? 0.0 : 1.0 / d.dx, rdy = EX == 1 ? 0.0 : 1.0 / d.dy;
+        // Full batches of ring entries after each chunk, the rest at the end.
+        auto chunk = [&](Chunk<Q>& A) {
+            mom_chunk<Q, EX, RING>(A, t, d, rdx, rdy, ring, tail);
+            tail = __builtin_amdgcn_readfirstlane(tail);
+            if (tail - head >= 64) {
+                wave_sync();
+                do {
+                    mom_ring_batch<EX, RING>(t, ring, head, 64, px, py, pq, d, gacc, status);
+                    head += 64;
+                } while (tail - head >= 64);
+            }
+        };
+        if (PIPE) {
+            Chunk<Q> A, B;
+            int base = 0;
+            if (base < total_q)
+                load_chunk<Q, false>(lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq, pvx, pvy, A);
+            while (base < total_q) {
+                const int b1 = base + step;
+                if (b1 < total_q)
+                    load_chunk<Q, false>(b1 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq, pvx,
+                                         pvy, B);
+                chunk(A);
+                if (b1 >= total_q) break;
+                const int b2 = b1 + step;
+                if (b2 < total_q)
+                    load_chunk<Q, false>(b2 + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq, pvx,
+                                         pvy, A);
+                chunk(B);
+                base = b2;
+            }
+        } else {
+            for (int base = 0; base < total_q; base += step) {
+                Chunk<Q> A;
+                load_chunk<Q, false>(base + lane * Q, total_q, th, t_qcum, t_qstart, t_rs, t_re, px, py, pq, pvx, pvy,
+                                     A);
+                chunk(A);
+            }
+        }
+        if (tail > head) {
+            wave_sync();
+            mom_ring_batch<EX, RING>(t, ring, head, tail - head, px, py, pq, d, gacc, status);
+        }
+    } else if (PIPE) {
         // Two register sets: the loads of one chunk are in flight while the
         // other is processed.
         Chunk<Q> A, B;
@@ -535,30 +780,53 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
     wave_sync();
     float* out = partial + (size_t)tile * g.wsize;
     if constexpr (S == TSC_MOM) {
-        // Window node (wx, wy) is tile node (wx - 1, wy - 1): the sum over the
-        // home nodes within one node of it, in a fixed order, of the weight
-        // of its offset mapped from the home node's moments.
-        for (int i = lane; i < g.wsize; i += 64) {
-            const int wy = i / g.WX, wx = i - wy * g.WX;
-            double sum = 0.0;
-            for (int b = 0; b < 3; ++b) {
-                const int hy = wy - b;
-                if (hy < 0 || hy >= t.th) continue;
-                for (int a = 0; a < 3; ++a) {
-                    const int hx = wx - a;
-                    if (hx < 0 || hx >= t.tw) continue;
-                    const double* m = t.win + (hy * t.mrow + hx) * St::V;
-                    double col[3];
+        // Node values of the window from the home nodes' moments: each lane
+        // maps one home node's nine moments to its 3 x 3 node values (the
+        // TSC weight polynomials, separable in x and y) and adds them to the
+        // FP64 node window; in each of the nine adds the lanes' target nodes
+        // differ.  Window node (wx, wy) is tile node (wx - 1, wy - 1).
+        // With at most 64 home nodes every lane holds its node's moments
+        // before the window is cleared, so the node window overlays them.
+        const bool overlay = g.T * g.R <= 64;
+        double* nw = overlay ? t.win : reinterpret_cast<double*>(t_qcum + g.R + 2 + mom_ring_size(K));
+        double mm[9];
+        if (overlay && lane < g.T * g.R) {
 #pragma unroll
-                    for (int j = 0; j < 3; ++j)
-                        col[j] = tsc_moment_coeff(a, 0) * m[3 * j] + tsc_moment_coeff(a, 1) * m[3 * j + 1] +
-                                 tsc_moment_coeff(a, 2) * m[3 * j + 2];
-                    sum += tsc_moment_coeff(b, 0) * col[0] + tsc_moment_coeff(b, 1) * col[1] +
-                           tsc_moment_coeff(b, 2) * col[2];
+            for (int k = 0; k < 9; ++k) mm[k] = t.win[(size_t)lane * St::V + k];
+        }
+        wave_sync();
+        for (int i = lane; i < g.wsize; i += 64) nw[i] = 0.0;
+        wave_sync();
+        for (int h = lane; h < g.T * g.R; h += 64) {
+            const int hy = h / g.T, hx = h - hy * g.T;
+            if (hx < t.tw && hy < t.th) {
+                if (!overlay) {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) mm[k] = t.win[(size_t)h * St::V + k];
+                }
+                // x: node h-1 1/8 m0 - 1/2 m1 + 1/2 m2, node h 3/4 m0 - m2,
+                // node h+1 1/8 m0 + 1/2 m1 + 1/2 m2 (m_i: moment order i in x)
+                double u[3][3];  // [x node a][y moment order j]
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double A = fma(0.125, mm[3 * j], 0.5 * mm[3 * j + 2]), B = 0.5 * mm[3 * j + 1];
+                    u[0][j] = A - B;
+                    u[1][j] = fma(0.75, mm[3 * j], -mm[3 * j + 2]);
+                    u[2][j] = A + B;
+                }
+                const int base = hy * g.WX + hx;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double A = fma(0.125, u[a][0], 0.5 * u[a][2]), B = 0.5 * u[a][1];
+                    atomicAdd(nw + base + a, A - B);
+                    atomicAdd(nw + base + g.WX + a, fma(0.75, u[a][0], -u[a][2]));
+                    atomicAdd(nw + base + 2 * g.WX + a, A + B);
                 }
             }
-            out[i] = (float)sum;
         }
+        wave_sync();
+        const double iv = (double)d.inv_vol;
+        for (int i = lane; i < g.wsize; i += 64) out[i] = (float)(nw[i] * iv);
     } else {
         for (int i = lane; i < g.wsize; i += 64) out[i] = (float)t.win[i];
     }

@@ -162,7 +162,7 @@ pic_status rebuild_work(pic_hip_ensemble* e, int m0 = 0, int m1 = -1) {
     std::map<Key, std::vector<int>> by;
     for (int i = m0; i < m1; ++i) {
         const EnsMember& m = e->mem[(size_t)i];
-        const int st = (m.scheme == PIC_SCHEME_TSC && tsc_uses_moments(m.g)) ? (int)TSC_MOM : m.scheme;
+        const int st = (m.scheme == PIC_SCHEME_TSC && tsc_uses_moments(m.g, m.ppl)) ? (int)TSC_MOM : m.scheme;
         by[Key{st, m.ppl, m.d.exact}].push_back(i);
     }
     std::vector<int2> items;
@@ -183,7 +183,7 @@ pic_status rebuild_work(pic_hip_ensemble* e, int m0 = 0, int m1 = -1) {
         offs.push_back(items.size());
         for (int i : list) {
             const EnsMember& m = e->mem[(size_t)i];
-            w.lds_per_wave = std::max(w.lds_per_wave, w.scheme == TSC_MOM ? moment_lds_per_wave(m.g) : m.g.lds_per_wave);
+            w.lds_per_wave = std::max(w.lds_per_wave, w.scheme == TSC_MOM ? moment_lds_per_wave(m.g, m.ppl) : m.g.lds_per_wave);
             for (int t = 0; t < m.g.ntiles; ++t) items.push_back(make_int2(i, t));
         }
         w.nitems = (int)(items.size() - offs.back());

@@ -396,9 +396,9 @@ struct Stencil<TSC_NODE> {
 // `b` of tsc_node_1d, in [-1/2, 1/2)).  The node weights are polynomials in d
 // (tsc_node_1d: node h-1 0.5 (1/2 - d)^2, node h 3/4 - d^2, node h+1
 // 0.5 (1/2 + d)^2), so the 3 x 3 node values of any sum of particles of one
-// home node follow linearly from its nine moment sums (tsc_moment_coeff).
-// The tile kernel adds moments per home node and maps them to node values
-// once per tile: fewer operations per particle than forming the weights.
+// home node follow linearly from its nine moment sums.  The tile kernel adds
+// moments per home node and maps them to node values once per tile: fewer
+// operations per particle than forming the weights.
 template <>
 struct Stencil<TSC_MOM> {
     static constexpr int SX = 3, SY = 3, V = 9, OX = -1, OY = -1;
@@ -429,10 +429,5 @@ struct Stencil<TSC_MOM> {
     }
 };
 
-// Coefficient of d^i in the TSC weight of node offset a - 1 (a = 0, 1, 2).
-__device__ __forceinline__ double tsc_moment_coeff(int a, int i) {
-    // node h-1: 1/8 - d/2 + d^2/2; node h: 3/4 - d^2; node h+1: 1/8 + d/2 + d^2/2
-    return i == 0 ? (a == 1 ? 0.75 : 0.125) : i == 1 ? (a == 0 ? -0.5 : a == 2 ? 0.5 : 0.0) : (a == 1 ? -1.0 : 0.5);
-}
 
 }  // namespace pichip

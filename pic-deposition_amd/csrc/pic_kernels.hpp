@@ -41,11 +41,12 @@ struct TileGeom {
 TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D = 0);
 
 // LDS floats per wave of the TSC moment stencil: the nine FP64 moments of
-// every home node of the tile plus the row tables.
-int moment_lds_per_wave(const TileGeom& g);
+// every home node of the tile, the row tables, the ring of particles left to
+// the per-particle path and the FP64 node window the moments map to.
+int moment_lds_per_wave(const TileGeom& g, int ppl);
 // TSC on the node-keyed order runs as charge moments: no drift halo and the
 // moment windows of four waves fit 64 KiB of LDS.
-bool tsc_uses_moments(const TileGeom& g);
+bool tsc_uses_moments(const TileGeom& g, int ppl);
 
 struct DepositArgs {
     const float *x, *y, *vx, *vy, *q;

@@ -372,6 +372,35 @@ def test_node_order_edge_positions(hip, orc, scheme):
     assert rel_l2(g, r) <= REL_L2
 
 
+@pytest.mark.parametrize("ppc", [64, 6, 300])
+@pytest.mark.parametrize("drift", [0.0, 0.3, 2.5])
+@pytest.mark.parametrize("tiling", [(16, 1, 4, 0), (16, 1, 8, 1), (32, 1, 4, 0), (48, 1, 4, 0), (5, 3, 4, 0),
+                                    (8, 8, 8, 0), (16, 8, 4, 1)])
+def test_tsc_moment_form_ring_and_overflow(hip, orc, tiling, drift, ppc):
+    """TSC on the node-keyed order without a drift halo (the moment form):
+    particles pushed up to `drift` cells after the sort leave their lanes'
+    reference nodes (the per-particle ring) or their tiles (the FP64 grid);
+    tiles as wide as the grid, ragged tiles, windows of more than 64 home
+    nodes, 4 and 8 particles per lane."""
+    say("tsc moments", tiling, drift, ppc)
+    Nx, Ny = 48, 40
+    dom = hip.make_domain(1.0, 1.0, Nx, Ny)
+    dt = 0.01
+    x, y, q, vx, vy = uniform_particles(Nx * Ny * ppc, seed=11 + ppc, vmax=max(drift, 1e-9) / Nx / dt)
+    with hip.Context(dom, len(x)) as c:
+        c.upload(x, y, q, vx, vy)
+        c.sort_nodes()
+        c.gather_push(dt, gather=False, push=True)
+        c.set_tiling(*tiling, scheme=2)
+        c.deposit(2, 2, dt)
+        g = c.rho().astype(np.float64)
+        p = c.download()
+    r = oracle_rho(orc, dom, p["x"], p["y"], p["q"], p["vx"], p["vy"], 2, dt)
+    err = rel_l2(g, r)
+    print(f"tsc moments {tiling} drift {drift} ppc {ppc} rel_l2 {err:.2e}", flush=True)
+    assert err <= REL_L2
+
+
 @pytest.mark.parametrize("ppc", [24, 6])
 @pytest.mark.parametrize("halo,tiling", [(2, (32, 4, 8, 0)), (3, (16, 16, 8, 1)), (4, (3, 3, 4, 0))])
 @pytest.mark.parametrize("scheme", [0, 2])
@@ -435,11 +464,14 @@ def test_default_tiling_follows_the_drift_halo(hip, orc):
         return (t.tile_x, t.tile_y, t.particles_per_lane)
 
     N = 1024
-    n = 8 * N * N  # 1024 particles per 128 x 1 tile
+    n = 8 * N * N  # 1024 particles per 128 x 1 tile, 128 per 16 x 1 tile
     dom = hip.make_domain(1.0, 1.0, N, N)
+    with hip.Context(dom, 64 * N * N) as c:
+        # TSC's long rows are 16 x 1 (1024 particles per tile at 64 per cell)
+        assert til(c, 2) == (16, 1, 4)
     with hip.Context(dom, n) as c:
         assert til(c, 1) == (128, 2, 4)
-        assert til(c, 2) == (128, 1, 8)
+        assert til(c, 2) == (32, 4, 8)
         c.set_tile_halo(2)
         assert til(c, 1) == (64, 4, 8)
         assert til(c, 2)[:2] == (32, 4)
