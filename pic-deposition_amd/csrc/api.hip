@@ -81,15 +81,18 @@ pic_status sync_tsc_cell(pic_hip_ctx* c) {
 // (kRowTiling) on grids that hold at least 4096 of them, without a halo (a
 // halo around a one- or two-row tile is mostly halo); otherwise the shape of
 // default_tile_shape, with 4 particles per lane below 1024 particles per tile.
-void default_tiling(const pic_hip_ctx* c, int k, int64_t capacity, int halo, int& tx, int& ty, int& ppl) {
+void default_tiling(const pic_hip_ctx* c, int k, int64_t capacity, int halo, int& tx, int& ty, int& ppl,
+                    int& pipe) {
     const int Nx = c->dom.Nx, Ny = c->dom.Ny;
     const int rx = kRowTiling[k][0], ry = kRowTiling[k][1];
     const int64_t row_tiles = (int64_t)((Nx + rx - 1) / rx) * ((Ny + ry - 1) / ry);
     const int64_t row_per_tile = capacity * rx * ry / std::max<int64_t>(c->G, 1);
+    pipe = 0;
     if (halo == 0 && rx <= Nx && row_tiles >= 4096 && row_per_tile >= 1024) {
         tx = rx;
         ty = ry;
         ppl = kRowTiling[k][2];
+        pipe = kRowTiling[k][3];
         return;
     }
     tx = kDefaultTiling[k][0];
@@ -233,9 +236,9 @@ pic_status pic_hip_create(const pic_domain* domain, size_t capacity, pic_hip_ctx
     // 0.215 -> 0.203, Esirkepov 0.389 -> 0.375).
     c->tile_capacity = (int64_t)capacity;
     for (int k = 0; k < 4; ++k) {
-        int tx, ty, ppl;
-        default_tiling(c.get(), k, c->tile_capacity, 0, tx, ty, ppl);
-        const pic_status st = apply_tiling(c.get(), k, tx, ty, ppl, 0);
+        int tx, ty, ppl, pipe;
+        default_tiling(c.get(), k, c->tile_capacity, 0, tx, ty, ppl, pipe);
+        const pic_status st = apply_tiling(c.get(), k, tx, ty, ppl, pipe);
         if (st != PIC_OK) return st;
         c->tiling_default[k] = true;
     }
@@ -302,9 +305,9 @@ pic_status pic_hip_set_tile_halo(pic_hip_ctx* ctx, int cells) {
     ctx->halo = cells;
     for (int k = 0; k < 4; ++k) {
         // a scheme still on its create-time tiling takes the default for this halo
-        int tx = saved[k][0], ty = saved[k][1], ppl = saved[k][2];
-        if (ctx->tiling_default[k]) default_tiling(ctx, k, ctx->tile_capacity, cells, tx, ty, ppl);
-        const pic_status st = apply_tiling(ctx, k, tx, ty, ppl, saved[k][3]);
+        int tx = saved[k][0], ty = saved[k][1], ppl = saved[k][2], pipe = saved[k][3];
+        if (ctx->tiling_default[k]) default_tiling(ctx, k, ctx->tile_capacity, cells, tx, ty, ppl, pipe);
+        const pic_status st = apply_tiling(ctx, k, tx, ty, ppl, pipe);
         if (st != PIC_OK) {
             const std::string msg = pic_hip_last_error();
             ctx->halo = old;

@@ -91,10 +91,11 @@ constexpr size_t kMaxDepositLds = 64 * 1024;
 // 1024^2 / 64 particles per cell: {tile_x, tile_y, particles per lane}.
 constexpr int kDefaultTiling[4][3] = {{64, 4, 8}, {64, 4, 8}, {32, 4, 8}, {32, 4, 4}};
 // Long-row tiling of a single grid with at least 4096 such tiles (the MI355X
-// sweep at 1024^2 / 64 ppc, interleaved repeats): {tile_x, tile_y, particles per lane}.
-// TSC takes short rows: its moment form (node-keyed order) is bound by VALU
-// issue and per-tile LDS, and 16 x 1 tiles keep the most waves resident.
-constexpr int kRowTiling[4][3] = {{128, 2, 4}, {128, 2, 4}, {16, 1, 4}, {128, 1, 4}};
+// sweep at 1024^2 / 64 ppc, interleaved repeats): {tile_x, tile_y, particles
+// per lane, pipelined}.  TSC takes short rows: its moment form (node-keyed
+// order) is bound by VALU issue and per-tile LDS, and 16 x 1 tiles keep the
+// most waves resident; with 48 VGPRs it double-buffers its loads.
+constexpr int kRowTiling[4][4] = {{128, 2, 4, 0}, {128, 2, 4, 0}, {16, 1, 4, 1}, {128, 1, 4, 0}};
 // Long rows of TSC on the cell-keyed order (16-slot stencil).
 constexpr int kRowTilingTscCell[3] = {128, 1, 8};
 

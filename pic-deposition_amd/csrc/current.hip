@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void current_tile_kernel(const float* __restri
                                                            int* __restrict__ status) {
     extern __shared__ double cur_lds[];
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar tile arithmetic
     const int tile = blockIdx.x * (blockDim.x >> 6) + wv;
     if (tile >= tg.ntiles) return;  // wave-uniform
     double* win = cur_lds + (size_t)wv * (4 * tg.wsize + (3 * kCurCells + 2) / 2);  // Jx, Jy, rho_old, rho_new

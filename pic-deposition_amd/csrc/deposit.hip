@@ -109,7 +109,8 @@ __global__ __launch_bounds__(256) void deposit_tile_kernel(const float* __restri
                                                            double* __restrict__ gacc, int* __restrict__ status) {
     extern __shared__ double smem_d[];
     float* smem = reinterpret_cast<float*>(smem_d);
-    const int wv = threadIdx.x >> 6;
+    // wave-uniform (scalar registers: the tile's index arithmetic runs on the SALU)
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tile = blockIdx.x * (blockDim.x >> 6) + wv;
     if (tile >= g.ntiles) return;  // wave-uniform
     deposit_one_tile<S, K, EX, PIPE, LA>(tile, smem + (size_t)wv * g.lds_per_wave, px, py, pvx, pvy, pq, cell_off, d,
@@ -145,7 +146,7 @@ void launch_atomic(const DepositArgs& a) {
 }  // namespace
 
 int moment_lds_per_wave(const TileGeom& g, int ppl) {
-    const int during = 2 * Stencil<TSC_MOM>::V * g.T * g.R + 4 * g.R + 2 + mom_ring_size(ppl);
+    const int during = 2 * Stencil<TSC_MOM>::V * g.T * g.R + 4 * g.R + 2 + 3 * mom_ring_cap(ppl);
     // The FP64 node window overlays the moments (and, once they are done
     // with, the row tables and the ring) for at most 64 home nodes, and
     // follows the ring otherwise.

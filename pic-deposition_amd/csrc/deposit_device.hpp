@@ -288,6 +288,15 @@ __device__ __forceinline__ void emit_run(const TileCtx& t, int k, const float (&
     }
 }
 
+// emit_run with the values negated (the negation folds into the FP64 conversion).
+template <int S, int V>
+__device__ __forceinline__ void emit_run_neg(const TileCtx& t, int k, const float (&z)[V]) {
+    float m[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) m[i] = -z[i];
+    emit_run<S, V>(t, k, m);
+}
+
 // Window key ((ly << 12) | lx) of a particle's home cell, -1 when the particle
 // is not this lane's (outside its row range) or not in the window.
 template <int S, int EX>
@@ -443,18 +452,16 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
 // key or run bookkeeping.  On node-sorted input a lane's particles share that
 // node except where a node's run ends inside the lane; such particles (and any
 // that drifted since the sort, or whose reference node is not one of the
-// tile's) are queued as particle indices in an LDS ring and deposited 64 at a
-// time, one per lane, by the general per-particle path.  The lanes then
+// tile's) are queued in an LDS ring and deposited 64 at a time, one per
+// lane, by the general per-particle path.  The lanes then
 // combine equal reference nodes with one segmented wave scan and one FP64 LDS
 // add per moment and run.  Window moments carry q without 1 / cell volume (the
 // tile's node values are scaled once when they are formed).
-// Lanes whose moment sums one segmented scan combines before the LDS adds
-// (2^4 = 16: one row of the wave).  Three steps time the same; two leave
-// enough adds to the same node in one instruction to be slower.
-constexpr int kMomScanSteps = 4;
-// Ring entries per wave for K particles per lane: at least the 63 + 64 K
-// entries that can be pending after a chunk.
-__host__ __device__ constexpr int mom_ring_size(int K) { return K <= 4 ? 512 : 1024; }
+// Ring capacity (particles) per wave for K particles per lane: the at most
+// 63 + 64 K entries pending after a chunk.  Entries hold the particle's x, y
+// and q (three float arrays of this length), so the per-particle path
+// re-reads nothing from global memory.
+__host__ __device__ constexpr int mom_ring_cap(int K) { return 64 + 64 * K; }
 
 // Offset in cells of coordinate x from node n.  With a power-of-two cell size
 // x * inv_d is exact in FP32 and so is the difference (the same value as the
@@ -468,10 +475,10 @@ __device__ __forceinline__ float node_offset(float x, float inv_d, double rinv, 
 
 // Fast path over one chunk.  Appends the indices of the particles it leaves to
 // the ring (`tail` counts appended entries, wave-uniform).
-template <int Q, int EX, int RING>
+template <int Q, int EX>
 __device__ __forceinline__ void mom_chunk(Chunk<Q>& c, const TileCtx& t, const DevDomain& d, double rdx, double rdy,
-                                          int* __restrict__ ring, int& tail) {
-    static_assert(63 + 64 * 4 * Q <= RING && (RING & (RING - 1)) == 0, "ring too small for one chunk's entries");
+                                          float* __restrict__ ring, int& tail) {
+    constexpr int CAP = mom_ring_cap(4 * Q);
     const int lane = threadIdx.x & 63;
     using St = Stencil<TSC_MOM>;
     constexpr int V = St::V;
@@ -563,9 +570,11 @@ __device__ __forceinline__ void mom_chunk(Chunk<Q>& c, const TileCtx& t, const D
             const uint64_t qm = __builtin_amdgcn_ballot_w64(untaken) & fast_lanes;
             if (qm) {
                 if (untaken && fast) {
-                    const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
-                    ring[(tail + r) & (RING - 1)] = c.p0[j] + e;
+                    const int r = tail + __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
+                    ring[r] = xx;
+                    ring[CAP + r] = yy;
+                    ring[2 * CAP + r] = qq;
                 }
                 tail += (int)__popcll(qm);
             }
@@ -582,38 +591,47 @@ __device__ __forceinline__ void mom_chunk(Chunk<Q>& c, const TileCtx& t, const D
             const uint64_t qm = __builtin_amdgcn_ballot_w64(queue);
             if (qm) {
                 if (queue) {
-                    const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
-                    ring[(tail + r) & (RING - 1)] = p;
+                    const int r = tail + __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
+                    ring[r] = e == 0 ? c.X[j].x : e == 1 ? c.X[j].y : e == 2 ? c.X[j].z : c.X[j].w;
+                    ring[CAP + r] = e == 0 ? c.Y[j].x : e == 1 ? c.Y[j].y : e == 2 ? c.Y[j].z : c.Y[j].w;
+                    ring[2 * CAP + r] = e == 0 ? c.QQ[j].x : e == 1 ? c.QQ[j].y : e == 2 ? c.QQ[j].z : c.QQ[j].w;
                 }
                 tail += (int)__popcll(qm);
             }
         }
     }
-    // Lanes of equal reference node (consecutive on sorted input) are summed
-    // by a segmented scan within each 16-lane row; the last lane of a run's
-    // piece in a row adds the moments to LDS (a few adds per node and chunk
-    // instead of two more scan steps over nine values).
-    const int kprev = dpp_i<kWaveShr1, 0xf>(-3, key);
-    const bool cont = fast && key == kprev;
-    wave_segmented_scan<V, kMomScanSteps>(s, cont ? 1.0f : 0.0f);
+    // Lanes of equal reference node are consecutive on sorted input.  Within
+    // each 16-lane row, P = the inclusive prefix of the lanes' sums (one
+    // fused DPP add per value and step); a run of lanes [s, e] of one node
+    // sums to P_e - P_{s-1}.  So the last lane of each run of equal keys adds
+    // +P_e to its node and, when another run follows in the row, -P_e to the
+    // next run's node (runs of lanes without a reference node add exact zeros
+    // to P and only pass -P on).
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        s[k] += dpp_z<kRowShr + 1, 0xf>(s[k]);
+        s[k] += dpp_z<kRowShr + 2, 0xf>(s[k]);
+        s[k] += dpp_z<kRowShr + 4, 0xf>(s[k]);
+        s[k] += dpp_z<kRowShr + 8, 0xf>(s[k]);
+    }
     const int knext = dpp_i<kWaveShl1, 0xf>(-3, key);
-    constexpr int kLast = (1 << kMomScanSteps) - 1;
-    if (fast && (knext != key || (lane & kLast) == kLast)) emit_run<TSC_MOM, V>(t, key, s);
+    const bool row_end = (lane & 15) == 15;
+    const bool run_end = row_end || knext != key;
+    if (run_end && fast) emit_run<TSC_MOM, V>(t, key, s);
+    if (run_end && !row_end && knext >= 0) emit_run_neg<TSC_MOM, V>(t, knext, s);
 }
 
-// Deposits ring entries [head, head + cnt), cnt <= 64, one per lane, with the
-// general path: nearest node, window test, moments into the window or node
-// weights to the FP64 accumulation grid.
-template <int EX, int RING>
-__device__ __forceinline__ void mom_ring_batch(const TileCtx& t, const int* __restrict__ ring, int head, int cnt,
-                                               const float* __restrict__ px, const float* __restrict__ py,
-                                               const float* __restrict__ pq, const DevDomain& d,
-                                               double* __restrict__ gacc, int* __restrict__ status) {
+// Deposits ring entries [first, first + cnt), cnt <= 64, one per lane, with
+// the general path: nearest node, window test, moments into the window or
+// node weights to the FP64 accumulation grid.
+template <int EX, int CAP>
+__device__ __forceinline__ void mom_ring_batch(const TileCtx& t, const float* __restrict__ ring, int first, int cnt,
+                                               const DevDomain& d, double* __restrict__ gacc,
+                                               int* __restrict__ status) {
     const int lane = threadIdx.x & 63;
     if (lane >= cnt) return;
-    const int p = ring[(head + lane) & (RING - 1)];
-    const float xx = px[p], yy = py[p], qq = pq[p];
+    const float xx = ring[first + lane], yy = ring[CAP + first + lane], qq = ring[2 * CAP + first + lane];
     int cx, cy;
     float fx, fy;
     node_cell<EX>(xx, yy, d, cx, cy, fx, fy);
@@ -629,6 +647,35 @@ __device__ __forceinline__ void mom_ring_batch(const TileCtx& t, const int* __re
         overflow_scatter<TSC_NODE>(true, cx, cy, w, xx, yy, 0.0f, 0.0f, qv, 0.0f, d, gacc, status);
         if (t.acc_mark) *t.acc_mark = t.epoch;  // the merge must fold the grid
     }
+}
+
+// Deposits the ring's full batches of 64 and moves the (< 64) entries left
+// to its front; returns their count.
+template <int EX, int CAP>
+__device__ __forceinline__ int mom_ring_drain(const TileCtx& t, float* __restrict__ ring, int tail,
+                                              const DevDomain& d, double* __restrict__ gacc,
+                                              int* __restrict__ status) {
+    const int lane = threadIdx.x & 63;
+    wave_sync();
+    int first = 0;
+    for (; tail - first >= 64; first += 64) mom_ring_batch<EX, CAP>(t, ring, first, 64, d, gacc, status);
+    const int left = tail - first;
+    if (first > 0 && left > 0) {
+        float a = 0.0f, b = 0.0f, q = 0.0f;
+        if (lane < left) {
+            a = ring[first + lane];
+            b = ring[CAP + first + lane];
+            q = ring[2 * CAP + first + lane];
+        }
+        wave_sync();
+        if (lane < left) {
+            ring[lane] = a;
+            ring[CAP + lane] = b;
+            ring[2 * CAP + lane] = q;
+        }
+    }
+    wave_sync();
+    return left;
 }
 
 // Deposits one tile with one wavefront: LDS window `lds` (g.lds_per_wave
@@ -699,21 +746,15 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
     const int step = 64 * Q;
 
     if constexpr (S == TSC_MOM) {
-        constexpr int RING = mom_ring_size(K);
-        int* ring = t_qcum + g.R + 2;  // 8-byte aligned: the node window follows it
-        int head = 0, tail = 0;
+        constexpr int CAP = mom_ring_cap(K);
+        float* ring = reinterpret_cast<float*>(t_qcum + g.R + 2);  // x, y, q arrays of CAP; 8-byte aligned
+        int tail = 0;
         const double rdx = EX == 1 ? 0.0 : 1.0 / d.dx, rdy = EX == 1 ? 0.0 : 1.0 / d.dy;
         // Full batches of ring entries after each chunk, the rest at the end.
         auto chunk = [&](Chunk<Q>& A) {
-            mom_chunk<Q, EX, RING>(A, t, d, rdx, rdy, ring, tail);
+            mom_chunk<Q, EX>(A, t, d, rdx, rdy, ring, tail);
             tail = __builtin_amdgcn_readfirstlane(tail);
-            if (tail - head >= 64) {
-                wave_sync();
-                do {
-                    mom_ring_batch<EX, RING>(t, ring, head, 64, px, py, pq, d, gacc, status);
-                    head += 64;
-                } while (tail - head >= 64);
-            }
+            if (tail >= 64) tail = mom_ring_drain<EX, CAP>(t, ring, tail, d, gacc, status);
         };
         if (PIPE) {
             Chunk<Q> A, B;
@@ -742,9 +783,9 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
                 chunk(A);
             }
         }
-        if (tail > head) {
+        if (tail > 0) {
             wave_sync();
-            mom_ring_batch<EX, RING>(t, ring, head, tail - head, px, py, pq, d, gacc, status);
+            mom_ring_batch<EX, CAP>(t, ring, 0, tail, d, gacc, status);
         }
     } else if (PIPE) {
         // Two register sets: the loads of one chunk are in flight while the
@@ -788,7 +829,7 @@ __device__ __forceinline__ void deposit_one_tile(int tile, float* __restrict__ l
         // With at most 64 home nodes every lane holds its node's moments
         // before the window is cleared, so the node window overlays them.
         const bool overlay = g.T * g.R <= 64;
-        double* nw = overlay ? t.win : reinterpret_cast<double*>(t_qcum + g.R + 2 + mom_ring_size(K));
+        double* nw = overlay ? t.win : reinterpret_cast<double*>(t_qcum + g.R + 2 + 3 * mom_ring_cap(K));
         double mm[9];
         if (overlay && lane < g.T * g.R) {
 #pragma unroll

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void ens_deposit_kernel(EnsParticles p, const 
                                                           int lds_per_wave) {
     extern __shared__ double smem_d[];
     const int lane = threadIdx.x & 63;
-    float* lds = reinterpret_cast<float*>(smem_d) + (size_t)(threadIdx.x >> 6) * lds_per_wave;
+    float* lds = reinterpret_cast<float*>(smem_d) + (size_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * lds_per_wave;
     for (;;) {
         int it = 0;
         if (lane == 0) it = atomicAdd(counter, 1);
