@@ -106,6 +106,11 @@ struct pic_hip_ctx {
     // post-deposit hook and not yet launched: the step's Poisson solve takes
     // it into its first pass (net_out null: none pending).
     pichip::LearnedApply pending_apply{};
+    // Learned step loop: the tiled deposit leaves its window merge to the
+    // denoiser's first layer (defer_merge), which takes pending_merge.
+    bool defer_merge = false;
+    bool merge_pending = false;
+    pichip::TileMerge pending_merge{};
     std::vector<hipEvent_t> prof_events;  // pairs (begin, end)
     int prof_max = 0, prof_used = 0;
     bool profiling = false;

@@ -26,6 +26,7 @@
 //    re-adds those sums in a fixed order in every block, decides the gate and
 //    renormalises: five launches from deposited rho to learned rho.
 
+#include "deposit_device.hpp"
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
 
@@ -84,12 +85,22 @@ struct OutStats {
     double inv_scale;
 };
 
+// Optional clear of the deposit's FP64 accumulation grid by a hidden layer
+// (after a fused merge read it): node p is zeroed with the output pixel p when
+// the deposit marked the grid (as the merge would have after folding it).
+struct AccClear {
+    double* acc;  // null: nothing to clear
+    const int* mark;
+    int epoch;
+};
+
 template <int CIN, int NT, int MODE, int MT>
 __global__ __launch_bounds__(256) void conv3x3_kernel(const void* __restrict__ in_, const bf16x8* __restrict__ wpk,
                                                       const float* __restrict__ bias, void* __restrict__ out_,
                                                       const __bf16* __restrict__ mask, CnnGeom g, int dil, InNorm nrm,
-                                                      OutStats ost) {
+                                                      OutStats ost, AccClear clr) {
     const int lane = threadIdx.x & 63;
+    const bool clear = MODE == kHidden && clr.acc != nullptr && (clr.mark == nullptr || *clr.mark == clr.epoch);
     // Single-channel input transform x * mul + add (identity without nrm.sum).
     float mul = 1.0f, add = 0.0f;
     const bool norm_in = CIN == 1 && nrm.sum != nullptr;
@@ -169,6 +180,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const void* __restrict__ i
                     const float v = acc[mt][nt][i];
                     if (MODE == kHidden) {
                         static_cast<__bf16*>(out_)[p * 32 + c] = (__bf16)fmaxf(v + bias[c], 0.0f);
+                        if (clear && c == 0) clr.acc[p] = 0.0;
                     } else if (MODE == kOutput) {
                         if (c == 0) {
                             const float o = v + bias[0];
@@ -195,6 +207,63 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const void* __restrict__ i
         if (threadIdx.x == 0) {
             ost.partial[2 * blockIdx.x] = su;
             ost.partial[2 * blockIdx.x + 1] = sr;
+        }
+    }
+}
+
+// First layer (one input channel, dilation 1) fused with the deposit's window
+// merge, for the learned step loop.  Workgroup y merges rows y-1, y, y+1 of
+// rho into LDS node by node exactly as deposit_merge_kernel adds them (the
+// FP64 accumulation grid folded in but not cleared: the neighbouring
+// workgroups read the same rows; the second layer clears it), one node per
+// thread, writes row y of rho and computes the layer's output for row y from
+// LDS.
+constexpr int kMergeConvThreads = 768;  // one node of rows y-1, y, y+1 per thread at W = 256
+
+template <int B>
+__global__ __launch_bounds__(kMergeConvThreads) void merge_conv1_kernel(const float* __restrict__ partial,
+                                                                        const double* __restrict__ gacc, DevDomain d,
+                                                                        TileGeom tg, float* __restrict__ rho,
+                                                                        const bf16x8* __restrict__ wpk,
+                                                                        const float* __restrict__ bias,
+                                                                        __bf16* __restrict__ out, InNorm nrm) {
+    extern __shared__ float rows[];  // [3][W]: rows y-1, y, y+1
+    const int W = d.Nx, H = d.Ny;
+    const int y = blockIdx.x;
+    const bool fold = tg.acc_mark == nullptr || *tg.acc_mark == tg.epoch;
+    for (int i = threadIdx.x; i < 3 * W; i += kMergeConvThreads) {
+        const int r = i / W, x = i - r * W;
+        const int gy = wrapn(y - 1 + r, H);
+        double s = merge_windows<B>(x, gy, partial, d, tg);
+        if (fold) {
+            const double o = gacc[(size_t)gy * W + x];
+            if (o != 0.0) s += o;
+        }
+        rows[i] = (float)s;
+        if (r == 1) rho[(size_t)y * W + x] = (float)s;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, grp = lane >> 4, col = lane & 15;
+    const double mean = *nrm.sum * nrm.sum_scale * nrm.inv_count;
+    const float mul = (float)(nrm.scale / mean), add = (float)(-nrm.scale);
+    const bf16x8 bw0 = wpk[lane], bw1 = wpk[64 + lane];
+    for (int tile = threadIdx.x >> 6; tile < W / 16; tile += kMergeConvThreads / 64) {
+        const int x = tile * 16 + col;  // A row: this lane's pixel
+        bf16x8 a;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = grp * 8 + j;  // tap
+            float v = 0.0f;
+            if (k < 9) v = fmaf(rows[(k / 3) * W + wrapn(x + k % 3 - 1, W)], mul, add);
+            a[j] = (__bf16)v;
+        }
+        const f32x4 z{0.0f, 0.0f, 0.0f, 0.0f};
+        const f32x4 acc0 = mfma(a, bw0, z), acc1 = mfma(a, bw1, z);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const size_t p = (size_t)y * W + tile * 16 + grp * 4 + i;
+            out[p * 32 + col] = (__bf16)fmaxf(acc0[i] + bias[col], 0.0f);
+            out[p * 32 + 16 + col] = (__bf16)fmaxf(acc1[i] + bias[16 + col], 0.0f);
         }
     }
 }
@@ -493,7 +562,8 @@ __global__ __launch_bounds__(256) void denoise_apply_kernel(const float* __restr
 // Launches one layer; returns the number of workgroups.
 template <int CIN, int NT, int MODE>
 int conv_launch(const void* in, const void* wpk, const float* bias, void* out, const void* mask, const CnnGeom& g,
-                int dil, int ncu, hipStream_t s, const InNorm& nrm = InNorm{}, const OutStats& ost = OutStats{}) {
+                int dil, int ncu, hipStream_t s, const InNorm& nrm = InNorm{}, const OutStats& ost = OutStats{},
+                const AccClear& clr = AccClear{}) {
     const int64_t rowtiles16 = (int64_t)g.B * g.H * (g.W / 16);
     // Four 16-pixel tiles per wave when there is enough work.  Otherwise the
     // 32 -> 32 layers take two (every wave loads the layer's 18 KB of packed
@@ -508,13 +578,13 @@ int conv_launch(const void* in, const void* wpk, const float* bias, void* out, c
     const auto* m = static_cast<const __bf16*>(mask);
     if (mt == 4)
         hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 4>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
-                           ost);
+                           ost, clr);
     else if (mt == 2)
         hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 2>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
-                           ost);
+                           ost, clr);
     else
         hipLaunchKernelGGL((conv3x3_kernel<CIN, NT, MODE, 1>), grid, dim3(256), 0, s, in, w, bias, out, m, g, dil, nrm,
-                           ost);
+                           ost, clr);
     return (int)grid.x;
 }
 
@@ -603,14 +673,38 @@ void launch_moments(const float* v, int64_t n, double* partial, int nb, hipStrea
     hipLaunchKernelGGL(charge_moments_kernel, dim3(nb), dim3(256), 0, s, v, n, partial);
 }
 
+bool denoise_can_merge(const CnnGeom& g, const int (&dil)[4]) {
+    return g.B == 1 && dil[0] == 1 && g.W % 16 == 0 && g.W <= 2048;
+}
+
 void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
-                    hipStream_t s, LearnedApply* defer) {
+                    hipStream_t s, LearnedApply* defer, const TileMerge* merge) {
     const __bf16* pk = reinterpret_cast<const __bf16*>(c.packed);
     const double inv_g = 1.0 / (double)a.G;
     const InNorm nrm{a.sum, a.sum_scale, inv_g, a.in_scale};
     const OutStats ost{a.stats, a.rho, a.sum, a.sum_scale, inv_g, a.inv_out_scale};
-    conv_launch<1, 2, kHidden>(a.rho, pk + 8 * 64 * 0, c.w + cnn::kB1, c.a1, nullptr, g, dil[0], ncu, s, nrm);
-    conv_launch<32, 2, kHidden>(c.a1, pk + 8 * cnn::kPackF2, c.w + cnn::kB2, c.a2, nullptr, g, dil[1], ncu, s);
+    AccClear clr{};
+    if (merge) {
+        const auto* w1 = reinterpret_cast<const bf16x8*>(pk);
+        const dim3 grid((unsigned)g.H);
+        const size_t lds = (size_t)3 * g.W * sizeof(float);
+        const TileMerge& m = *merge;
+        __bf16* a1 = static_cast<__bf16*>(c.a1);
+        if (m.geom.mbatch <= 4)
+            hipLaunchKernelGGL(merge_conv1_kernel<4>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom, m.geom, a.rho,
+                               w1, c.w + cnn::kB1, a1, nrm);
+        else if (m.geom.mbatch <= 8)
+            hipLaunchKernelGGL(merge_conv1_kernel<8>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom, m.geom, a.rho,
+                               w1, c.w + cnn::kB1, a1, nrm);
+        else
+            hipLaunchKernelGGL(merge_conv1_kernel<16>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom, m.geom, a.rho,
+                               w1, c.w + cnn::kB1, a1, nrm);
+        clr = AccClear{m.acc, m.geom.acc_mark, m.geom.epoch};
+    } else {
+        conv_launch<1, 2, kHidden>(a.rho, pk + 8 * 64 * 0, c.w + cnn::kB1, c.a1, nullptr, g, dil[0], ncu, s, nrm);
+    }
+    conv_launch<32, 2, kHidden>(c.a1, pk + 8 * cnn::kPackF2, c.w + cnn::kB2, c.a2, nullptr, g, dil[1], ncu, s,
+                                InNorm{}, OutStats{}, clr);
     conv_launch<32, 2, kHidden>(c.a2, pk + 8 * cnn::kPackF3, c.w + cnn::kB3, c.a3, nullptr, g, dil[2], ncu, s);
     const int nbs =
         conv_launch<32, 1, kOutput>(c.a3, pk + 8 * cnn::kPackF4, c.w + cnn::kB4, c.out, nullptr, g, dil[3], ncu, s,

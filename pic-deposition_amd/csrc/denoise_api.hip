@@ -154,8 +154,13 @@ void enqueue_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double
     a.fallback_count = fallback_count;
     // Inside a step loop whose solve runs in LDS, the gate and the write of the
     // learned density are left to the solve's first pass (ctx->pending_apply).
+    const TileMerge* merge = nullptr;
+    if (in_run && ctx->merge_pending) {
+        merge = &ctx->pending_merge;
+        ctx->merge_pending = false;
+    }
     launch_denoise(net->c, geom(net, 1), net->dil, net->ncu, a, ctx->stream,
-                   in_run && ctx->lds_fft ? &ctx->pending_apply : nullptr);
+                   in_run && ctx->lds_fft ? &ctx->pending_apply : nullptr, merge);
 }
 
 double kappa_of(int scheme) {
@@ -442,8 +447,14 @@ pic_status pic_hip_run_learned(pic_hip_ctx* ctx, pic_hip_denoiser* net, const pi
         PIC_HIP_TRY(hipGetLastError());
         return PIC_OK;
     };
+    // The first layer merges the deposit's tile windows (one launch fewer per
+    // step) when its dilation is 1.
+    ctx->defer_merge = denoise_can_merge(geom(net, 1), net->dil);
+    ctx->merge_pending = false;
     pic_status st = run_steps(ctx, cfg, nsteps, record_from, field_energy, mode1_amplitude, kinetic_energy,
                               charge_error, hook);
+    ctx->defer_merge = false;
+    ctx->merge_pending = false;
     if (st != PIC_OK) return st;
     if (fallbacks) {
         PIC_HIP_TRY(hipMemcpyAsync(fallbacks, net->fallbacks, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

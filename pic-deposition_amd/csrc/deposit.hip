@@ -170,6 +170,7 @@ void launch_tile(const DepositArgs& a) {
     hipLaunchKernelGGL((deposit_tile_kernel<S, K, EX, PIPE, LA>), dim3(blocks), dim3(64 * waves_per_block), lds, a.stream,
                        a.x, a.y, a.vx, a.vy, a.q, a.cell_off, a.dom, g, a.dt, a.partial, a.acc, a.status);
     if (a.ev_end) (void)hipEventRecord(a.ev_end, a.stream);
+    if (a.skip_merge) return;
     const dim3 mgrid((unsigned)((a.dom.Nx + 255) / 256), (unsigned)a.dom.Ny);
     if (g.mbatch <= 4)
         hipLaunchKernelGGL(deposit_merge_kernel<4>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);

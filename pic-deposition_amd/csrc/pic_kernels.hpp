@@ -66,6 +66,16 @@ struct DepositArgs {
     hipStream_t stream;
     hipEvent_t ev_begin = nullptr;  // if set, recorded around the main scatter kernel
     hipEvent_t ev_end = nullptr;
+    bool skip_merge = false;  // tile backend: leave the window merge to the caller (TileMerge)
+};
+
+// What a deferred window merge needs: the tile windows, the FP64 accumulation
+// grid, the deposit's domain and tiling (with its overflow marker and epoch).
+struct TileMerge {
+    const float* partial;
+    double* acc;
+    DevDomain dom;
+    TileGeom geom;
 };
 
 hipError_t launch_deposit(const DepositArgs& a);
@@ -416,8 +426,13 @@ struct DenoiseArgs {
 };
 // With `defer`, the final gate / renormalisation launch is not made: its
 // arguments are written to *defer for launch_poisson_lds (bg left 0).
+// With `merge`, rho is not yet written: the first layer merges the deposit's
+// tile windows itself (writing rho) and the second clears the FP64
+// accumulation grid the merge folded (needs dil[0] == 1, see
+// denoise_can_merge).
 void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
-                    hipStream_t s, LearnedApply* defer = nullptr);
+                    hipStream_t s, LearnedApply* defer = nullptr, const TileMerge* merge = nullptr);
+bool denoise_can_merge(const CnnGeom& g, const int (&dil)[4]);
 size_t denoise_stats_doubles(int ncu);
 
 }  // namespace pichip

@@ -122,6 +122,23 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(float* __restrict__ rh
     const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
     const bool pair = r1 < H;  // a grid of one row transforms it alone
     const double mean = (*rho_sum * sum_scale + sum_add + bg * (double)G) * cv / (double)G;
+    // With a deferred learned density, the rows' plain and learned values of
+    // up to kPre nodes per lane are requested before the gate's sums, so their
+    // loads overlap its reduction.
+    constexpr int kPre = 4;
+    float pre_r[2][kPre], pre_n[2][kPre];
+    const bool prefetch = la.net_out && W <= 64 * kPre;
+    if (prefetch) {
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int n = lane + 64 * j;
+            const bool in = n < W;
+            pre_r[0][j] = in ? rho[(size_t)r0 * W + n] : 0.0f;
+            pre_n[0][j] = in ? la.net_out[(size_t)r0 * W + n] : 0.0f;
+            pre_r[1][j] = in && pair ? rho[(size_t)r1 * W + n] : 0.0f;
+            pre_n[1][j] = in && pair ? la.net_out[(size_t)r1 * W + n] : 0.0f;
+        }
+    }
     LearnedGate gt{};
     if (la.net_out) {
         gt = learned_gate_wave(la, G);
@@ -133,12 +150,13 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(float* __restrict__ rh
             if (!gt.ok && la.fallback_count) *la.fallback_count += 1;
         }
     }
-    // rho of node i, replaced first by the learned density when one is deferred
-    auto node = [&](size_t i) -> float {
-        float v = rho[i];
+    // rho of node i (plain value pr, learned output nv), replaced first by the
+    // learned density when one is deferred
+    auto node_v = [&](size_t i, float pr, float nv) -> float {
+        float v = pr;
         if (la.net_out) {
             if (gt.ok) {
-                const double lv = gt.mean * (1.0 + (double)la.net_out[i] * la.inv_scale) * gt.scale;
+                const double lv = gt.mean * (1.0 + (double)nv * la.inv_scale) * gt.scale;
                 la.out64[i] = lv;
                 v = (float)lv;
             } else {
@@ -149,11 +167,28 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(float* __restrict__ rh
         }
         return v;
     };
-    for (int n = lane; n < W; n += 64) {
-        const float u = (float)(((double)node((size_t)r0 * W + n) + bg) * cv - mean);
-        const float v = pair ? (float)(((double)node((size_t)r1 * W + n) + bg) * cv - mean) : 0.0f;
-        a[n] = make_float2(u, v);
-        t[n] = tw[n];
+    auto node = [&](size_t i) -> float { return node_v(i, rho[i], la.net_out ? la.net_out[i] : 0.0f); };
+    if (prefetch) {
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int n = lane + 64 * j;
+            if (n < W) {
+                const float u =
+                    (float)(((double)node_v((size_t)r0 * W + n, pre_r[0][j], pre_n[0][j]) + bg) * cv - mean);
+                const float v =
+                    pair ? (float)(((double)node_v((size_t)r1 * W + n, pre_r[1][j], pre_n[1][j]) + bg) * cv - mean)
+                         : 0.0f;
+                a[n] = make_float2(u, v);
+                t[n] = tw[n];
+            }
+        }
+    } else {
+        for (int n = lane; n < W; n += 64) {
+            const float u = (float)(((double)node((size_t)r0 * W + n) + bg) * cv - mean);
+            const float v = pair ? (float)(((double)node((size_t)r1 * W + n) + bg) * cv - mean) : 0.0f;
+            a[n] = make_float2(u, v);
+            t[n] = tw[n];
+        }
     }
     __syncthreads();
     const float2* Z = wave_fft<-1, false>(a, b, W, t);

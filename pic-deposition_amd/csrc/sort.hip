@@ -358,6 +358,7 @@ __global__ __launch_bounds__(kRT, 2) void rscatter_kernel(const int* __restrict_
 
 // ---- exclusive scan of uint32 (3 kernels) ---------------------------------
 
+template <int NW = kWaves>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     uint32_t incl = v;
@@ -370,7 +371,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     __syncthreads();
     uint32_t wpre = 0, tot = 0;
 #pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
+    for (int ww = 0; ww < NW; ++ww) {
         const uint32_t c = s_wave[ww];
         if (ww < w) wpre += c;
         tot += c;
@@ -430,6 +431,10 @@ __global__ __launch_bounds__(256) void scan_top_kernel(uint32_t* __restrict__ su
     }
 }
 
+// LOCAL: `sums` holds the blocks' totals (at most kThreads of them), and each
+// block adds those before it itself; otherwise `sums` holds their exclusive
+// prefix (scan_top_kernel).
+template <bool LOCAL>
 __global__ __launch_bounds__(256) void scan_down_kernel(const uint32_t* __restrict__ in, size_t m,
                                                         const uint32_t* __restrict__ sums,
                                                         uint32_t* __restrict__ out, const int* __restrict__ mode) {
@@ -438,11 +443,20 @@ __global__ __launch_bounds__(256) void scan_down_kernel(const uint32_t* __restri
     const size_t base = (size_t)blockIdx.x * kBlockItems + (size_t)threadIdx.x * kItems;
     uint32_t v[kItems];
     load16(in, base, m, v);
+    uint32_t carry;
+    if (LOCAL) {
+        const uint32_t c = threadIdx.x < blockIdx.x ? sums[threadIdx.x] : 0u;
+        uint32_t before;
+        (void)block_exclusive_scan(c, s_wave, before);
+        carry = before;
+    } else {
+        carry = sums[blockIdx.x];
+    }
     uint32_t s = 0;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) s += v[j];
     uint32_t total;
-    uint32_t run = block_exclusive_scan(s, s_wave, total) + sums[blockIdx.x];
+    uint32_t run = block_exclusive_scan(s, s_wave, total) + carry;
 #pragma unroll
     for (int q4 = 0; q4 < kItems / 4; ++q4) {
         const size_t b = base + 4 * q4;
@@ -465,8 +479,12 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, size_t m, uint32_t* tmp, 
     const size_t nb = (m + kBlockItems - 1) / kBlockItems;
     if (nb == 0) return;
     hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, mode);
+    if (nb <= kThreads) {  // every down-sweep block adds the totals before it itself: no top-level launch
+        hipLaunchKernelGGL(scan_down_kernel<true>, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, out, mode);
+        return;
+    }
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kThreads), 0, s, tmp, nb, mode);
-    hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, out, mode);
+    hipLaunchKernelGGL(scan_down_kernel<false>, dim3((unsigned)nb), dim3(kThreads), 0, s, in, m, tmp, out, mode);
 }
 
 // off[c] = first sorted index with key >= c, off[G] = n.  Four keys per thread.

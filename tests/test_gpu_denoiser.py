@@ -258,6 +258,36 @@ def test_learned_run_matches_denoise(hip, trained, background):
         assert abs(learned.sum() / (N * N) - q) <= 1e-13 * abs(q)
 
 
+@pytest.mark.parametrize("halo", [0, 1])
+def test_learned_run_fallback_equals_plain_run(hip, trained, halo):
+    """A learned run whose gate always falls back (tolerance 0) keeps the plain
+    deposit every step, so it must reproduce the plain step loop bit for bit:
+    the first layer's fused window merge (with particles drifting past the
+    tiles' halo into the FP64 overflow grid, folded there and cleared by the
+    second layer) writes the same rho as the deposit's own merge."""
+    say("learned fallback vs plain run, halo", halo)
+    N = 64
+    n = N * N * 8
+    dom = hip.make_domain(1.0, 1.0, N, N)
+    steps = 12
+    with hip.Context(dom, n) as a, hip.Context(dom, n) as b:
+        for c in (a, b):
+            c.set_tile_halo(halo)
+            # thermal speed ~1.5 cells per step: many particles leave their tile's reach between sorts
+            c.init_particles(n, kind=1, seed=43, amp=0.02, mode=2, vth=1.5 / N / 0.1,
+                             q=hip.unit_plasma_charge(n, N, N), m=1.0)
+        rec_a, fallbacks = trained.run(a, steps, 2, 2, dt=0.1, sort_interval=5, tolerance=0.0)
+        assert fallbacks == steps
+        rec_b = b.run(steps, 2, 2, 0.1, 0.0, hip.ORDER_PUSH_FIRST, sort_interval=5, use_graph=True)
+        assert np.array_equal(a.rho(), b.rho())
+        pa, pb = a.download(), b.download()
+        for k in ("x", "y", "vx", "vy"):
+            assert np.array_equal(pa[k], pb[k]), k
+        # same fields; the learned run also records particle sums, which
+        # changes the diagnostics' reduction layout
+        np.testing.assert_allclose(rec_a["field_energy"], rec_b["field_energy"], rtol=1e-12, atol=0.0)
+
+
 def test_invalid_denoiser_arguments(hip):
     say("denoiser errors")
     with pytest.raises(hip.PicError, match="multiple of 32"):
