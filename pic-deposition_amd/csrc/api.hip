@@ -692,9 +692,11 @@ pic_status enqueue_solve(pic_hip_ctx* ctx, const double* sum, double sum_scale, 
         launch_poisson_lds(ctx->rho, ctx->G, 0.0, cv, sum, sum_scale, sum_add, ctx->spec, ctx->spec_ex, ctx->spec_ey,
                            ctx->ptab, ctx->dev, ctx->tw_x, ctx->tw_y, ctx->stream, la);
         if (ctx->dom.Ny > 1) {
-            launch_poisson_lds_rows(ctx->spec_ex, ctx->spec_ey, ctx->Ex, ctx->Ey, ctx->dev, ctx->tw_x, ctx->stream);
+            launch_poisson_lds_rows(ctx->spec_ex, ctx->spec_ey, ctx->Ex, ctx->Ey, ctx->dev, ctx->tw_x, ctx->stream,
+                                    ctx->rho, la);
         } else {  // one row: Ey is exactly zero (phi does not vary along y)
-            launch_poisson_lds_rows(ctx->spec_ex, nullptr, ctx->Ex, nullptr, ctx->dev, ctx->tw_x, ctx->stream);
+            launch_poisson_lds_rows(ctx->spec_ex, nullptr, ctx->Ex, nullptr, ctx->dev, ctx->tw_x, ctx->stream, ctx->rho,
+                                    la);
             fill_async(ctx->Ey, (size_t)ctx->G, 0.0f, ctx->stream);
         }
         if (with_phi) launch_poisson_lds_rows(ctx->spec, nullptr, ctx->phi, nullptr, ctx->dev, ctx->tw_x, ctx->stream);

@@ -444,8 +444,10 @@ void launch_esirkepov_current_tiled(const float* x, const float* y, const float*
         hipLaunchKernelGGL(current_merge_kernel<4>, mgrid, dim3(256), 0, s, partial, d, tg, g);
     else if (tg.mbatch <= 8)
         hipLaunchKernelGGL(current_merge_kernel<8>, mgrid, dim3(256), 0, s, partial, d, tg, g);
-    else
+    else if (tg.mbatch <= 16)
         hipLaunchKernelGGL(current_merge_kernel<16>, mgrid, dim3(256), 0, s, partial, d, tg, g);
+    else
+        hipLaunchKernelGGL(current_merge_kernel<32>, mgrid, dim3(256), 0, s, partial, d, tg, g);
 }
 
 void launch_continuity_residual(const CurrentGrids& g, const DevDomain& d, double dt, double* partial, int nblocks,

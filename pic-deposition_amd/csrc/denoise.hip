@@ -8,9 +8,7 @@
 // tap t = 3 * ty + tx reads pixel (y + (ty - 1) * dil, x + (tx - 1) * dil),
 // wrapped periodically.
 //
-// MFMA 16x16x32 operand layout (lane l, g = l / 16, c = l % 16): A holds row c,
-// k = 8g .. 8g + 7; B holds column c, k = 8g .. 8g + 7; the accumulator holds
-// column c, rows 4g .. 4g + 3.
+// (MFMA operand layout: cnn_device.hpp.)
 //
 //  * conv3x3_kernel: implicit GEMM with M = 16 pixels of one row per tile,
 //    N = output channels, K = (tap, input channel) = 32 per MFMA.  For a
@@ -26,6 +24,7 @@
 //    re-adds those sums in a fixed order in every block, decides the gate and
 //    renormalises: five launches from deposited rho to learned rho.
 
+#include "cnn_device.hpp"
 #include "deposit_device.hpp"
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
@@ -34,15 +33,6 @@ namespace pichip {
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-// i + o wrapped into [0, n) for |o| < n.
-__device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 __device__ __forceinline__ int64_t pix(const CnnGeom& g, int b, int y, int x) {
     return ((int64_t)b * g.H + y) * g.W + x;
@@ -234,11 +224,10 @@ __global__ __launch_bounds__(kMergeConvThreads) void merge_conv1_kernel(const fl
     for (int i = threadIdx.x; i < 3 * W; i += kMergeConvThreads) {
         const int r = i / W, x = i - r * W;
         const int gy = wrapn(y - 1 + r, H);
+        // the overflow grid's value is requested with the windows' (used when marked)
+        const double o = gacc[(size_t)gy * W + x];
         double s = merge_windows<B>(x, gy, partial, d, tg);
-        if (fold) {
-            const double o = gacc[(size_t)gy * W + x];
-            if (o != 0.0) s += o;
-        }
+        if (fold && o != 0.0) s += o;
         rows[i] = (float)s;
         if (r == 1) rho[(size_t)y * W + x] = (float)s;
     }
@@ -678,7 +667,7 @@ bool denoise_can_merge(const CnnGeom& g, const int (&dil)[4]) {
 }
 
 void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
-                    hipStream_t s, LearnedApply* defer, const TileMerge* merge) {
+                    hipStream_t s, LearnedApply* defer, const TileMerge* merge, float2* spec_l) {
     const __bf16* pk = reinterpret_cast<const __bf16*>(c.packed);
     const double inv_g = 1.0 / (double)a.G;
     const InNorm nrm{a.sum, a.sum_scale, inv_g, a.in_scale};
@@ -694,11 +683,14 @@ void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], 
             hipLaunchKernelGGL(merge_conv1_kernel<4>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom, m.geom, a.rho,
                                w1, c.w + cnn::kB1, a1, nrm);
         else if (m.geom.mbatch <= 8)
-            hipLaunchKernelGGL(merge_conv1_kernel<8>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom, m.geom, a.rho,
-                               w1, c.w + cnn::kB1, a1, nrm);
+            hipLaunchKernelGGL(merge_conv1_kernel<8>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom,
+                               m.geom, a.rho, w1, c.w + cnn::kB1, a1, nrm);
+        else if (m.geom.mbatch <= 16)
+            hipLaunchKernelGGL(merge_conv1_kernel<16>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom,
+                               m.geom, a.rho, w1, c.w + cnn::kB1, a1, nrm);
         else
-            hipLaunchKernelGGL(merge_conv1_kernel<16>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom, m.geom, a.rho,
-                               w1, c.w + cnn::kB1, a1, nrm);
+            hipLaunchKernelGGL(merge_conv1_kernel<32>, grid, dim3(kMergeConvThreads), lds, s, m.partial, m.acc, m.dom,
+                               m.geom, a.rho, w1, c.w + cnn::kB1, a1, nrm);
         clr = AccClear{m.acc, m.geom.acc_mark, m.geom.epoch};
     } else {
         conv_launch<1, 2, kHidden>(a.rho, pk + 8 * 64 * 0, c.w + cnn::kB1, c.a1, nullptr, g, dil[0], ncu, s, nrm);
@@ -706,6 +698,17 @@ void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], 
     conv_launch<32, 2, kHidden>(c.a1, pk + 8 * cnn::kPackF2, c.w + cnn::kB2, c.a2, nullptr, g, dil[1], ncu, s,
                                 InNorm{}, OutStats{}, clr);
     conv_launch<32, 2, kHidden>(c.a2, pk + 8 * cnn::kPackF3, c.w + cnn::kB3, c.a3, nullptr, g, dil[2], ncu, s);
+    if (defer && spec_l) {  // the output layer runs in the solve's row pass
+        LearnedApply la{c.out, a.out64, a.sum, a.sum_scale, a.inv_out_scale, a.stats, (g.H + 1) / 2, a.moments,
+                        a.nbm, a.cv, a.kappa, a.tol, a.result, a.fallback_count, 0.0f};
+        la.a3 = c.a3;
+        la.w4 = pk + 8 * cnn::kPackF4;
+        la.bias4 = c.w + cnn::kB4;
+        la.dil4 = dil[3];
+        la.spec_l = spec_l;
+        *defer = la;
+        return;
+    }
     const int nbs =
         conv_launch<32, 1, kOutput>(c.a3, pk + 8 * cnn::kPackF4, c.w + cnn::kB4, c.out, nullptr, g, dil[3], ncu, s,
                                     InNorm{}, ost);

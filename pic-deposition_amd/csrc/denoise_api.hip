@@ -41,6 +41,7 @@ struct pic_hip_denoiser {
     double* scratch = nullptr;
     double* result = nullptr;     // [4]
     double* out64 = nullptr;      // [H][W]
+    float2* spec_l = nullptr;     // [H][W/2+1] learned half spectrum of a step loop's solve
     double* h_result = nullptr;   // pinned [4]
     int* fallbacks = nullptr;     // gate fallbacks counted during pic_hip_run_learned
     // Work on the denoiser's buffers is enqueued on its own stream (training,
@@ -55,7 +56,7 @@ struct pic_hip_denoiser {
         void* bufs[] = {c.w,       c.grad,      c.m,         c.v,       c.packed, c.x0,     c.out,     c.target,
                         c.dout,    c.a1,        c.a2,        c.a3,      c.d2,     c.d3,     c.wpart,   c.bias_part,
                         c.loss_part, inputs,    targets,     losses,    items,    diag_part, diag_out, scratch,
-                        result,    out64,       fallbacks};
+                        result,    out64,       fallbacks, spec_l};
         for (void* b : bufs) (void)hipFree(b);
         (void)hipHostFree(h_result);
         if (stream) (void)hipStreamDestroy(stream);
@@ -159,8 +160,12 @@ void enqueue_denoise(pic_hip_ctx* ctx, pic_hip_denoiser* net, int scheme, double
         merge = &ctx->pending_merge;
         ctx->merge_pending = false;
     }
-    launch_denoise(net->c, geom(net, 1), net->dil, net->ncu, a, ctx->stream,
-                   in_run && ctx->lds_fft ? &ctx->pending_apply : nullptr, merge);
+    // Inside a step loop whose solve runs in LDS, its row pass also runs the
+    // output layer where the grid allows.
+    const bool defer = in_run && ctx->lds_fft;
+    float2* spec_l = defer && poisson_rows_can_run_output_layer(net->W, net->H, net->dil[3]) ? net->spec_l : nullptr;
+    launch_denoise(net->c, geom(net, 1), net->dil, net->ncu, a, ctx->stream, defer ? &ctx->pending_apply : nullptr,
+                   merge, spec_l);
 }
 
 double kappa_of(int scheme) {
@@ -234,6 +239,7 @@ pic_status pic_hip_denoiser_create(int32_t Nx, int32_t Ny, int32_t max_batch, in
                         std::max<size_t>(4 * kMomentBlocks, 2 * kMomentBlocks + denoise_stats_doubles(n->ncu))));
     PIC_HIP_TRY(dmalloc(&n->result, 4));
     PIC_HIP_TRY(dmalloc(&n->out64, P));
+    PIC_HIP_TRY(dmalloc(&n->spec_l, (size_t)n->H * (n->W / 2 + 1)));
     PIC_HIP_TRY(dmalloc(&n->fallbacks, 1));
     PIC_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&n->h_result), 4 * sizeof(double), hipHostMallocDefault));
     const std::vector<float> w0 = initial_params(seed);

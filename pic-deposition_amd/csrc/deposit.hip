@@ -176,8 +176,10 @@ void launch_tile(const DepositArgs& a) {
         hipLaunchKernelGGL(deposit_merge_kernel<4>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);
     else if (g.mbatch <= 8)
         hipLaunchKernelGGL(deposit_merge_kernel<8>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);
-    else
+    else if (g.mbatch <= 16)
         hipLaunchKernelGGL(deposit_merge_kernel<16>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);
+    else
+        hipLaunchKernelGGL(deposit_merge_kernel<32>, mgrid, dim3(256), 0, a.stream, a.partial, a.acc, a.dom, g, a.rho);
 }
 
 // With a drift halo the particles have moved since the sort.  The lookahead
@@ -301,7 +303,7 @@ TileGeom make_tile_geom(int Nx, int Ny, int T, int R, int D) {
     g.uniform = ux && uy;
     g.fast_rows = !g.fast && g.D == 0 && g.T >= 3 && g.R < 3 && g.ntx >= 3 && (rx == 0 || rx >= 3) && uy;
     const int most = ((g.T + 2 + 2 * g.D) / g.T + 1) * ((g.R + 2 + 2 * g.D) / g.R + 1);
-    g.mbatch = most <= 4 ? 4 : (most <= 8 ? 8 : 16);
+    g.mbatch = most <= 4 ? 4 : most <= 8 ? 8 : most <= 16 ? 16 : 32;
     return g;
 }
 

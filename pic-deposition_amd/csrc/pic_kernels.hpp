@@ -28,7 +28,7 @@ struct TileGeom {
     int fast;          // merge fast path: no halo, tiles >= 3 cells on each axis, >= 3 tiles per axis
     int fast_rows;     // merge row path: as fast along x; tiles of 1 or 2 rows dividing the y axis
     int uniform;       // merge range path: equal tiles dividing each axis, one window image per node
-    int mbatch = 16;   // range path: windows per node loaded together (4, 8 or 16; the most per node when <= 16)
+    int mbatch = 16;   // range path: windows per node loaded together (4, 8, 16 or 32: the most per node, up to 32)
     // Overflow marker of the single-context deposit: a tile that sends a
     // particle to the FP64 accumulation grid stores `epoch` at *acc_mark, and
     // the merge reads that grid only when *acc_mark == epoch.  A marker left
@@ -206,11 +206,11 @@ bool poisson_lds_supported(int Nx, int Ny);
 // plain deposit when the gate rejects it, then rho += bg, before the solve
 // reads them.  One launch fewer per step of a learned run.
 struct LearnedApply {
-    const float* net_out;  // null: nothing deferred
+    float* net_out;  // null: nothing deferred
     double* out64;
     const double* sum;
     double sum_scale, inv_scale;
-    const double* stats;
+    double* stats;  // per-block gate sums (written by the layer that produces them)
     int nbs;
     const double* moments;
     int nbm;
@@ -218,13 +218,31 @@ struct LearnedApply {
     double* result;
     int* fallback_count;
     float bg;
+    // Output layer run by the solve's row pass (a3 set): it computes the
+    // network output (net_out) for its rows from the third layer's
+    // activations (BF16 NHWC, 32 channels) with the packed weights w4 and
+    // bias *bias4 at dilation dil4, writes per-block gate sums (stats, nbs =
+    // row blocks) and transforms the plain rows and the learned rows (into
+    // spec_l, without the renormalising scale); the column pass takes the gate
+    // decision and picks the spectrum, and the inverse-row pass writes rho and
+    // out64.
+    const void* a3 = nullptr;
+    const void* w4 = nullptr;
+    const float* bias4 = nullptr;
+    int dil4 = 1;
+    float2* spec_l = nullptr;
 };
+// The row pass can run the output layer (LearnedApply::a3) on this grid.
+bool poisson_rows_can_run_output_layer(int Nx, int Ny, int dil);
 void launch_poisson_lds(float* rho, int64_t G, double bg, double cv, const double* rho_sum, double sum_scale,
                         double sum_add, float2* spec, float2* spec_ex, float2* spec_ey, const PoissonTables& t,
                         const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s,
                         const LearnedApply& la = LearnedApply{});
+// With la.a3 set (the Ex / Ey call of a learned step), also writes the
+// learned or plain density into rho and la.out64 as the gate decided.
 void launch_poisson_lds_rows(const float2* X, const float2* Y, float* ox, float* oy, const DevDomain& d,
-                             const float2* tw_x, hipStream_t s);
+                             const float2* tw_x, hipStream_t s, float* rho = nullptr,
+                             const LearnedApply& la = LearnedApply{});
 // Ex = -(phi(i+1,j) - phi(i,j))/dx, Ey = -(phi(i,j+1) - phi(i,j))/dy.
 void launch_efield(const float* phi, float* Ex, float* Ey, const DevDomain& d, hipStream_t s);
 void launch_add_scalar(float* p, int64_t n, float v, hipStream_t s);
@@ -430,8 +448,12 @@ struct DenoiseArgs {
 // tile windows itself (writing rho) and the second clears the FP64
 // accumulation grid the merge folded (needs dil[0] == 1, see
 // denoise_can_merge).
+// With `spec_l` (and `defer`), the output layer is left to the solve's row
+// pass as well (poisson_rows_can_run_output_layer; spec_l holds the learned
+// half spectrum).
 void launch_denoise(const CnnBuffers& c, const CnnGeom& g, const int (&dil)[4], int ncu, const DenoiseArgs& a,
-                    hipStream_t s, LearnedApply* defer = nullptr, const TileMerge* merge = nullptr);
+                    hipStream_t s, LearnedApply* defer = nullptr, const TileMerge* merge = nullptr,
+                    float2* spec_l = nullptr);
 bool denoise_can_merge(const CnnGeom& g, const int (&dil)[4]);
 size_t denoise_stats_doubles(int ncu);
 

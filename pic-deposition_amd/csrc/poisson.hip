@@ -26,6 +26,7 @@
 // length is not a power of four, twiddles from one table e^{2 pi i m / L}
 // tabulated in FP64 and rounded to FP32.
 
+#include "cnn_device.hpp"
 #include "learned_gate.hpp"
 #include "pic_common.hpp"
 #include "pic_kernels.hpp"
@@ -201,14 +202,158 @@ __global__ __launch_bounds__(64) void poisson_rows_kernel(float* __restrict__ rh
     }
 }
 
+// Row pass of a learned step that runs the network's output layer (32 -> 1
+// channels, 3x3, dilation la.dil4) itself: one workgroup of 16 waves per pair
+// of rows.  The waves compute the output for the two rows on MFMA (a tile of
+// 16 pixels per instruction group, nine taps of K = 32 channels) and store it
+// (la.net_out); the per-block FP64 gate sums are added in a fixed order; wave
+// 0 transforms the plain rows exactly as poisson_rows_kernel does (the same
+// pairing, so a fallback step is bit for bit the plain solve) and wave 1 the
+// learned rows' fluctuation mean * out / out_scale (the renormalising scale,
+// a real factor, is applied by the column pass; the mean only moves the zero
+// mode, which the solve drops).
+constexpr int kRowsNetThreads = 1024;
+
+__global__ __launch_bounds__(kRowsNetThreads) void poisson_rows_net_kernel(
+    const float* __restrict__ rho, float2* __restrict__ spec, int W, int H, int64_t G, double bg, double cv,
+    const double* __restrict__ rho_sum, double sum_scale, double sum_add, const float2* __restrict__ tw,
+    LearnedApply la) {
+    extern __shared__ float2 sm[];
+    float2* a = sm;           // plain rows r0 + i r1
+    float2* b = sm + W;       // scratch
+    float2* l = sm + 2 * W;   // learned rows
+    float2* lb = sm + 3 * W;  // scratch
+    float2* t = sm + 4 * W;
+    float* orow = reinterpret_cast<float*>(sm + 5 * W);  // [2][W] network output
+    __shared__ double red[2][kRowsNetThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int grp = lane >> 4, col = lane & 15;
+    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+    const bool pair = r1 < H;
+    // plain values of this thread's column (W <= kRowsNetThreads)
+    const bool has = tid < W;
+    const float p0 = has ? rho[(size_t)r0 * W + tid] : 0.0f;
+    const float p1 = has && pair ? rho[(size_t)r1 * W + tid] : 0.0f;
+    // output layer, two 16-pixel tiles per pass of a wave
+    const bf16x8* wpk = static_cast<const bf16x8*>(la.w4);
+    const __bf16* act = static_cast<const __bf16*>(la.a3);
+    bf16x8 bw[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) bw[k] = wpk[k * 64 + lane];
+    const float bias = *la.bias4;
+    const int tpr = W / 16;
+    const int ntile = (pair ? 2 : 1) * tpr;
+    constexpr int nw = kRowsNetThreads / 64;
+    for (int tb = wv; tb < ntile; tb += 2 * nw) {
+        bf16x8 av[2][9];
+        int yt[2], xt[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int tile = min(tb + u * nw, ntile - 1);  // a missing second tile repeats the last (not stored)
+            const int rl = tile / tpr;
+            yt[u] = r0 + rl;
+            xt[u] = (tile - rl * tpr) * 16;
+            const int x = xt[u] + col;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int yy = wrapn(yt[u] + (k / 3 - 1) * la.dil4, H);
+                const int xx = wrapn(x + (k % 3 - 1) * la.dil4, W);
+                av[u][k] = *reinterpret_cast<const bf16x8*>(act + ((size_t)yy * W + xx) * 32 + grp * 8);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (tb + u * nw >= ntile) break;  // wave-uniform
+            f32x4 acc{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc = mfma(av[u][k], bw[k], acc);
+            if (col == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int xo = xt[u] + grp * 4 + i;
+                    const float o = acc[i] + bias;
+                    orow[(yt[u] - r0) * W + xo] = o;
+                    la.net_out[(size_t)yt[u] * W + xo] = o;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // gate sums (u = 1 + out / out_scale, (mean u - rho_plain)^2) and the
+    // transform inputs
+    const double mean = (*rho_sum * sum_scale + sum_add + bg * (double)G) * cv / (double)G;
+    const double mean_l = *la.sum * la.sum_scale / (double)G;
+    double su = 0.0, sr = 0.0;
+    if (has) {
+        const float o0 = orow[tid];
+        const double u0 = 1.0 + (double)o0 * la.inv_scale;
+        const double e0 = mean_l * u0 - (double)p0;
+        su += u0;
+        sr += e0 * e0;
+        float v0 = p0, v1 = p1;
+        if (la.bg != 0.0f) {
+            v0 += la.bg;
+            v1 += la.bg;
+        }
+        float lv1 = 0.0f;
+        if (pair) {
+            const float o1 = orow[W + tid];
+            const double u1 = 1.0 + (double)o1 * la.inv_scale;
+            const double e1 = mean_l * u1 - (double)p1;
+            su += u1;
+            sr += e1 * e1;
+            lv1 = (float)(mean_l * la.inv_scale * cv * (double)o1);
+        }
+        a[tid] = make_float2((float)(((double)v0 + bg) * cv - mean),
+                             pair ? (float)(((double)v1 + bg) * cv - mean) : 0.0f);
+        l[tid] = make_float2((float)(mean_l * la.inv_scale * cv * (double)o0), lv1);
+        t[tid] = tw[tid];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        su += __shfl_xor(su, o, 64);
+        sr += __shfl_xor(sr, o, 64);
+    }
+    if (lane == 0) {
+        red[0][wv] = su;
+        red[1][wv] = sr;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            s0 += red[0][w];
+            s1 += red[1][w];
+        }
+        la.stats[2 * blockIdx.x] = s0;
+        la.stats[2 * blockIdx.x + 1] = s1;
+    }
+    const int hx = W / 2 + 1;
+    if (wv < 2) {  // wave-uniform: the plain pair on wave 0, the learned pair on wave 1
+        float2* src = wv == 0 ? a : l;
+        float2* dst = wv == 0 ? spec : la.spec_l;
+        const float2* Z = wave_fft<-1, false>(src, wv == 0 ? b : lb, W, t);
+        for (int k = lane; k < hx; k += 64) {
+            const float2 p = Z[k], m = Z[(W - k) & (W - 1)];
+            dst[(size_t)r0 * hx + k] = make_float2(0.5f * (p.x + m.x), 0.5f * (p.y - m.y));
+            if (pair) dst[(size_t)r1 * hx + k] = make_float2(0.5f * (p.y + m.y), -0.5f * (p.x - m.x));
+        }
+    }
+}
+
 // Three wavefronts per column: the forward transform and the Green's function
 // by all of them, then one inverse transform each (phi, Ex, Ey) side by side.
 constexpr int kColsThreads = 192;
 
+// With the output layer in the row pass (la.a3), every wave takes the gate
+// decision (fixed-order sums: all blocks agree), block 0 records it, and the
+// column is read from the learned spectrum times the renormalising scale, or
+// from the plain one when the gate falls back.
 __global__ __launch_bounds__(kColsThreads) void poisson_cols_kernel(float2* __restrict__ spec,
                                                                     float2* __restrict__ sex,
                                                                     float2* __restrict__ sey, PoissonTables tb,
-                                                                    DevDomain d, const float2* __restrict__ tw) {
+                                                                    DevDomain d, const float2* __restrict__ tw,
+                                                                    LearnedApply la) {
     extern __shared__ float2 sm[];
     const int H = d.Ny;
     float2* a = sm;
@@ -221,9 +366,39 @@ __global__ __launch_bounds__(kColsThreads) void poisson_cols_kernel(float2* __re
     const int tid = threadIdx.x;
     const int hx = d.Nx / 2 + 1;
     const int kx = blockIdx.x;
-    for (int y = tid; y < H; y += kColsThreads) {
-        a[y] = spec[(size_t)y * hx + kx];
-        t[y] = tw[y];
+    if (la.a3) {
+        // both columns requested before the gate's sums, so their loads overlap
+        constexpr int kPre = (1024 + kColsThreads - 1) / kColsThreads;
+        float2 vp[kPre], vl[kPre];
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int y = tid + j * kColsThreads;
+            const bool in = y < H;
+            vp[j] = in ? spec[(size_t)y * hx + kx] : make_float2(0.0f, 0.0f);
+            vl[j] = in ? la.spec_l[(size_t)y * hx + kx] : make_float2(0.0f, 0.0f);
+        }
+        const LearnedGate gt = learned_gate_wave(la, (int64_t)d.Nx * d.Ny);
+        if (blockIdx.x == 0 && tid == 0) {
+            la.result[0] = gt.ratio;
+            la.result[1] = gt.ok ? 0.0 : 1.0;
+            la.result[2] = gt.scale;
+            la.result[3] = gt.q1;
+            if (!gt.ok && la.fallback_count) *la.fallback_count += 1;
+        }
+        const float sc = (float)gt.scale;
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int y = tid + j * kColsThreads;
+            if (y < H) {
+                a[y] = gt.ok ? make_float2(vl[j].x * sc, vl[j].y * sc) : vp[j];
+                t[y] = tw[y];
+            }
+        }
+    } else {
+        for (int y = tid; y < H; y += kColsThreads) {
+            a[y] = spec[(size_t)y * hx + kx];
+            t[y] = tw[y];
+        }
     }
     __syncthreads();
     float2* F = wave_fft<-1>(a, b, H, t);
@@ -255,10 +430,14 @@ __global__ __launch_bounds__(kColsThreads) void poisson_cols_kernel(float2* __re
     }
 }
 
+// With la.a3 set it also writes the row's density as the gate decided (the
+// column pass's record): the learned value mean (1 + out / out_scale) scale in
+// FP64 to la.out64 and, rounded, to rho, or the plain value; plus la.bg.
 __global__ __launch_bounds__(64) void poisson_rows_inv_kernel(const float2* __restrict__ X,
                                                               const float2* __restrict__ Y, float* __restrict__ ox,
                                                               float* __restrict__ oy, int W,
-                                                              const float2* __restrict__ tw) {
+                                                              const float2* __restrict__ tw, float* __restrict__ rho,
+                                                              LearnedApply la) {
     extern __shared__ float2 sm[];
     float2* a = sm;
     float2* b = sm + W;
@@ -280,11 +459,48 @@ __global__ __launch_bounds__(64) void poisson_rows_inv_kernel(const float2* __re
     }
     for (int n = lane; n < W; n += 64) t[n] = tw[n];
     __syncthreads();
+    // The learned step's density: the row's plain values and network output
+    // are requested before the transform, which their loads then overlap.
+    constexpr int kPre = 16;  // W <= 1024
+    float pr[kPre], po[kPre];
+    double ok_d = 0.0, scale = 0.0, lsum = 0.0;
+    if (la.a3) {
+        ok_d = la.result[1];
+        scale = la.result[2];
+        lsum = *la.sum;
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int n = lane + 64 * j;
+            pr[j] = n < W ? rho[row * W + n] : 0.0f;
+            po[j] = n < W ? la.net_out[row * W + n] : 0.0f;
+        }
+    }
     const float2* z = wave_fft<1, false>(a, b, W, t);
     for (int n = lane; n < W; n += 64) {
         const float2 v = z[n];
         ox[row * W + n] = v.x;
         if (oy) oy[row * W + n] = v.y;
+    }
+    if (la.a3) {
+        const bool ok = ok_d == 0.0;
+        const double mean = lsum * la.sum_scale / ((double)W * (double)gridDim.x);
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int n = lane + 64 * j;
+            if (n < W) {
+                const size_t i = row * W + n;
+                float v = pr[j];
+                if (ok) {
+                    const double lv = mean * (1.0 + (double)po[j] * la.inv_scale) * scale;
+                    la.out64[i] = lv;
+                    v = (float)lv;
+                } else {
+                    la.out64[i] = (double)v;  // fallback: the plain deposit stays
+                }
+                if (la.bg != 0.0f) v += la.bg;
+                rho[i] = v;
+            }
+        }
     }
 }
 
@@ -300,16 +516,26 @@ void launch_poisson_lds(float* rho, int64_t G, double bg, double cv, const doubl
                         const DevDomain& d, const float2* tw_x, const float2* tw_y, hipStream_t s,
                         const LearnedApply& la) {
     const int W = d.Nx, H = d.Ny;
-    hipLaunchKernelGGL(poisson_rows_kernel, dim3((unsigned)((H + 1) / 2)), dim3(64), (size_t)3 * W * sizeof(float2),
-                       s, rho, spec, W, H, G, bg, cv, rho_sum, sum_scale, sum_add, tw_x, la);
+    if (la.a3)
+        hipLaunchKernelGGL(poisson_rows_net_kernel, dim3((unsigned)((H + 1) / 2)), dim3(kRowsNetThreads),
+                           (size_t)5 * W * sizeof(float2) + (size_t)2 * W * sizeof(float), s, rho, spec, W, H, G, bg,
+                           cv, rho_sum, sum_scale, sum_add, tw_x, la);
+    else
+        hipLaunchKernelGGL(poisson_rows_kernel, dim3((unsigned)((H + 1) / 2)), dim3(64),
+                           (size_t)3 * W * sizeof(float2), s, rho, spec, W, H, G, bg, cv, rho_sum, sum_scale, sum_add,
+                           tw_x, la);
     hipLaunchKernelGGL(poisson_cols_kernel, dim3((unsigned)(W / 2 + 1)), dim3(kColsThreads),
-                       (size_t)7 * H * sizeof(float2), s, spec, spec_ex, spec_ey, t, d, tw_y);
+                       (size_t)7 * H * sizeof(float2), s, spec, spec_ex, spec_ey, t, d, tw_y, la);
+}
+
+bool poisson_rows_can_run_output_layer(int Nx, int Ny, int dil) {
+    return poisson_lds_supported(Nx, Ny) && Nx % 16 == 0 && Nx <= kRowsNetThreads && dil < Nx && dil < Ny;
 }
 
 void launch_poisson_lds_rows(const float2* X, const float2* Y, float* ox, float* oy, const DevDomain& d,
-                             const float2* tw_x, hipStream_t s) {
+                             const float2* tw_x, hipStream_t s, float* rho, const LearnedApply& la) {
     hipLaunchKernelGGL(poisson_rows_inv_kernel, dim3((unsigned)d.Ny), dim3(64), (size_t)3 * d.Nx * sizeof(float2), s,
-                       X, Y, ox, oy, d.Nx, tw_x);
+                       X, Y, ox, oy, d.Nx, tw_x, rho, la);
 }
 
 }  // namespace pichip
