@@ -175,8 +175,7 @@ pic_status pic_hip_init_particles(pic_hip_ctx* ctx, const pic_init_params* param
  * keys, perm[i] = source index of the i-th sorted particle, and the Nx*Ny+1
  * cell offsets; NULL outputs are skipped.  The first pic_hip_sort or
  * pic_hip_run of a context allocates 32 bytes per particle of capacity for
- * record staging: a permutation far from the identity moves each particle's
- * streams as one or two 16-byte records (decided on the device per sort). */
+ * the intermediate streams of the radix passes. */
 pic_status pic_hip_sort(pic_hip_ctx* ctx);
 /* Sort by nearest node instead of by cell (key floor(x/dx) + [fraction >= 1/2],
  * wrapped; build-defined, no reference counterpart).  On this order the
@@ -187,6 +186,14 @@ pic_status pic_hip_sort(pic_hip_ctx* ctx);
  * pic_hip_deposit_host does for GPU_PRIV NGP and TSC. */
 pic_status pic_hip_sort_nodes(pic_hip_ctx* ctx);
 pic_status pic_hip_sort_perm(pic_hip_ctx* ctx, int32_t* keys_sorted, int32_t* perm, int32_t* cell_offsets);
+/* How the context's last sort (pic_hip_sort, pic_hip_sort_nodes or a sort
+ * inside pic_hip_run) moved the particles, decided on the device: 0 the radix
+ * passes, which carry the streams with the keys, 2 a copy (the order was
+ * still sorted); -1 before any sort, or for sets below 2^22 particles, which
+ * always take (key, index) radix passes and one gather.  Synchronous.
+ * Build-defined, no reference counterpart (the reference's std::sort has no
+ * such choice). */
+pic_status pic_hip_sort_path(pic_hip_ctx* ctx, int32_t* path);
 /* Keys of the current order (synchronous copy out). */
 pic_status pic_hip_get_keys(pic_hip_ctx* ctx, int32_t* keys);
 

@@ -463,7 +463,8 @@ void swap_permuted(pic_hip_ctx* ctx) {
         if (!ctx->uniform[i]) std::swap(ctx->s[i], ctx->alt[i]);
 }
 
-// Buffers of the staged permutation (outside stream capture).
+// Buffers of the staged sort: the intermediate streams of the radix passes,
+// the per-block counts and the mode word (outside stream capture).
 pic_status ensure_sort_staging(pic_hip_ctx* ctx) {
     if (ctx->sort_rec) return PIC_OK;
     PIC_HIP_TRY(dmalloc(&ctx->far_part, sort_part_elems((int64_t)ctx->cap)));
@@ -488,20 +489,18 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back, bool node) {
         back.dst[back.count++] = ctx->s[i];
     }
     const int64_t n = (int64_t)ctx->n;
+    ctx->last_sort_staged = ctx->sort_rec && n >= kStagedSortMin;
     if (ctx->sort_rec && n >= kStagedSortMin) {
-        // The device decides per sort between a direct gather of each stream
-        // (most particles kept their cell since the last sort, so the
-        // permutation moves long runs), packed records (more than an eighth
-        // changed cell: the runs are short, and one or two 16-byte records per
-        // particle read fewer lines than one word per stream), and a copy (the
-        // order is still sorted).  Below kStagedSortMin particles the random
-        // gather is short and the decision's own launches would cost more than
-        // they save.
-        launch_keys_hist(ctx->s[0], ctx->s[1], n, ctx->dev, (int)ctx->G, ctx->sb, 0, node,
+        // Keys, the first digit's histogram and the already-sorted test (mode
+        // 2: the passes only copy); then the radix passes move the streams
+        // themselves (the staging buffer holds the intermediate set), so no
+        // permutation is gathered.  Below kStagedSortMin particles the sort
+        // keeps (key, index) passes and one gather, whose few launches cost
+        // less than the staged path's.
+        launch_keys_hist(ctx->s[0], ctx->s[1], n, ctx->dev, (int)ctx->G, ctx->sb, -1, node,
                          ctx->far_part, ctx->permute_mode, ctx->stream);
-        launch_pack_streams(fwd, n, ctx->permute_mode, ctx->sort_rec, ctx->stream);
-        PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream, ctx->permute_mode, true));
-        launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream, ctx->permute_mode, ctx->sort_rec);
+        PIC_HIP_TRY(launch_sort_records(n, (int)ctx->G, ctx->sb, fwd, reinterpret_cast<float*>(ctx->sort_rec),
+                                        ctx->cap, ctx->stream, ctx->permute_mode));
     } else {
         launch_keys(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, ctx->stream, node);
         PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));
@@ -547,6 +546,18 @@ pic_status pic_hip_sort_perm(pic_hip_ctx* ctx, int32_t* keys_sorted, int32_t* pe
         PIC_HIP_TRY(hipMemcpyAsync(cell_offsets, ctx->sb.cell_off, ((size_t)ctx->G + 1) * sizeof(int),
                                    hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PIC_OK;
+}
+
+pic_status pic_hip_sort_path(pic_hip_ctx* ctx, int32_t* path) {
+    if (check_ctx(ctx) != PIC_OK) return PIC_ERR_INVALID_ARGUMENT;
+    if (!path) return fail(PIC_ERR_INVALID_ARGUMENT, "null path");
+    *path = -1;
+    if (!ctx->last_sort_staged || !ctx->permute_mode) return PIC_OK;
+    int m = -1;
+    PIC_HIP_TRY(hipMemcpyAsync(&m, ctx->permute_mode, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *path = m;
     return PIC_OK;
 }
 

@@ -37,14 +37,16 @@ struct pic_hip_ctx {
     // Stream k holds one value for every particle (charges and masses of a
     // single species): a permutation leaves it unchanged, so sorts skip it.
     bool uniform[kStreams] = {};
-    // Sort of a far-from-sorted order: the permuted streams go through packed
-    // records (sort_rec, 2 float4 per particle) chosen on the device per sort
-    // (far_part, per-block counts of far moves, and permute_mode); allocated before
-    // the first sort that can use it.
+    // Staged sort (at least kStagedSortMin particles): the radix passes carry
+    // the streams, the intermediate set in sort_rec (8 floats per particle of
+    // capacity); far_part holds per-block counts from which the device sets
+    // permute_mode (2: the order is still sorted, the passes only copy);
+    // allocated before the first sort.
     float4* sort_rec = nullptr;
     unsigned* far_part = nullptr;  // sort_part_elems(cap): per-block far / descent counts
     int* permute_mode = nullptr;
     SortBuffers sb{};
+    bool last_sort_staged = false;  // the last sort set permute_mode (pic_hip_sort_path)
     bool order_valid = false;
     bool order_node = false;  // the valid order is the node-keyed one (pic_hip_sort_nodes)
     float* rho = nullptr;

@@ -127,6 +127,13 @@ struct StreamSet {
 // W = 1 for up to four streams, else 2).
 void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s,
                             const int* mode = nullptr, const float4* rec = nullptr);
+// Stable sort of the keys that launch_keys_hist left in sb.keys (with the
+// first pass's histogram in sb.hist and the mode in *mode), the particle
+// streams moving with the keys through the radix passes (ss.src -> ss.dst;
+// tmp: ss.count intermediate streams of `stride` floats each), then cell_off.
+// Mode 2 (already in order) copies the streams.
+hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const StreamSet& ss, float* tmp,
+                               size_t stride, hipStream_t s, const int* mode);
 // Cell keys (node keys with `node`) of the current order into sb.keys (which
 // holds the keys of the last sort's order), the first radix pass's histogram
 // table, and from per-block counts (`part`, sort_part_elems(n) values) the
@@ -140,6 +147,7 @@ size_t sort_part_elems(int64_t n);
 void launch_permute_mode(const unsigned* part, int nparts, int64_t n, int* mode, hipStream_t s);
 void launch_keys_hist(const float* x, const float* y, int64_t n, const DevDomain& d, int key_range, SortBuffers& sb,
                       int far, bool node, unsigned* part, int* mode, hipStream_t s);
+
 // rec: n (up to 4 streams) or 2n (5 or 6 streams) float4 records.
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s);
 // Cell keys (particleCellIndex), or with `node` the keys of the node-keyed

@@ -133,29 +133,31 @@ __global__ __launch_bounds__(kRT) void rhist_kernel(const int* __restrict__ keys
 // block: part[2 lb] the particles whose key moved by more than `far` from the
 // key at the same position after the last sort, part[2 lb + 1] the positions
 // whose key is below the previous position's (0 everywhere: already sorted).
-template <int I>
-__global__ __launch_bounds__(kRT) void keys_rhist_kernel(const float* __restrict__ x, const float* __restrict__ y,
+template <int I, int NT = kRT>
+__global__ __launch_bounds__(NT) void keys_rhist_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                          int64_t n, DevDomain d, int node, int* __restrict__ keys,
                                                          int far, int shift, int bins, int nblocks, int q,
                                                          uint32_t* __restrict__ table, unsigned* __restrict__ part) {
     const int lb = logical_block(q);
     if (lb >= nblocks) return;
     __shared__ uint32_t h[kMaxBins];
-    __shared__ unsigned cnt[2][kRW];
-    for (int i = threadIdx.x; i < bins; i += kRT) h[i] = 0;
+    constexpr int NW = NT / 64;
+    __shared__ unsigned cnt[2][NW];
+    for (int i = threadIdx.x; i < bins; i += NT) h[i] = 0;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t wb = (int64_t)lb * (kRT * I) + (int64_t)w * (64 * I);
+    const int64_t wb = (int64_t)lb * (NT * I) + (int64_t)w * (64 * I);
     int k[I];
     unsigned nfar = 0, ndesc = 0;  // wave totals, kept by every lane
     // all loads first (clamped to the last particle, so none is conditional)
     float xs[I], ys[I];
     int olds[I];
+    const bool cmp_old = far >= 0;  // far < 0: no comparison with the last sort's keys (not read)
 #pragma unroll
     for (int j = 0; j < I; ++j) {
         const int64_t i = min(wb + j * 64 + lane, n - 1);
         xs[j] = x[i];
         ys[j] = y[i];
-        olds[j] = keys[i];
+        olds[j] = cmp_old ? keys[i] : 0;
     }
 #pragma unroll
     for (int j = 0; j < I; ++j) {
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(kRT) void keys_rhist_kernel(const float* __restrict
             k[j] = node ? node_key_exact(xs[j], ys[j], d) : cell_key_exact(xs[j], ys[j], d);
             keys[i] = k[j];
             const long long dk = (long long)k[j] - (long long)olds[j];
-            f = dk > far || dk < -far;
+            f = cmp_old && (dk > far || dk < -far);
         }
         nfar += (unsigned)__popcll(__ballot(f));
     }
@@ -189,24 +191,130 @@ __global__ __launch_bounds__(kRT) void keys_rhist_kernel(const float* __restrict
         cnt[1][w] = ndesc;
     }
     __syncthreads();
-    for (int dd = threadIdx.x; dd < bins; dd += kRT) table[(size_t)dd * nblocks + lb] = h[dd];
+    for (int dd = threadIdx.x; dd < bins; dd += NT) table[(size_t)dd * nblocks + lb] = h[dd];
     if (threadIdx.x < 2) {
         unsigned s = 0;
-        for (int ww = 0; ww < kRW; ++ww) s += cnt[threadIdx.x][ww];
+        for (int ww = 0; ww < NW; ++ww) s += cnt[threadIdx.x][ww];
         part[2 * (size_t)lb + threadIdx.x] = s;
     }
 }
 
-// Stable scatter of one pass.  Each wave ranks its contiguous eighth of the
-// block slot by slot: lanes holding the same digit are matched with one
-// ballot per digit bit, the group's first lane adds the group size to the
-// wave's LDS histogram (returning the count before it) and the others read
-// it from that lane, so ranks follow item order.  Then, per digit, the waves'
-// counts become exclusive offsets and the digit totals a block-wide exclusive
-// scan; every item goes to its digit-ordered slot of an LDS stage, from which
-// the block writes out run by run.  On an order already sorted (*mode == 2)
-// the pass only copies the keys through when it is the last of an odd number
-// of passes (so that the sorted keys end in the same buffer).
+// Stable ranking of one block's items by digit, shared by the scatter
+// kernels.  Each wave ranks its contiguous quarter of the block slot by slot:
+// lanes holding the same digit are matched (digit_peers), the group's first
+// lane adds the group size to the wave's LDS histogram (returning the count
+// before it) and the others read it from that lane, so ranks follow item
+// order.  Then, per digit, the waves' counts become exclusive offsets and the
+// digit totals a block-wide exclusive scan.  On return rank[j] is item j's
+// slot in the digit-ordered block (items with key < 0 excluded), and
+// gbase[d] + i the global position of the digit-ordered block's item i when
+// its digit is d.  wh (kRW * 2^nbits words) is dead after the final barrier.
+template <int I, int NT = kRT>
+__device__ __forceinline__ void block_digit_rank(const int (&key)[I], int shift, int nbits, int nblocks, int lb,
+                                                 const uint32_t* __restrict__ scan, uint32_t* wh, uint32_t* gbase,
+                                                 uint32_t* bpre, uint32_t* wtot, uint32_t (&rank)[I]) {
+    const int bins = 1 << nbits;
+    const unsigned dmask = (unsigned)(bins - 1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int NW = NT / 64;
+    for (int i = threadIdx.x; i < NW * bins; i += NT) wh[i] = 0;
+    for (int dd = threadIdx.x; dd < bins; dd += NT) gbase[dd] = scan[(size_t)dd * nblocks + lb];
+    __syncthreads();  // zeroed wave histograms, gbase loaded
+    uint32_t* myh = wh + w * bins;
+    // Ranking four slots at a time, each group in three phases so that the
+    // slots' latencies overlap: the lanes sharing a digit (no memory access),
+    // then the group leaders' histogram adds in slot order (LDS atomics of
+    // one wave execute in program order, which keeps the ranks in item
+    // order), then the broadcasts of the returned counts.  Groups of four
+    // keep the peer masks of only four slots live (register pressure sets
+    // the kernels' occupancy).
+    constexpr int GR = I < 4 ? I : 4;
+#pragma unroll
+    for (int j0 = 0; j0 < I; j0 += GR) {
+        uint64_t mm[GR];
+        uint32_t old[GR], below[GR];
+#pragma unroll
+        for (int g = 0; g < GR; ++g) {
+            const int j = j0 + g;
+            mm[g] = digit_peers<10>(key[j] >= 0, ((unsigned)key[j] >> shift) & dmask, nbits);
+        }
+#pragma unroll
+        for (int g = 0; g < GR; ++g) {
+            const int j = j0 + g;
+            // group members below this lane: its rank in the group
+            below[g] =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(mm[g] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mm[g], 0u));
+            old[g] = 0;
+            const unsigned dg = ((unsigned)key[j] >> shift) & dmask;
+            if (key[j] >= 0 && below[g] == 0) old[g] = atomicAdd(&myh[dg], (uint32_t)__popcll(mm[g]));
+        }
+#pragma unroll
+        for (int g = 0; g < GR; ++g) {
+            const int j = j0 + g;
+            const int leader = key[j] >= 0 ? __ffsll((unsigned long long)mm[g]) - 1 : lane;
+            rank[j] = (uint32_t)__shfl((int)old[g], leader, 64) + below[g];
+        }
+    }
+    __syncthreads();
+    // wave counts -> exclusive offsets per digit; digit totals into bpre
+    for (int dd = threadIdx.x; dd < bins; dd += NT) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+            const uint32_t c = wh[ww * bins + dd];
+            wh[ww * bins + dd] = s;
+            s += c;
+        }
+        bpre[dd] = s;
+    }
+    __syncthreads();
+    // block-wide exclusive scan of the digit totals, kDpt consecutive digits
+    // per thread
+    {
+        constexpr int kDpt = kMaxBins / NT;
+        const int d0 = kDpt * threadIdx.x;
+        uint32_t t[kDpt];
+        uint32_t v = 0;
+#pragma unroll
+        for (int e = 0; e < kDpt; ++e) {
+            t[e] = d0 + e < bins ? bpre[d0 + e] : 0u;
+            v += t[e];
+        }
+        uint32_t incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) wtot[w] = incl;
+        __syncthreads();
+        uint32_t ex = incl - v;
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww)
+            if (ww < w) ex += wtot[ww];
+#pragma unroll
+        for (int e = 0; e < kDpt; ++e) {
+            if (d0 + e < bins) {
+                bpre[d0 + e] = ex;
+                gbase[d0 + e] -= ex;
+            }
+            ex += t[e];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const unsigned dg = ((unsigned)key[j] >> shift) & dmask;
+        rank[j] += key[j] >= 0 ? bpre[dg] + myh[dg] : 0u;  // local position in the digit-ordered block
+    }
+    __syncthreads();  // the wave histograms are read: their memory may be reused
+}
+
+// Stable scatter of (key, value) pairs for one pass (block_digit_rank, then
+// every item to its digit-ordered slot of an LDS stage, from which the block
+// writes out run by run).  On an order already sorted (*mode == 2) the pass
+// only copies the keys through when it is the last of an odd number of
+// passes (so that the sorted keys end in the same buffer).
 template <int I>
 __global__ __launch_bounds__(kRT, 2) void rscatter_kernel(const int* __restrict__ keys_in,
                                                        const int* __restrict__ vals_in, int64_t n, int shift,
@@ -231,16 +339,11 @@ __global__ __launch_bounds__(kRT, 2) void rscatter_kernel(const int* __restrict_
         uint32_t wh[kRW * kMaxBins];
     };
     __shared__ Lds lds;
-    int2* stage = lds.stage;
-    uint32_t* wh = lds.wh;
-    __shared__ uint32_t gbase[kMaxBins];  // later: global position of the block's first item of each digit minus bpre
+    __shared__ uint32_t gbase[kMaxBins];
     __shared__ uint32_t bpre[kMaxBins];
     __shared__ uint32_t wtot[kRW];
-    const int bins = 1 << nbits;
-    const unsigned dmask = (unsigned)(bins - 1);
+    const unsigned dmask = (unsigned)((1 << nbits) - 1);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < kRW * bins; i += kRT) wh[i] = 0;
-    for (int dd = threadIdx.x; dd < bins; dd += kRT) gbase[dd] = scan[(size_t)dd * nblocks + lb];
     const int64_t wb = b0 + (int64_t)w * (64 * I);
     int key[I], val[I];
 #pragma unroll
@@ -253,94 +356,18 @@ __global__ __launch_bounds__(kRT, 2) void rscatter_kernel(const int* __restrict_
             val[j] = vals_in ? vals_in[i] : (int)i;
         }
     }
-    // Ranking in three phases so that the slots' latencies overlap: the
-    // lanes sharing a digit in every slot (no memory access), then the group
-    // leaders' histogram adds in slot order (LDS atomics of one wave execute
-    // in program order, which keeps the ranks in item order), then the
-    // broadcasts of the returned counts.
-    uint64_t mm[I];
-#pragma unroll
-    for (int j = 0; j < I; ++j) mm[j] = digit_peers<10>(key[j] >= 0, ((unsigned)key[j] >> shift) & dmask, nbits);
-    __syncthreads();  // zeroed wave histograms, gbase loaded
-    uint32_t* myh = wh + w * bins;
-    uint32_t old[I], below[I];
-#pragma unroll
-    for (int j = 0; j < I; ++j) {
-        // group members below this lane: its rank in the group
-        below[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mm[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mm[j], 0u));
-        old[j] = 0;
-        const unsigned dg = ((unsigned)key[j] >> shift) & dmask;
-        if (key[j] >= 0 && below[j] == 0) old[j] = atomicAdd(&myh[dg], (uint32_t)__popcll(mm[j]));
-    }
     uint32_t rank[I];
-#pragma unroll
-    for (int j = 0; j < I; ++j) {
-        const int leader = key[j] >= 0 ? __ffsll((unsigned long long)mm[j]) - 1 : lane;
-        rank[j] = (uint32_t)__shfl((int)old[j], leader, 64) + below[j];
-    }
-    __syncthreads();
-    // wave counts -> exclusive offsets per digit; digit totals into bpre
-    for (int dd = threadIdx.x; dd < bins; dd += kRT) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int ww = 0; ww < kRW; ++ww) {
-            const uint32_t c = wh[ww * bins + dd];
-            wh[ww * bins + dd] = s;
-            s += c;
-        }
-        bpre[dd] = s;
-    }
-    __syncthreads();
-    // block-wide exclusive scan of the digit totals, kDpt consecutive digits
-    // per thread
-    {
-        constexpr int kDpt = kMaxBins / kRT;
-        const int d0 = kDpt * threadIdx.x;
-        uint32_t t[kDpt];
-        uint32_t v = 0;
-#pragma unroll
-        for (int e = 0; e < kDpt; ++e) {
-            t[e] = d0 + e < bins ? bpre[d0 + e] : 0u;
-            v += t[e];
-        }
-        uint32_t incl = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += u;
-        }
-        if (lane == 63) wtot[w] = incl;
-        __syncthreads();
-        uint32_t ex = incl - v;
-#pragma unroll
-        for (int ww = 0; ww < kRW; ++ww)
-            if (ww < w) ex += wtot[ww];
-#pragma unroll
-        for (int e = 0; e < kDpt; ++e) {
-            if (d0 + e < bins) {
-                bpre[d0 + e] = ex;
-                gbase[d0 + e] -= ex;
-            }
-            ex += t[e];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < I; ++j) {
-        const unsigned dg = ((unsigned)key[j] >> shift) & dmask;
-        rank[j] += key[j] >= 0 ? bpre[dg] + myh[dg] : 0u;  // local position in the digit-ordered block
-    }
-    __syncthreads();  // the wave histograms are read: their memory becomes the stage
+    block_digit_rank<I>(key, shift, nbits, nblocks, lb, scan, lds.wh, gbase, bpre, wtot, rank);
 #pragma unroll
     for (int j = 0; j < I; ++j)
-        if (key[j] >= 0) stage[rank[j]] = make_int2(key[j], val[j]);
+        if (key[j] >= 0) lds.stage[rank[j]] = make_int2(key[j], val[j]);
     __syncthreads();
     // write-out: item i of the digit-ordered stage goes to gbase[digit] + i
     int2 kv[I];
 #pragma unroll
     for (int j = 0; j < I; ++j) {
         const int i = threadIdx.x + j * kRT;
-        kv[j] = i < nb ? stage[i] : make_int2(0, 0);
+        kv[j] = i < nb ? lds.stage[i] : make_int2(0, 0);
     }
     uint32_t off[I];
 #pragma unroll
@@ -352,6 +379,97 @@ __global__ __launch_bounds__(kRT, 2) void rscatter_kernel(const int* __restrict_
             const uint32_t pos = off[j] + (uint32_t)i;
             keys_out[pos] = kv[j].x;
             vals_out[pos] = kv[j].y;
+        }
+    }
+}
+
+// Stable scatter of one pass that moves the particle streams with the keys
+// (no permutation array, no gather afterwards).  After block_digit_rank the
+// keys go through a 16 KiB LDS stage and out run by run, and then each stream
+// in turn through the same stage, to the positions the keys took.  io.src /
+// io.dst: this pass's input and output streams; on an order already sorted
+// (*mode == 2) only the last pass runs, and copies orig (the unsorted
+// streams, which are already in order) to io.dst, plus the keys when
+// copy_keys (an odd number of passes).
+template <int I, int NT>
+__global__ __launch_bounds__(NT) void rrec_kernel(const int* __restrict__ keys_in, int64_t n, int shift, int nbits,
+                                                   int nblocks, int q, const uint32_t* __restrict__ scan,
+                                                   int* __restrict__ keys_out, StreamSet io, StreamSet orig,
+                                                   const int* __restrict__ mode, int last, int copy_keys) {
+    constexpr int B = NT * I;
+    constexpr int NW = NT / 64;
+    const int lb = logical_block(q);
+    if (lb >= nblocks) return;
+    const int64_t b0 = (int64_t)lb * B;
+    const int nb = (int)min((int64_t)B, n - b0);
+    if (mode && *mode == 2) {
+        if (last) {
+            // 16-byte copies (block ranges start at multiples of B; the
+            // stream buffers are 256-byte aligned)
+            const int nv = nb >> 2;
+            for (int s = 0; s < io.count; ++s) {
+                const float4* src = reinterpret_cast<const float4*>(orig.src[s] + b0);
+                float4* dst = reinterpret_cast<float4*>(io.dst[s] + b0);
+                for (int i = threadIdx.x; i < nv; i += NT) dst[i] = src[i];
+                for (int i = 4 * nv + threadIdx.x; i < nb; i += NT) io.dst[s][b0 + i] = orig.src[s][b0 + i];
+            }
+            if (copy_keys)
+                for (int i = threadIdx.x; i < nb; i += NT) keys_out[b0 + i] = keys_in[b0 + i];
+        }
+        return;
+    }
+    union Lds {
+        uint32_t stage[B];
+        uint32_t wh[NW * kMaxBins];
+    };
+    __shared__ Lds lds;
+    __shared__ uint32_t gbase[kMaxBins];
+    __shared__ uint32_t bpre[kMaxBins];
+    __shared__ uint32_t wtot[NW];
+    const unsigned dmask = (unsigned)((1 << nbits) - 1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // 32-bit item indices (n < 2^31): loads and stores address as a scalar
+    // base plus a 32-bit lane offset, which keeps 64-bit addresses out of
+    // the registers
+    const uint32_t n32 = (uint32_t)n;
+    const uint32_t wb = (uint32_t)b0 + (uint32_t)w * (64 * I);
+    int key[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const uint32_t i = wb + j * 64 + lane;
+        key[j] = i < n32 ? keys_in[i] : -1;
+    }
+    uint32_t rank[I];
+    block_digit_rank<I, NT>(key, shift, nbits, nblocks, lb, scan, lds.wh, gbase, bpre, wtot, rank);
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        if (key[j] >= 0)
+            lds.stage[rank[j]] = (uint32_t)key[j];
+        else
+            rank[j] = ~0u;  // no item: from here on rank alone marks it
+    }
+    __syncthreads();
+    uint32_t pos[I];
+#pragma unroll
+    for (int j = 0; j < I; ++j) {
+        const int i = threadIdx.x + j * NT;
+        const uint32_t kv = i < nb ? lds.stage[i] : 0u;
+        pos[j] = gbase[(kv >> shift) & dmask] + (uint32_t)i;
+        if (i < nb) keys_out[pos[j]] = (int)kv;
+    }
+    for (int s = 0; s < io.count; ++s) {
+        float v[I];
+#pragma unroll
+        for (int j = 0; j < I; ++j) v[j] = io.src[s][min(wb + j * 64 + lane, n32 - 1)];
+        __syncthreads();  // the stage's previous contents are read
+#pragma unroll
+        for (int j = 0; j < I; ++j)
+            if (rank[j] != ~0u) lds.stage[rank[j]] = __float_as_uint(v[j]);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < I; ++j) {
+            const int i = threadIdx.x + j * NT;
+            if (i < nb) io.dst[s][pos[j]] = __uint_as_float(lds.stage[i]);
         }
     }
 }
@@ -706,6 +824,66 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
     return launch_sort_precomputed(n, d.Nx * d.Ny, sb, s);
 }
 
+hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const StreamSet& ss, float* tmp,
+                               size_t stride, hipStream_t s, const int* mode) {
+    if (n > 0) {
+        const RadixPlan p = radix_plan(n, key_range);
+        const int bins = 1 << p.bits;
+        const size_t m = (size_t)p.nblocks * bins;
+        const dim3 grid((unsigned)(p.q * kXcds));
+        int* kin = sb.keys;
+        int* kout = sb.keys_alt;
+        const float* in[kMaxStreams] = {};
+        float* tmps[kMaxStreams] = {};
+        for (int k = 0; k < ss.count; ++k) {
+            in[k] = ss.src[k];
+            tmps[k] = tmp + (size_t)k * stride;
+        }
+        for (int pass = 0; pass < p.passes; ++pass) {
+            const int shift = p.bits * pass;
+            if (pass > 0) {
+                if (p.items == 4)
+                    hipLaunchKernelGGL(rhist_kernel<4>, grid, dim3(kRT), 0, s, kin, n, shift, bins, p.nblocks, p.q,
+                                       sb.hist, mode);
+                else
+                    hipLaunchKernelGGL(rhist_kernel<16>, grid, dim3(kRT), 0, s, kin, n, shift, bins, p.nblocks, p.q,
+                                       sb.hist, mode);
+            }
+            exclusive_scan(sb.hist, sb.hist, m, sb.scan_tmp, s, mode);
+            const int last = pass == p.passes - 1 ? 1 : 0;
+            const int copy_keys = (last && (p.passes & 1)) ? 1 : 0;
+            // the last pass writes ss.dst, the ones before alternate with tmp
+            const bool to_dst = ((p.passes - 1 - pass) & 1) == 0;
+            StreamSet io{};
+            io.count = ss.count;
+            for (int k = 0; k < ss.count; ++k) {
+                io.src[k] = in[k];
+                io.dst[k] = to_dst ? ss.dst[k] : tmps[k];
+            }
+            // blocks of 256 * items: 4096-item blocks run 512 threads of 8
+            // items (fewer registers per thread, twice the waves per CU)
+            if (p.items == 4)
+                hipLaunchKernelGGL((rrec_kernel<4, kRT>), grid, dim3(kRT), 0, s, kin, n, shift, p.bits, p.nblocks,
+                                   p.q, sb.hist, kout, io, ss, mode, last, copy_keys);
+            else
+                hipLaunchKernelGGL((rrec_kernel<8, 2 * kRT>), grid, dim3(2 * kRT), 0, s, kin, n, shift, p.bits,
+                                   p.nblocks, p.q, sb.hist, kout, io, ss, mode, last, copy_keys);
+            int* kt = kin;
+            kin = kout;
+            kout = kt;
+            for (int k = 0; k < ss.count; ++k) in[k] = io.dst[k];
+        }
+        if (kin != sb.keys) {
+            int* t = sb.keys;
+            sb.keys = sb.keys_alt;
+            sb.keys_alt = t;
+        }
+    }
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3((unsigned)((n / 4 + 1 + 255) / 256)), dim3(256), 0, s, sb.keys, n,
+                       key_range, sb.cell_off);
+    return hipGetLastError();
+}
+
 void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s, const int* mode,
                             const float4* rec) {
     if (n > 0)
@@ -723,7 +901,8 @@ void launch_keys_hist(const float* x, const float* y, int64_t n, const DevDomain
             hipLaunchKernelGGL(keys_rhist_kernel<4>, grid, dim3(kRT), 0, s, x, y, n, d, node ? 1 : 0, sb.keys, far, 0,
                                bins, p.nblocks, p.q, sb.hist, part);
         else
-            hipLaunchKernelGGL(keys_rhist_kernel<16>, grid, dim3(kRT), 0, s, x, y, n, d, node ? 1 : 0, sb.keys, far,
+            hipLaunchKernelGGL((keys_rhist_kernel<8, 2 * kRT>), grid, dim3(2 * kRT), 0, s, x, y, n, d, node ? 1 : 0,
+                               sb.keys, far,
                                0, bins, p.nblocks, p.q, sb.hist, part);
     }
     hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, part, n > 0 ? p.nblocks : 0, n, mode);
@@ -743,5 +922,7 @@ void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n,
     if (n > 0)
         hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, perm, n);
 }
+
+
 
 }  // namespace pichip

@@ -181,6 +181,7 @@ _SIGS = {
     "pic_hip_sort_nodes": (ctypes.c_int, [_P]),
     "pic_hip_sort_perm": (ctypes.c_int, [_P, _i32p, _i32p, _i32p]),
     "pic_hip_get_keys": (ctypes.c_int, [_P, _i32p]),
+    "pic_hip_sort_path": (ctypes.c_int, [_P, _i32p]),
     "pic_hip_deposit": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "pic_hip_get_rho": (ctypes.c_int, [_P, _f32p]),
     "pic_hip_get_rho_f64": (ctypes.c_int, [_P, _f64p]),
@@ -444,6 +445,14 @@ class Context:
         _check(_lib.pic_hip_sort_perm(self._h, _ptr(keys, ctypes.c_int32), _ptr(perm, ctypes.c_int32),
                                       _ptr(off, ctypes.c_int32)))
         return keys, perm, off
+
+    def sort_path(self) -> int:
+        """pic_hip_sort_path: how the last sort moved the particles (0 radix
+        passes carrying the streams, 2 copy of a sorted order, -1 unstaged /
+        none)."""
+        v = ctypes.c_int32(-1)
+        _check(_lib.pic_hip_sort_path(self._h, ctypes.byref(v)))
+        return int(v.value)
 
     def keys(self) -> np.ndarray:
         k = np.empty(self.count, dtype=np.int32)
