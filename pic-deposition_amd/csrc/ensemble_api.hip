@@ -44,6 +44,7 @@ struct pic_hip_ensemble {
     int64_t nslots = 0, nchunks = 0, Gtot = 0, Stot = 0, keys_total = 0, gblocks = 0, sblocks = 0;
     int64_t tab_total = 0, x_total = 0;
     int ncu = 1;
+    size_t cap = 0;  // slots of each stream buffer (nslots plus a float4 over-read pad)
     hipStream_t stream = nullptr;
     float* s[kStreams] = {};
     float* alt[kStreams] = {};
@@ -57,8 +58,9 @@ struct pic_hip_ensemble {
     float* partial = nullptr;
     size_t partial_cap = 0;
     int* status = nullptr;
-    // sort: per-chunk key-change / descent counts, the device-chosen
-    // permutation mode, and the packed-record staging (two float4 per slot)
+    // sort: per-chunk key-change / descent counts, the device-chosen mode
+    // (2: still in order), and the intermediate streams of the radix passes
+    // (eight floats per slot of cap)
     unsigned* sort_part = nullptr;
     int* permute_mode = nullptr;
     float4* sort_rec = nullptr;
@@ -229,9 +231,9 @@ pic_status ensure_fft(pic_hip_ensemble* e) {
     return PIC_OK;
 }
 
-// The permutation is chosen on the device per sort (as for a context): a
-// copy when the order is still sorted, packed records when more than an
-// eighth of the slots changed key, else a gather per stream.
+// As for a context: the radix passes carry the six streams with the keys
+// (the staging buffer holds the intermediate set), and an order that is
+// still sorted (decided on the device) is copied.
 pic_status enqueue_sort(pic_hip_ensemble* e, bool copy_back) {
     StreamSet fwd{}, back{};
     fwd.count = back.count = kStreams;
@@ -243,9 +245,8 @@ pic_status enqueue_sort(pic_hip_ensemble* e, bool copy_back) {
     }
     launch_ens_keys(e->particles(), e->tables(), e->nchunks, e->sb.keys, e->stream, e->sort_part);
     launch_permute_mode(e->sort_part, (int)e->nchunks, e->nslots, e->permute_mode, e->stream);
-    launch_pack_streams(fwd, e->nslots, e->permute_mode, e->sort_rec, e->stream);
-    PIC_HIP_TRY(launch_sort_precomputed(e->nslots, (int)e->keys_total, e->sb, e->stream, e->permute_mode));
-    launch_permute_streams(fwd, e->sb.perm, e->nslots, e->stream, e->permute_mode, e->sort_rec);
+    PIC_HIP_TRY(launch_sort_records(e->nslots, (int)e->keys_total, e->sb, fwd, reinterpret_cast<float*>(e->sort_rec),
+                                    (size_t)e->cap, e->stream, e->permute_mode, false));
     if (copy_back)
         launch_permute_streams(back, nullptr, e->nslots, e->stream);
     else
@@ -448,6 +449,7 @@ pic_status pic_hip_ensemble_create(const pic_ensemble_member* members, int32_t c
 
     PIC_HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     const size_t cap = (size_t)slots + 64;  // float4 over-read pad
+    e->cap = cap;
     for (int i = 0; i < kStreams; ++i) {
         PIC_HIP_TRY(dmalloc(&e->s[i], cap));
         PIC_HIP_TRY(dmalloc(&e->alt[i], cap));
@@ -456,8 +458,7 @@ pic_status pic_hip_ensemble_create(const pic_ensemble_member* members, int32_t c
     }
     PIC_HIP_TRY(dmalloc(&e->sb.keys, cap));
     PIC_HIP_TRY(dmalloc(&e->sb.keys_alt, cap));
-    PIC_HIP_TRY(dmalloc(&e->sb.perm, cap));
-    PIC_HIP_TRY(dmalloc(&e->sb.perm_alt, cap));
+    // no permutation array: the radix passes move the streams
     e->sb.hist_cap = sort_hist_elems((int64_t)cap);
     PIC_HIP_TRY(dmalloc(&e->sb.hist, e->sb.hist_cap));
     e->sb.scan_tmp_cap = sort_scan_tmp_elems(e->sb.hist_cap);

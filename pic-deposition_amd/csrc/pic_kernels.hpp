@@ -127,13 +127,14 @@ struct StreamSet {
 // W = 1 for up to four streams, else 2).
 void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hipStream_t s,
                             const int* mode = nullptr, const float4* rec = nullptr);
-// Stable sort of the keys that launch_keys_hist left in sb.keys (with the
-// first pass's histogram in sb.hist and the mode in *mode), the particle
+// Stable sort of the keys in sb.keys (each in [0, key_range)), the particle
 // streams moving with the keys through the radix passes (ss.src -> ss.dst;
 // tmp: ss.count intermediate streams of `stride` floats each), then cell_off.
-// Mode 2 (already in order) copies the streams.
+// Mode 2 (already in order, set by launch_keys_hist / launch_permute_mode)
+// copies the streams.  first_hist_done: launch_keys_hist left the first
+// pass's histogram in sb.hist.
 hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const StreamSet& ss, float* tmp,
-                               size_t stride, hipStream_t s, const int* mode);
+                               size_t stride, hipStream_t s, const int* mode, bool first_hist_done = true);
 // Cell keys (node keys with `node`) of the current order into sb.keys (which
 // holds the keys of the last sort's order), the first radix pass's histogram
 // table, and from per-block counts (`part`, sort_part_elems(n) values) the

@@ -1,7 +1,6 @@
-// Cell sort for gfx950: exact cell keys, a stable LSD radix sort of (key,
-// index) pairs with digits of up to 10 bits (two passes for 2^20 cells), the
-// permutation of the particle streams, and the cell-offset table that the
-// tiled deposit reads.
+// Cell sort for gfx950: exact cell keys, a stable LSD radix sort with digits
+// of up to 9 bits (three passes for 2^20 cells, two for 2^18), and the
+// cell-offset table that the tiled deposit reads.
 //
 // Keys follow particleCellIndex (reference src/sorting/cell_sort.cpp:8-13) in
 // FP64, so they are bit-identical to the reference's.  The reference orders
@@ -11,10 +10,14 @@
 // One radix pass = a histogram kernel (per block and digit), an exclusive
 // scan of that table (digit-major), and a scatter kernel.  A scatter block of
 // 4096 items ranks its items stably in LDS (each wave ranks a contiguous
-// quarter of the block in item order against a wave-private digit histogram;
-// a block-wide scan then combines the waves), stages them in
-// digit order in LDS and writes them out so that consecutive threads write
-// consecutive addresses of a digit's run.  Blocks map to item ranges
+// part of the block in item order against a wave-private digit histogram; a
+// block-wide scan then combines the waves), stages them in digit order in
+// LDS and writes them out so that consecutive threads write consecutive
+// addresses of a digit's run.  The context's sort (rrec_kernel) moves the
+// particle streams themselves through the passes, one stream at a time
+// through the same stage; the (key, index) form (rscatter_kernel) serves the
+// permutation output of pic_hip_sort_perm and sets below 2^22 particles,
+// whose streams are then gathered once.  Blocks map to item ranges
 // XCD-contiguously (blocks are dispatched round-robin over the 8 XCDs; block
 // b takes range (b mod 8) * q + b / 8), so the runs of neighbouring ranges,
 // which are adjacent in the output, are written through the same L2.
@@ -33,7 +36,8 @@ constexpr int kWaves = kThreads / 64;
 
 constexpr int kRT = 256;        // radix block threads
 constexpr int kRW = kRT / 64;   // waves per radix block
-constexpr int kMaxBins = 1024;  // digits of at most 10 bits
+constexpr int kMaxBins = 1024;     // LDS histogram size (digits of up to 10 bits)
+constexpr int kMaxDigitBits = 9;   // digit width the passes use at most
 constexpr int kXcds = 8;        // MI355X: workgroups are dispatched round-robin over 8 XCDs
 
 __global__ __launch_bounds__(256) void keys_kernel(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
@@ -742,7 +746,11 @@ RadixPlan radix_plan(int64_t n, int key_range) {
     p.nblocks = (int)std::max<int64_t>(1, (n + bi - 1) / bi);
     p.q = (p.nblocks + kXcds - 1) / kXcds;
     const int kb = key_bits(key_range);
-    p.passes = kb == 0 ? 1 : (kb + 9) / 10;
+    // Digits of at most 9 bits: a 4096-item block then holds 8 or more items
+    // per digit on average, so its runs reach whole 32-byte sectors of each
+    // stream (10-bit digits leave runs of 4 items: a 2^20-key sort of a
+    // random order ran 25% slower in two 10-bit passes than in three of 7).
+    p.passes = kb == 0 ? 1 : (kb + kMaxDigitBits - 1) / kMaxDigitBits;
     p.bits = kb == 0 ? 1 : (kb + p.passes - 1) / p.passes;
     return p;
 }
@@ -825,7 +833,7 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
 }
 
 hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const StreamSet& ss, float* tmp,
-                               size_t stride, hipStream_t s, const int* mode) {
+                               size_t stride, hipStream_t s, const int* mode, bool first_hist_done) {
     if (n > 0) {
         const RadixPlan p = radix_plan(n, key_range);
         const int bins = 1 << p.bits;
@@ -841,7 +849,7 @@ hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const 
         }
         for (int pass = 0; pass < p.passes; ++pass) {
             const int shift = p.bits * pass;
-            if (pass > 0) {
+            if (pass > 0 || !first_hist_done) {
                 if (p.items == 4)
                     hipLaunchKernelGGL(rhist_kernel<4>, grid, dim3(kRT), 0, s, kin, n, shift, bins, p.nblocks, p.q,
                                        sb.hist, mode);
