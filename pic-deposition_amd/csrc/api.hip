@@ -498,9 +498,10 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back, bool node) {
         // keeps (key, index) passes and one gather, whose few launches cost
         // less than the staged path's.
         launch_keys_hist(ctx->s[0], ctx->s[1], n, ctx->dev, (int)ctx->G, ctx->sb, -1, node,
-                         ctx->far_part, ctx->permute_mode, ctx->stream);
+                         ctx->far_part, ctx->permute_mode, ctx->stream, false);
         PIC_HIP_TRY(launch_sort_records(n, (int)ctx->G, ctx->sb, fwd, reinterpret_cast<float*>(ctx->sort_rec),
-                                        ctx->cap, ctx->stream, ctx->permute_mode));
+                                        ctx->cap, ctx->stream, ctx->permute_mode, true, ctx->s[0], ctx->s[1],
+                                        &ctx->dev, node));
     } else {
         launch_keys(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, ctx->stream, node);
         PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));

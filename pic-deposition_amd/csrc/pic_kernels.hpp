@@ -132,9 +132,13 @@ void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hip
 // tmp: ss.count intermediate streams of `stride` floats each), then cell_off.
 // Mode 2 (already in order, set by launch_keys_hist / launch_permute_mode)
 // copies the streams.  first_hist_done: launch_keys_hist left the first
-// pass's histogram in sb.hist.
+// pass's histogram in sb.hist.  With kx, ky (the positions, in the unsorted
+// order) and the domain, the keys are formed from them (cell keys, or node
+// keys with `node`) instead of read from sb.keys.
 hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const StreamSet& ss, float* tmp,
-                               size_t stride, hipStream_t s, const int* mode, bool first_hist_done = true);
+                               size_t stride, hipStream_t s, const int* mode, bool first_hist_done = true,
+                               const float* kx = nullptr, const float* ky = nullptr, const DevDomain* d = nullptr,
+                               bool node = false);
 // Cell keys (node keys with `node`) of the current order into sb.keys (which
 // holds the keys of the last sort's order), the first radix pass's histogram
 // table, and from per-block counts (`part`, sort_part_elems(n) values) the
@@ -146,8 +150,10 @@ hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const 
 size_t sort_part_elems(int64_t n);
 // *mode from nparts pairs (changed, descents) of per-block counts.
 void launch_permute_mode(const unsigned* part, int nparts, int64_t n, int* mode, hipStream_t s);
+// far < 0: no comparison with the last sort's keys; write_keys false: the
+// keys are not stored (launch_sort_records forms them again from x, y).
 void launch_keys_hist(const float* x, const float* y, int64_t n, const DevDomain& d, int key_range, SortBuffers& sb,
-                      int far, bool node, unsigned* part, int* mode, hipStream_t s);
+                      int far, bool node, unsigned* part, int* mode, hipStream_t s, bool write_keys = true);
 
 // rec: n (up to 4 streams) or 2n (5 or 6 streams) float4 records.
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s);

@@ -141,7 +141,8 @@ template <int I, int NT = kRT>
 __global__ __launch_bounds__(NT) void keys_rhist_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                          int64_t n, DevDomain d, int node, int* __restrict__ keys,
                                                          int far, int shift, int bins, int nblocks, int q,
-                                                         uint32_t* __restrict__ table, unsigned* __restrict__ part) {
+                                                         uint32_t* __restrict__ table, unsigned* __restrict__ part,
+                                                         int write_keys) {
     const int lb = logical_block(q);
     if (lb >= nblocks) return;
     __shared__ uint32_t h[kMaxBins];
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(NT) void keys_rhist_kernel(const float* __restrict_
         bool f = false;
         if (i < n) {
             k[j] = node ? node_key_exact(xs[j], ys[j], d) : cell_key_exact(xs[j], ys[j], d);
-            keys[i] = k[j];
+            if (write_keys) keys[i] = k[j];
             const long long dk = (long long)k[j] - (long long)olds[j];
             f = cmp_old && (dk > far || dk < -far);
         }
@@ -391,15 +392,20 @@ __global__ __launch_bounds__(kRT, 2) void rscatter_kernel(const int* __restrict_
 // (no permutation array, no gather afterwards).  After block_digit_rank the
 // keys go through a 16 KiB LDS stage and out run by run, and then each stream
 // in turn through the same stage, to the positions the keys took.  io.src /
-// io.dst: this pass's input and output streams; on an order already sorted
-// (*mode == 2) only the last pass runs, and copies orig (the unsorted
-// streams, which are already in order) to io.dst, plus the keys when
-// copy_keys (an odd number of passes).
+// io.dst: this pass's input and output streams.  With kx (the positions of
+// the unsorted order) the first pass forms the keys itself, as
+// launch_keys_hist did, instead of reading them.  On an order already sorted
+// (*mode == 2) only the last pass runs: it copies orig (the unsorted
+// streams, which are already in order) to io.dst, and writes the keys (from
+// kx, ky; without them copies keys_in when copy_keys, an odd number of
+// passes).
 template <int I, int NT>
 __global__ __launch_bounds__(NT) void rrec_kernel(const int* __restrict__ keys_in, int64_t n, int shift, int nbits,
                                                    int nblocks, int q, const uint32_t* __restrict__ scan,
                                                    int* __restrict__ keys_out, StreamSet io, StreamSet orig,
-                                                   const int* __restrict__ mode, int last, int copy_keys) {
+                                                   const int* __restrict__ mode, int last, int copy_keys,
+                                                   const float* __restrict__ kx, const float* __restrict__ ky,
+                                                   DevDomain d, int node, int first) {
     constexpr int B = NT * I;
     constexpr int NW = NT / 64;
     const int lb = logical_block(q);
@@ -417,8 +423,14 @@ __global__ __launch_bounds__(NT) void rrec_kernel(const int* __restrict__ keys_i
                 for (int i = threadIdx.x; i < nv; i += NT) dst[i] = src[i];
                 for (int i = 4 * nv + threadIdx.x; i < nb; i += NT) io.dst[s][b0 + i] = orig.src[s][b0 + i];
             }
-            if (copy_keys)
+            if (kx) {
+                for (int i = threadIdx.x; i < nb; i += NT) {
+                    const float xv = kx[b0 + i], yv = ky[b0 + i];
+                    keys_out[b0 + i] = node ? node_key_exact(xv, yv, d) : cell_key_exact(xv, yv, d);
+                }
+            } else if (copy_keys) {
                 for (int i = threadIdx.x; i < nb; i += NT) keys_out[b0 + i] = keys_in[b0 + i];
+            }
         }
         return;
     }
@@ -438,10 +450,25 @@ __global__ __launch_bounds__(NT) void rrec_kernel(const int* __restrict__ keys_i
     const uint32_t n32 = (uint32_t)n;
     const uint32_t wb = (uint32_t)b0 + (uint32_t)w * (64 * I);
     int key[I];
+    if (first && kx) {
+        float xs[I], ys[I];
 #pragma unroll
-    for (int j = 0; j < I; ++j) {
-        const uint32_t i = wb + j * 64 + lane;
-        key[j] = i < n32 ? keys_in[i] : -1;
+        for (int j = 0; j < I; ++j) {
+            const uint32_t i = min(wb + j * 64 + lane, n32 - 1);
+            xs[j] = kx[i];
+            ys[j] = ky[i];
+        }
+#pragma unroll
+        for (int j = 0; j < I; ++j) {
+            const uint32_t i = wb + j * 64 + lane;
+            key[j] = i < n32 ? (node ? node_key_exact(xs[j], ys[j], d) : cell_key_exact(xs[j], ys[j], d)) : -1;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < I; ++j) {
+            const uint32_t i = wb + j * 64 + lane;
+            key[j] = i < n32 ? keys_in[i] : -1;
+        }
     }
     uint32_t rank[I];
     block_digit_rank<I, NT>(key, shift, nbits, nblocks, lb, scan, lds.wh, gbase, bpre, wtot, rank);
@@ -647,25 +674,34 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ s
 // sorted: the passes are skipped and the streams copied), else 1 (staged
 // gather through packed records) when more than an eighth of the particles
 // moved by more than `far` keys, else 0.  One block sums the counts.
-__global__ __launch_bounds__(256) void permute_mode_kernel(const unsigned* __restrict__ part, int nparts, int64_t n,
-                                                           int* __restrict__ mode) {
-    __shared__ unsigned long long sh[2][256];
+__global__ __launch_bounds__(1024) void permute_mode_kernel(const unsigned* __restrict__ part, int nparts,
+                                                            int64_t n, int* __restrict__ mode) {
+    __shared__ unsigned long long sh[2][16];
     unsigned long long f = 0, dsc = 0;
-    for (int i = threadIdx.x; i < nparts; i += 256) {
-        f += part[2 * (size_t)i];
-        dsc += part[2 * (size_t)i + 1];
+    const uint2* pairs = reinterpret_cast<const uint2*>(part);
+    for (int i = threadIdx.x; i < nparts; i += 1024) {
+        const uint2 v = pairs[i];
+        f += v.x;
+        dsc += v.y;
     }
-    sh[0][threadIdx.x] = f;
-    sh[1][threadIdx.x] = dsc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        f += __shfl_xor(f, o, 64);
+        dsc += __shfl_xor(dsc, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = f;
+        sh[1][threadIdx.x >> 6] = dsc;
+    }
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+    if (threadIdx.x == 0) {
+        unsigned long long tf = 0, td = 0;
+        for (int w = 0; w < 16; ++w) {
+            tf += sh[0][w];
+            td += sh[1][w];
         }
-        __syncthreads();
+        *mode = td == 0 ? 2 : ((int64_t)tf > n / 8 ? 1 : 0);
     }
-    if (threadIdx.x == 0) *mode = sh[1][0] == 0 ? 2 : ((int64_t)sh[0][0] > n / 8 ? 1 : 0);
 }
 
 // The stream kernels below take one item per thread: their reads are random
@@ -833,7 +869,9 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
 }
 
 hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const StreamSet& ss, float* tmp,
-                               size_t stride, hipStream_t s, const int* mode, bool first_hist_done) {
+                               size_t stride, hipStream_t s, const int* mode, bool first_hist_done,
+                               const float* kx, const float* ky, const DevDomain* d, bool node) {
+    const DevDomain dd = d ? *d : DevDomain{};
     if (n > 0) {
         const RadixPlan p = radix_plan(n, key_range);
         const int bins = 1 << p.bits;
@@ -870,12 +908,15 @@ hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const 
             }
             // blocks of 256 * items: 4096-item blocks run 512 threads of 8
             // items (fewer registers per thread, twice the waves per CU)
+            const int first = pass == 0 ? 1 : 0;
             if (p.items == 4)
                 hipLaunchKernelGGL((rrec_kernel<4, kRT>), grid, dim3(kRT), 0, s, kin, n, shift, p.bits, p.nblocks,
-                                   p.q, sb.hist, kout, io, ss, mode, last, copy_keys);
+                                   p.q, sb.hist, kout, io, ss, mode, last, copy_keys, kx, ky, dd, node ? 1 : 0,
+                                   first);
             else
                 hipLaunchKernelGGL((rrec_kernel<8, 2 * kRT>), grid, dim3(2 * kRT), 0, s, kin, n, shift, p.bits,
-                                   p.nblocks, p.q, sb.hist, kout, io, ss, mode, last, copy_keys);
+                                   p.nblocks, p.q, sb.hist, kout, io, ss, mode, last, copy_keys, kx, ky, dd,
+                                   node ? 1 : 0, first);
             int* kt = kin;
             kin = kout;
             kout = kt;
@@ -900,24 +941,24 @@ void launch_permute_streams(const StreamSet& ss, const int* perm, int64_t n, hip
 }
 
 void launch_keys_hist(const float* x, const float* y, int64_t n, const DevDomain& d, int key_range, SortBuffers& sb,
-                      int far, bool node, unsigned* part, int* mode, hipStream_t s) {
+                      int far, bool node, unsigned* part, int* mode, hipStream_t s, bool write_keys) {
     const RadixPlan p = radix_plan(n, key_range);
     const int bins = 1 << p.bits;
     const dim3 grid((unsigned)(p.q * kXcds));
     if (n > 0) {
         if (p.items == 4)
             hipLaunchKernelGGL(keys_rhist_kernel<4>, grid, dim3(kRT), 0, s, x, y, n, d, node ? 1 : 0, sb.keys, far, 0,
-                               bins, p.nblocks, p.q, sb.hist, part);
+                               bins, p.nblocks, p.q, sb.hist, part, write_keys ? 1 : 0);
         else
             hipLaunchKernelGGL((keys_rhist_kernel<8, 2 * kRT>), grid, dim3(2 * kRT), 0, s, x, y, n, d, node ? 1 : 0,
                                sb.keys, far,
-                               0, bins, p.nblocks, p.q, sb.hist, part);
+                               0, bins, p.nblocks, p.q, sb.hist, part, write_keys ? 1 : 0);
     }
-    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, part, n > 0 ? p.nblocks : 0, n, mode);
+    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(1024), 0, s, part, n > 0 ? p.nblocks : 0, n, mode);
 }
 
 void launch_permute_mode(const unsigned* part, int nparts, int64_t n, int* mode, hipStream_t s) {
-    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(256), 0, s, part, nparts, n, mode);
+    hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(1024), 0, s, part, nparts, n, mode);
 }
 
 void launch_pack_streams(const StreamSet& ss, int64_t n, const int* mode, float4* rec, hipStream_t s) {
