@@ -107,6 +107,14 @@ __device__ float2* wave_fft(float2* a, float2* b, int L, const float2* tw) {
     return a;
 }
 
+// The buffer in which wave_fft(a, b, L, ...) leaves its result (one swap per
+// stage).
+__device__ __forceinline__ float2* wave_fft_result(float2* a, float2* b, int L) {
+    int stages = 0;
+    for (int Ls = 1; Ls < L; Ls *= (L / Ls >= 4) ? 4 : 2) ++stages;
+    return (stages & 1) ? b : a;
+}
+
 // With a deferred learned density (la.net_out set), each block first takes the
 // gate decision and then writes its rows of rho (and la.out64) as
 // denoise_apply_kernel would, plus la.bg, before transforming them.
@@ -366,42 +374,60 @@ __global__ __launch_bounds__(kColsThreads) void poisson_cols_kernel(float2* __re
     const int tid = threadIdx.x;
     const int hx = d.Nx / 2 + 1;
     const int kx = blockIdx.x;
+    float2* F;
     if (la.a3) {
-        // both columns requested before the gate's sums, so their loads overlap
-        constexpr int kPre = (1024 + kColsThreads - 1) / kColsThreads;
-        float2 vp[kPre], vl[kPre];
-#pragma unroll
-        for (int j = 0; j < kPre; ++j) {
-            const int y = tid + j * kColsThreads;
-            const bool in = y < H;
-            vp[j] = in ? spec[(size_t)y * hx + kx] : make_float2(0.0f, 0.0f);
-            vl[j] = in ? la.spec_l[(size_t)y * hx + kx] : make_float2(0.0f, 0.0f);
+        // Both columns are transformed (one wave each) while the third wave
+        // takes the gate decision, then the chosen transform is kept, the
+        // learned one times the renormalising scale (the transform is linear;
+        // a fallback keeps the plain transform, formed exactly as without a
+        // learned density).
+        __shared__ int gate_ok;
+        __shared__ float gate_scale;
+        for (int y = tid; y < H; y += kColsThreads) {
+            a[y] = spec[(size_t)y * hx + kx];
+            s1[y] = la.spec_l[(size_t)y * hx + kx];
+            t[y] = tw[y];
         }
-        const LearnedGate gt = learned_gate_wave(la, (int64_t)d.Nx * d.Ny);
-        if (blockIdx.x == 0 && tid == 0) {
-            la.result[0] = gt.ratio;
-            la.result[1] = gt.ok ? 0.0 : 1.0;
-            la.result[2] = gt.scale;
-            la.result[3] = gt.q1;
-            if (!gt.ok && la.fallback_count) *la.fallback_count += 1;
-        }
-        const float sc = (float)gt.scale;
-#pragma unroll
-        for (int j = 0; j < kPre; ++j) {
-            const int y = tid + j * kColsThreads;
-            if (y < H) {
-                a[y] = gt.ok ? make_float2(vl[j].x * sc, vl[j].y * sc) : vp[j];
-                t[y] = tw[y];
+        __syncthreads();
+        const int wv = tid >> 6;  // wave-uniform
+        float2* fp = nullptr;
+        float2* fl = nullptr;
+        if (wv == 0) {
+            fp = wave_fft<-1, false>(a, b, H, t);
+        } else if (wv == 1) {
+            fl = wave_fft<-1, false>(s1, s2, H, t);
+        } else {
+            const LearnedGate gt = learned_gate_wave(la, (int64_t)d.Nx * d.Ny);
+            if ((tid & 63) == 0) {
+                gate_ok = gt.ok ? 1 : 0;
+                gate_scale = (float)gt.scale;
+                if (blockIdx.x == 0) {
+                    la.result[0] = gt.ratio;
+                    la.result[1] = gt.ok ? 0.0 : 1.0;
+                    la.result[2] = gt.scale;
+                    la.result[3] = gt.q1;
+                    if (!gt.ok && la.fallback_count) *la.fallback_count += 1;
+                }
             }
         }
+        // every wave learns where the transforms landed (both took the same stages)
+        if (wv != 0) fp = wave_fft_result(a, b, H);
+        if (wv != 1) fl = wave_fft_result(s1, s2, H);
+        __syncthreads();
+        F = fp;
+        if (gate_ok) {
+            const float sc = gate_scale;
+            for (int y = tid; y < H; y += kColsThreads) F[y] = make_float2(fl[y].x * sc, fl[y].y * sc);
+        }
+        __syncthreads();
     } else {
         for (int y = tid; y < H; y += kColsThreads) {
             a[y] = spec[(size_t)y * hx + kx];
             t[y] = tw[y];
         }
+        __syncthreads();
+        F = wave_fft<-1>(a, b, H, t);
     }
-    __syncthreads();
-    float2* F = wave_fft<-1>(a, b, H, t);
     float2* S = F == a ? b : a;  // free buffer
     const float2 sx = tb.dx_sym[kx];
     for (int ky = tid; ky < H; ky += kColsThreads) {
