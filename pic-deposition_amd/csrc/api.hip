@@ -503,9 +503,8 @@ pic_status sort_particles(pic_hip_ctx* ctx, bool copy_back, bool node) {
                                         ctx->cap, ctx->stream, ctx->permute_mode, true, ctx->s[0], ctx->s[1],
                                         &ctx->dev, node));
     } else {
-        launch_keys(ctx->s[0], ctx->s[1], n, ctx->dev, ctx->sb.keys, ctx->stream, node);
-        PIC_HIP_TRY(launch_sort_precomputed(n, (int)ctx->G, ctx->sb, ctx->stream));
-        launch_permute_streams(fwd, ctx->sb.perm, n, ctx->stream);
+        PIC_HIP_TRY(launch_sort_gather(ctx->s[0], ctx->s[1], n, ctx->dev, (int)ctx->G, node, ctx->sb, fwd,
+                                       ctx->stream));
     }
     if (copy_back)
         launch_permute_streams(back, nullptr, (int64_t)ctx->n, ctx->stream);

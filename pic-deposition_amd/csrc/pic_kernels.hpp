@@ -110,9 +110,10 @@ hipError_t launch_sort_keys_perm(const float* x, const float* y, int64_t n, cons
 // same outputs as launch_sort_keys_perm, cell_off with key_range + 1 entries.
 // With `mode` (set by launch_keys_hist), mode 2 (keys already in order) skips
 // the radix passes and leaves sb.perm unset; first_hist_done: the first
-// pass's histogram table is already in sb.hist (launch_keys_hist).
+// pass's histogram table is already in sb.hist (launch_keys_hist); offsets
+// false: cell_off is left to the caller.
 hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s,
-                                   const int* mode = nullptr, bool first_hist_done = false);
+                                   const int* mode = nullptr, bool first_hist_done = false, bool offsets = true);
 // dst[i] = src[perm[i]] for one float stream.
 void launch_gather_f32(const float* src, float* dst, const int* perm, int64_t n, hipStream_t s);
 // dst_k[i] = src_k[perm[i]] for ss.count streams in one pass (perm == nullptr: copy).
@@ -139,6 +140,12 @@ hipError_t launch_sort_records(int64_t n, int key_range, SortBuffers& sb, const 
                                size_t stride, hipStream_t s, const int* mode, bool first_hist_done = true,
                                const float* kx = nullptr, const float* ky = nullptr, const DevDomain* d = nullptr,
                                bool node = false);
+// The (key, index) sort with the streams gathered once: keys and the first
+// pass's histogram in one kernel, the radix passes on (key, index) pairs, then
+// one kernel that gathers ss.src into ss.dst by the permutation and writes
+// cell_off.  Fewest launches, for sets too small to fill the GPU.
+hipError_t launch_sort_gather(const float* x, const float* y, int64_t n, const DevDomain& d, int key_range,
+                              bool node, SortBuffers& sb, const StreamSet& ss, hipStream_t s);
 // Cell keys (node keys with `node`) of the current order into sb.keys (which
 // holds the keys of the last sort's order), the first radix pass's histogram
 // table, and from per-block counts (`part`, sort_part_elems(n) values) the

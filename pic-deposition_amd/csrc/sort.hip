@@ -134,9 +134,10 @@ __global__ __launch_bounds__(kRT) void rhist_kernel(const int* __restrict__ keys
 
 // Keys of the current order (into `keys`, which holds the keys of the last
 // sort's order on entry) and the first pass's histogram table, with per
-// block: part[2 lb] the particles whose key moved by more than `far` from the
-// key at the same position after the last sort, part[2 lb + 1] the positions
-// whose key is below the previous position's (0 everywhere: already sorted).
+// block (unless part is null): part[2 lb] the particles whose key moved by
+// more than `far` from the key at the same position after the last sort,
+// part[2 lb + 1] the positions whose key is below the previous position's (0
+// everywhere: already sorted).
 template <int I, int NT = kRT>
 __global__ __launch_bounds__(NT) void keys_rhist_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                          int64_t n, DevDomain d, int node, int* __restrict__ keys,
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(NT) void keys_rhist_kernel(const float* __restrict_
     }
     __syncthreads();
     for (int dd = threadIdx.x; dd < bins; dd += NT) table[(size_t)dd * nblocks + lb] = h[dd];
-    if (threadIdx.x < 2) {
+    if (part && threadIdx.x < 2) {
         unsigned s = 0;
         for (int ww = 0; ww < NW; ++ww) s += cnt[threadIdx.x][ww];
         part[2 * (size_t)lb + threadIdx.x] = s;
@@ -756,6 +757,28 @@ __global__ __launch_bounds__(256) void permute_streams_kernel(StreamSet ss, cons
         if (k < ss.count) ss.dst[k][i] = v[k];
 }
 
+// The (key, index) sort's last step: dst_k[i] = src_k[perm[i]] and the
+// cell-offset table of the sorted keys (thread i writes off[c] = i for the
+// keys c in (keys[i-1], keys[i]]; thread n the tail up to G).
+__global__ __launch_bounds__(256) void permute_offsets_kernel(StreamSet ss, const int* __restrict__ perm,
+                                                              const int* __restrict__ keys, int64_t n, int G,
+                                                              int* __restrict__ off) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const int lo = i == 0 ? -1 : keys[i - 1];
+    const int hi = i == n ? G : keys[i];
+    for (int c = lo + 1; c <= hi; ++c) off[c] = (int)i;
+    if (i == n) return;
+    const int64_t p = perm[i];
+    float v[kMaxStreams];
+#pragma unroll
+    for (int k = 0; k < kMaxStreams; ++k)
+        if (k < ss.count) v[k] = ss.src[k][p];
+#pragma unroll
+    for (int k = 0; k < kMaxStreams; ++k)
+        if (k < ss.count) ss.dst[k][i] = v[k];
+}
+
 int key_bits(int G) {
     int bits = 0;
     while (bits < 31 && (1ll << bits) < (long long)G) ++bits;
@@ -810,7 +833,7 @@ void launch_keys(const float* x, const float* y, int64_t n, const DevDomain& d, 
 }
 
 hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hipStream_t s, const int* mode,
-                                   bool first_hist_done) {
+                                   bool first_hist_done, bool offsets) {
     if (n > 0) {
         const RadixPlan p = radix_plan(n, key_range);
         const int bins = 1 << p.bits;
@@ -857,8 +880,9 @@ hipError_t launch_sort_precomputed(int64_t n, int key_range, SortBuffers& sb, hi
             sb.perm_alt = t;
         }
     }
-    hipLaunchKernelGGL(cell_offsets_kernel, dim3((unsigned)((n / 4 + 1 + 255) / 256)), dim3(256), 0, s, sb.keys, n,
-                       key_range, sb.cell_off);
+    if (offsets)
+        hipLaunchKernelGGL(cell_offsets_kernel, dim3((unsigned)((n / 4 + 1 + 255) / 256)), dim3(256), 0, s, sb.keys,
+                           n, key_range, sb.cell_off);
     return hipGetLastError();
 }
 
@@ -955,6 +979,26 @@ void launch_keys_hist(const float* x, const float* y, int64_t n, const DevDomain
                                0, bins, p.nblocks, p.q, sb.hist, part, write_keys ? 1 : 0);
     }
     hipLaunchKernelGGL(permute_mode_kernel, dim3(1), dim3(1024), 0, s, part, n > 0 ? p.nblocks : 0, n, mode);
+}
+
+hipError_t launch_sort_gather(const float* x, const float* y, int64_t n, const DevDomain& d, int key_range,
+                              bool node, SortBuffers& sb, const StreamSet& ss, hipStream_t s) {
+    if (n > 0) {
+        const RadixPlan p = radix_plan(n, key_range);
+        const int bins = 1 << p.bits;
+        const dim3 grid((unsigned)(p.q * kXcds));
+        if (p.items == 4)
+            hipLaunchKernelGGL(keys_rhist_kernel<4>, grid, dim3(kRT), 0, s, x, y, n, d, node ? 1 : 0, sb.keys, -1, 0,
+                               bins, p.nblocks, p.q, sb.hist, nullptr, 1);
+        else
+            hipLaunchKernelGGL((keys_rhist_kernel<8, 2 * kRT>), grid, dim3(2 * kRT), 0, s, x, y, n, d, node ? 1 : 0,
+                               sb.keys, -1, 0, bins, p.nblocks, p.q, sb.hist, nullptr, 1);
+        const hipError_t e = launch_sort_precomputed(n, key_range, sb, s, nullptr, true, false);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(permute_offsets_kernel, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, ss, sb.perm,
+                       sb.keys, n, key_range, sb.cell_off);
+    return hipGetLastError();
 }
 
 void launch_permute_mode(const unsigned* part, int nparts, int64_t n, int* mode, hipStream_t s) {
