@@ -918,6 +918,27 @@ pic_status capture_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
     return PIC_OK;
 }
 
+// One sort interval in one graph: the sorting step and the `len` - 1 steps
+// after it, so that a step loop of small sets launches one graph per
+// interval instead of one per step.
+pic_status capture_chunk(pic_hip_ctx* ctx, const pic_step_config& cfg, int len, bool diag, int record_from, int cap,
+                         const StepRecords& rec, GraphPair& g, const StepHook& post_deposit) {
+    PIC_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    pic_status st = PIC_OK;
+    for (int k = 0; k < len && st == PIC_OK; ++k)
+        st = enqueue_step(ctx, cfg, k == 0, false, diag, record_from, cap, rec, post_deposit);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != PIC_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    PIC_HIP_TRY(e);
+    g.graph = graph;
+    PIC_HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    return PIC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -993,11 +1014,33 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
                 (st = capture_step(ctx, *cfg, false, d, record_from, cap, rec, gp[v][0], post_deposit)) != PIC_OK)
                 return st;
         }
+        // Whole sort intervals as one graph each (gc[v][p], out of set p), for
+        // the intervals that end before the last step (whose push the host
+        // switches off) and lie on one side of the first recorded step.
+        GraphPair gc[2][2];
+        const int si = cfg->sort_interval;
+        const auto variant = [&](int step) { return step >= diag_from ? 1 : 0; };
+        if (priv && si >= 2) {
+            bool used[2] = {false, false};
+            for (int step = 0; step + si < nsteps; step += si)
+                if (variant(step) == variant(step + si - 1)) used[variant(step)] = true;
+            for (int v = 0; v < 2; ++v) {
+                if (!used[v]) continue;
+                for (int p = 0; p < 2; ++p)
+                    if ((st = capture_chunk(ctx, *cfg, si, v == 1, record_from, cap, rec, gc[v][p], post_deposit)) !=
+                        PIC_OK)
+                        return st;
+            }
+        }
         int parity = 0;
         for (int step = 0; step < nsteps; ++step) {
             if (fused && step == nsteps - 1) launch_set_int(ctx->push_flag, 0, ctx->stream);
-            const int v = step >= diag_from ? 1 : 0;
-            if (priv && step % cfg->sort_interval == 0) {
+            const int v = variant(step);
+            if (priv && step % si == 0 && step + si < nsteps && gc[v][parity].exec && variant(step + si - 1) == v) {
+                PIC_HIP_TRY(hipGraphLaunch(gc[v][parity].exec, ctx->stream));
+                parity ^= 1;
+                step += si - 1;
+            } else if (priv && step % si == 0) {
                 PIC_HIP_TRY(hipGraphLaunch(gs[v][parity].exec, ctx->stream));
                 parity ^= 1;
             } else {
