@@ -918,6 +918,8 @@ pic_status capture_step(pic_hip_ctx* ctx, const pic_step_config& cfg, bool with_
     return PIC_OK;
 }
 
+constexpr int kMaxIntervalGraph = 32;
+
 // One sort interval in one graph: the sorting step and the `len` - 1 steps
 // after it, so that a step loop of small sets launches one graph per
 // interval instead of one per step.
@@ -1017,10 +1019,12 @@ pic_status run_steps(pic_hip_ctx* ctx, const pic_step_config* cfg, int32_t nstep
         // Whole sort intervals as one graph each (gc[v][p], out of set p), for
         // the intervals that end before the last step (whose push the host
         // switches off) and lie on one side of the first recorded step.
+        // Intervals longer than kMaxIntervalGraph steps replay step by step
+        // (bounded graph size and capture time).
         GraphPair gc[2][2];
         const int si = cfg->sort_interval;
         const auto variant = [&](int step) { return step >= diag_from ? 1 : 0; };
-        if (priv && si >= 2) {
+        if (priv && si >= 2 && si <= kMaxIntervalGraph) {
             bool used[2] = {false, false};
             for (int step = 0; step + si < nsteps; step += si)
                 if (variant(step) == variant(step + si - 1)) used[variant(step)] = true;
