@@ -1299,8 +1299,11 @@ pic_status pic_hip_esirkepov_current(pic_hip_ctx* ctx, double dt) {
         for (int k = 0; k < 8; ++k) *f[k] = base + (size_t)k * ctx->G;
     }
     if (ctx->order_valid && !ctx->order_node && ctx->n > 0) {
-        // Cell-sorted particles: tiles of 64 x 2 cells (smaller grids: the grid).
-        const TileGeom tg = make_tile_geom(ctx->dom.Nx, ctx->dom.Ny, 64, 2, 0);
+        // Cell-sorted particles: tiles of 32 x 2 cells (smaller grids: the
+        // grid), the size at which the kernel's LDS and registers leave room
+        // for 8 waves per CU (64 x 2: 6 waves, 0.76 ms at 67M particles; 32 x
+        // 2: 0.60 ms; 16 x 2: 0.64 ms).
+        const TileGeom tg = make_tile_geom(ctx->dom.Nx, ctx->dom.Ny, 32, 2, 0);
         const size_t need = current_partial_doubles(tg);
         if (need > ctx->cur_partial_cap) {
             (void)hipFree(ctx->cur_partial);
