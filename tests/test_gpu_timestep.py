@@ -253,6 +253,41 @@ def test_full_size_config3_timestep_properties(hip):
     np.testing.assert_allclose(rec["charge_error"], 1.0, rtol=1e-5)
 
 
+def test_full_size_config3_dynamics_match_oracle_loop(hip, orc):
+    """512x512, 100 ppc (26,214,400 particles), TSC, in a regime with real
+    dynamics: unit plasma frequency, the two beams at the mode-1 resonant
+    velocity with a 1% velocity perturbation and thermal spread 0.01, dt 0.2,
+    sort every step.  The mode-1 field grows and the beams' kinetic energy
+    changes (mode-1 amplitude x6, field energy x19, kinetic energy +1.1e-4 in
+    the oracle over these 8 steps), and the device's per-step field energy,
+    mode-1 amplitude and kinetic energy equal the FP64 oracle loop's on the
+    same FP32 particles."""
+    say("config3 full dynamics")
+    G, ppc = 512, 100
+    n = G * G * ppc
+    dom = hip.make_domain(1.0, 1.0, G, G)
+    od = orc.domain(1.0, 1.0, G, G)
+    q = hip.unit_plasma_charge(n, G, G)
+    v0 = np.sqrt(3.0 / 8.0) / (2.0 * np.pi)  # k v0 = sqrt(3/8) omega_p for mode 1 at omega_p = 1
+    dt, steps = 0.2, 8
+    with hip.Context(dom, n) as c:
+        c.init_particles(n, kind=2, seed=42, amp=v0, pert=0.01 * v0, vth=0.01, q=q, m=1.0)
+        p = {k: v.astype(np.float64) for k, v in c.download().items()}
+        rec = c.run(steps, 2, 2, dt, 0.0, 0, sort_interval=1, record_from=0, record_particles=True, use_graph=True)
+    _, ref = orc.run_loop(p, od, 2, 0, dt, 0.0, steps, 0)
+    fe = ref["field_energy"]
+    print(f"field energy {fe[0]:.4e} -> {fe[-1]:.4e}, kinetic {ref['kinetic'][0]:.6e} -> {ref['kinetic'][-1]:.6e}",
+          flush=True)
+    # the mode-1 field grows: not a ballistic run
+    assert ref["mode1"][-1] > 1.5 * ref["mode1"][0] and fe[-1] > 1.5 * fe[0]
+    np.testing.assert_allclose(rec["field_energy"], fe, rtol=2e-4)
+    np.testing.assert_allclose(rec["mode1"], ref["mode1"], rtol=2e-4)
+    np.testing.assert_allclose(rec["kinetic"], ref["kinetic"], rtol=1e-6)
+    # without a background the charge error is round-off on both sides: the
+    # device's (FP32 rho, FP64 sums) stays at FP32 level
+    assert np.all(rec["charge_error"] < 1e-5)
+
+
 @pytest.mark.parametrize("scheme,backend,interval", [(1, 2, 1), (1, 2, 10), (2, 2, 3), (1, 1, 0), (3, 2, 2)])
 def test_run_profiled_phases_and_state(hip, scheme, backend, interval):
     """pic_hip_run_profiled (Simulation::runProfiled, simulation.cpp:147-292):
