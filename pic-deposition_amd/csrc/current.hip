@@ -412,13 +412,22 @@ __global__ __launch_bounds__(64 * kCurWaves) void current_tile_kernel(
         float r[kCurV];
 #pragma unroll
         for (int k = 0; k < kCurV; ++k) r[k] = 0.0f;
+        // the item's particles from the stage, all read before the arithmetic
+        // (slots past the item read its first particle)
+        float xs[kCurItem], ys[kCurItem], vxs[kCurItem], vys[kCurItem], qs[kCurItem];
+#pragma unroll
+        for (int v = 0; v < kCurItem; ++v) {
+            const int w = cur_stage_word(cur.p0 + v < cur.p1 ? sl + v : (live ? sl : 0));
+            xs[v] = stage[w];
+            ys[v] = stage[kCurStage + w];
+            vxs[v] = stage[2 * kCurStage + w];
+            vys[v] = stage[3 * kCurStage + w];
+            qs[v] = stage[4 * kCurStage + w];
+        }
 #pragma unroll
         for (int v = 0; v < kCurItem; ++v) {
             if (cur.p0 + v >= cur.p1) break;
-            const int w = cur_stage_word(sl + v);
-            const float xv = stage[w], yv = stage[kCurStage + w];
-            const float vxv = stage[2 * kCurStage + w], vyv = stage[3 * kCurStage + w];
-            const float qq = stage[4 * kCurStage + w];
+            const float xv = xs[v], yv = ys[v], vxv = vxs[v], vyv = vys[v], qq = qs[v];
             int cx, cy;
             float f0x, f1x, f0y, f1y;
             const bool okx = cur_axis<EX>(xv, vxv, dt, dtx, d.inv_dx, d.dx, d.exact, cx, f0x, f1x);
