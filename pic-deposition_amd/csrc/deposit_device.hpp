@@ -421,15 +421,17 @@ __device__ __forceinline__ void process_chunk(const Chunk<Q>& cur, const TileCtx
             }
         }
     }
-    // Combine the runs that straddle lanes.  Lane i's outgoing run is acc
-    // with key ck; it continues lane i-1's outgoing run when hk == ck(i-1).
+    // Combine the runs that straddle lanes within each row of 16 lanes.  Lane
+    // i's outgoing run is acc with key ck; it continues lane i-1's outgoing
+    // run when hk == ck(i-1) and both lanes are in one row (a run crossing a
+    // row boundary is added once per row: four scan steps instead of six).
     const int ck_prev = dpp_i<kWaveShr1, 0xf>(-2, ck);
-    const bool cont = hk >= 0 && hk == ck_prev;
+    const bool cont = hk >= 0 && hk == ck_prev && (threadIdx.x & 15) != 0;
     const float a = (cont && !multi) ? 1.0f : 0.0f;
     float Z[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) Z[k] = acc[k];
-    wave_segmented_scan<V>(Z, a);
+    wave_segmented_scan<V, 4>(Z, a);
     float Zp[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) Zp[k] = dpp_z<kWaveShr1, 0xf>(Z[k]);
